@@ -1,0 +1,64 @@
+// runtime.h -- device context of the fitness evaluator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "cpd.h"
+#include "trace_core.h"
+
+namespace cpd {
+
+// Grow-only device allocation.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    cpd_status reserve(size_t want, bool zero, hipStream_t s);
+    void release();
+    template <class T> T *as() const { return (T *)ptr; }
+};
+
+// Grow-only pinned host allocation (staging for one-copy uploads).
+struct HostBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    cpd_status reserve(size_t want);
+    void release();
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;  // serialises calls on this context
+    int cu_count = 0;
+    int clock_khz = 0;
+    char name[256] = {0};
+    uint32_t waves_per_cu = 16;   // resident wavefronts per CU of the persistent grid
+    uint32_t tile_rays_hint = 0;  // rays per work item (0: the default)
+
+    // beam quantile tables z_k, k = 1..z_n
+    uint32_t z_n = 0;
+    DeviceBuffer z32, z64;
+    // batch state
+    DeviceBuffer stage, acc, last_bin, results, counters, row_buf, prop_buf;
+    HostBuffer host_stage, host_out;
+
+    static cpd_status create(int device, Context **out);
+    ~Context();
+
+    cpd_status ensure_beam_table(uint32_t n);
+    uint32_t choose_tile(uint32_t n_rays) const;
+    cpd_status run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid,
+                   bool persistent, const double *wavelengths, cpd_fitness *out, double *rows,
+                   unsigned long long *ops_out);
+    cpd_status propagate(const cpd_design &g, const float *wcdf, const float *rcdf, size_t n, uint32_t *bin,
+                         float *t);
+};
+
+cpd_status default_context(Context **out);
+cpd_status check_design_public(const cpd_design &g, const cpd_sampling &s);
+
+}  // namespace cpd

@@ -1,0 +1,416 @@
+// runtime.hip -- device contexts and the GPU entry points of cpd.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "cpd.h"
+#include "cpd_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+
+namespace cpd {
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? CPD_ERR_OUT_OF_MEMORY : CPD_ERR_DEVICE, "%s: %s", \
+                        #expr, hipGetErrorString(e_));                                                 \
+    } while (0)
+
+cpd_status DeviceBuffer::reserve(size_t want, bool zero, hipStream_t s) {
+    if (want <= bytes) return CPD_OK;
+    if (ptr) {
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    size_t sz = std::max<size_t>(want, 256);
+    HIP_TRY(hipMalloc(&ptr, sz));
+    bytes = sz;
+    if (zero) HIP_TRY(hipMemsetAsync(ptr, 0, sz, s));
+    return CPD_OK;
+}
+
+void DeviceBuffer::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+cpd_status HostBuffer::reserve(size_t want) {
+    if (want <= bytes) return CPD_OK;
+    if (ptr) HIP_TRY(hipHostFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    size_t sz = std::max<size_t>(want, 4096);
+    HIP_TRY(hipHostMalloc(&ptr, sz, hipHostMallocDefault));
+    bytes = sz;
+    return CPD_OK;
+}
+
+void HostBuffer::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// Fill the beam-quantile tables for rays k = 1..n (grow-only).
+cpd_status Context::ensure_beam_table(uint32_t n) {
+    if (n <= z_n) return CPD_OK;
+    uint32_t want = std::max<uint32_t>(n, 65536);
+    cpd_status st = z32.reserve((size_t)want * sizeof(float), false, stream);
+    if (st != CPD_OK) return st;
+    st = z64.reserve((size_t)want * sizeof(double), false, stream);
+    if (st != CPD_OK) return st;
+    HIP_TRY(launch_beam_table(stream, want, z32.as<float>(), z64.as<double>()));
+    z_n = want;
+    return CPD_OK;
+}
+
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+cpd_status check_design(const cpd_design &g, const cpd_sampling &s) {
+    if (g.n_glass < 1 || g.n_glass > CPD_MAX_PRISMS) return fail(CPD_ERR_INVALID_ARGUMENT, "n_glass out of range");
+    if (g.bin_count < 1 || g.bin_count > CPD_MAX_BINS) return fail(CPD_ERR_INVALID_ARGUMENT, "bin_count out of range");
+    if (s.n_wavelengths < 1 || s.n_rays < 1) return fail(CPD_ERR_INVALID_ARGUMENT, "empty sampling");
+    if (s.n_rays > (1u << 30)) return fail(CPD_ERR_INVALID_ARGUMENT, "n_rays above 2^30");
+    if (s.mode != CPD_MODE_FIXED) return fail(CPD_ERR_INVALID_ARGUMENT, "only CPD_MODE_FIXED is implemented on the GPU");
+    if (!(g.det_slope > 0.0)) return fail(CPD_ERR_INVALID_ARGUMENT, "detector slope must be > 0");
+    return CPD_OK;
+}
+
+}  // namespace
+
+uint32_t Context::choose_tile(uint32_t n_rays) const {
+    // one wavefront per wavelength when that is at most kMaxTileRays rays;
+    // otherwise tiles of the largest size that keeps tiles equal
+    uint32_t want = std::min<uint32_t>(tile_rays_hint ? tile_rays_hint : kMaxTileRays, kMaxTileRays);
+    uint32_t r = (n_rays + 63) / 64 * 64;
+    return std::max<uint32_t>(64, std::min(r, want));
+}
+
+// Stage a batch on the device and run trace + finalize.  `valid` lists the
+// indices of the designs to run; out/rows receive results for them.
+cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid,
+                        bool persistent, const double *wavelengths, cpd_fitness *out, double *rows,
+                        unsigned long long *ops_out) {
+    const uint32_t n = (uint32_t)valid.size();
+    if (n == 0) return CPD_OK;
+    HIP_TRY(hipSetDevice(device));
+    // one tile size for the whole launch
+    uint32_t max_rays = 0, nbin_max = 0;
+    for (uint32_t i : valid) {
+        max_rays = std::max(max_rays, sampling[i].n_rays);
+        nbin_max = std::max(nbin_max, designs[i].bin_count);
+    }
+    const uint32_t tile = choose_tile(max_rays);
+    const uint32_t cols = nbin_max <= 128 ? 64 : 32;
+    // host staging: designs | traces | item_begin | acc_begin | lb_begin | row_begin
+    const size_t off_designs = 0;
+    const size_t off_traces = align_up(off_designs + n * sizeof(cpd_design), 256);
+    const size_t off_items = align_up(off_traces + n * sizeof(TraceDesign), 256);
+    const size_t off_acc = align_up(off_items + (n + 1) * sizeof(uint64_t), 256);
+    const size_t off_lb = align_up(off_acc + n * sizeof(uint64_t), 256);
+    const size_t off_rows = align_up(off_lb + n * sizeof(uint64_t), 256);
+    const size_t off_wl = align_up(off_rows + n * sizeof(uint64_t), 256);
+    const uint32_t n_wl = wavelengths ? sampling[valid[0]].n_wavelengths : 0;
+    const size_t staged = align_up(off_wl + n_wl * sizeof(double), 256);
+    cpd_status st = host_stage.reserve(staged);
+    if (st != CPD_OK) return st;
+    char *h = (char *)host_stage.ptr;
+    cpd_design *hd = (cpd_design *)(h + off_designs);
+    TraceDesign *ht = (TraceDesign *)(h + off_traces);
+    uint64_t *hitem = (uint64_t *)(h + off_items);
+    uint64_t *hacc = (uint64_t *)(h + off_acc);
+    uint64_t *hlb = (uint64_t *)(h + off_lb);
+    uint64_t *hrow = (uint64_t *)(h + off_rows);
+    uint64_t items = 0, acc_n = 0, lb_n = 0;
+    for (uint32_t q = 0; q < n; ++q) {
+        const cpd_design &g = designs[valid[q]];
+        const cpd_sampling &s = sampling[valid[q]];
+        hd[q] = g;
+        make_trace_design(g, s.n_wavelengths, s.n_rays, ht[q]);
+        hitem[q] = items;
+        hacc[q] = acc_n;
+        hlb[q] = lb_n;
+        hrow[q] = acc_n;
+        const uint64_t tiles = (s.n_rays + tile - 1) / tile;
+        items += (uint64_t)s.n_wavelengths * tiles;
+        acc_n += (uint64_t)s.n_wavelengths * g.bin_count;
+        lb_n += s.n_wavelengths;
+    }
+    hitem[n] = items;
+    if (n_wl) memcpy(h + off_wl, wavelengths, n_wl * sizeof(double));
+    if (items >= (1ull << 32)) return fail(CPD_ERR_INVALID_ARGUMENT, "batch too large (>= 2^32 work items)");
+
+    if ((st = ensure_beam_table(max_rays)) != CPD_OK) return st;
+    if ((st = stage.reserve(staged, false, stream)) != CPD_OK) return st;
+    if ((st = acc.reserve(acc_n * sizeof(unsigned long long), true, stream)) != CPD_OK) return st;
+    if ((st = last_bin.reserve(lb_n * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
+    if ((st = results.reserve(n * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
+    if ((st = counters.reserve(64, true, stream)) != CPD_OK) return st;
+    if (rows && (st = row_buf.reserve(acc_n * sizeof(double), false, stream)) != CPD_OK) return st;
+
+    char *dstage = (char *)stage.ptr;
+    HIP_TRY(hipMemcpyAsync(dstage, h, staged, hipMemcpyHostToDevice, stream));
+    unsigned long long *dops = (unsigned long long *)counters.ptr;
+    unsigned int *dwork = (unsigned int *)((char *)counters.ptr + 16);
+    if (ops_out) HIP_TRY(hipMemsetAsync(dops, 0, sizeof(unsigned long long), stream));
+
+    BatchArgs a{};
+    a.designs = (const cpd_design *)(dstage + off_designs);
+    a.traces = (const TraceDesign *)(dstage + off_traces);
+    a.item_begin = (const uint64_t *)(dstage + off_items);
+    a.acc_begin = (const uint64_t *)(dstage + off_acc);
+    a.lb_begin = (const uint64_t *)(dstage + off_lb);
+    a.acc = acc.as<unsigned long long>();
+    a.last_bin = last_bin.as<uint32_t>();
+    a.z32 = z32.as<float>();
+    a.z64 = z64.as<double>();
+    a.ops = ops_out ? dops : nullptr;
+    a.wavelengths = n_wl ? (const double *)(dstage + off_wl) : nullptr;
+    a.n_designs = n;
+    a.tile_rays = tile;
+    a.cols = cols;
+    a.nbin_max = nbin_max;
+    a.n_items = items;
+    uint32_t grid;
+    if (persistent) {
+        HIP_TRY(launch_zero_u32(stream, dwork, 1));
+        a.work_counter = dwork;
+        const uint64_t resident = (uint64_t)cu_count * waves_per_cu;
+        grid = (uint32_t)std::min<uint64_t>(items, resident);
+    } else {
+        a.work_counter = nullptr;
+        grid = (uint32_t)items;
+    }
+    HIP_TRY(launch_trace(stream, a, grid, ops_out != nullptr));
+
+    FinalizeArgs f{};
+    f.designs = a.designs;
+    f.traces = a.traces;
+    f.acc_begin = a.acc_begin;
+    f.lb_begin = a.lb_begin;
+    f.acc = a.acc;
+    f.last_bin = a.last_bin;
+    f.out = out ? results.as<cpd_fitness>() : nullptr;
+    f.rows = rows ? row_buf.as<double>() : nullptr;
+    f.row_begin = (const uint64_t *)(dstage + off_rows);
+    f.n_designs = n;
+    HIP_TRY(launch_finalize(stream, f));
+    if (out) {
+        st = host_out.reserve(n * sizeof(cpd_fitness));
+        if (st != CPD_OK) return st;
+        HIP_TRY(hipMemcpyAsync(host_out.ptr, results.ptr, n * sizeof(cpd_fitness), hipMemcpyDeviceToHost, stream));
+    }
+    if (rows) HIP_TRY(hipMemcpyAsync(rows, row_buf.ptr, acc_n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (ops_out) HIP_TRY(hipMemcpyAsync(ops_out, dops, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (out) {
+        const cpd_fitness *r = (const cpd_fitness *)host_out.ptr;
+        for (uint32_t q = 0; q < n; ++q) out[valid[q]] = r[q];
+    }
+    return CPD_OK;
+}
+
+Context::~Context() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    z32.release();
+    z64.release();
+    stage.release();
+    acc.release();
+    last_bin.release();
+    results.release();
+    counters.release();
+    row_buf.release();
+    prop_buf.release();
+    host_stage.release();
+    host_out.release();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+cpd_status Context::create(int device, Context **out) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CPD_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(CPD_ERR_NO_DEVICE, "device %d not present (%d devices)", device, count);
+    Context *c = new Context();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CPD_ERR_DEVICE, "context creation: %s", hipGetErrorString(e));
+    }
+    c->cu_count = prop.multiProcessorCount;
+    c->clock_khz = prop.clockRate;
+    snprintf(c->name, sizeof c->name, "%s", prop.name);
+    *out = c;
+    return CPD_OK;
+}
+
+namespace {
+std::mutex g_default_mu;
+Context *g_default = nullptr;
+}  // namespace
+
+cpd_status default_context(Context **out) {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default) {
+        cpd_status st = Context::create(0, &g_default);
+        if (st != CPD_OK) return st;
+    }
+    *out = g_default;
+    return CPD_OK;
+}
+
+}  // namespace cpd
+
+using cpd::Context;
+
+static cpd_status resolve(cpd_ctx *ctx, Context **out) {
+    if (ctx) {
+        *out = reinterpret_cast<Context *>(ctx);
+        return CPD_OK;
+    }
+    return cpd::default_context(out);
+}
+
+extern "C" {
+
+cpd_status cpd_ctx_create(int device, cpd_ctx **ctx_out) {
+    if (!ctx_out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL output");
+    Context *c = nullptr;
+    cpd_status st = Context::create(device, &c);
+    if (st != CPD_OK) return st;
+    *ctx_out = reinterpret_cast<cpd_ctx *>(c);
+    return CPD_OK;
+}
+
+void cpd_ctx_destroy(cpd_ctx *ctx) { delete reinterpret_cast<Context *>(ctx); }
+
+cpd_status cpd_ctx_info(cpd_ctx *ctx, char *name, size_t name_len, int *cu_count, int *clock_khz) {
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    if (name && name_len) snprintf(name, name_len, "%s", c->name);
+    if (cu_count) *cu_count = c->cu_count;
+    if (clock_khz) *clock_khz = c->clock_khz;
+    return CPD_OK;
+}
+
+cpd_status cpd_fitness_gpu(cpd_ctx *ctx, const cpd_design *design, const cpd_sampling *sampling, cpd_fitness *out) {
+    if (!design || !sampling || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    cpd_status st = cpd::check_design_public(*design, *sampling);
+    if (st != CPD_OK) return st;
+    Context *c;
+    if ((st = resolve(ctx, &c)) != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    std::vector<uint32_t> valid{0};
+    return c->run(design, sampling, valid, false, nullptr, out, nullptr, nullptr);
+}
+
+cpd_status cpd_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                             cpd_fitness *out, cpd_status *per_design) {
+    if (n == 0) return CPD_OK;
+    if (!designs || !sampling || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n > (1u << 24)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "batch above 2^24 designs");
+    std::vector<uint32_t> valid;
+    valid.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+        cpd_status s = cpd::check_design_public(designs[i], sampling[i]);
+        if (per_design) per_design[i] = s;
+        if (s == CPD_OK) {
+            valid.push_back((uint32_t)i);
+        } else {
+            out[i].size = out[i].info = out[i].deviation = NAN;
+        }
+    }
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    st = c->run(designs, sampling, valid, true, nullptr, out, nullptr, nullptr);
+    if (st != CPD_OK && per_design)
+        for (uint32_t i : valid) per_design[i] = st;
+    return st;
+}
+
+cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const double *wavelengths,
+                                   size_t n_wavelengths, uint32_t max_m, uint32_t mode, double *out) {
+    if (!design || (!wavelengths && n_wavelengths) || (!out && n_wavelengths))
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n_wavelengths == 0) return CPD_OK;
+    if (n_wavelengths > (1u << 24)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "too many wavelengths");
+    cpd_sampling s{(uint32_t)n_wavelengths, max_m, mode, 0};
+    cpd_status st = cpd::check_design_public(*design, s);
+    if (st != CPD_OK) return st;
+    Context *c;
+    if ((st = resolve(ctx, &c)) != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    std::vector<uint32_t> valid{0};
+    return c->run(design, &s, valid, true, wavelengths, nullptr, out, nullptr);
+}
+
+cpd_status cpd_propagate_rays(cpd_ctx *ctx, const cpd_design *design, const float *wavelength_cdf,
+                              const float *ray_cdf, size_t n, uint32_t *bin_out, float *t_out) {
+    if (n == 0) return CPD_OK;
+    if (!design || !wavelength_cdf || !ray_cdf || !bin_out || !t_out)
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n > (1u << 28)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "too many rays");
+    cpd_sampling s{1, 1, CPD_MODE_FIXED, 0};
+    cpd_status st = cpd::check_design_public(*design, s);
+    if (st != CPD_OK) return st;
+    Context *c;
+    if ((st = resolve(ctx, &c)) != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return c->propagate(*design, wavelength_cdf, ray_cdf, n, bin_out, t_out);
+}
+
+}  // extern "C"
+
+namespace cpd {
+
+cpd_status check_design_public(const cpd_design &g, const cpd_sampling &s) { return check_design(g, s); }
+
+cpd_status Context::propagate(const cpd_design &g, const float *wcdf, const float *rcdf, size_t n, uint32_t *bin,
+                              float *t) {
+    HIP_TRY(hipSetDevice(device));
+    const size_t off_g = 0, off_t = align_up(sizeof(cpd_design), 256);
+    const size_t off_w = align_up(off_t + sizeof(TraceDesign), 256);
+    const size_t off_r = align_up(off_w + n * sizeof(float), 256);
+    const size_t off_b = align_up(off_r + n * sizeof(float), 256);
+    const size_t off_o = align_up(off_b + n * sizeof(uint32_t), 256);
+    const size_t total = align_up(off_o + n * sizeof(float), 256);
+    cpd_status st = prop_buf.reserve(total, false, stream);
+    if (st != CPD_OK) return st;
+    if ((st = host_stage.reserve(off_b)) != CPD_OK) return st;
+    char *h = (char *)host_stage.ptr;
+    memcpy(h + off_g, &g, sizeof g);
+    TraceDesign td;
+    make_trace_design(g, 1, 1, td);
+    memcpy(h + off_t, &td, sizeof td);
+    memcpy(h + off_w, wcdf, n * sizeof(float));
+    memcpy(h + off_r, rcdf, n * sizeof(float));
+    char *d = (char *)prop_buf.ptr;
+    HIP_TRY(hipMemcpyAsync(d, h, off_b, hipMemcpyHostToDevice, stream));
+    HIP_TRY(launch_propagate(stream, (const cpd_design *)(d + off_g), (const TraceDesign *)(d + off_t),
+                             (const float *)(d + off_w), (const float *)(d + off_r), (uint32_t)n,
+                             (uint32_t *)(d + off_b), (float *)(d + off_o)));
+    HIP_TRY(hipMemcpyAsync(bin, d + off_b, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(t, d + off_o, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return CPD_OK;
+}
+
+}  // namespace cpd

@@ -1,0 +1,251 @@
+/*
+ * cpd.h -- C ABI of the MI355X compound-prism fitness evaluator.
+ *
+ * The evaluator traces quasi-Monte Carlo rays through a 2D compound prism
+ * (P planar faces and one circular-arc exit lens) onto a linear detector
+ * array, and reduces the per-bin hit probabilities p(d|lambda) to the mutual
+ * information I(Lambda; D), next to the design's size and deviation.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * cpd_status and never lets a C++ exception cross the boundary.  Buffers are
+ * owned by the caller.  Host-only entry points (construction, positioning,
+ * glass fit, workload generation) never touch the GPU and can be called on a
+ * machine without one.
+ *
+ * Each declaration names the reference interface it replaces
+ * (jdtatz/compound-prism-designer, path:line).
+ */
+#ifndef CPD_H
+#define CPD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPD_ABI_VERSION 1
+#define CPD_MAX_PRISMS 6      /* src/pylib.rs:458 MAX_GLASS */
+#define CPD_GLASS_COEFFS 6    /* Glass<F, 6>, src/spectrometer/glasscat.rs:7-22 */
+#define CPD_MAX_BINS 1024
+
+/* Status codes.  1..4 are the reference's RayTraceError variants
+ * (src/spectrometer/ray.rs:11-16), in the same order. */
+typedef enum cpd_status {
+    CPD_OK = 0,
+    CPD_ERR_NO_SURFACE_INTERSECTION = 1,
+    CPD_ERR_OUT_OF_BOUNDS = 2,
+    CPD_ERR_TOTAL_INTERNAL_REFLECTION = 3,
+    CPD_ERR_ANGULAR_RESPONSE_TOO_WEAK = 4,
+    CPD_ERR_INVALID_ARGUMENT = 16,
+    CPD_ERR_NOT_CONVERGED = 17,   /* cuda_fitness returned None, src/cuda_fitness.rs:403-422 */
+    CPD_ERR_NOT_READY = 18,       /* server request still in flight */
+    CPD_ERR_NO_DEVICE = 32,
+    CPD_ERR_DEVICE = 33,          /* CudaError, src/pylib.rs:26-35 */
+    CPD_ERR_OUT_OF_MEMORY = 34,
+    CPD_ERR_CAPACITY = 35         /* request larger than a server's admission budget */
+} cpd_status;
+
+/* Parameters of one spectrometer as the reference's Python constructors take
+ * them: CompoundPrism(glasses, angles, lengths, curvature, height, width,
+ * ar_coated) src/pylib.rs:689-754, DetectorArray(...) src/pylib.rs:867-904,
+ * GaussianBeam(width, y_mean) src/pylib.rs:363-372,
+ * UniformWavelengthDistribution(bounds) src/pylib.rs:304-316.
+ * Lengths are in the caller's unit (cm in the benchmarks), wavelengths in um,
+ * angles in radians. */
+typedef struct cpd_params {
+    uint32_t n_glass;                                   /* P, 1..CPD_MAX_PRISMS */
+    uint32_t ar_coated;                                 /* bool */
+    double glass[CPD_MAX_PRISMS][CPD_GLASS_COEFFS];     /* n(l) coefficients, Horner order */
+    double angles[CPD_MAX_PRISMS + 1];                  /* P + 1 face angles */
+    double lengths[CPD_MAX_PRISMS];                     /* P separation lengths */
+    double curvature;                                   /* signed normalised lens curvature */
+    double height;
+    double width;
+    /* detector array */
+    uint32_t bin_count;
+    uint32_t flipped;                                   /* bool */
+    double bin_size;
+    double linear_slope;
+    double linear_intercept;
+    double det_length;
+    double max_incident_angle;
+    double det_angle;
+    double det_position[2];
+    /* gaussian beam */
+    double beam_width;
+    double y_mean;
+    /* uniform wavelength distribution */
+    double wavelength_lo;
+    double wavelength_hi;
+} cpd_params;
+
+/* A constructed spectrometer (FP64): the resolved geometry the ray trace reads.
+ * Face 0 is the entrance face, faces 1..P-1 the interior faces; the exit face
+ * is the arc.  Mirrors Spectrometer<f64, ...> src/spectrometer/spectrometer.rs:387-394. */
+typedef struct cpd_design {
+    uint32_t n_glass;
+    uint32_t bin_count;
+    uint32_t ar_coated;
+    uint32_t reserved;
+    double glass[CPD_MAX_PRISMS][CPD_GLASS_COEFFS];
+    double plane_point[CPD_MAX_PRISMS][2];
+    double plane_normal[CPD_MAX_PRISMS][2];
+    double height;                                      /* PrismBounds height of every plane */
+    double arc_center[2];
+    double arc_radius;
+    double arc_midpt[2];                                /* RadialBounds centre */
+    double arc_max_dist_sq;                             /* RadialBounds radius_squared */
+    double det_position[2];
+    double det_direction[2];
+    double det_normal[2];
+    double det_min_ci;
+    double det_length;
+    double det_slope;
+    double det_intercept;
+    double det_bin_size;
+    double beam_width;
+    double y_mean;
+    double wavelength_lo;
+    double wavelength_hi;
+} cpd_design;
+
+/* Sampling of one fitness evaluation.
+ * CPD_MODE_FIXED traces every wavelength sample j = 1..n_wavelengths and every
+ * ray k = 1..n_rays (the reference's loops with both SEM thresholds at 0).
+ * CPD_MODE_ADAPTIVE keeps the reference's SEM <= 5e-3 early exits
+ * (src/fitness.rs:20-28, 108-114). */
+enum { CPD_MODE_FIXED = 0, CPD_MODE_ADAPTIVE = 1 };
+
+typedef struct cpd_sampling {
+    uint32_t n_wavelengths;   /* max_n */
+    uint32_t n_rays;          /* max_m */
+    uint32_t mode;
+    uint32_t reserved;
+} cpd_sampling;
+
+/* DesignFitness, src/fitness.rs:36-41. */
+typedef struct cpd_fitness {
+    double size;
+    double info;
+    double deviation;
+} cpd_fitness;
+
+typedef struct cpd_ctx cpd_ctx;
+typedef struct cpd_server cpd_server;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+
+int cpd_abi_version(void);
+/* Name of a status code ("NoSurfaceIntersection", ...): RayTraceError's
+ * Into<&'static str>, src/spectrometer/ray.rs:18-29. */
+const char *cpd_status_name(cpd_status status);
+/* Message of the last failing call on this thread ("" if none). */
+const char *cpd_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side construction (FP64, no GPU)                                     */
+/* ------------------------------------------------------------------------ */
+
+/* detector_array_positioning, src/spectrometer/spectrometer.rs:322-364, as
+ * exposed by position_detector_array, src/pylib.rs:814-838.  Reads the prism,
+ * beam and wavelength fields of params plus det_length and det_angle. */
+cpd_status cpd_position_detector_array(const cpd_params *params, double acceptance,
+                                       double position_out[2], uint32_t *flipped_out);
+
+/* CompoundPrism::new src/spectrometer/ray.rs:137-183 (via
+ * create_joined_trapezoids src/spectrometer/geometry.rs:430-489 and the
+ * Plane/CurvedPlane from_hyperplane constructors geometry.rs:225-243,280-315),
+ * LinearDetectorArray::new src/spectrometer/spectrometer.rs:183-219 and
+ * Spectrometer src/pylib.rs:985-997. */
+cpd_status cpd_build_design(const cpd_params *params, cpd_design *out);
+
+/* Spectrometer::size_and_deviation, src/spectrometer/spectrometer.rs:429-437. */
+cpd_status cpd_size_and_deviation(const cpd_design *design, double *size, double *deviation);
+
+/* The ray leaving the compound prism (origin x, y; direction x, y) for a
+ * beam ray starting at (0, y) with direction (1, 0): CompoundPrism.exit_ray,
+ * src/pylib.rs:563-573, 768-773 via GenericCompoundPrism::propagate,
+ * src/spectrometer/ray.rs:485-499. */
+cpd_status cpd_exit_ray(const cpd_design *design, double y, double wavelength, double out[4]);
+
+/* Propagation path of one ray (Spectrometer.ray_trace, src/pylib.rs:1060-1083;
+ * propagation_path, src/spectrometer/spectrometer.rs:439-464): rows of
+ * (origin x, origin y, direction x, direction y) -- the start, each surface
+ * crossed, and the detector hit.  out holds up to CPD_MAX_PRISMS + 3 rows;
+ * *n_rows receives the count. */
+cpd_status cpd_ray_trace(const cpd_design *design, double wavelength, double y, double *out, uint32_t *n_rows);
+
+/* Sellmeier1 n(l) fitted by a degree-5 Horner polynomial over npts equally
+ * spaced wavelengths in [lo, hi]: AbstractGlass.horner_polynomial_fit /
+ * into_glass, compound_prism_designer/glasscat.py:114-127, 144-152.
+ * b and c are (B1,B2,B3) and (C1,C2,C3).  chisq_out may be NULL. */
+cpd_status cpd_fit_sellmeier1(const double b[3], const double c[3], double lo, double hi,
+                              uint32_t npts, double coeffs_out[CPD_GLASS_COEFFS],
+                              double *chisq_out);
+
+/* Synthetic workloads of the benchmark configurations (no reference
+ * counterpart; the reference's optimiser draws designs in designer.py:203-265).
+ * config 0/1: 3 prisms, 32 bins, 64 wavelengths x 16384 rays;
+ * config 2:   1..6 prisms, 16..128 bins, 32..256 wavelengths x 4096..65536 rays;
+ * config 4:   6 prisms, 256 bins, 1024 wavelengths x 2^20 rays.
+ * Design `index` of a configuration is reproducible on its own.  Draws that
+ * fail construction or positioning are redrawn; *attempts_out (may be NULL)
+ * receives the number of draws used. */
+cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *params_out,
+                               cpd_sampling *sampling_out, uint32_t *attempts_out);
+
+/* ------------------------------------------------------------------------ */
+/* Device context                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* A context owns a HIP stream and its device buffers on one GPU.  Calls on
+ * one context are serialised by the context; different contexts run
+ * independently.  The reference keeps one global context behind a mutex,
+ * src/cuda_fitness.rs:322-363; passing ctx = NULL to the entry points below
+ * uses the same kind of process-wide default context on device 0. */
+cpd_status cpd_ctx_create(int device, cpd_ctx **ctx_out);
+void cpd_ctx_destroy(cpd_ctx *ctx);
+/* Name of the context's GPU, its compute-unit count and clock (kHz). */
+cpd_status cpd_ctx_info(cpd_ctx *ctx, char *name, size_t name_len, int *cu_count, int *clock_khz);
+
+/* ------------------------------------------------------------------------ */
+/* Fitness on the GPU                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Plain one-design launch: the design is copied to the device, traced by one
+ * wavefront per wavelength sample, reduced to DesignFitness on the device and
+ * copied back before return.  Replaces fitness(), src/fitness.rs:130-147, with
+ * the per-design launch structure of src/cuda_fitness.rs:199-245. */
+cpd_status cpd_fitness_gpu(cpd_ctx *ctx, const cpd_design *design, const cpd_sampling *sampling,
+                           cpd_fitness *out);
+
+/* Batched evaluation of n designs with per-design sampling in one persistent
+ * launch.  per_design (may be NULL) receives each design's status.  Replaces
+ * n calls of fitness()/cuda_fitness(), src/fitness.rs:130-147,
+ * src/cuda_fitness.rs:347-423, as driven by designer.py:267-300. */
+cpd_status cpd_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling,
+                             size_t n, cpd_fitness *out, cpd_status *per_design);
+
+/* Per-bin transmission probabilities p(d|l) for the given wavelengths with
+ * max_m rays each; out is row-major [n_wavelengths][bin_count].  Replaces
+ * p_dets_l_wavelength, src/fitness.rs:48-69, as exposed by
+ * Spectrometer.transmission_probability, src/pylib.rs:1028-1058. */
+cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const double *wavelengths,
+                                   size_t n_wavelengths, uint32_t max_m, uint32_t mode, double *out);
+
+/* One ray per (wavelength cdf, ray cdf) pair through the FP32 device trace:
+ * bin index (bin_count on a miss) and transmittance (0 on a miss).  Replaces
+ * propagation_test_kernel, src/spectrometer/kernel.rs:195-224, launched by
+ * src/cuda_fitness.rs:247-317. */
+cpd_status cpd_propagate_rays(cpd_ctx *ctx, const cpd_design *design, const float *wavelength_cdf,
+                              const float *ray_cdf, size_t n, uint32_t *bin_out, float *t_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPD_H */
