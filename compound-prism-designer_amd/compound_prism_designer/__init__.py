@@ -1,0 +1,464 @@
+"""compound_prism_designer -- MI355X fitness evaluator for 2D compound-prism spectrometers.
+
+The Python surface mirrors the reference's pyo3 module (src/pylib.rs,
+compound_prism_designer/compound_prism_designer.pyi): Glass, DesignFitness,
+Vector2D, UnitVector2D, UniformWavelengthDistribution, GaussianBeam,
+CompoundPrism, position_detector_array, DetectorArray, Spectrometer,
+RayTraceError and CudaError.  Construction runs on the host in FP64; every
+fitness and probability evaluation runs on the GPU through libcpd.so
+(include/cpd.h).  There is no CPU fitness path: the methods that the
+reference ran on the CPU (cpu_fitness, transmission_probability) evaluate the
+same estimator on the GPU.
+
+Beyond the reference: fitness_batch() evaluates many designs in one
+persistent launch, and Context selects a GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from ._native import Design, Fitness, Params, Sampling
+
+__all__ = [
+    "RayTraceError",
+    "CudaError",
+    "GpuError",
+    "Glass",
+    "DesignFitness",
+    "Vector2D",
+    "UnitVector2D",
+    "UniformWavelengthDistribution",
+    "GaussianBeam",
+    "FiberBeam",
+    "CompoundPrism",
+    "position_detector_array",
+    "DetectorArray",
+    "Spectrometer",
+    "Context",
+    "fitness_batch",
+    "generate_design",
+]
+
+
+class RayTraceError(Exception):
+    """A ray (or the design's construction) failed: src/pylib.rs:16-24."""
+
+
+class CudaError(Exception):
+    """A GPU error.  The reference's name (src/pylib.rs:26-35) is kept for source compatibility."""
+
+
+GpuError = CudaError
+
+_libm = ctypes.CDLL("libm.so.6")
+_libm.fma.argtypes = [ctypes.c_double] * 3
+_libm.fma.restype = ctypes.c_double
+_fma = _libm.fma
+
+_RAY_ERRORS = {N.ERR_NO_SURFACE_INTERSECTION, N.ERR_OUT_OF_BOUNDS, N.ERR_TOTAL_INTERNAL_REFLECTION,
+               N.ERR_ANGULAR_RESPONSE_TOO_WEAK}
+
+
+def _check(status: int) -> None:
+    if status == N.OK:
+        return
+    if status in _RAY_ERRORS:
+        raise RayTraceError(N.status_name(status))
+    msg = N.last_error() or N.status_name(status)
+    if status == N.ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    raise CudaError(f"{N.status_name(status)}: {msg}")
+
+
+class Glass:
+    """Glass n(l) as a degree-5 polynomial in Horner order (src/pylib.rs:37-95)."""
+
+    ORDER = 5
+
+    def __init__(self, name: str, coefficents: Sequence[float]):
+        c = np.asarray(coefficents, dtype=np.float64).ravel()
+        if c.size != self.ORDER + 1:
+            raise ValueError(f"expected {self.ORDER + 1} coefficients")
+        self.name = str(name)
+        self._c = c
+
+    @property
+    def glass(self) -> list:
+        return self._c.tolist()
+
+    def __call__(self, wavelength: float) -> float:
+        # Horner with fused multiply-adds, as Glass::calc_n (src/spectrometer/glasscat.rs:17-22)
+        s = float(self._c[0])
+        for c in self._c[1:]:
+            s = _fma(s, float(wavelength), float(c))
+        return s
+
+    def __getnewargs__(self):
+        return (self.name, self._c.copy())
+
+    def __repr__(self):
+        return f"Glass({self.name!r}, {self._c.tolist()})"
+
+
+class DesignFitness:
+    """size, info (bits), deviation: src/pylib.rs:103-153."""
+
+    __slots__ = ("size", "info", "deviation")
+
+    def __init__(self, size: float, info: float, deviation: float):
+        self.size = float(size)
+        self.info = float(info)
+        self.deviation = float(deviation)
+
+    def __getnewargs__(self):
+        return (self.size, self.info, self.deviation)
+
+    def __iter__(self):
+        return iter((self.size, self.info, self.deviation))
+
+    def __repr__(self):
+        return f"DesignFitness(size={self.size}, info={self.info}, deviation={self.deviation})"
+
+
+class Vector2D:
+    def __init__(self, x: float, y: float):
+        self.x, self.y = float(x), float(y)
+
+    def __iter__(self):
+        return iter((self.x, self.y))
+
+    def __getnewargs__(self):
+        return (self.x, self.y)
+
+    def __repr__(self):
+        return f"Vector2D({self.x}, {self.y})"
+
+
+class UnitVector2D(Vector2D):
+    def __init__(self, x: float, y: float):
+        super().__init__(x, y)
+        if not math.isclose(math.hypot(x, y), 1.0, rel_tol=1e-3):
+            raise ValueError("not a unit vector")
+
+    def __repr__(self):
+        return f"UnitVector2D({self.x}, {self.y})"
+
+
+class UniformWavelengthDistribution:
+    """Uniform wavelengths in bounds (um): src/pylib.rs:293-325."""
+
+    def __init__(self, bounds):
+        lo, hi = (float(b) for b in bounds)
+        if hi < lo:
+            raise ValueError("lower bound > upper bound")
+        self.bounds = (lo, hi)
+
+    def __getnewargs__(self):
+        return (self.bounds,)
+
+
+class GaussianBeam:
+    """Collimated Gaussian beam, 1/e^2 width and mean height: src/pylib.rs:351-378."""
+
+    def __init__(self, width: float, y_mean: float):
+        self.width = float(width)
+        self.y_mean = float(y_mean)
+
+    def __getnewargs__(self):
+        return (self.width, self.y_mean)
+
+
+class FiberBeam:
+    """Fiber beam of the 3D toric variant (src/pylib.rs:380-430); not evaluated by this build."""
+
+    def __init__(self, core_radius: float, numerical_aperture: float, center_y: float):
+        self.core_radius = float(core_radius)
+        self.numerical_aperture = float(numerical_aperture)
+        self.center_y = float(center_y)
+
+    @property
+    def y_mean(self):
+        return self.center_y
+
+    @property
+    def width(self):
+        return self.core_radius
+
+
+class CompoundPrism:
+    """CompoundPrism(glasses, angles, lengths, curvature, height, width, ar_coated): src/pylib.rs:666-812."""
+
+    def __init__(self, glasses: Sequence[Glass], angles: Sequence[float], lengths: Sequence[float],
+                 curvature, height: float, width: float, ar_coated: bool):
+        glasses = list(glasses)
+        angles = [float(a) for a in angles]
+        lengths = [float(v) for v in lengths]
+        if len(angles) < 2:
+            raise ValueError("`angles` must have at least 2 elements")
+        if len(glasses) > N.MAX_PRISMS:
+            raise ValueError(f"Maximum of {N.MAX_PRISMS} prisms are allowed")
+        if len(glasses) != len(angles) - 1:
+            raise ValueError("len(glasses) + 1 != len(angles)")
+        if len(glasses) != len(lengths):
+            raise ValueError("len(glasses) != len(lengths)")
+        if isinstance(curvature, (tuple, list)):
+            raise NotImplementedError("the 3D toric compound prism is not part of this build")
+        self.glasses = glasses
+        self.angles = angles
+        self.lengths = lengths
+        self.curvature = float(curvature)
+        self.height = float(height)
+        self.width = float(width)
+        self.ar_coated = bool(ar_coated)
+
+    def _fill(self, p: Params) -> None:
+        p.n_glass = len(self.glasses)
+        p.ar_coated = int(self.ar_coated)
+        for i, g in enumerate(self.glasses):
+            for j, c in enumerate(g.glass):
+                p.glass[i][j] = c
+        for i, a in enumerate(self.angles):
+            p.angles[i] = a
+        for i, v in enumerate(self.lengths):
+            p.lengths[i] = v
+        p.curvature = self.curvature
+        p.height = self.height
+        p.width = self.width
+
+    def _design(self) -> Design:
+        p = Params()
+        self._fill(p)
+        p.bin_count = 1
+        p.linear_slope = 1.0
+        p.bin_size = 1.0
+        d = Design()
+        _check(N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)))
+        return d
+
+    def exit_ray(self, y: float, wavelength: float):
+        out = (ctypes.c_double * 4)()
+        _check(N.lib.cpd_exit_ray(ctypes.byref(self._design()), y, wavelength, out))
+        return Vector2D(out[0], out[1]), UnitVector2D(out[2], out[3])
+
+    def final_midpt_and_normal(self):
+        d = self._design()
+        a = self.angles[-1]
+        return Vector2D(d.arc_midpt[0], d.arc_midpt[1]), UnitVector2D(math.cos(a), math.sin(a))
+
+    def __getnewargs__(self):
+        return (self.glasses, self.angles, self.lengths, self.curvature, self.height, self.width, self.ar_coated)
+
+
+def _beam_fields(beam) -> tuple:
+    if not isinstance(beam, GaussianBeam):
+        raise NotImplementedError("No kernel compiled for this Spectrometer")
+    return beam.width, beam.y_mean
+
+
+def position_detector_array(length: float, angle: float, compound_prism: CompoundPrism,
+                            wavelengths: UniformWavelengthDistribution, beam, acceptance: float):
+    """src/pylib.rs:814-838: ((x, y), flipped) of the detector array."""
+    p = Params()
+    compound_prism._fill(p)
+    p.beam_width, p.y_mean = _beam_fields(beam)
+    p.wavelength_lo, p.wavelength_hi = wavelengths.bounds
+    p.det_length = length
+    p.det_angle = angle
+    pos = (ctypes.c_double * 2)()
+    flipped = ctypes.c_uint32()
+    _check(N.lib.cpd_position_detector_array(ctypes.byref(p), acceptance, pos, ctypes.byref(flipped)))
+    return (pos[0], pos[1]), bool(flipped.value)
+
+
+class DetectorArray:
+    """DetectorArray(bin_count, bin_size, linear_slope, linear_intercept, length,
+    max_incident_angle, angle, position, flipped): src/pylib.rs:840-955."""
+
+    def __init__(self, bin_count: int, bin_size: float, linear_slope: float, linear_intercept: float,
+                 length: float, max_incident_angle: float, angle: float, position, flipped: bool):
+        self.bin_count = int(bin_count)
+        self.bin_size = float(bin_size)
+        self.linear_slope = float(linear_slope)
+        self.linear_intercept = float(linear_intercept)
+        self.length = float(length)
+        self.max_incident_angle = float(max_incident_angle)
+        self.angle = float(angle)
+        self.position = (float(position[0]), float(position[1]))
+        self.flipped = bool(flipped)
+
+    def __getnewargs__(self):
+        return (self.bin_count, self.bin_size, self.linear_slope, self.linear_intercept, self.length,
+                self.max_incident_angle, self.angle, self.position, self.flipped)
+
+
+class Context:
+    """A device context (HIP stream and buffers) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(N.lib.cpd_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        name = ctypes.create_string_buffer(256)
+        cu, clk = ctypes.c_int(), ctypes.c_int()
+        _check(N.lib.cpd_ctx_info(self._h, name, 256, ctypes.byref(cu), ctypes.byref(clk)))
+        return {"name": name.value.decode(), "compute_units": cu.value, "clock_khz": clk.value}
+
+    def close(self):
+        if self._h:
+            N.lib.cpd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ctx_handle(ctx: Optional[Context]):
+    return ctx.handle if ctx is not None else None
+
+
+class Spectrometer:
+    """Spectrometer(compound_prism, detector_array, wavelengths, beam): src/pylib.rs:957-1170."""
+
+    def __init__(self, compound_prism: CompoundPrism, detector_array: DetectorArray,
+                 wavelengths: UniformWavelengthDistribution, beam):
+        self.compound_prism = compound_prism
+        self.detector_array = detector_array
+        self.wavelengths = wavelengths
+        self.beam = beam
+
+    def __getnewargs__(self):
+        return (self.compound_prism, self.detector_array, self.wavelengths, self.beam)
+
+    def params(self) -> Params:
+        p = Params()
+        self.compound_prism._fill(p)
+        da = self.detector_array
+        p.bin_count = da.bin_count
+        p.flipped = int(da.flipped)
+        p.bin_size = da.bin_size
+        p.linear_slope = da.linear_slope
+        p.linear_intercept = da.linear_intercept
+        p.det_length = da.length
+        p.max_incident_angle = da.max_incident_angle
+        p.det_angle = da.angle
+        p.det_position[0], p.det_position[1] = da.position
+        p.beam_width, p.y_mean = _beam_fields(self.beam)
+        p.wavelength_lo, p.wavelength_hi = self.wavelengths.bounds
+        return p
+
+    def design(self) -> Design:
+        d = Design()
+        _check(N.lib.cpd_build_design(ctypes.byref(self.params()), ctypes.byref(d)))
+        return d
+
+    @property
+    def position(self):
+        d = self.design()
+        return (d.det_position[0], d.det_position[1])
+
+    @property
+    def direction(self):
+        d = self.design()
+        return (d.det_direction[0], d.det_direction[1])
+
+    def size_and_deviation(self):
+        s, dv = ctypes.c_double(), ctypes.c_double()
+        _check(N.lib.cpd_size_and_deviation(ctypes.byref(self.design()), ctypes.byref(s), ctypes.byref(dv)))
+        return s.value, dv.value
+
+    def cpu_fitness(self, *, max_n: int = 16_384, max_m: int = 16_384, ctx: Optional[Context] = None
+                    ) -> DesignFitness:
+        """fitness(spec, max_n, max_m) (src/pylib.rs:1008-1026, src/fitness.rs:130-147), evaluated on
+        the GPU: max_n wavelength samples x max_m rays, every sample traced."""
+        return self.fitness(max_n=max_n, max_m=max_m, ctx=ctx)
+
+    def fitness(self, *, max_n: int = 64, max_m: int = 16_384, ctx: Optional[Context] = None) -> DesignFitness:
+        s = Sampling(max_n, max_m, N.MODE_FIXED, 0)
+        f = Fitness()
+        _check(N.lib.cpd_fitness_gpu(_ctx_handle(ctx), ctypes.byref(self.design()), ctypes.byref(s),
+                                     ctypes.byref(f)))
+        return DesignFitness(f.size, f.info, f.deviation)
+
+    def transmission_probability(self, wavelengths, *, max_m: int = 16_384, ctx: Optional[Context] = None
+                                 ) -> np.ndarray:
+        """p(d|l) rows for the given wavelengths: src/pylib.rs:1028-1058."""
+        w = np.ascontiguousarray(wavelengths, dtype=np.float64).ravel()
+        out = np.empty((w.size, self.detector_array.bin_count), np.float64)
+        if w.size:
+            _check(N.lib.cpd_p_dets_l_wavelength(
+                _ctx_handle(ctx), ctypes.byref(self.design()), w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                w.size, max_m, N.MODE_FIXED, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def ray_trace(self, wavelength: float, inital_y: float) -> np.ndarray:
+        """Propagation path rows (x, y, ux, uy): src/pylib.rs:1060-1083."""
+        out = np.zeros((N.MAX_PRISMS + 3, 4), np.float64)
+        n = ctypes.c_uint32()
+        _check(N.lib.cpd_ray_trace(ctypes.byref(self.design()), wavelength, inital_y,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+        return out[: n.value].copy()
+
+    def propagate_rays(self, wavelength_cdf, ray_cdf, *, ctx: Optional[Context] = None):
+        """Per-ray bin and transmittance (the reference's propagation test kernel)."""
+        w = np.ascontiguousarray(wavelength_cdf, dtype=np.float32).ravel()
+        r = np.ascontiguousarray(ray_cdf, dtype=np.float32).ravel()
+        if w.size != r.size:
+            raise ValueError("wavelength_cdf and ray_cdf differ in length")
+        bins = np.empty(w.size, np.uint32)
+        ts = np.empty(w.size, np.float32)
+        if w.size:
+            fp = ctypes.POINTER(ctypes.c_float)
+            _check(N.lib.cpd_propagate_rays(_ctx_handle(ctx), ctypes.byref(self.design()), w.ctypes.data_as(fp),
+                                            r.ctypes.data_as(fp), w.size,
+                                            bins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                            ts.ctypes.data_as(fp)))
+        return bins, ts
+
+
+def fitness_batch(designs: Sequence[Design], sampling, *, ctx: Optional[Context] = None):
+    """Evaluate many designs in one persistent launch.
+
+    designs: cpd_design records (Spectrometer.design() or generate_design()).
+    sampling: one (n_wavelengths, n_rays) pair for all, or a sequence of Sampling.
+    Returns (fitness float64 [n, 3], status int32 [n]).
+    """
+    n = len(designs)
+    darr = (Design * n)(*designs)
+    if isinstance(sampling, Sampling):
+        sarr = (Sampling * n)(*([sampling] * n))
+    elif isinstance(sampling, tuple) and len(sampling) == 2 and not isinstance(sampling[0], Sampling):
+        s = Sampling(int(sampling[0]), int(sampling[1]), N.MODE_FIXED, 0)
+        sarr = (Sampling * n)(*([s] * n))
+    else:
+        sarr = (Sampling * n)(*sampling)
+    out = (Fitness * n)()
+    status = (ctypes.c_int * n)()
+    st = N.lib.cpd_fitness_batch(_ctx_handle(ctx), darr, sarr, n, out, status)
+    if st != N.OK and st != N.ERR_INVALID_ARGUMENT:
+        _check(st)
+    res = np.array([[f.size, f.info, f.deviation] for f in out], np.float64).reshape(n, 3)
+    return res, np.array(status[:], np.int32)
+
+
+def generate_design(config: int, index: int):
+    """Synthetic design `index` of benchmark configuration `config` (0, 1, 2 or 4):
+    returns (Params, Design, Sampling, attempts)."""
+    p, s, att = Params(), Sampling(), ctypes.c_uint32()
+    _check(N.lib.cpd_generate_design(config, index, ctypes.byref(p), ctypes.byref(s), ctypes.byref(att)))
+    d = Design()
+    _check(N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)))
+    return p, d, s, att.value

@@ -152,11 +152,24 @@ def lib() -> ctypes.CDLL:
     return L
 
 
+def _as(obj, cls):
+    """Reinterpret a same-layout ctypes struct (e.g. the package's binding of
+    cpd_params / cpd_design) as this module's mirror type."""
+    if isinstance(obj, cls):
+        return obj
+    if ctypes.sizeof(obj) != ctypes.sizeof(cls):
+        raise TypeError(f"{type(obj).__name__} does not have the layout of {cls.__name__}")
+    out = cls()
+    ctypes.memmove(ctypes.byref(out), ctypes.byref(obj), ctypes.sizeof(cls))
+    return out
+
+
 def _ptr(a: np.ndarray, ct):
     return a.ctypes.data_as(ctypes.POINTER(ct))
 
 
 def build_design(params: Params) -> Design:
+    params = _as(params, Params)
     d = Design()
     st = lib().ora_build_design(ctypes.byref(params), ctypes.byref(d))
     if st != 0:
@@ -165,6 +178,7 @@ def build_design(params: Params) -> Design:
 
 
 def build_design_f32(params: Params, acceptance: float) -> Design:
+    params = _as(params, Params)
     d = Design()
     st = lib().ora_build_design_f32(ctypes.byref(params), acceptance, ctypes.byref(d))
     if st != 0:
@@ -173,6 +187,7 @@ def build_design_f32(params: Params, acceptance: float) -> Design:
 
 
 def position_detector_array(params: Params, acceptance: float):
+    params = _as(params, Params)
     pos = (ctypes.c_double * 2)()
     flipped = ctypes.c_uint32()
     st = lib().ora_position_detector_array(ctypes.byref(params), acceptance, pos, ctypes.byref(flipped))
@@ -180,6 +195,7 @@ def position_detector_array(params: Params, acceptance: float):
 
 
 def size_and_deviation(design: Design):
+    design = _as(design, Design)
     s, dv = ctypes.c_double(), ctypes.c_double()
     lib().ora_size_and_deviation(ctypes.byref(design), ctypes.byref(s), ctypes.byref(dv))
     return s.value, dv.value
@@ -201,6 +217,7 @@ def qmc(n: int, kind: int = QRNG_ITERATIVE, precision: int = 64) -> np.ndarray:
 
 def propagate(design: Design, wavelength: float, y: float, precision: int = 64):
     """(status, bin, t) of one ray starting at (0, y)."""
+    design = _as(design, Design)
     b = ctypes.c_uint32()
     if precision == 64:
         t = ctypes.c_double()
@@ -212,6 +229,7 @@ def propagate(design: Design, wavelength: float, y: float, precision: int = 64):
 
 
 def propagate_many(design: Design, wavelength, y, precision: int = 64):
+    design = _as(design, Design)
     w = np.ascontiguousarray(np.broadcast_to(np.asarray(wavelength, np.float64), np.shape(y)))
     y = np.ascontiguousarray(y, np.float64)
     n = y.size
@@ -223,6 +241,7 @@ def propagate_many(design: Design, wavelength, y, precision: int = 64):
 
 
 def propagation_test_f32(design: Design, wavelength_cdf: np.ndarray, ray_cdf: np.ndarray):
+    design = _as(design, Design)
     w = np.ascontiguousarray(wavelength_cdf, np.float32)
     r = np.ascontiguousarray(ray_cdf, np.float32)
     n = w.size
@@ -234,6 +253,7 @@ def propagation_test_f32(design: Design, wavelength_cdf: np.ndarray, ray_cdf: np
 
 
 def ray_trace(design: Design, wavelength: float, y: float) -> np.ndarray:
+    design = _as(design, Design)
     out = np.zeros((MAX_PRISMS + 3, 4), np.float64)
     k = lib().ora_ray_trace_f64(ctypes.byref(design), wavelength, y, _ptr(out, ctypes.c_double))
     return out[:k].copy()
@@ -241,6 +261,7 @@ def ray_trace(design: Design, wavelength: float, y: float) -> np.ndarray:
 
 def p_dets_l_wavelength(design: Design, wavelength: float, max_m: int, precision: int = 64,
                         adaptive: bool = False, qrng: int = QRNG_ITERATIVE):
+    design = _as(design, Design)
     out = np.empty(design.bin_count, np.float64)
     rays = ctypes.c_uint64()
     lib().ora_p_dets_l_wavelength(ctypes.byref(design), wavelength, max_m, precision, int(adaptive), qrng,
@@ -251,6 +272,7 @@ def p_dets_l_wavelength(design: Design, wavelength: float, max_m: int, precision
 def fitness(design: Design, max_n: int, max_m: int, precision: int = 64, adaptive: bool = False,
             qrng: int = QRNG_ITERATIVE):
     """(size, info, deviation), rays traced."""
+    design = _as(design, Design)
     out = (ctypes.c_double * 3)()
     rays = ctypes.c_uint64()
     lib().ora_fitness(ctypes.byref(design), max_n, max_m, precision, int(adaptive), qrng, out, ctypes.byref(rays))
@@ -261,7 +283,7 @@ def fitness_many(designs, max_n, max_m, precision: int = 64, adaptive: bool = Fa
                  qrng: int = QRNG_FIXED_POINT, threads: int = 0):
     """Fitness of many designs over a host thread pool; returns (out[n,3], rays)."""
     n = len(designs)
-    arr = (Design * n)(*designs)
+    arr = (Design * n)(*[_as(d, Design) for d in designs])
     mn = np.ascontiguousarray(np.broadcast_to(np.asarray(max_n, np.uint32), (n,)))
     mm = np.ascontiguousarray(np.broadcast_to(np.asarray(max_m, np.uint32), (n,)))
     out = np.zeros((n, 3), np.float64)

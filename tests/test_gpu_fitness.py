@@ -1,0 +1,130 @@
+"""GPU parity: the HIP fitness path against the FP64 oracle on the same inputs.
+
+Tolerances (from the fixed-count estimator's definition):
+  info        |GPU - oracle FP64| <= 1e-4 bits
+  size, dev   relative <= 1e-5 (both are FP64 on either side)
+Sample indices are the same integers k on both sides (R1 sequence in 64-bit
+fixed point, oracle QRNG_FIXED_POINT)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import compound_prism_designer as cpd
+from compound_prism_designer import _native as N
+from oracle import oracle as O
+import kat
+
+pytestmark = pytest.mark.gpu
+
+INFO_TOL = 1e-4
+REL_TOL = 1e-5
+
+
+def to_oracle(d: N.Design) -> O.Design:
+    od = O.Design()
+    ctypes.memmove(ctypes.byref(od), ctypes.byref(d), ctypes.sizeof(od))
+    return od
+
+
+def kat_design() -> N.Design:
+    p = kat.fill_params(N.Params())
+    pos = (ctypes.c_double * 2)()
+    fl = ctypes.c_uint32()
+    assert N.lib.cpd_position_detector_array(ctypes.byref(p), 1.0, pos, ctypes.byref(fl)) == 0
+    p.det_position[0], p.det_position[1] = pos[0], pos[1]
+    p.flipped = fl.value
+    d = N.Design()
+    assert N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)) == 0
+    return d
+
+
+def gpu_fitness(d, n_lambda, n_rays):
+    s = N.Sampling(n_lambda, n_rays, N.MODE_FIXED, 0)
+    f = N.Fitness()
+    st = N.lib.cpd_fitness_gpu(None, ctypes.byref(d), ctypes.byref(s), ctypes.byref(f))
+    assert st == 0, N.last_error()
+    return np.array([f.size, f.info, f.deviation])
+
+
+def check_close(gpu, ref, label):
+    assert abs(gpu[1] - ref[1]) <= INFO_TOL, f"{label}: info {gpu[1]} vs {ref[1]}"
+    for i in (0, 2):
+        assert abs(gpu[i] - ref[i]) <= REL_TOL * max(abs(ref[i]), 1e-300), f"{label}: {gpu} vs {ref}"
+
+
+def test_kat_fixed_count_matches_oracle():
+    d = kat_design()
+    gpu = gpu_fitness(d, 64, 16384)
+    ref, rays = O.fitness(to_oracle(d), 64, 16384, 64, adaptive=False, qrng=O.QRNG_FIXED_POINT)
+    assert rays == 64 * 16384
+    print("KAT 64x16384 gpu", gpu, "oracle", ref, "dI", gpu[1] - ref[1])
+    check_close(gpu, np.array(ref), "KAT")
+
+
+def test_kat_known_answer_on_gpu():
+    # src/fitness.rs:255-257 with the sample counts the reference's benchmark uses
+    d = kat_design()
+    gpu = gpu_fitness(d, 1024, 16384)
+    assert abs(gpu[0] - kat.KAT_SIZE) <= kat.KAT_SIZE_RMAX * kat.KAT_SIZE
+    assert abs(gpu[1] - kat.KAT_INFO) <= kat.KAT_INFO_RMAX * kat.KAT_INFO
+    assert abs(gpu[2] - kat.KAT_DEVIATION) <= kat.KAT_DEVIATION_RMAX * kat.KAT_DEVIATION
+
+
+@pytest.mark.parametrize("config,n,nl,nr", [(0, 24, 16, 4096), (2, 24, 8, 2048)])
+def test_generated_batch_matches_oracle(config, n, nl, nr):
+    designs = [cpd.generate_design(config, i)[1] for i in range(n)]
+    samp = [N.Sampling(nl, nr, N.MODE_FIXED, 0) for _ in range(n)]
+    res, status = cpd.fitness_batch(designs, samp)
+    assert (status == 0).all()
+    ref, _ = O.fitness_many([to_oracle(d) for d in designs], nl, nr, precision=64, qrng=O.QRNG_FIXED_POINT)
+    err = np.abs(res[:, 1] - ref[:, 1])
+    print(f"config {config}: max |dI| {err.max():.3e}, infos {np.round(ref[:, 1], 4)}")
+    for i in range(n):
+        check_close(res[i], ref[i], f"design {i}")
+
+
+def test_batch_result_independent_of_companions():
+    # integer accumulation: a design's result is bit-identical alone and in a batch
+    designs = [cpd.generate_design(2, i)[1] for i in range(12)]
+    samp = [N.Sampling(8 + i, 1024 + 64 * i, N.MODE_FIXED, 0) for i in range(12)]
+    batch, _ = cpd.fitness_batch(designs, samp)
+    sub, _ = cpd.fitness_batch(designs[5:9][::-1], samp[5:9][::-1])
+    for q, i in enumerate(range(8, 4, -1)):
+        alone = gpu_fitness(designs[i], samp[i].n_wavelengths, samp[i].n_rays)
+        assert np.array_equal(alone, batch[i]), (i, alone, batch[i])
+        assert np.array_equal(sub[q], batch[i])
+
+
+def test_propagation_matches_oracle_per_ray():
+    # src/cuda_fitness.rs:533-607: 1024 seeded (wavelength cdf, ray cdf) pairs on the known design
+    d = kat_design()
+    rng = np.random.default_rng(0)
+    wcdf = rng.random(1024, dtype=np.float32)
+    rcdf = rng.random(1024, dtype=np.float32)
+    bins = np.empty(1024, np.uint32)
+    ts = np.empty(1024, np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    st = N.lib.cpd_propagate_rays(None, ctypes.byref(d), wcdf.ctypes.data_as(fp), rcdf.ctypes.data_as(fp), 1024,
+                                  bins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ts.ctypes.data_as(fp))
+    assert st == 0
+    lam = 0.5 + (0.82 - 0.5) * wcdf.astype(np.float64)
+    y = 0.95 - 0.2 * np.array([O.norminv(float(u)) for u in rcdf])
+    ob, ot = O.propagate_many(to_oracle(d), lam, y, precision=64)
+    assert np.array_equal(bins, ob)
+    hit = ob < 32
+    assert hit.sum() > 100
+    np.testing.assert_allclose(ts[hit], ot[hit], rtol=1e-5)
+
+
+def test_transmission_probability_matches_oracle():
+    d = kat_design()
+    w = np.linspace(0.5, 0.82, 7)
+    out = np.empty((7, 32))
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = N.lib.cpd_p_dets_l_wavelength(None, ctypes.byref(d), w.ctypes.data_as(dp), 7, 4096, N.MODE_FIXED,
+                                       out.ctypes.data_as(dp))
+    assert st == 0
+    for j in range(7):
+        ref, _ = O.p_dets_l_wavelength(to_oracle(d), float(w[j]), 4096, qrng=O.QRNG_FIXED_POINT)
+        np.testing.assert_allclose(out[j], ref, atol=2e-6)
