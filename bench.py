@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the compound-prism fitness evaluator on one MI355X.
+
+Workload (N = 1): 1024 synthetic designs of the 3-prism distribution (32
+bins, 64 wavelengths x 16384 rays each, FP32 trace, FP64 entropy sums),
+evaluated in one persistent launch per step.  A step is one batched
+evaluation of all 1024 designs, with the design records already staged in
+host memory (the upload of ~1 MB of design records is inside the step).
+
+Printed as ONE JSON line:
+  value            designs/s of the batched path (steps x 1024 / timed seconds)
+  plain            designs/s of the plain one-design-per-launch path (the
+                   reference's cuda_fitness structure) and the batched/plain ratio
+  mi_error_bits    max |info_GPU - info_FP64 oracle| over a checked subset
+  roofline         trace kernel: algorithmic operations per launch / mean
+                   kernel time (HIP events on the context's stream) against
+                   MI355X's 157.3 TFLOP/s FP32 peak
+  cpu_baseline     the FP64 oracle (C++ restatement of the reference's
+                   fitness.rs) on the host cores, one design per thread
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "compound-prism-designer_amd"), str(ROOT)]
+
+METRIC = "designs/sec (batched vs plain GPU) at ≤1e-4 bit MI error; p50/p99 req latency"
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--designs", type=int, default=1024)
+    ap.add_argument("--plain", type=int, default=128, help="designs timed through the plain path")
+    ap.add_argument("--check", type=int, default=16, help="designs checked against the FP64 oracle")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (designs shard across GPUs "
+              "with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import ctypes
+
+    import numpy as np
+
+    import compound_prism_designer as cpd
+    from compound_prism_designer import _native as N
+
+    ctx = cpd.Context(0)
+    info = ctx.info()
+    log(f"device {info}")
+    n = args.designs
+    t0 = time.time()
+    gen = [cpd.generate_design(1, i) for i in range(n)]
+    designs = [g[1] for g in gen]
+    attempts = sum(g[3] for g in gen)
+    log(f"generated {n} designs in {time.time() - t0:.1f}s ({attempts} draws)")
+    samp = N.Sampling(64, 16384, N.MODE_FIXED, 0)
+    darr = (N.Design * n)(*designs)
+    sarr = (N.Sampling * n)(*([samp] * n))
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+
+    def batch():
+        st = N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status)
+        if st != 0:
+            raise RuntimeError(N.last_error())
+
+    # algorithmic work of one launch (counted once; the timed runs do not count)
+    ctx.set_option(N.OPT_COUNT_OPS, 1)
+    batch()
+    stats = ctx.last_stats()
+    ops_per_launch = stats["ops"]
+    ctx.set_option(N.OPT_COUNT_OPS, 0)
+    infos = np.array([f.info for f in out])
+    log(f"ops per launch {ops_per_launch:.4e}, items {stats['items']}, grid {stats['grid']}")
+
+    for _ in range(args.warmup):
+        batch()
+    step_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        batch()
+        step_ms.append((time.perf_counter() - t) * 1e3)
+    elapsed = time.perf_counter() - t_start
+    value = n * args.steps / elapsed
+    log(f"batched: {elapsed / args.steps * 1e3:.3f} ms/step, {value:.0f} designs/s")
+
+    # kernel time from HIP events on the context's stream
+    ctx.set_option(N.OPT_PROFILE, 1)
+    trace_ms, fin_ms = [], []
+    for _ in range(max(3, min(args.steps, 10))):
+        batch()
+        s = ctx.last_stats()
+        trace_ms.append(s["trace_ms"])
+        fin_ms.append(s["finalize_ms"])
+    ctx.set_option(N.OPT_PROFILE, 0)
+    trace_mean = float(np.mean(trace_ms))
+    achieved = ops_per_launch / (trace_mean * 1e-3) / 1e12
+    nominal_ops = n * (64 * 16384 * (85 + 42 * 3) + 64 * (12 * 3 + 3 * 32))
+    log(f"trace kernel {trace_mean:.3f} ms, finalize {np.mean(fin_ms):.3f} ms, {achieved:.2f} TFLOP/s")
+
+    # plain path: one design per launch, synchronised and copied back
+    f1 = N.Fitness()
+    n_plain = min(args.plain, n)
+    N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(designs[0]), ctypes.byref(samp), ctypes.byref(f1))
+    lat = []
+    t = time.perf_counter()
+    plain_info = []
+    for i in range(n_plain):
+        t1 = time.perf_counter()
+        st = N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(designs[i]), ctypes.byref(samp), ctypes.byref(f1))
+        lat.append((time.perf_counter() - t1) * 1e3)
+        if st != 0:
+            raise RuntimeError(N.last_error())
+        plain_info.append(f1.info)
+    plain_value = n_plain / (time.perf_counter() - t)
+    plain_equal = bool(np.array_equal(np.array(plain_info), infos[:n_plain]))
+    log(f"plain: {plain_value:.0f} designs/s; identical to batched: {plain_equal}")
+
+    # accuracy against the FP64 oracle on a subset
+    mi_err = None
+    n_check = min(args.check, n)
+    if n_check:
+        from oracle import oracle as O
+        ref, _ = O.fitness_many(designs[:n_check], 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT)
+        mi_err = float(np.max(np.abs(ref[:, 1] - infos[:n_check])))
+        log(f"max |dI| vs FP64 oracle over {n_check} designs: {mi_err:.3e} bits")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        done, t = 0, time.perf_counter()
+        while time.perf_counter() - t < args.cpu_seconds and done < n:
+            chunk = designs[done: done + threads]
+            O.fitness_many(chunk, 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT, threads=threads)
+            done += len(chunk)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": done / cpu_s, "unit": "designs/s", "cores": threads, "kind": "port",
+               "sample": f"{done} of the {n} designs, 64 wavelengths x 16384 rays each, FP64, "
+                         f"fixed-count, one design per thread, {cpu_s:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.2f} designs/s on {threads} threads")
+
+    lat_sorted = sorted(lat)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "designs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 trace, f64 entropy",
+        "data": "synthetic (seeded SplitMix64 designs)",
+        "config": {"workload": f"{n} designs x (3 prisms, 32 bins, 64 wavelengths x 16384 rays), "
+                               "one persistent launch per step"},
+        "plain": {"value": plain_value, "unit": "designs/s", "designs": n_plain,
+                  "batched_over_plain": value / plain_value, "identical_results": plain_equal},
+        "latency_ms": {"plain_p50": float(np.percentile(lat_sorted, 50)),
+                       "plain_p99": float(np.percentile(lat_sorted, 99)),
+                       "batch_p50": float(np.percentile(step_ms, 50)),
+                       "batch_p99": float(np.percentile(step_ms, 99))},
+        "mi_error_bits": mi_err,
+        "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "ops_per_launch": ops_per_launch, "nominal_ops_per_launch": nominal_ops,
+                     "kernel_ms": trace_mean},
+        "cpu_baseline": cpu,
+        "device": info,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -314,6 +314,14 @@ class Context:
         _check(N.lib.cpd_ctx_info(self._h, name, 256, ctypes.byref(cu), ctypes.byref(clk)))
         return {"name": name.value.decode(), "compute_units": cu.value, "clock_khz": clk.value}
 
+    def set_option(self, option: int, value: int) -> None:
+        _check(N.lib.cpd_ctx_set_option(self._h, option, int(value)))
+
+    def last_stats(self) -> dict:
+        st = N.RunStats()
+        _check(N.lib.cpd_ctx_last_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in N.RunStats._fields_}
+
     def close(self):
         if self._h:
             N.lib.cpd_ctx_destroy(self._h)

@@ -95,6 +95,24 @@ class Sampling(ctypes.Structure):
     ]
 
 
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("trace_ms", ctypes.c_double),
+        ("finalize_ms", ctypes.c_double),
+        ("ops", ctypes.c_uint64),
+        ("items", ctypes.c_uint64),
+        ("rays", ctypes.c_uint64),
+        ("grid", ctypes.c_uint32),
+        ("tile_rays", ctypes.c_uint32),
+    ]
+
+
+OPT_PROFILE = 1
+OPT_COUNT_OPS = 2
+OPT_TILE_RAYS = 3
+OPT_WAVES_PER_CU = 4
+
+
 class Fitness(ctypes.Structure):
     _fields_ = [("size", ctypes.c_double), ("info", ctypes.c_double), ("deviation", ctypes.c_double)]
 
@@ -124,6 +142,8 @@ def _load() -> ctypes.CDLL:
         "cpd_ctx_create": ([i32, P(vp)], st),
         "cpd_ctx_destroy": ([vp], None),
         "cpd_ctx_info": ([vp, ctypes.c_char_p, sz, P(i32), P(i32)], st),
+        "cpd_ctx_set_option": ([vp, i32, ctypes.c_int64], st),
+        "cpd_ctx_last_stats": ([vp, P(RunStats)], st),
         "cpd_fitness_gpu": ([vp, P(Design), P(Sampling), P(Fitness)], st),
         "cpd_fitness_batch": ([vp, P(Design), P(Sampling), sz, P(Fitness), P(st)], st),
         "cpd_p_dets_l_wavelength": ([vp, P(Design), P(d), sz, u32, u32, P(d)], st),

@@ -38,6 +38,10 @@ struct Context {
     char name[256] = {0};
     uint32_t waves_per_cu = 16;   // resident wavefronts per CU of the persistent grid
     uint32_t tile_rays_hint = 0;  // rays per work item (0: the default)
+    bool profile = false;
+    bool count_ops = false;
+    cpd_run_stats last{};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
     // beam quantile tables z_k, k = 1..z_n
     uint32_t z_n = 0;

@@ -163,7 +163,8 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     HIP_TRY(hipMemcpyAsync(dstage, h, staged, hipMemcpyHostToDevice, stream));
     unsigned long long *dops = (unsigned long long *)counters.ptr;
     unsigned int *dwork = (unsigned int *)((char *)counters.ptr + 16);
-    if (ops_out) HIP_TRY(hipMemsetAsync(dops, 0, sizeof(unsigned long long), stream));
+    const bool want_ops = ops_out != nullptr || count_ops;
+    if (want_ops) HIP_TRY(hipMemsetAsync(dops, 0, sizeof(unsigned long long), stream));
 
     BatchArgs a{};
     a.designs = (const cpd_design *)(dstage + off_designs);
@@ -175,7 +176,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.last_bin = last_bin.as<uint32_t>();
     a.z32 = z32.as<float>();
     a.z64 = z64.as<double>();
-    a.ops = ops_out ? dops : nullptr;
+    a.ops = want_ops ? dops : nullptr;
     a.wavelengths = n_wl ? (const double *)(dstage + off_wl) : nullptr;
     a.n_designs = n;
     a.tile_rays = tile;
@@ -192,7 +193,9 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
         a.work_counter = nullptr;
         grid = (uint32_t)items;
     }
-    HIP_TRY(launch_trace(stream, a, grid, ops_out != nullptr));
+    if (profile) HIP_TRY(hipEventRecord(ev[0], stream));
+    HIP_TRY(launch_trace(stream, a, grid, want_ops));
+    if (profile) HIP_TRY(hipEventRecord(ev[1], stream));
 
     FinalizeArgs f{};
     f.designs = a.designs;
@@ -206,14 +209,31 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     f.row_begin = (const uint64_t *)(dstage + off_rows);
     f.n_designs = n;
     HIP_TRY(launch_finalize(stream, f));
+    if (profile) HIP_TRY(hipEventRecord(ev[2], stream));
     if (out) {
         st = host_out.reserve(n * sizeof(cpd_fitness));
         if (st != CPD_OK) return st;
         HIP_TRY(hipMemcpyAsync(host_out.ptr, results.ptr, n * sizeof(cpd_fitness), hipMemcpyDeviceToHost, stream));
     }
     if (rows) HIP_TRY(hipMemcpyAsync(rows, row_buf.ptr, acc_n * sizeof(double), hipMemcpyDeviceToHost, stream));
-    if (ops_out) HIP_TRY(hipMemcpyAsync(ops_out, dops, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    unsigned long long ops_count = 0;
+    if (want_ops)
+        HIP_TRY(hipMemcpyAsync(&ops_count, dops, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    if (ops_out) *ops_out = ops_count;
+    last = cpd_run_stats{};
+    last.items = items;
+    last.grid = grid;
+    last.tile_rays = tile;
+    last.ops = ops_count;
+    for (uint32_t q = 0; q < n; ++q) last.rays += (uint64_t)sampling[valid[q]].n_wavelengths * sampling[valid[q]].n_rays;
+    if (profile) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        last.trace_ms = ms;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2]));
+        last.finalize_ms = ms;
+    }
     if (out) {
         const cpd_fitness *r = (const cpd_fitness *)host_out.ptr;
         for (uint32_t q = 0; q < n; ++q) out[valid[q]] = r[q];
@@ -235,6 +255,8 @@ Context::~Context() {
     prop_buf.release();
     host_stage.release();
     host_out.release();
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -251,6 +273,11 @@ cpd_status Context::create(int device, Context **out) {
     if (e != hipSuccess) {
         delete c;
         return fail(CPD_ERR_DEVICE, "context creation: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CPD_ERR_DEVICE, "event creation: %s", hipGetErrorString(e));
     }
     c->cu_count = prop.multiProcessorCount;
     c->clock_khz = prop.clockRate;
@@ -306,6 +333,37 @@ cpd_status cpd_ctx_info(cpd_ctx *ctx, char *name, size_t name_len, int *cu_count
     if (name && name_len) snprintf(name, name_len, "%s", c->name);
     if (cu_count) *cu_count = c->cu_count;
     if (clock_khz) *clock_khz = c->clock_khz;
+    return CPD_OK;
+}
+
+cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value) {
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    switch (option) {
+        case CPD_OPT_PROFILE: c->profile = value != 0; return CPD_OK;
+        case CPD_OPT_COUNT_OPS: c->count_ops = value != 0; return CPD_OK;
+        case CPD_OPT_TILE_RAYS:
+            if (value < 0 || value > (int64_t)cpd::kMaxTileRays || value % 64)
+                return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "tile rays must be a multiple of 64 in [0, 16384]");
+            c->tile_rays_hint = (uint32_t)value;
+            return CPD_OK;
+        case CPD_OPT_WAVES_PER_CU:
+            if (value < 1 || value > 64) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "waves per CU must be in 1..64");
+            c->waves_per_cu = (uint32_t)value;
+            return CPD_OK;
+    }
+    return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown option %d", option);
+}
+
+cpd_status cpd_ctx_last_stats(cpd_ctx *ctx, cpd_run_stats *out) {
+    if (!out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL output");
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *out = c->last;
     return CPD_OK;
 }
 

@@ -212,6 +212,28 @@ void cpd_ctx_destroy(cpd_ctx *ctx);
 /* Name of the context's GPU, its compute-unit count and clock (kHz). */
 cpd_status cpd_ctx_info(cpd_ctx *ctx, char *name, size_t name_len, int *cu_count, int *clock_khz);
 
+/* Context options (no reference counterpart: measurement and scheduling
+ * knobs of this implementation). */
+enum {
+    CPD_OPT_PROFILE = 1,       /* record HIP events around each kernel of a run (0/1) */
+    CPD_OPT_COUNT_OPS = 2,     /* count the algorithmic operations a run performs (0/1) */
+    CPD_OPT_TILE_RAYS = 3,     /* rays per work item, multiple of 64, <= 16384 (0: automatic) */
+    CPD_OPT_WAVES_PER_CU = 4   /* resident wavefronts per CU of the persistent grid */
+};
+cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value);
+
+/* Statistics of the last fitness run on a context. */
+typedef struct cpd_run_stats {
+    double trace_ms;      /* trace kernel, HIP events on the context's stream (CPD_OPT_PROFILE) */
+    double finalize_ms;   /* entropy reduction kernel (CPD_OPT_PROFILE) */
+    uint64_t ops;         /* algorithmic operations (CPD_OPT_COUNT_OPS) */
+    uint64_t items;       /* work items (design, wavelength, ray tile) */
+    uint64_t rays;        /* rays traced */
+    uint32_t grid;        /* workgroups launched */
+    uint32_t tile_rays;
+} cpd_run_stats;
+cpd_status cpd_ctx_last_stats(cpd_ctx *ctx, cpd_run_stats *out);
+
 /* ------------------------------------------------------------------------ */
 /* Fitness on the GPU                                                        */
 /* ------------------------------------------------------------------------ */

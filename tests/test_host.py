@@ -1,0 +1,149 @@
+"""Host-side product code (no GPU): the C ABI loads and exports what cpd.h
+declares; FP64 construction and positioning agree with the oracle bit for bit;
+the Sellmeier fit matches scipy's curve_fit; the workload generator is
+deterministic; the Python surface validates like the reference's pyo3 classes."""
+import ctypes
+import json
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import compound_prism_designer as cpd
+from compound_prism_designer import _native as N
+from oracle import oracle as O
+import kat
+
+ROOT = Path(__file__).resolve().parent.parent
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "cpd.h").read_text()
+    names = set(re.findall(r"\b(cpd_[a-z0-9_]+)\s*\(", header))
+    assert len(names) >= 15
+    for name in sorted(names):
+        assert hasattr(N.lib, name), name
+
+
+def test_struct_layouts_agree():
+    assert ctypes.sizeof(N.Params) == O.lib().ora_sizeof_params()
+    assert ctypes.sizeof(N.Design) == O.lib().ora_sizeof_design()
+    assert [f[0] for f in N.Params._fields_] == [f[0] for f in O.Params._fields_]
+    assert [f[0] for f in N.Design._fields_] == [f[0] for f in O.Design._fields_]
+
+
+def test_status_names_follow_ray_trace_error():
+    # src/spectrometer/ray.rs:18-29
+    assert [N.status_name(i) for i in (1, 2, 3, 4)] == [
+        "NoSurfaceIntersection", "OutOfBounds", "TotalInternalReflection", "SpectrometerAngularResponseTooWeak"]
+
+
+@pytest.mark.parametrize("config", [0, 2, 4])
+def test_construction_is_bit_identical_to_oracle(config):
+    for i in range(25):
+        p, d, s, _ = cpd.generate_design(config, i)
+        assert bytes(O.build_design(p)) == bytes(d)
+        st, pos, fl = O.position_detector_array(p, 1.0)
+        assert st == 0 and pos == (p.det_position[0], p.det_position[1]) and fl == bool(p.flipped)
+        size, dev = ctypes.c_double(), ctypes.c_double()
+        assert N.lib.cpd_size_and_deviation(ctypes.byref(d), ctypes.byref(size), ctypes.byref(dev)) == 0
+        assert (size.value, dev.value) == O.size_and_deviation(d)
+
+
+def test_known_design_size_and_deviation():
+    p = kat.fill_params(N.Params())
+    pos = (ctypes.c_double * 2)()
+    fl = ctypes.c_uint32()
+    assert N.lib.cpd_position_detector_array(ctypes.byref(p), 1.0, pos, ctypes.byref(fl)) == 0
+    p.det_position[0], p.det_position[1] = pos[0], pos[1]
+    p.flipped = fl.value
+    d = N.Design()
+    assert N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)) == 0
+    size, dev = ctypes.c_double(), ctypes.c_double()
+    N.lib.cpd_size_and_deviation(ctypes.byref(d), ctypes.byref(size), ctypes.byref(dev))
+    assert abs(size.value - kat.KAT_SIZE) <= kat.KAT_SIZE_RMAX * kat.KAT_SIZE
+    assert abs(dev.value - kat.KAT_DEVIATION) <= kat.KAT_DEVIATION_RMAX * kat.KAT_DEVIATION
+
+
+def test_ray_trace_and_exit_ray_match_oracle():
+    p, d, _, _ = cpd.generate_design(0, 1)
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        w, y = rng.uniform(0.5, 0.82), rng.uniform(0, p.height)
+        out = np.zeros((9, 4))
+        n = ctypes.c_uint32()
+        assert N.lib.cpd_ray_trace(ctypes.byref(d), w, y, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   ctypes.byref(n)) == 0
+        ref = O.ray_trace(d, w, y)
+        assert n.value == len(ref)
+        np.testing.assert_array_equal(out[: n.value], ref)
+
+
+def test_sellmeier_fit_matches_curve_fit():
+    # compound_prism_designer/glasscat.py:114-127; the least-squares optimum is
+    # unique, so the fitted n(l) agrees closely even where coefficients of the
+    # ill-conditioned monomial basis differ
+    for case in json.loads((GOLDEN / "sellmeier.json").read_text()):
+        b = (ctypes.c_double * 3)(*case["b"])
+        c = (ctypes.c_double * 3)(*case["c"])
+        coeffs = (ctypes.c_double * 6)()
+        chisq = ctypes.c_double()
+        assert N.lib.cpd_fit_sellmeier1(b, c, case["lo"], case["hi"], 100, coeffs, ctypes.byref(chisq)) == 0
+        g = cpd.Glass("fit", list(coeffs))
+        n_ours = np.array([g(w) for w in case["check_w"]])
+        np.testing.assert_allclose(n_ours, case["check_n_fit"], rtol=0, atol=1e-6)
+        # no worse a fit than scipy's
+        assert chisq.value <= case["chisq"] * (1 + 1e-6) + 1e-18
+
+
+def test_generator_is_deterministic_and_in_range():
+    for cfg in (0, 2, 4):
+        for i in (0, 7):
+            a = cpd.generate_design(cfg, i)
+            b = cpd.generate_design(cfg, i)
+            assert bytes(a[0]) == bytes(b[0]) and bytes(a[1]) == bytes(b[1])
+    p, d, s, _ = cpd.generate_design(0, 3)
+    assert (p.n_glass, p.bin_count, s.n_wavelengths, s.n_rays) == (3, 32, 64, 16384)
+    seen_p, seen_bins = set(), set()
+    for i in range(60):
+        p, d, s, _ = cpd.generate_design(2, i)
+        seen_p.add(p.n_glass)
+        seen_bins.add(p.bin_count)
+        assert 32 <= s.n_wavelengths <= 256 and 4096 <= s.n_rays <= 65536 and s.n_rays % 64 == 0
+        assert all(-math.pi / 2 <= p.angles[k] <= math.pi / 2 for k in range(p.n_glass + 1))
+        assert 0.5 <= p.height <= 2 and 0 < p.curvature <= 1
+    assert seen_p == {1, 2, 3, 4, 5, 6} and seen_bins == {16, 32, 64, 128}
+
+
+def test_python_surface_validation():
+    g = cpd.Glass("g", [0, 0, 0, 0, 0, 1.5])
+    assert g(0.6) == 1.5
+    with pytest.raises(ValueError):
+        cpd.CompoundPrism([g] * 7, [0.0] * 8, [1.0] * 7, 0.2, 2.0, 2.0, False)
+    with pytest.raises(ValueError):
+        cpd.CompoundPrism([g], [0.0, 0.1, 0.2], [1.0], 0.2, 2.0, 2.0, False)
+    with pytest.raises(ValueError):
+        cpd.CompoundPrism([g], [0.0, 0.1], [1.0, 2.0], 0.2, 2.0, 2.0, False)
+    with pytest.raises(ValueError):
+        cpd.UniformWavelengthDistribution((0.8, 0.5))
+    with pytest.raises(NotImplementedError):
+        cpd.CompoundPrism([g], [0.0, 0.1], [1.0], ((0.1, 0.2), (0.1, 0.2)), 2.0, 2.0, False)
+
+
+def test_python_api_builds_known_spectrometer():
+    glasses = [cpd.Glass(n, c) for n, c in zip(["N-PK52A", "N-SF57", "N-FK58"], kat.GLASSES)]
+    prism = cpd.CompoundPrism(glasses, [math.radians(a) for a in kat.ANGLES_DEG], [0.0] * 3, 0.21, 2.5, 2.0, False)
+    wl = cpd.UniformWavelengthDistribution((0.5, 0.82))
+    beam = cpd.GaussianBeam(0.2, 0.95)
+    pos, flipped = cpd.position_detector_array(3.2, 0.0, prism, wl, beam, 1.0)
+    det = cpd.DetectorArray(32, 0.1, 0.1, 0.0, 3.2, math.radians(60), 0.0, pos, flipped)
+    spec = cpd.Spectrometer(prism, det, wl, beam)
+    size, dev = spec.size_and_deviation()
+    assert abs(size - kat.KAT_SIZE) <= kat.KAT_SIZE_RMAX * kat.KAT_SIZE
+    assert spec.ray_trace(0.6, 0.95).shape[1] == 4
+    with pytest.raises(cpd.RayTraceError):
+        bad = cpd.CompoundPrism(glasses, [0.0, 1.5, -1.5, 1.5], [0.0] * 3, 0.21, 2.5, 2.0, False)
+        cpd.position_detector_array(3.2, 0.0, bad, wl, beam, 1.0)
