@@ -6,9 +6,11 @@ if libcpd.so is missing or fails to load, importing the package raises.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libcpd.so"
+# CPD_LIBRARY selects an alternate build of libcpd.so (same ABI).
+LIB_PATH = Path(os.environ.get("CPD_LIBRARY") or Path(__file__).resolve().parent / "libcpd.so")
 
 MAX_PRISMS = 6
 COEFFS = 6

@@ -35,6 +35,18 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Copy of a wave-uniform record into scalar registers.
+template <class T>
+__device__ __forceinline__ T to_sgprs(const T &v) {
+    static_assert(sizeof(T) % 4 == 0, "record of 32-bit words");
+    T out;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&v);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
+
 __device__ __forceinline__ uint32_t find_design(const uint64_t *begin, uint32_t n, uint64_t item) {
     uint32_t lo = 0, hi = n;  // begin[lo] <= item < begin[hi]
     while (hi - lo > 1) {
@@ -93,19 +105,25 @@ template <int P, bool COUNT>
 __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     const uint32_t lane = threadIdx.x;
     const uint32_t col = lane & (c.cols - 1);
-    const TraceDesign &td = *c.td;
+    // register copies of the per-item records: the loop stores through
+    // last_bin, so loads through c.td would be repeated every iteration
+    const TraceDesign td = to_sgprs(*c.td);
+    const LambdaConsts lc = to_sgprs(c.lc);
+    const float *__restrict__ z32 = c.z32;
+    uint32_t *__restrict__ hist = c.hist;
+    const uint32_t k_end = c.k_end, n_rays = c.n_rays, nbin = c.nbin, cols = c.cols;
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
-    for (uint32_t base = c.k_begin; base < c.k_end; base += 64) {
+    for (uint32_t base = c.k_begin; base < k_end; base += 64) {
         const uint32_t k = base + lane;
-        const bool active = k < c.k_end;
-        const float z = active ? c.z32[k - 1] : 0.f;
+        const bool active = k < k_end;
+        const float z = active ? z32[k - 1] : 0.f;
         const float y = td.y_mean - td.beam_w * z;
-        const Trace32Result r = trace32<P>(td, c.lc, y);
+        const Trace32Result r = trace32<P, COUNT>(td, lc, y);
         if (COUNT && active) ops += r.ops;
         const bool flagged = active && r.guard;
         if (active && !r.guard) {
-            if (r.bin < c.nbin) atomicAdd(&c.hist[r.bin * c.cols + col], quant32(r.t));
-            if (k == c.n_rays) *c.last_bin = r.bin;
+            if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], quant32(r.t));
+            if (k == n_rays) *c.last_bin = r.bin;
         }
         const uint64_t mask = __ballot(flagged);
         if (mask) {
@@ -138,7 +156,7 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         if (a.work_counter) {
             uint32_t v = 0;
             if (lane == 0) v = atomicAdd(a.work_counter, 1u);
-            item = (uint64_t)__shfl(v, 0);
+            item = (uint64_t)__builtin_amdgcn_readfirstlane(v);
         }
         if (item >= a.n_items) break;
         const uint32_t d = find_design(a.item_begin, a.n_designs, item);
@@ -269,7 +287,7 @@ __global__ void zero_u32_kernel(unsigned int *p, uint32_t n) {
 
 template <int P>
 __device__ Trace32Result trace32_p(const TraceDesign &td, const LambdaConsts &lc, float y) {
-    return trace32<P>(td, lc, y);
+    return trace32<P, false>(td, lc, y);
 }
 
 __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, const float *wcdf, const float *rcdf,

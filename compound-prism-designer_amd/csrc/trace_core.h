@@ -225,27 +225,30 @@ CPD_HD uint32_t trace64(const cpd_design &g, double wavelength, double y, double
 struct alignas(16) TraceDesign {
     uint32_t n_glass, nbin, ar_coated, n_lambda;
     uint32_t n_rays, reserved0, reserved1, reserved2;
-    // entrance face: hit at x = (c0 - y n0y) / n0x for beam height y in [0, h]
-    float c0, n0y, inv_n0x, h;
-    // faces 1..P-1 (slot 0 unused)
-    float px[CPD_MAX_PRISMS], py[CPD_MAX_PRISMS], nx[CPD_MAX_PRISMS], ny[CPD_MAX_PRISMS];
-    // exit lens relative to its chord midpoint m: centre offset c - m,
-    // kappa = |c - m|^2 - R^2, 1/R, bound radius^2
+    // entrance face: a beam ray at height y meets it at x = a0 + b0 y; the
+    // prism spans heights [0, h], tested as |y - h/2| <= h/2
+    float a0, b0, h2, nbin_f;
+    // faces 1..P-1 (slot 0 unused): normal and point.normal
+    float nx[CPD_MAX_PRISMS], ny[CPD_MAX_PRISMS], pn[CPD_MAX_PRISMS];
+    // exit lens relative to its chord midpoint m: centre offset cm = c - m,
+    // kappa = |cm|^2 - R^2, 1/R, bound radius^2, error scales
     float mx, my, cmx, cmy;
     float kappa, inv_r, max_dist_sq, lens_scale;
-    // detector, relative to m: m - P0, normal, direction
+    float bound_scale, reserved3, reserved4, reserved5;
+    // detector, relative to m: m - P0, normal, direction, |pos - L/2| <= L/2
     float mpx, mpy, dnx, dny;
-    float ddx, ddy, min_ci, length;
+    float ddx, ddy, min_ci, half_len;
     float slope, inv_slope, intercept, bin_size;
-    float beam_w, y_mean, reserved5, reserved6;
+    float beam_w, y_mean, reserved6, reserved7;
 };
 
 struct LambdaConsts {
-    float r[CPD_MAX_PRISMS + 1];   // n_{i-1}/n_i at faces 1..P-1, n_{P-1} at the lens
-    float r2[CPD_MAX_PRISMS + 1];  // r^2
-    float dx0, dy0, ts0, tp0;      // beam after the entrance face
-    float rs0;                     // its s-polarised Fresnel amplitude
-    int ok;                        // 0 when the entrance face reflects totally
+    float r[CPD_MAX_PRISMS + 1];     // n_{i-1}/n_i at faces 1..P-1, n_{P-1} at the lens
+    float r2[CPD_MAX_PRISMS + 1];    // r^2
+    float omr2[CPD_MAX_PRISMS + 1];  // 1 - r^2
+    float dx0, dy0, ts0, tp0;        // beam after the entrance face
+    float amp0;                      // angular amplification of the entrance refraction, >= 1
+    int ok;                          // 0 when the entrance face reflects totally
     double lambda;
 };
 
@@ -259,15 +262,15 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.n_rays = n_rays;
     t.reserved0 = t.reserved1 = t.reserved2 = 0;
     const double n0x = g.plane_normal[0][0], n0y = g.plane_normal[0][1];
-    t.c0 = (float)(g.plane_point[0][0] * n0x + g.plane_point[0][1] * n0y);
-    t.n0y = (float)n0y;
-    t.inv_n0x = (float)(1.0 / n0x);
-    t.h = (float)g.height;
+    const double c0 = g.plane_point[0][0] * n0x + g.plane_point[0][1] * n0y;
+    t.a0 = (float)(c0 / n0x);
+    t.b0 = (float)(-n0y / n0x);
+    t.h2 = (float)(g.height * 0.5);
+    t.nbin_f = (float)g.bin_count;
     for (int i = 0; i < CPD_MAX_PRISMS; ++i) {
-        t.px[i] = (float)g.plane_point[i][0];
-        t.py[i] = (float)g.plane_point[i][1];
         t.nx[i] = (float)g.plane_normal[i][0];
         t.ny[i] = (float)g.plane_normal[i][1];
+        t.pn[i] = (float)(g.plane_point[i][0] * g.plane_normal[i][0] + g.plane_point[i][1] * g.plane_normal[i][1]);
     }
     const double cmx = g.arc_center[0] - g.arc_midpt[0], cmy = g.arc_center[1] - g.arc_midpt[1];
     t.mx = (float)g.arc_midpt[0];
@@ -278,6 +281,8 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.inv_r = (float)(1.0 / g.arc_radius);
     t.max_dist_sq = (float)g.arc_max_dist_sq;
     t.lens_scale = (float)(g.arc_radius + sqrt(g.arc_max_dist_sq));
+    t.bound_scale = (float)(4.0 * (sqrt(g.arc_max_dist_sq) + 1.0));
+    t.reserved3 = t.reserved4 = t.reserved5 = 0.f;
     t.mpx = (float)(g.arc_midpt[0] - g.det_position[0]);
     t.mpy = (float)(g.arc_midpt[1] - g.det_position[1]);
     t.dnx = (float)g.det_normal[0];
@@ -285,14 +290,14 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.ddx = (float)g.det_direction[0];
     t.ddy = (float)g.det_direction[1];
     t.min_ci = (float)g.det_min_ci;
-    t.length = (float)g.det_length;
+    t.half_len = (float)(g.det_length * 0.5);
     t.slope = (float)g.det_slope;
     t.inv_slope = (float)(1.0 / g.det_slope);
     t.intercept = (float)g.det_intercept;
     t.bin_size = (float)g.det_bin_size;
     t.beam_w = (float)g.beam_width;
     t.y_mean = (float)g.y_mean;
-    t.reserved5 = t.reserved6 = 0.f;
+    t.reserved6 = t.reserved7 = 0.f;
 }
 
 // Wavelength sample j (1-based) of the R1 sequence and its per-face
@@ -317,11 +322,11 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.ts0 = (float)ray.ts;
     lc.tp0 = (float)ray.tp;
     {
-        // s amplitude of the entrance refraction of the direction (1, 0)
-        double n0 = n[0], ci = -g.plane_normal[0][0];
-        double cr2 = 1.0 - (1.0 / n0) * (1.0 / n0) * (1.0 - ci * ci);
+        double ci = -g.plane_normal[0][0], r = 1.0 / n[0];
+        double cr2 = 1.0 - r * r * (1.0 - ci * ci);
         double cr = cr2 > 0.0 ? sqrt(cr2) : 0.0;
-        lc.rs0 = (float)((ci - n0 * cr) / (ci + n0 * cr));
+        double a = cr > 0.0 ? r * ci / cr : 1e30;
+        lc.amp0 = (float)(a > 1.0 ? a : 1.0);
     }
     for (int i = 0; i <= CPD_MAX_PRISMS; ++i) {
         double r = 1.0;
@@ -329,6 +334,7 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
         if (i == P) r = n[P - 1];
         lc.r[i] = (float)r;
         lc.r2[i] = (float)(r * r);
+        lc.omr2[i] = (float)(1.0 - r * r);
     }
 }
 
@@ -338,13 +344,16 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
 #if defined(__HIP_DEVICE_COMPILE__)
 CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// true when any lane of the wavefront has b set
+__device__ inline bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 #else
 CPD_HD float fast_rcp(float x) { return 1.0f / x; }
 CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
+inline bool wave_any(bool b) { return b; }
 #endif
 
-// Relative error budget of an FP32 position or distance per unit of path
-// length travelled, and of dimensionless quantities (cosines, cos^2).
+// Error budget of an FP32 length per unit of path travelled, and of an FP32
+// cosine, before angular amplification.
 static constexpr float kGuardRel = 8.0e-6f;
 static constexpr float kGuardCos = 2.0e-5f;
 
@@ -360,176 +369,176 @@ struct Trace32Result {
     uint32_t bin;  // nbin on a miss
     float t;
     bool guard;    // the outcome is within the FP32 error margin of a decision
-    uint32_t ops;  // algorithmic operations of the stages the ray reached
+    uint32_t ops;  // algorithmic operations of the stages the ray reached (COUNT)
 };
 
-CPD_HD bool near_f(float a, float b, float eps) { return fabsf(a - b) <= eps; }
-
-// Margin bookkeeping: a refraction with angular amplification
-// A = r cos(ti) / cos(tt) multiplies the direction error it receives by A.
-// With rs = (A - 1) / (A + 1) (the s-polarised Fresnel amplitude the trace
-// computes anyway), A = (1 + rs) / (1 - rs); the running product of
-// max(A, 1) is kept as num / den so that margin tests need no division:
-// |x - b| <= kGuardRel * scale * num / den  <=>  |x - b| * den <= kGuardRel * scale * num.
-struct Margin {
-    float num = 1.f, den = 1.f;
-    CPD_HD void refraction(float rs) {
-        float a = fmaxf(rs, 0.f);
-        num *= 1.f + a;
-        den *= 1.f - a;
-    }
-    CPD_HD bool near(float x, float b, float scale) const { return fabsf(x - b) * den <= kGuardRel * scale * num; }
+// Snell refraction of (dx, dy) at unit normal (nx, ny) with ci = -d.n and
+// ratio r = n1/n2 (r2 = r^2, omr2 = 1 - r^2), plus the Fresnel factors
+// 1 - rs^2 and 1 - rp^2 with rs = (r ci - cr) / (r ci + cr) and
+// rp = (r cr - ci) / (r cr + ci) (the reference's amplitudes divided through
+// by n2) from one reciprocal.  The angular amplification of the refraction,
+// A = r ci / cr, is folded into the margin as amp_num / amp_den.
+struct Refract32 {
+    float vx, vy, fs, fp;
+    float cr_sq;
+    float amp_num, amp_den;
 };
 
-template <int P>
+CPD_HD Refract32 refract32(float dx, float dy, float nx, float ny, float ci, float r, float r2, float omr2) {
+    Refract32 o;
+    o.cr_sq = fmaf(r2, ci * ci, omr2);
+    const float cr = fast_sqrt(fmaxf(o.cr_sq, 0.f));
+    const float rci = r * ci;
+    const float k = rci - cr;
+    o.vx = fmaf(nx, k, dx * r);
+    o.vy = fmaf(ny, k, dy * r);
+    const float a = rci + cr;
+    const float b = fmaf(r, cr, ci);
+    const float inv = fast_rcp(a * b);
+    const float as = k * (b * inv);
+    const float ap = fmaf(r, cr, -ci) * (a * inv);
+    o.fs = fmaf(-as, as, 1.f);
+    o.fp = fmaf(-ap, ap, 1.f);
+    o.amp_num = fmaxf(rci, cr);
+    o.amp_den = cr;
+    return o;
+}
+
+template <int P, bool COUNT>
 CPD_HD Trace32Result trace32(const TraceDesign &d, const LambdaConsts &lc, float y) {
-    Trace32Result res{d.nbin, 0.f, false, kOpsBeam + kOpsFaceHit};
-    Margin m;
-    // entrance face: the beam starts at (0, y) travelling +x and meets the
-    // face at height y, so only the height bound decides
-    float scale = fabsf(y) + 1.f;
-    res.guard = m.near(y, 0.f, scale) || m.near(y, d.h, scale);
-    if (!(y >= 0.f && y <= d.h)) return res;
-    float ox = (d.c0 - y * d.n0y) * d.inv_n0x;
-    float oy = y;
-    res.guard = res.guard || m.near(ox, 0.f, scale);
-    if (!(ox >= 0.f)) return res;
-    float path = fabsf(ox) + scale;
-    res.ops += kOpsFaceRefract;
-    float dx = lc.dx0, dy = lc.dy0, ts = lc.ts0, tp = lc.tp0;
-    m.refraction(lc.rs0);
+    Trace32Result res{d.nbin, 0.f, false, 0};
     const bool ar = d.ar_coated != 0;
+    // entrance face: the beam ray (0, y) travelling +x meets it at height y
+    const float ay = y - d.h2;
+    bool alive = fabsf(ay) <= d.h2;
+    bool guard = fabsf(fabsf(ay) - d.h2) <= kGuardRel * (fabsf(y) + 1.f);
+    float ox = fmaf(d.b0, y, d.a0), oy = y;
+    alive = alive && ox >= 0.f;
+    float path = fabsf(ox) + fabsf(y) + 1.f;
+    float num = lc.amp0, den = 1.f;
+    float dx = lc.dx0, dy = lc.dy0, ts = lc.ts0, tp = lc.tp0;
+    if (COUNT) res.ops = kOpsBeam + kOpsFaceHit + (alive ? kOpsFaceRefract : 0u);
 #pragma unroll
     for (int i = 1; i < P; ++i) {
-        res.ops += kOpsFaceHit;
+        if (!wave_any(alive)) {
+            res.guard = guard;
+            return res;
+        }
+        if (COUNT) res.ops += alive ? kOpsFaceHit : 0u;
         const float nx = d.nx[i], ny = d.ny[i];
-        float ci_p = dx * nx + dy * ny;
-        float num = (d.px[i] - ox) * nx + (d.py[i] - oy) * ny;
-        float dist = num * fast_rcp(ci_p);
+        const float ci_p = fmaf(dx, nx, dy * ny);
+        const float dist = (d.pn[i] - fmaf(ox, nx, oy * ny)) * fast_rcp(ci_p);
+        const float hx = fmaf(dx, dist, ox), hy = fmaf(dy, dist, oy);
         path += fabsf(dist);
-        res.guard = res.guard || m.near(dist, 0.f, path);
-        if (!(dist >= 0.f)) return res;
-        float hx = fmaf(dx, dist, ox), hy = fmaf(dy, dist, oy);
-        res.guard = res.guard || m.near(hy, 0.f, path) || m.near(hy, d.h, path);
-        if (!(hy >= 0.f && hy <= d.h)) return res;
+        const float a = hy - d.h2;
+        guard = guard || (alive && fabsf(fabsf(a) - d.h2) * den <= kGuardRel * path * num);
+        alive = alive && dist >= 0.f && fabsf(a) <= d.h2;
+        if (COUNT) res.ops += alive ? kOpsFaceRefract : 0u;
         ox = hx;
         oy = hy;
-        res.ops += kOpsFaceRefract;
-        float ci = -ci_p;
-        float r = lc.r[i];
-        float cr_sq = 1.f - lc.r2[i] * (1.f - ci * ci);
-        res.guard = res.guard || fabsf(cr_sq) <= kGuardCos;
-        if (!(cr_sq >= 0.f)) return res;
-        float cr = fast_sqrt(cr_sq);
-        float rci = r * ci;
-        float k = rci - cr;
-        float vx = dx * r + nx * k;
-        float vy = dy * r + ny * k;
-        float a = k * fast_rcp(rci + cr);
-        if (ar && ci > 0.5f) {
-            ts *= 0.99f;
-            tp *= 0.99f;
+        const float ci = -ci_p;
+        const Refract32 f = refract32(dx, dy, nx, ny, ci, lc.r[i], lc.r2[i], lc.omr2[i]);
+        guard = guard || (alive && fabsf(f.cr_sq) <= kGuardCos);
+        alive = alive && f.cr_sq >= 0.f;
+        if (ar) {
+            guard = guard || (alive && fabsf(ci - 0.5f) * den <= kGuardCos * num);
+            const bool coat = ci > 0.5f;
+            ts *= coat ? 0.99f : f.fs;
+            tp *= coat ? 0.99f : f.fp;
         } else {
-            float rcr = r * cr;
-            float b = (rcr - ci) * fast_rcp(rcr + ci);
-            ts *= 1.f - a * a;
-            tp *= 1.f - b * b;
+            ts *= f.fs;
+            tp *= f.fp;
         }
-        if (ar) res.guard = res.guard || near_f(ci, 0.5f, kGuardCos);
-        m.refraction(a);
-        dx = vx;
-        dy = vy;
+        num *= f.amp_num;
+        den *= f.amp_den;
+        dx = f.vx;
+        dy = f.vy;
+    }
+    if (!wave_any(alive)) {
+        res.guard = guard;
+        return res;
     }
     // exit lens, relative to its chord midpoint m: e = o - m, c = m + cm.
     // |o + s dir - c|^2 = R^2  <=>  s^2 + 2 b s + c2 = 0 with
-    // b = dir.(e - cm), c2 = |e|^2 - 2 e.cm + (|cm|^2 - R^2); the roots are
-    // formed without cancellation (their product is c2).
+    // b = dir.(e - cm) and c2 = |e|^2 - 2 e.cm + (|cm|^2 - R^2); the root of
+    // larger magnitude is -(b + sign(b) sqrt(disc)) and the other one is
+    // c2 over it, so neither root cancels.
+    if (COUNT) res.ops += alive ? kOpsLensHit : 0u;
     float qx, qy;
-    res.ops += kOpsLensHit;
     {
-        float ex = ox - d.mx, ey = oy - d.my;
-        float b = dx * (ex - d.cmx) + dy * (ey - d.cmy);
-        float c2 = (ex * ex + ey * ey) - 2.f * (ex * d.cmx + ey * d.cmy) + d.kappa;
-        float disc = b * b - c2;
-        res.guard = res.guard || m.near(disc, 0.f, 4.f * (fabsf(b) + d.lens_scale) * path);
-        if (!(disc > 0.f)) return res;
-        float sd = fast_sqrt(disc);
-        float d1, d2;
-        if (b > 0.f) {
-            d1 = -b - sd;
-            d2 = c2 * fast_rcp(d1);
-        } else {
-            d2 = -b + sd;
-            d1 = c2 * fast_rcp(d2);
-        }
-        float q1x = fmaf(dx, d1, ex), q1y = fmaf(dy, d1, ey);
-        float q1 = q1x * q1x + q1y * q1y;
-        const float bscale = 4.f * (sqrtf(d.max_dist_sq) + 1.f) * path;
-        res.guard = res.guard || m.near(d1, 0.f, path) || m.near(q1, d.max_dist_sq, bscale);
-        if (!signbit(d1) && q1 <= d.max_dist_sq) {
-            qx = q1x;
-            qy = q1y;
-            path += fabsf(d1);
-        } else {
-            float q2x = fmaf(dx, d2, ex), q2y = fmaf(dy, d2, ey);
-            float q2 = q2x * q2x + q2y * q2y;
-            res.guard = res.guard || m.near(d2, 0.f, path) || m.near(q2, d.max_dist_sq, bscale);
-            if (!(!signbit(d2) && q2 <= d.max_dist_sq)) return res;
-            qx = q2x;
-            qy = q2y;
-            path += fabsf(d2);
-        }
-        res.ops += kOpsLensRefract;
+        const float ex = ox - d.mx, ey = oy - d.my;
+        const float b = fmaf(dx, ex - d.cmx, dy * (ey - d.cmy));
+        const float e2 = fmaf(ex, ex, ey * ey);
+        const float ec = fmaf(ex, d.cmx, ey * d.cmy);
+        const float c2 = fmaf(-2.f, ec, e2 + d.kappa);
+        const float disc = fmaf(b, b, -c2);
+        const float lim = kGuardRel * path * num;
+        guard = guard || (alive && fabsf(disc) * den <= lim * 4.f * (fabsf(b) + d.lens_scale));
+        alive = alive && disc > 0.f;
+        const float sd = fast_sqrt(fmaxf(disc, 0.f));
+        const float big = -(b + copysignf(sd, b));
+        const float small = c2 * fast_rcp(big);
+        const bool bpos = !signbit(b);
+        const float d1 = bpos ? big : small, d2 = bpos ? small : big;
+        const float q1x = fmaf(dx, d1, ex), q1y = fmaf(dy, d1, ey);
+        const float q2x = fmaf(dx, d2, ex), q2y = fmaf(dy, d2, ey);
+        const float s1 = fmaf(q1x, q1x, q1y * q1y), s2 = fmaf(q2x, q2x, q2y * q2y);
+        const bool ok1 = d1 >= 0.f && s1 <= d.max_dist_sq;
+        const bool ok2 = d2 >= 0.f && s2 <= d.max_dist_sq;
+        const float blim = lim * d.bound_scale;
+        guard = guard || (alive && (fabsf(s1 - d.max_dist_sq) * den <= blim ||
+                                    (!ok1 && fabsf(s2 - d.max_dist_sq) * den <= blim)));
+        alive = alive && (ok1 || ok2);
+        if (COUNT) res.ops += alive ? kOpsLensRefract : 0u;
+        qx = ok1 ? q1x : q2x;
+        qy = ok1 ? q1y : q2y;
+        path += fabsf(ok1 ? d1 : d2);
         // normal (c - p) / R = (cm - q) / R
-        float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
-        float ci = -(dx * nx + dy * ny);
-        float r = lc.r[P];
-        float cr_sq = 1.f - lc.r2[P] * (1.f - ci * ci);
-        res.guard = res.guard || fabsf(cr_sq) <= kGuardCos;
-        if (!(cr_sq >= 0.f)) return res;
-        float cr = fast_sqrt(cr_sq);
-        float rci = r * ci;
-        float k = rci - cr;
-        float vx = dx * r + nx * k;
-        float vy = dy * r + ny * k;
-        float a = k * fast_rcp(rci + cr);
-        if (ar && ci > 0.5f) {
-            ts *= 0.99f;
-            tp *= 0.99f;
+        const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
+        const float ci = -fmaf(dx, nx, dy * ny);
+        const Refract32 f = refract32(dx, dy, nx, ny, ci, lc.r[P], lc.r2[P], lc.omr2[P]);
+        guard = guard || (alive && fabsf(f.cr_sq) <= kGuardCos);
+        alive = alive && f.cr_sq >= 0.f;
+        if (ar) {
+            guard = guard || (alive && fabsf(ci - 0.5f) * den <= kGuardCos * num);
+            const bool coat = ci > 0.5f;
+            ts *= coat ? 0.99f : f.fs;
+            tp *= coat ? 0.99f : f.fp;
         } else {
-            float rcr = r * cr;
-            float bb = (rcr - ci) * fast_rcp(rcr + ci);
-            ts *= 1.f - a * a;
-            tp *= 1.f - bb * bb;
+            ts *= f.fs;
+            tp *= f.fp;
         }
-        if (ar) res.guard = res.guard || near_f(ci, 0.5f, kGuardCos);
-        m.refraction(a);
-        dx = vx;
-        dy = vy;
+        num *= f.amp_num;
+        den *= f.amp_den;
+        dx = f.vx;
+        dy = f.vy;
     }
     // detector array: w = o - P0 = (m - P0) + q
-    res.ops += kOpsDetector;
+    if (COUNT) res.ops += alive ? kOpsDetector : 0u;
     {
-        float ci = -(dx * d.dnx + dy * d.dny);
-        res.guard = res.guard || m.near(ci, d.min_ci, kGuardCos / kGuardRel);
-        if (!(ci > d.min_ci)) return res;
-        float wx = d.mpx + qx, wy = d.mpy + qy;
-        float dist = (wx * d.dnx + wy * d.dny) * fast_rcp(ci);
-        float ux = fmaf(dx, dist, wx), uy = fmaf(dy, dist, wy);
-        float pos = ux * d.ddx + uy * d.ddy;
-        float scale_d = path + fabsf(dist) + fabsf(wx) + fabsf(wy);
-        res.guard = res.guard || m.near(pos, 0.f, scale_d) || m.near(pos, d.length, scale_d);
-        if (!(pos >= 0.f && pos <= d.length)) return res;
-        float v = pos - d.intercept;
-        float qb = floorf(v * d.inv_slope);
-        float rem = fmaf(-qb, d.slope, v);
-        res.guard = res.guard || m.near(rem, 0.f, scale_d) || m.near(rem, d.slope, scale_d) ||
-                    m.near(rem, d.bin_size, scale_d);
-        if (!(qb >= 0.f && qb < (float)d.nbin && rem < d.bin_size)) return res;
-        res.bin = (uint32_t)qb;
-        res.t = (ts + tp) * 0.5f;
-        res.ops += kOpsHit;
+        const float ci = -fmaf(dx, d.dnx, dy * d.dny);
+        guard = guard || (alive && fabsf(ci - d.min_ci) * den <= kGuardCos * num);
+        alive = alive && ci > d.min_ci;
+        const float wx = d.mpx + qx, wy = d.mpy + qy;
+        const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(ci);
+        const float pos = fmaf(fmaf(dx, dist, wx), d.ddx, fmaf(dy, dist, wy) * d.ddy);
+        const float lim = kGuardRel * (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * num;
+        const float ap = pos - d.half_len;
+        guard = guard || (alive && fabsf(fabsf(ap) - d.half_len) * den <= lim);
+        alive = alive && fabsf(ap) <= d.half_len;
+        const float v = pos - d.intercept;
+        const float qb = floorf(v * d.inv_slope);
+        const float rem = fmaf(-qb, d.slope, v);
+        const float edge = fminf(fminf(fabsf(rem), fabsf(d.slope - rem)), fabsf(rem - d.bin_size));
+        guard = guard || (alive && edge * den <= lim);
+        alive = alive && qb >= 0.f && qb < d.nbin_f && rem < d.bin_size;
+        if (alive) {
+            res.bin = (uint32_t)qb;
+            res.t = (ts + tp) * 0.5f;
+        }
+        if (COUNT) res.ops += alive ? kOpsHit : 0u;
     }
+    res.guard = guard;
     return res;
 }
 
