@@ -107,8 +107,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     const uint32_t col = lane & (c.cols - 1);
     // register copies of the per-item records: the loop stores through
     // last_bin, so loads through c.td would be repeated every iteration
-    const TraceDesign td = to_sgprs(*c.td);
-    const LambdaConsts lc = to_sgprs(c.lc);
+    const HotRecord<P> hot = to_sgprs(make_hot<P>(*c.td, c.lc));
     const float *__restrict__ z32 = c.z32;
     uint32_t *__restrict__ hist = c.hist;
     const uint32_t k_end = c.k_end, n_rays = c.n_rays, nbin = c.nbin, cols = c.cols;
@@ -117,8 +116,8 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         const uint32_t k = base + lane;
         const bool active = k < k_end;
         const float z = active ? z32[k - 1] : 0.f;
-        const float y = td.y_mean - td.beam_w * z;
-        const Trace32Result r = trace32<P, COUNT>(td, lc, y);
+        const float y = hot.y_mean - hot.beam_w * z;
+        const Trace32Result r = trace32<P, COUNT>(hot, y);
         if (COUNT && active) ops += r.ops;
         const bool flagged = active && r.guard;
         if (active && !r.guard) {
@@ -287,7 +286,7 @@ __global__ void zero_u32_kernel(unsigned int *p, uint32_t n) {
 
 template <int P>
 __device__ Trace32Result trace32_p(const TraceDesign &td, const LambdaConsts &lc, float y) {
-    return trace32<P, false>(td, lc, y);
+    return trace32<P, false>(make_hot<P>(td, lc), y);
 }
 
 __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, const float *wcdf, const float *rcdf,

@@ -344,12 +344,9 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
 #if defined(__HIP_DEVICE_COMPILE__)
 CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-// true when any lane of the wavefront has b set
-__device__ inline bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 #else
 CPD_HD float fast_rcp(float x) { return 1.0f / x; }
 CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
-inline bool wave_any(bool b) { return b; }
 #endif
 
 // Error budget of an FP32 length per unit of path travelled, and of an FP32
@@ -404,8 +401,76 @@ CPD_HD Refract32 refract32(float dx, float dy, float nx, float ny, float ci, flo
     return o;
 }
 
+// The constants one ray of a P-glass design at one wavelength reads: the
+// hot subset of TraceDesign and LambdaConsts, small enough to live in scalar
+// registers for the whole ray loop.  Faces 1..P-1 are stored at [0, P-2].
+template <int P>
+struct alignas(4) HotRecord {
+    static constexpr int F = P > 1 ? P - 1 : 1;
+    uint32_t nbin, ar_coated;
+    float a0, b0, h2, y_mean, beam_w, nbin_f;
+    float dx0, dy0, ts0, tp0, amp0;
+    float fnx[F], fny[F], fpn[F], fr[F], fr2[F], fomr2[F];
+    float mx, my, cmx, cmy, kappa, inv_r, max_dist_sq, lens_scale, bound_scale;
+    float lr, lr2, lomr2;
+    float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
+    float slope, inv_slope, intercept, bin_size;
+};
+
+template <int P>
+CPD_HD HotRecord<P> make_hot(const TraceDesign &d, const LambdaConsts &lc) {
+    HotRecord<P> h;
+    h.nbin = d.nbin;
+    h.ar_coated = d.ar_coated;
+    h.a0 = d.a0;
+    h.b0 = d.b0;
+    h.h2 = d.h2;
+    h.y_mean = d.y_mean;
+    h.beam_w = d.beam_w;
+    h.nbin_f = d.nbin_f;
+    h.dx0 = lc.dx0;
+    h.dy0 = lc.dy0;
+    h.ts0 = lc.ts0;
+    h.tp0 = lc.tp0;
+    h.amp0 = lc.amp0;
+    for (int i = 0; i < HotRecord<P>::F; ++i) {
+        const int f = i + 1 < CPD_MAX_PRISMS ? i + 1 : CPD_MAX_PRISMS - 1;
+        h.fnx[i] = d.nx[f];
+        h.fny[i] = d.ny[f];
+        h.fpn[i] = d.pn[f];
+        h.fr[i] = lc.r[f];
+        h.fr2[i] = lc.r2[f];
+        h.fomr2[i] = lc.omr2[f];
+    }
+    h.mx = d.mx;
+    h.my = d.my;
+    h.cmx = d.cmx;
+    h.cmy = d.cmy;
+    h.kappa = d.kappa;
+    h.inv_r = d.inv_r;
+    h.max_dist_sq = d.max_dist_sq;
+    h.lens_scale = d.lens_scale;
+    h.bound_scale = d.bound_scale;
+    h.lr = lc.r[P];
+    h.lr2 = lc.r2[P];
+    h.lomr2 = lc.omr2[P];
+    h.mpx = d.mpx;
+    h.mpy = d.mpy;
+    h.dnx = d.dnx;
+    h.dny = d.dny;
+    h.ddx = d.ddx;
+    h.ddy = d.ddy;
+    h.min_ci = d.min_ci;
+    h.half_len = d.half_len;
+    h.slope = d.slope;
+    h.inv_slope = d.inv_slope;
+    h.intercept = d.intercept;
+    h.bin_size = d.bin_size;
+    return h;
+}
+
 template <int P, bool COUNT>
-CPD_HD Trace32Result trace32(const TraceDesign &d, const LambdaConsts &lc, float y) {
+CPD_HD Trace32Result trace32(const HotRecord<P> &d, float y) {
     Trace32Result res{d.nbin, 0.f, false, 0};
     const bool ar = d.ar_coated != 0;
     // entrance face: the beam ray (0, y) travelling +x meets it at height y
@@ -415,19 +480,15 @@ CPD_HD Trace32Result trace32(const TraceDesign &d, const LambdaConsts &lc, float
     float ox = fmaf(d.b0, y, d.a0), oy = y;
     alive = alive && ox >= 0.f;
     float path = fabsf(ox) + fabsf(y) + 1.f;
-    float num = lc.amp0, den = 1.f;
-    float dx = lc.dx0, dy = lc.dy0, ts = lc.ts0, tp = lc.tp0;
+    float num = d.amp0, den = 1.f;
+    float dx = d.dx0, dy = d.dy0, ts = d.ts0, tp = d.tp0;
     if (COUNT) res.ops = kOpsBeam + kOpsFaceHit + (alive ? kOpsFaceRefract : 0u);
 #pragma unroll
     for (int i = 1; i < P; ++i) {
-        if (!wave_any(alive)) {
-            res.guard = guard;
-            return res;
-        }
         if (COUNT) res.ops += alive ? kOpsFaceHit : 0u;
-        const float nx = d.nx[i], ny = d.ny[i];
+        const float nx = d.fnx[i - 1], ny = d.fny[i - 1];
         const float ci_p = fmaf(dx, nx, dy * ny);
-        const float dist = (d.pn[i] - fmaf(ox, nx, oy * ny)) * fast_rcp(ci_p);
+        const float dist = (d.fpn[i - 1] - fmaf(ox, nx, oy * ny)) * fast_rcp(ci_p);
         const float hx = fmaf(dx, dist, ox), hy = fmaf(dy, dist, oy);
         path += fabsf(dist);
         const float a = hy - d.h2;
@@ -437,7 +498,7 @@ CPD_HD Trace32Result trace32(const TraceDesign &d, const LambdaConsts &lc, float
         ox = hx;
         oy = hy;
         const float ci = -ci_p;
-        const Refract32 f = refract32(dx, dy, nx, ny, ci, lc.r[i], lc.r2[i], lc.omr2[i]);
+        const Refract32 f = refract32(dx, dy, nx, ny, ci, d.fr[i - 1], d.fr2[i - 1], d.fomr2[i - 1]);
         guard = guard || (alive && fabsf(f.cr_sq) <= kGuardCos);
         alive = alive && f.cr_sq >= 0.f;
         if (ar) {
@@ -453,10 +514,6 @@ CPD_HD Trace32Result trace32(const TraceDesign &d, const LambdaConsts &lc, float
         den *= f.amp_den;
         dx = f.vx;
         dy = f.vy;
-    }
-    if (!wave_any(alive)) {
-        res.guard = guard;
-        return res;
     }
     // exit lens, relative to its chord midpoint m: e = o - m, c = m + cm.
     // |o + s dir - c|^2 = R^2  <=>  s^2 + 2 b s + c2 = 0 with
@@ -496,7 +553,7 @@ CPD_HD Trace32Result trace32(const TraceDesign &d, const LambdaConsts &lc, float
         // normal (c - p) / R = (cm - q) / R
         const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
         const float ci = -fmaf(dx, nx, dy * ny);
-        const Refract32 f = refract32(dx, dy, nx, ny, ci, lc.r[P], lc.r2[P], lc.omr2[P]);
+        const Refract32 f = refract32(dx, dy, nx, ny, ci, d.lr, d.lr2, d.lomr2);
         guard = guard || (alive && fabsf(f.cr_sq) <= kGuardCos);
         alive = alive && f.cr_sq >= 0.f;
         if (ar) {
