@@ -48,7 +48,19 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="designs checked against the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (counter calibration for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (designs shard across GPUs "
               "with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -118,7 +130,7 @@ def main() -> int:
 
     # plain path: one design per launch, synchronised and copied back
     f1 = N.Fitness()
-    n_plain = min(args.plain, n)
+    n_plain = max(1, min(args.plain, n))
     N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(designs[0]), ctypes.byref(samp), ctypes.byref(f1))
     lat = []
     t = time.perf_counter()
@@ -148,13 +160,13 @@ def main() -> int:
         from oracle import oracle as O
         threads = len(os.sched_getaffinity(0))
         done, t = 0, time.perf_counter()
-        while time.perf_counter() - t < args.cpu_seconds and done < n:
-            chunk = designs[done: done + threads]
+        while time.perf_counter() - t < args.cpu_seconds:
+            chunk = [designs[(done + i) % n] for i in range(threads)]
             O.fitness_many(chunk, 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT, threads=threads)
             done += len(chunk)
         cpu_s = time.perf_counter() - t
         cpu = {"value": done / cpu_s, "unit": "designs/s", "cores": threads, "kind": "port",
-               "sample": f"{done} of the {n} designs, 64 wavelengths x 16384 rays each, FP64, "
+               "sample": f"{done} design evaluations cycling over the {n} designs, 64 wavelengths x 16384 rays each, FP64, "
                          f"fixed-count, one design per thread, {cpu_s:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.2f} designs/s on {threads} threads")
 

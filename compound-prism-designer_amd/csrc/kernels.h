@@ -58,7 +58,9 @@ struct FinalizeArgs {
 size_t trace_lds_bytes(uint32_t nbin_max, uint32_t cols);
 
 hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64);
-hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops);
+// start/stop (may be null) are recorded immediately around the kernel.
+hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start = nullptr,
+                        hipEvent_t stop = nullptr);
 hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n);
 // Single rays: wavelength from its CDF value, beam height from its CDF value.

@@ -19,6 +19,7 @@
 // Neumaier-compensated sums and a fixed reduction order
 // (src/fitness.rs:79-123).  It clears the accumulators it read, so the next
 // launch starts from zero without a memset.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -331,13 +332,14 @@ hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64)
     return hipGetLastError();
 }
 
-hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops) {
+hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start,
+                        hipEvent_t stop) {
     if (grid == 0 || a.n_items == 0) return hipSuccess;
     const size_t lds = trace_lds_bytes(a.nbin_max, a.cols);
     if (count_ops)
-        hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, a);
+        hipExtLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, start, stop, 0, a);
     else
-        hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(64), lds, s, a);
+        hipExtLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(64), lds, s, start, stop, 0, a);
     return hipGetLastError();
 }
 

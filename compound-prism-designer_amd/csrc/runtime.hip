@@ -193,9 +193,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
         a.work_counter = nullptr;
         grid = (uint32_t)items;
     }
-    if (profile) HIP_TRY(hipEventRecord(ev[0], stream));
-    HIP_TRY(launch_trace(stream, a, grid, want_ops));
-    if (profile) HIP_TRY(hipEventRecord(ev[1], stream));
+    HIP_TRY(launch_trace(stream, a, grid, want_ops, profile ? ev[0] : nullptr, profile ? ev[1] : nullptr));
 
     FinalizeArgs f{};
     f.designs = a.designs;
