@@ -21,6 +21,7 @@ Printed as ONE JSON line:
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -38,6 +39,49 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed):
+    """Poisson arrivals at `rate` req/s for `duration` s; each request holds
+    U{100..2000} designs drawn from the pool.  Returns latency percentiles and
+    the served throughput (designs completed / time from first arrival to last
+    completion)."""
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        arrivals.append(t)
+    sizes = rng.integers(100, 2001, len(arrivals))
+    picks = [rng.integers(0, len(pool_d), n) for n in sizes]
+    reqs = [((N.Design * len(ix))(*[pool_d[i] for i in ix]), (N.Sampling * len(ix))(*[pool_s[i] for i in ix]))
+            for ix in picks]
+    srv = cpd.Server(ctx)
+    ids = []
+    t0 = time.perf_counter()
+    for at, (darr, sarr), n in zip(arrivals, reqs, sizes):
+        now = time.perf_counter() - t0
+        if at > now:
+            time.sleep(at - now)
+        rid = ctypes.c_uint64()
+        st = N.lib.cpd_server_submit(srv._h, darr, sarr, int(n), ctypes.byref(rid))
+        if st != 0:
+            raise RuntimeError(N.last_error())
+        srv._sizes[rid.value] = int(n)
+        ids.append(rid.value)
+    lat = []
+    for rid in ids:
+        _, status, ms = srv.wait(rid)
+        lat.append(ms)
+    t_end = time.perf_counter()
+    stats = srv.stats()
+    srv.close()
+    designs = int(sizes.sum())
+    return {"rate_req_s": rate, "arrival_window_s": duration, "requests": len(ids), "designs": designs,
+            "p50_ms": float(np.percentile(lat, 50)) if lat else None,
+            "p99_ms": float(np.percentile(lat, 99)) if lat else None,
+            "designs_per_s": designs / (t_end - t0), "steps": stats["steps"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +92,10 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="designs checked against the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="req/s of the short served phase (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="served runs at 20, 50, 100 and 200 req/s (the served configuration), 2 s of arrivals each")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -65,8 +113,6 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (designs shard across GPUs "
               "with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         return 2
-
-    import ctypes
 
     import numpy as np
 
@@ -170,6 +216,20 @@ def main() -> int:
                          f"fixed-count, one design per thread, {cpu_s:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.2f} designs/s on {threads} threads")
 
+    serve = None
+    if args.serve_rate > 0 or args.serve_sweep:
+        t = time.time()
+        _, pool_d, pool_s, _ = cpd.generate_designs(2, 0, 2048)
+        log(f"serving pool: 2048 uneven designs in {time.time() - t:.1f}s")
+        rates = [20.0, 50.0, 100.0, 200.0] if args.serve_sweep else [args.serve_rate]
+        window = 2.0 if args.serve_sweep else args.serve_seconds
+        serve = []
+        for q, r in enumerate(rates):
+            res = serve_run(cpd, N, np, ctx, pool_d, pool_s, r, window, 0x5E12 + q)
+            log(f"served {r:g} req/s: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
+                f"{res['designs_per_s']:.0f} designs/s over {res['requests']} requests")
+            serve.append(res)
+
     lat_sorted = sorted(lat)
     line = {
         "metric": METRIC,
@@ -192,6 +252,7 @@ def main() -> int:
                        "plain_p99": float(np.percentile(lat_sorted, 99)),
                        "batch_p50": float(np.percentile(step_ms, 50)),
                        "batch_p99": float(np.percentile(step_ms, 99))},
+        "served": serve,
         "mi_error_bits": mi_err,
         "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

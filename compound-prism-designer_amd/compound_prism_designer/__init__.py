@@ -41,7 +41,9 @@ __all__ = [
     "Spectrometer",
     "Context",
     "fitness_batch",
+    "Server",
     "generate_design",
+    "generate_designs",
 ]
 
 
@@ -462,6 +464,84 @@ def fitness_batch(designs: Sequence[Design], sampling, *, ctx: Optional[Context]
     return res, np.array(status[:], np.int32)
 
 
+def _sampling_array(sampling, n):
+    if isinstance(sampling, Sampling):
+        return (Sampling * n)(*([sampling] * n))
+    if isinstance(sampling, tuple) and len(sampling) == 2 and not isinstance(sampling[0], Sampling):
+        s = Sampling(int(sampling[0]), int(sampling[1]), N.MODE_FIXED, 0)
+        return (Sampling * n)(*([s] * n))
+    return (Sampling * n)(*sampling)
+
+
+class Server:
+    """Continuous-batching fitness server on one GPU (cpd_server_* in include/cpd.h).
+
+    submit() queues a batch of designs and returns a request id; poll()/wait()
+    return (fitness [n, 3], status [n], latency_ms) once the request is done.
+    Requests that arrive while others run join the next scheduling step."""
+
+    def __init__(self, ctx: Optional[Context] = None, *, memory_budget: int = 0, step_items: int = 0,
+                 max_designs: int = 0):
+        h = ctypes.c_void_p()
+        opts = N.ServerOpts(memory_budget, step_items, max_designs)
+        _check(N.lib.cpd_server_create(_ctx_handle(ctx), ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        self._sizes = {}
+
+    def submit(self, designs: Sequence[Design], sampling) -> int:
+        n = len(designs)
+        darr = (Design * n)(*designs)
+        sarr = _sampling_array(sampling, n)
+        rid = ctypes.c_uint64()
+        _check(N.lib.cpd_server_submit(self._h, darr, sarr, n, ctypes.byref(rid)))
+        self._sizes[rid.value] = n
+        return rid.value
+
+    def _collect(self, rid: int, timeout_ms: Optional[int]):
+        n = self._sizes[rid]
+        out = (Fitness * max(n, 1))()
+        status = (ctypes.c_int * max(n, 1))()
+        lat = ctypes.c_double()
+        if timeout_ms is None:
+            st = N.lib.cpd_server_poll(self._h, rid, out, status, ctypes.byref(lat))
+        else:
+            st = N.lib.cpd_server_wait(self._h, rid, timeout_ms, out, status, ctypes.byref(lat))
+        if st == N.ERR_NOT_READY:
+            return None
+        _check(st)
+        del self._sizes[rid]
+        res = np.array([[f.size, f.info, f.deviation] for f in out[:n]], np.float64).reshape(n, 3)
+        return res, np.array(status[:n], np.int32), lat.value
+
+    def poll(self, rid: int):
+        return self._collect(rid, None)
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        return self._collect(rid, timeout_ms)
+
+    def stats(self) -> dict:
+        st = N.ServerStats()
+        _check(N.lib.cpd_server_stats_get(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in N.ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            N.lib.cpd_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def generate_design(config: int, index: int):
     """Synthetic design `index` of benchmark configuration `config` (0, 1, 2 or 4):
     returns (Params, Design, Sampling, attempts)."""
@@ -470,3 +550,13 @@ def generate_design(config: int, index: int):
     d = Design()
     _check(N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)))
     return p, d, s, att.value
+
+
+def generate_designs(config: int, first: int, n: int, threads: int = 0):
+    """Designs first..first+n-1 of a benchmark configuration, generated on host
+    threads: returns (params list, design list, sampling list, attempts array)."""
+    P, D, S = (Params * n)(), (Design * n)(), (Sampling * n)()
+    att = (ctypes.c_uint32 * n)()
+    _check(N.lib.cpd_generate_designs(config, first, n, threads, P, D, S, att))
+    return list(P), list(D), list(S), np.array(att[:], np.uint32)
+

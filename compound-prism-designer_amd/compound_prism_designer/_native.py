@@ -115,6 +115,18 @@ OPT_TILE_RAYS = 3
 OPT_WAVES_PER_CU = 4
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("step_items", ctypes.c_uint32), ("max_designs", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+        ("steps", ctypes.c_uint64), ("items", ctypes.c_uint64), ("designs", ctypes.c_uint64),
+        ("queued", ctypes.c_uint32), ("admitted", ctypes.c_uint32), ("busy_ms", ctypes.c_double),
+    ]
+
+
 class Fitness(ctypes.Structure):
     _fields_ = [("size", ctypes.c_double), ("info", ctypes.c_double), ("deviation", ctypes.c_double)]
 
@@ -141,11 +153,18 @@ def _load() -> ctypes.CDLL:
         "cpd_ray_trace": ([P(Design), d, d, P(d), P(u32)], st),
         "cpd_fit_sellmeier1": ([P(d), P(d), d, d, u32, P(d), P(d)], st),
         "cpd_generate_design": ([u32, u64, P(Params), P(Sampling), P(u32)], st),
+        "cpd_generate_designs": ([u32, u64, sz, u32, P(Params), P(Design), P(Sampling), P(u32)], st),
         "cpd_ctx_create": ([i32, P(vp)], st),
         "cpd_ctx_destroy": ([vp], None),
         "cpd_ctx_info": ([vp, ctypes.c_char_p, sz, P(i32), P(i32)], st),
         "cpd_ctx_set_option": ([vp, i32, ctypes.c_int64], st),
         "cpd_ctx_last_stats": ([vp, P(RunStats)], st),
+        "cpd_server_create": ([vp, P(ServerOpts), P(vp)], st),
+        "cpd_server_submit": ([vp, P(Design), P(Sampling), sz, P(u64)], st),
+        "cpd_server_poll": ([vp, u64, P(Fitness), P(st), P(d)], st),
+        "cpd_server_wait": ([vp, u64, ctypes.c_int64, P(Fitness), P(st), P(d)], st),
+        "cpd_server_stats_get": ([vp, P(ServerStats)], st),
+        "cpd_server_destroy": ([vp], None),
         "cpd_fitness_gpu": ([vp, P(Design), P(Sampling), P(Fitness)], st),
         "cpd_fitness_batch": ([vp, P(Design), P(Sampling), sz, P(Fitness), P(st)], st),
         "cpd_p_dets_l_wavelength": ([vp, P(Design), P(d), sz, u32, u32, P(d)], st),

@@ -5,7 +5,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "cpd.h"
 #include "cpd_internal.h"
@@ -503,6 +506,36 @@ cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *para
         return CPD_OK;
     }
     return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "no valid design after 1e6 draws");
+}
+
+cpd_status cpd_generate_designs(uint32_t config, uint64_t first_index, size_t n, uint32_t threads,
+                                cpd_params *params_out, cpd_design *designs_out, cpd_sampling *sampling_out,
+                                uint32_t *attempts_out) {
+    if (n && (!params_out || !designs_out || !sampling_out)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL output");
+    unsigned hw = std::thread::hardware_concurrency();
+    const unsigned T = (unsigned)std::max<size_t>(1, std::min<size_t>(n, threads ? threads : (hw ? hw : 1)));
+    std::atomic<size_t> next{0};
+    std::atomic<int> err{CPD_OK};
+    auto work = [&]() {
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n || err.load() != CPD_OK) break;
+            uint32_t att = 0;
+            cpd_status st = cpd_generate_design(config, first_index + i, &params_out[i], &sampling_out[i], &att);
+            if (st == CPD_OK) st = cpd_build_design(&params_out[i], &designs_out[i]);
+            if (st != CPD_OK) {
+                int expected = CPD_OK;
+                err.compare_exchange_strong(expected, st);
+                break;
+            }
+            if (attempts_out) attempts_out[i] = att;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < T; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    return (cpd_status)err.load();
 }
 
 }  // extern "C"

@@ -16,18 +16,24 @@ static constexpr double kQuantScale = 4194304.0;
 static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 re-trace
 static constexpr uint32_t kMaxTileRays = 16384;
 
-// One batch on the device.  Designs are laid out as parallel arrays; the work
-// of design d is items [item_begin[d], item_begin[d+1]): wavelength sample j
-// and ray tile t of tile_rays rays, item = item_begin[d] + j * n_tiles + t.
-// Per-(design, wavelength, bin) sums live at acc[acc_begin[d] + j * nbin + b];
-// the bin of ray k = n_rays (the last ray, which the reference counts once
-// more, src/fitness.rs:20-32) at last_bin[lb_begin[d] + j].
+// One launch's work on the device.  Design records live in a pool
+// (designs/traces, indexed by slot); the launch covers n_entries entries,
+// entry e being pool slot slot[e] (e itself when slot is null).  The work of
+// entry e is launch items [item_begin[e], item_begin[e+1]), which are items
+// item_offset[e] + 0, 1, ... of its design (offset 0 when item_offset is
+// null).  Item i of a design is wavelength sample j = i / n_tiles and ray
+// tile i % n_tiles of tile_rays rays.  Per-(design, wavelength, bin) sums
+// live at acc[acc_begin[slot] + j * nbin + b]; the bin of ray k = n_rays (the
+// last ray, which the reference counts once more, src/fitness.rs:20-32) at
+// last_bin[lb_begin[slot] + j].
 struct BatchArgs {
     const cpd_design *designs;
     const TraceDesign *traces;
-    const uint64_t *item_begin;  // [n_designs + 1]
-    const uint64_t *acc_begin;   // [n_designs]
-    const uint64_t *lb_begin;    // [n_designs]
+    const uint32_t *slot;        // [n_entries] (may be null)
+    const uint64_t *item_begin;  // [n_entries + 1]
+    const uint64_t *item_offset; // [n_entries] (may be null)
+    const uint64_t *acc_begin;   // [pool slots]
+    const uint64_t *lb_begin;    // [pool slots]
     unsigned long long *acc;
     uint32_t *last_bin;
     const float *z32;            // beam quantiles z_k = norminv(u_k), k = 1..n, FP32
@@ -35,7 +41,7 @@ struct BatchArgs {
     unsigned long long *ops;     // algorithmic operation count (may be null)
     unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)
-    uint32_t n_designs;
+    uint32_t n_designs;          // entries
     uint32_t tile_rays;          // multiple of 64, <= kMaxTileRays
     uint32_t cols;               // LDS histogram columns, 64 or 32
     uint32_t nbin_max;
@@ -45,6 +51,8 @@ struct BatchArgs {
 struct FinalizeArgs {
     const cpd_design *designs;
     const TraceDesign *traces;
+    const uint32_t *slot;        // [n_designs] pool slots to finalize (may be null: 0..n-1)
+    uint32_t out_stride_slot;    // results indexed by slot (1) or by entry (0)
     const uint64_t *acc_begin;
     const uint64_t *lb_begin;
     unsigned long long *acc;

@@ -159,12 +159,13 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
             item = (uint64_t)__builtin_amdgcn_readfirstlane(v);
         }
         if (item >= a.n_items) break;
-        const uint32_t d = find_design(a.item_begin, a.n_designs, item);
+        const uint32_t e = find_design(a.item_begin, a.n_designs, item);
+        const uint32_t d = a.slot ? a.slot[e] : e;
         const cpd_design *g = a.designs + d;
         const TraceDesign *td = a.traces + d;
         const uint32_t M = td->n_rays, nbin = td->nbin, P = td->n_glass;
         const uint32_t n_tiles = (M + a.tile_rays - 1) / a.tile_rays;
-        const uint64_t local = item - a.item_begin[d];
+        const uint64_t local = item - a.item_begin[e] + (a.item_offset ? a.item_offset[e] : 0);
         const uint32_t j = (uint32_t)(local / n_tiles), tile = (uint32_t)(local % n_tiles);
 
         TileCtx c;
@@ -217,7 +218,8 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
 }
 
 __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
-    const uint32_t d = blockIdx.x;
+    const uint32_t e = blockIdx.x;
+    const uint32_t d = a.slot ? a.slot[e] : e;
     const uint32_t lane = threadIdx.x;
     const cpd_design &g = a.designs[d];
     const TraceDesign &td = a.traces[d];
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
         f.size = size;
         f.info = (hs + hc) / (double)N - (ds + dc);
         f.deviation = deviation;
-        a.out[d] = f;
+        a.out[a.out_stride_slot ? d : e] = f;
     }
 }
 

@@ -1,0 +1,573 @@
+// server.hip -- continuous batching of fitness requests (cpd_server_* in cpd.h).
+//
+// A scheduling thread owns a HIP stream and three device pools: design
+// records (one slot per design), u64 accumulators and last-ray bins.  Requests
+// are admitted first come, first served while their pool regions fit.  Each
+// step gives every admitted request an equal share of the step's work items
+// (leftovers go to the oldest), launches one persistent trace over all the
+// shares, finalizes the requests whose last item ran, copies their results
+// back and frees their regions.  A request arriving during a step is admitted
+// before the next one, so its work runs alongside the requests already in
+// flight.  Accumulation is exact integer arithmetic, so a design's result does
+// not depend on which requests share its steps.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "cpd.h"
+#include "cpd_internal.h"
+#include "kernels.h"
+#include "runtime.h"
+
+namespace cpd {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+#define SRV_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) return fail(CPD_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// First-fit allocator of ranges in [0, capacity).
+class RangeAlloc {
+   public:
+    void reset(uint64_t cap) {
+        free_.clear();
+        if (cap) free_[0] = cap;
+    }
+    bool alloc(uint64_t n, uint64_t &off) {
+        if (n == 0) {
+            off = 0;
+            return true;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const uint64_t rest = it->second - n, start = it->first + n;
+                free_.erase(it);
+                if (rest) free_[start] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        if (n == 0) return;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+   private:
+    std::map<uint64_t, uint64_t> free_;
+};
+
+struct Request {
+    uint64_t id = 0;
+    size_t n_total = 0;
+    std::vector<cpd_design> designs;  // the valid designs
+    std::vector<cpd_sampling> sampling;
+    std::vector<uint32_t> index;      // position of each valid design in the submission
+    std::vector<uint64_t> items;      // work items per valid design
+    std::vector<cpd_status> status;   // per submitted design
+    std::vector<cpd_fitness> out;     // per submitted design
+    uint64_t total_items = 0, done_items = 0;
+    uint32_t cursor = 0;              // next design with work
+    uint64_t cursor_item = 0;         // next item of that design
+    uint64_t slot0 = 0, acc0 = 0, lb0 = 0, acc_n = 0, lb_n = 0;
+    uint32_t max_rays = 0;
+    bool finished = false;
+    cpd_status error = CPD_OK;
+    Clock::time_point t_submit, t_done;
+};
+
+struct Entry {
+    uint32_t slot;
+    uint64_t offset, count;
+};
+
+}  // namespace
+
+struct Server {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    uint32_t step_items = 0;
+    uint32_t waves_per_cu = 16;
+    uint64_t max_designs = 0, acc_cap = 0, lb_cap = 0;
+    DeviceBuffer d_designs, d_traces, d_acc_begin, d_lb_begin, d_acc, d_last_bin, d_results, d_z32, d_z64, d_step,
+        d_counter;
+    HostBuffer h_stage, h_results;
+    uint32_t z_n = 0;
+    RangeAlloc slots, accs, lbs;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::vector<std::shared_ptr<Request>> active;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> all;
+    uint64_t next_id = 1;
+    bool stop = false;
+    cpd_status fatal = CPD_OK;
+    cpd_server_stats stats{};
+    std::thread worker;
+
+    cpd_status init(Context *ctx, const cpd_server_opts *o);
+    void run();
+    cpd_status step();
+    bool admit(Request &r);
+    void release(Request &r);
+    cpd_status upload(const std::vector<std::shared_ptr<Request>> &fresh);
+    cpd_status ensure_tables(uint32_t n);
+};
+
+cpd_status Server::init(Context *ctx, const cpd_server_opts *o) {
+    device = ctx->device;
+    cu_count = ctx->cu_count;
+    waves_per_cu = ctx->waves_per_cu;
+    const uint64_t budget = o && o->memory_budget ? o->memory_budget : (4ull << 30);
+    max_designs = o && o->max_designs ? o->max_designs : 65536;
+    step_items = o && o->step_items ? o->step_items : (uint32_t)(4 * cu_count * waves_per_cu);
+    const uint64_t record_bytes = max_designs * (sizeof(cpd_design) + sizeof(TraceDesign) + 2 * sizeof(uint64_t) +
+                                                 sizeof(cpd_fitness));
+    if (budget <= record_bytes + (1 << 20))
+        return fail(CPD_ERR_INVALID_ARGUMENT, "memory budget below the %llu bytes of design slots",
+                    (unsigned long long)record_bytes);
+    const uint64_t rest = budget - record_bytes;
+    acc_cap = rest / 10;  // 80% of the rest as u64 accumulators
+    lb_cap = rest / 20;   // 20% as u32 last-ray bins
+    SRV_TRY(hipSetDevice(device));
+    SRV_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    cpd_status st;
+    if ((st = d_designs.reserve(max_designs * sizeof(cpd_design), false, stream)) != CPD_OK) return st;
+    if ((st = d_traces.reserve(max_designs * sizeof(TraceDesign), false, stream)) != CPD_OK) return st;
+    if ((st = d_acc_begin.reserve(max_designs * sizeof(uint64_t), false, stream)) != CPD_OK) return st;
+    if ((st = d_lb_begin.reserve(max_designs * sizeof(uint64_t), false, stream)) != CPD_OK) return st;
+    if ((st = d_results.reserve(max_designs * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
+    if ((st = d_acc.reserve(acc_cap * sizeof(unsigned long long), true, stream)) != CPD_OK) return st;
+    if ((st = d_last_bin.reserve(lb_cap * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
+    if ((st = d_counter.reserve(256, true, stream)) != CPD_OK) return st;
+    if ((st = h_results.reserve(max_designs * sizeof(cpd_fitness))) != CPD_OK) return st;
+    SRV_TRY(hipStreamSynchronize(stream));
+    slots.reset(max_designs);
+    accs.reset(acc_cap);
+    lbs.reset(lb_cap);
+    worker = std::thread([this] { run(); });
+    return CPD_OK;
+}
+
+bool Server::admit(Request &r) {
+    uint64_t s, a, l;
+    if (!slots.alloc(r.designs.size(), s)) return false;
+    if (!accs.alloc(r.acc_n, a)) {
+        slots.release(s, r.designs.size());
+        return false;
+    }
+    if (!lbs.alloc(r.lb_n, l)) {
+        slots.release(s, r.designs.size());
+        accs.release(a, r.acc_n);
+        return false;
+    }
+    r.slot0 = s;
+    r.acc0 = a;
+    r.lb0 = l;
+    return true;
+}
+
+void Server::release(Request &r) {
+    slots.release(r.slot0, r.designs.size());
+    accs.release(r.acc0, r.acc_n);
+    lbs.release(r.lb0, r.lb_n);
+}
+
+cpd_status Server::ensure_tables(uint32_t n) {
+    if (n <= z_n) return CPD_OK;
+    const uint32_t want = std::max<uint32_t>(n, 65536);
+    cpd_status st;
+    if ((st = d_z32.reserve((size_t)want * sizeof(float), false, stream)) != CPD_OK) return st;
+    if ((st = d_z64.reserve((size_t)want * sizeof(double), false, stream)) != CPD_OK) return st;
+    SRV_TRY(launch_beam_table(stream, want, d_z32.as<float>(), d_z64.as<double>()));
+    z_n = want;
+    return CPD_OK;
+}
+
+// Copy the records of newly admitted requests into their pool slots.
+cpd_status Server::upload(const std::vector<std::shared_ptr<Request>> &fresh) {
+    for (const auto &rp : fresh) {
+        Request &r = *rp;
+        const size_t n = r.designs.size();
+        const size_t bytes = n * (sizeof(cpd_design) + sizeof(TraceDesign) + 2 * sizeof(uint64_t));
+        cpd_status st = h_stage.reserve(bytes);
+        if (st != CPD_OK) return st;
+        SRV_TRY(hipStreamSynchronize(stream));  // staging reuse
+        char *h = (char *)h_stage.ptr;
+        cpd_design *hd = (cpd_design *)h;
+        TraceDesign *ht = (TraceDesign *)(h + n * sizeof(cpd_design));
+        uint64_t *ha = (uint64_t *)(h + n * (sizeof(cpd_design) + sizeof(TraceDesign)));
+        uint64_t *hl = ha + n;
+        uint64_t acc = r.acc0, lb = r.lb0;
+        for (size_t q = 0; q < n; ++q) {
+            hd[q] = r.designs[q];
+            make_trace_design(r.designs[q], r.sampling[q].n_wavelengths, r.sampling[q].n_rays, ht[q]);
+            ha[q] = acc;
+            hl[q] = lb;
+            acc += (uint64_t)r.sampling[q].n_wavelengths * r.designs[q].bin_count;
+            lb += r.sampling[q].n_wavelengths;
+        }
+        SRV_TRY(hipMemcpyAsync(d_designs.as<cpd_design>() + r.slot0, hd, n * sizeof(cpd_design),
+                               hipMemcpyHostToDevice, stream));
+        SRV_TRY(hipMemcpyAsync(d_traces.as<TraceDesign>() + r.slot0, ht, n * sizeof(TraceDesign),
+                               hipMemcpyHostToDevice, stream));
+        SRV_TRY(hipMemcpyAsync(d_acc_begin.as<uint64_t>() + r.slot0, ha, n * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, stream));
+        SRV_TRY(hipMemcpyAsync(d_lb_begin.as<uint64_t>() + r.slot0, hl, n * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, stream));
+    }
+    return CPD_OK;
+}
+
+// One scheduling step over the admitted requests.
+cpd_status Server::step() {
+    std::vector<std::shared_ptr<Request>> fresh, act;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        while (!queue.empty()) {
+            std::shared_ptr<Request> r = queue.front();
+            if (!admit(*r)) break;
+            queue.pop_front();
+            active.push_back(r);
+            fresh.push_back(r);
+        }
+        act = active;
+        stats.queued = (uint32_t)queue.size();
+        stats.admitted = (uint32_t)active.size();
+    }
+    if (act.empty()) return CPD_OK;
+    const Clock::time_point t0 = Clock::now();
+    SRV_TRY(hipSetDevice(device));
+    cpd_status st = upload(fresh);
+    if (st != CPD_OK) return st;
+    uint32_t max_rays = 0;
+    for (const auto &r : act) max_rays = std::max(max_rays, r->max_rays);
+    if ((st = ensure_tables(max_rays)) != CPD_OK) return st;
+
+    // shares: equal split of the step, leftovers to the oldest requests
+    const size_t A = act.size();
+    std::vector<uint64_t> take(A, 0);
+    uint64_t budget = step_items;
+    const uint64_t share = std::max<uint64_t>(1, budget / A);
+    for (size_t i = 0; i < A && budget; ++i) {
+        take[i] = std::min<uint64_t>({share, act[i]->total_items - act[i]->done_items, budget});
+        budget -= take[i];
+    }
+    for (size_t i = 0; i < A && budget; ++i) {
+        const uint64_t more = std::min<uint64_t>(act[i]->total_items - act[i]->done_items - take[i], budget);
+        take[i] += more;
+        budget -= more;
+    }
+    std::vector<Entry> entries;
+    std::vector<uint32_t> finishing;  // slots to finalize
+    std::vector<size_t> done_req;
+    uint32_t nbin_max = 1;
+    for (size_t i = 0; i < A; ++i) {
+        Request &r = *act[i];
+        uint64_t left = take[i];
+        while (left) {
+            const uint64_t avail = r.items[r.cursor] - r.cursor_item;
+            const uint64_t t = std::min(avail, left);
+            entries.push_back(Entry{(uint32_t)(r.slot0 + r.cursor), r.cursor_item, t});
+            nbin_max = std::max(nbin_max, r.designs[r.cursor].bin_count);
+            left -= t;
+            r.cursor_item += t;
+            if (r.cursor_item == r.items[r.cursor]) {
+                ++r.cursor;
+                r.cursor_item = 0;
+            }
+        }
+        r.done_items += take[i];
+        if (r.done_items == r.total_items) {
+            done_req.push_back(i);
+            for (size_t q = 0; q < r.designs.size(); ++q) finishing.push_back((uint32_t)(r.slot0 + q));
+        }
+    }
+    // step arrays: slot[E] | item_begin[E+1] | item_offset[E] | finalize slots[F]
+    const size_t E = entries.size(), F = finishing.size();
+    const size_t off_begin = (E * sizeof(uint32_t) + 255) / 256 * 256;
+    const size_t off_offset = off_begin + ((E + 1) * sizeof(uint64_t) + 255) / 256 * 256;
+    const size_t off_fin = off_offset + (E * sizeof(uint64_t) + 255) / 256 * 256;
+    const size_t bytes = off_fin + F * sizeof(uint32_t) + 256;
+    SRV_TRY(hipStreamSynchronize(stream));  // staging reuse after uploads
+    if ((st = h_stage.reserve(bytes)) != CPD_OK) return st;
+    if ((st = d_step.reserve(bytes, false, stream)) != CPD_OK) return st;
+    char *h = (char *)h_stage.ptr;
+    uint32_t *hslot = (uint32_t *)h;
+    uint64_t *hbegin = (uint64_t *)(h + off_begin);
+    uint64_t *hoff = (uint64_t *)(h + off_offset);
+    uint32_t *hfin = (uint32_t *)(h + off_fin);
+    uint64_t items = 0;
+    for (size_t e = 0; e < E; ++e) {
+        hslot[e] = entries[e].slot;
+        hbegin[e] = items;
+        hoff[e] = entries[e].offset;
+        items += entries[e].count;
+    }
+    hbegin[E] = items;
+    for (size_t f = 0; f < F; ++f) hfin[f] = finishing[f];
+    char *d = (char *)d_step.ptr;
+    SRV_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+
+    BatchArgs a{};
+    a.designs = d_designs.as<cpd_design>();
+    a.traces = d_traces.as<TraceDesign>();
+    a.slot = (const uint32_t *)d;
+    a.item_begin = (const uint64_t *)(d + off_begin);
+    a.item_offset = (const uint64_t *)(d + off_offset);
+    a.acc_begin = d_acc_begin.as<uint64_t>();
+    a.lb_begin = d_lb_begin.as<uint64_t>();
+    a.acc = d_acc.as<unsigned long long>();
+    a.last_bin = d_last_bin.as<uint32_t>();
+    a.z32 = d_z32.as<float>();
+    a.z64 = d_z64.as<double>();
+    a.work_counter = d_counter.as<unsigned int>();
+    a.n_designs = (uint32_t)E;
+    a.tile_rays = kMaxTileRays;
+    a.cols = nbin_max <= 128 ? 64 : 32;
+    a.nbin_max = nbin_max;
+    a.n_items = items;
+    SRV_TRY(launch_zero_u32(stream, a.work_counter, 1));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)cu_count * waves_per_cu);
+    SRV_TRY(launch_trace(stream, a, grid, false));
+    if (F) {
+        FinalizeArgs f{};
+        f.designs = a.designs;
+        f.traces = a.traces;
+        f.slot = (const uint32_t *)(d + off_fin);
+        f.out_stride_slot = 1;
+        f.acc_begin = a.acc_begin;
+        f.lb_begin = a.lb_begin;
+        f.acc = a.acc;
+        f.last_bin = a.last_bin;
+        f.out = d_results.as<cpd_fitness>();
+        f.n_designs = (uint32_t)F;
+        SRV_TRY(launch_finalize(stream, f));
+        for (size_t i : done_req) {
+            const Request &r = *act[i];
+            SRV_TRY(hipMemcpyAsync((cpd_fitness *)h_results.ptr + r.slot0, d_results.as<cpd_fitness>() + r.slot0,
+                                   r.designs.size() * sizeof(cpd_fitness), hipMemcpyDeviceToHost, stream));
+        }
+    }
+    SRV_TRY(hipStreamSynchronize(stream));
+    const Clock::time_point t1 = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i : done_req) {
+            Request &r = *act[i];
+            const cpd_fitness *res = (const cpd_fitness *)h_results.ptr + r.slot0;
+            for (size_t q = 0; q < r.designs.size(); ++q) r.out[r.index[q]] = res[q];
+            r.finished = true;
+            r.t_done = t1;
+            release(r);
+            stats.completed++;
+            stats.designs += r.designs.size();
+        }
+        if (!done_req.empty()) {
+            std::vector<std::shared_ptr<Request>> keep;
+            for (const auto &r : active)
+                if (!r->finished) keep.push_back(r);
+            active.swap(keep);
+        }
+        stats.steps++;
+        stats.items += items;
+        stats.busy_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    if (!done_req.empty()) cv_done.notify_all();
+    return CPD_OK;
+}
+
+void Server::run() {
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+            if (stop) break;
+        }
+        const cpd_status st = step();
+        if (st != CPD_OK) {
+            std::lock_guard<std::mutex> lk(mu);
+            fatal = st;
+            for (auto &kv : all) {
+                Request &r = *kv.second;
+                if (!r.finished) {
+                    r.finished = true;
+                    r.error = st;
+                    r.t_done = Clock::now();
+                }
+            }
+            queue.clear();
+            active.clear();
+            cv_done.notify_all();
+            break;
+        }
+    }
+}
+
+}  // namespace cpd
+
+using cpd::Server;
+
+extern "C" {
+
+cpd_status cpd_server_create(cpd_ctx *ctx, const cpd_server_opts *opts, cpd_server **out) {
+    if (!out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL output");
+    cpd::Context *c = nullptr;
+    cpd_status st = ctx ? CPD_OK : cpd::default_context(&c);
+    if (st != CPD_OK) return st;
+    if (ctx) c = reinterpret_cast<cpd::Context *>(ctx);
+    Server *s = new Server();
+    st = s->init(c, opts);
+    if (st != CPD_OK) {
+        delete s;
+        return st;
+    }
+    *out = reinterpret_cast<cpd_server *>(s);
+    return CPD_OK;
+}
+
+cpd_status cpd_server_submit(cpd_server *srv, const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                             uint64_t *request_id) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s || !request_id || (n && (!designs || !sampling)))
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    auto r = std::make_shared<cpd::Request>();
+    r->n_total = n;
+    r->status.assign(n, CPD_OK);
+    r->out.assign(n, cpd_fitness{NAN, NAN, NAN});
+    for (size_t i = 0; i < n; ++i) {
+        const cpd_status ds = cpd::check_design_public(designs[i], sampling[i]);
+        r->status[i] = ds;
+        if (ds != CPD_OK) continue;
+        r->index.push_back((uint32_t)i);
+        r->designs.push_back(designs[i]);
+        r->sampling.push_back(sampling[i]);
+        const uint64_t tiles = (sampling[i].n_rays + cpd::kMaxTileRays - 1) / cpd::kMaxTileRays;
+        r->items.push_back((uint64_t)sampling[i].n_wavelengths * tiles);
+        r->total_items += r->items.back();
+        r->acc_n += (uint64_t)sampling[i].n_wavelengths * designs[i].bin_count;
+        r->lb_n += sampling[i].n_wavelengths;
+        r->max_rays = std::max(r->max_rays, sampling[i].n_rays);
+    }
+    if (r->designs.size() > s->max_designs || r->acc_n > s->acc_cap || r->lb_n > s->lb_cap) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.rejected++;
+        return cpd::fail(CPD_ERR_CAPACITY, "request needs %zu slots, %llu accumulators: above the server's budget",
+                         r->designs.size(), (unsigned long long)r->acc_n);
+    }
+    r->t_submit = cpd::Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fatal != CPD_OK) return cpd::fail(s->fatal, "server stopped after a device error");
+    r->id = s->next_id++;
+    *request_id = r->id;
+    s->all[r->id] = r;
+    s->stats.submitted++;
+    if (r->designs.empty()) {
+        r->finished = true;
+        r->t_done = r->t_submit;
+        s->cv_done.notify_all();
+        return CPD_OK;
+    }
+    s->queue.push_back(r);
+    s->cv_work.notify_one();
+    return CPD_OK;
+}
+
+static cpd_status collect(Server *s, std::unique_lock<std::mutex> &lk, uint64_t id, int64_t timeout_ms,
+                          cpd_fitness *out, cpd_status *per_design, double *latency_ms) {
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown request %llu", (unsigned long long)id);
+    std::shared_ptr<cpd::Request> r = it->second;
+    if (!r->finished && timeout_ms != 0) {
+        auto pred = [&] { return r->finished; };
+        if (timeout_ms < 0)
+            s->cv_done.wait(lk, pred);
+        else
+            s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred);
+    }
+    if (!r->finished) return CPD_ERR_NOT_READY;
+    if (out) memcpy(out, r->out.data(), r->n_total * sizeof(cpd_fitness));
+    if (per_design)
+        for (size_t i = 0; i < r->n_total; ++i) per_design[i] = r->error != CPD_OK ? r->error : r->status[i];
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    s->all.erase(id);
+    return r->error;
+}
+
+cpd_status cpd_server_poll(cpd_server *srv, uint64_t request_id, cpd_fitness *out, cpd_status *per_design,
+                           double *latency_ms) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    return collect(s, lk, request_id, 0, out, per_design, latency_ms);
+}
+
+cpd_status cpd_server_wait(cpd_server *srv, uint64_t request_id, int64_t timeout_ms, cpd_fitness *out,
+                           cpd_status *per_design, double *latency_ms) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    return collect(s, lk, request_id, timeout_ms, out, per_design, latency_ms);
+}
+
+cpd_status cpd_server_stats_get(cpd_server *srv, cpd_server_stats *out) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->queued = (uint32_t)s->queue.size();
+    out->admitted = (uint32_t)s->active.size();
+    return CPD_OK;
+}
+
+void cpd_server_destroy(cpd_server *srv) {
+    Server *s = reinterpret_cast<Server *>(srv);
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (cpd::DeviceBuffer *b : {&s->d_designs, &s->d_traces, &s->d_acc_begin, &s->d_lb_begin, &s->d_acc,
+                                 &s->d_last_bin, &s->d_results, &s->d_z32, &s->d_z64, &s->d_step, &s->d_counter})
+        b->release();
+    s->h_stage.release();
+    s->h_results.release();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"

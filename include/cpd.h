@@ -198,6 +198,12 @@ cpd_status cpd_fit_sellmeier1(const double b[3], const double c[3], double lo, d
 cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *params_out,
                                cpd_sampling *sampling_out, uint32_t *attempts_out);
 
+/* Designs first_index .. first_index + n - 1 of a configuration, generated
+ * on `threads` host threads (0: all); arrays of n.  attempts_out may be NULL. */
+cpd_status cpd_generate_designs(uint32_t config, uint64_t first_index, size_t n, uint32_t threads,
+                                cpd_params *params_out, cpd_design *designs_out, cpd_sampling *sampling_out,
+                                uint32_t *attempts_out);
+
 /* ------------------------------------------------------------------------ */
 /* Device context                                                            */
 /* ------------------------------------------------------------------------ */
@@ -265,6 +271,49 @@ cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const
  * src/cuda_fitness.rs:247-317. */
 cpd_status cpd_propagate_rays(cpd_ctx *ctx, const cpd_design *design, const float *wavelength_cdf,
                               const float *ray_cdf, size_t n, uint32_t *bin_out, float *t_out);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of fitness requests                          */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart: the reference evaluates one individual per call
+ * (designer.py:267-300).  A server owns a HIP stream and device pools on its
+ * context's GPU and a scheduling thread.  Submitted requests (batches of
+ * designs) are admitted first come, first served while their records and
+ * accumulators fit the memory budget.  Every scheduling step is one
+ * persistent launch over a share of the remaining work of each admitted
+ * request, so requests that arrive while others run join the next step and
+ * their work overlaps on the device.  A request completes when its last work
+ * item has run; its results are the same as from cpd_fitness_batch. */
+typedef struct cpd_server_opts {
+    uint64_t memory_budget;    /* device bytes for admitted requests (0: 4 GiB) */
+    uint32_t step_items;       /* work items per step (0: 4 per resident wavefront) */
+    uint32_t max_designs;      /* design slots (0: 65536) */
+} cpd_server_opts;
+
+typedef struct cpd_server_stats {
+    uint64_t submitted, completed, rejected;
+    uint64_t steps, items, designs;
+    uint32_t queued, admitted;
+    double busy_ms;            /* time with work on the device */
+} cpd_server_stats;
+
+cpd_status cpd_server_create(cpd_ctx *ctx, const cpd_server_opts *opts, cpd_server **out);
+/* Queue n designs; *request_id identifies the request.  Fails with
+ * CPD_ERR_CAPACITY when the request can never fit the budget. */
+cpd_status cpd_server_submit(cpd_server *srv, const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                             uint64_t *request_id);
+/* CPD_ERR_NOT_READY while the request runs; otherwise fills out[n] and
+ * per_design[n] (may be NULL) and forgets the request.  *latency_ms (may be
+ * NULL) receives the time from submission to completion. */
+cpd_status cpd_server_poll(cpd_server *srv, uint64_t request_id, cpd_fitness *out, cpd_status *per_design,
+                           double *latency_ms);
+/* As poll, waiting up to timeout_ms (< 0: no limit). */
+cpd_status cpd_server_wait(cpd_server *srv, uint64_t request_id, int64_t timeout_ms, cpd_fitness *out,
+                           cpd_status *per_design, double *latency_ms);
+cpd_status cpd_server_stats_get(cpd_server *srv, cpd_server_stats *out);
+/* Stops the scheduling thread after the step in flight; queued requests are dropped. */
+void cpd_server_destroy(cpd_server *srv);
 
 #ifdef __cplusplus
 }

@@ -936,6 +936,32 @@ int ora_fitness(const cpd_design *d, uint64_t max_n, uint64_t max_m, int precisi
     return CPD_OK;
 }
 
+// fitness with both counters: out[3], counts[2] = {wavelength rows, rays}.
+int ora_fitness_counts(const cpd_design *d, uint64_t max_n, uint64_t max_m, int precision, int adaptive,
+                       int qrng_kind, double *out, uint64_t *counts) {
+    Counters c;
+    if (precision == 32) {
+        Spec<float> s{};
+        spec_from_design(*d, s);
+        float size, dev;
+        size_and_deviation(s, size, dev);
+        out[1] = mutual_information<float>(s, max_n, max_m, adaptive != 0, qrng_kind, &c);
+        out[0] = size;
+        out[2] = dev;
+    } else {
+        Spec<double> s{};
+        spec_from_design(*d, s);
+        double size, dev;
+        size_and_deviation(s, size, dev);
+        out[1] = mutual_information<double>(s, max_n, max_m, adaptive != 0, qrng_kind, &c);
+        out[0] = size;
+        out[2] = dev;
+    }
+    counts[0] = c.wavelengths;
+    counts[1] = c.rays;
+    return CPD_OK;
+}
+
 // n designs over `threads` host threads, one design per thread at a time
 // (designer.py:190-192 ThreadPool + GIL release at pylib.rs:1019).
 // out is [n][3]; returns the total number of rays traced.

@@ -144,6 +144,7 @@ def lib() -> ctypes.CDLL:
     L.ora_ray_trace_f64.restype = sz
     L.ora_p_dets_l_wavelength.argtypes = [P(Design), d, u64, i32, i32, i32, P(d), P(u64)]
     L.ora_fitness.argtypes = [P(Design), u64, u64, i32, i32, i32, P(d), P(u64)]
+    L.ora_fitness_counts.argtypes = [P(Design), u64, u64, i32, i32, i32, P(d), P(u64)]
     L.ora_fitness_many.argtypes = [P(Design), P(u32), P(u32), sz, i32, i32, i32, i32, P(d)]
     L.ora_fitness_many.restype = u64
     L.ora_sizeof_params.restype = sz
@@ -277,6 +278,16 @@ def fitness(design: Design, max_n: int, max_m: int, precision: int = 64, adaptiv
     rays = ctypes.c_uint64()
     lib().ora_fitness(ctypes.byref(design), max_n, max_m, precision, int(adaptive), qrng, out, ctypes.byref(rays))
     return (out[0], out[1], out[2]), rays.value
+
+
+def fitness_counts(design: Design, max_n: int, max_m: int, precision: int = 64, adaptive: bool = True,
+                   qrng: int = QRNG_ITERATIVE):
+    """(size, info, deviation), (wavelength rows, rays) of one evaluation."""
+    design = _as(design, Design)
+    out = (ctypes.c_double * 3)()
+    cnt = (ctypes.c_uint64 * 2)()
+    lib().ora_fitness_counts(ctypes.byref(design), max_n, max_m, precision, int(adaptive), qrng, out, cnt)
+    return (out[0], out[1], out[2]), (cnt[0], cnt[1])
 
 
 def fitness_many(designs, max_n, max_m, precision: int = 64, adaptive: bool = False,

@@ -1,0 +1,65 @@
+"""Continuous-batching server: requests that arrive while others are on the
+GPU join the next step, and every design's result is bit-identical to the
+same design evaluated in a standalone batch (integer accumulation)."""
+import time
+
+import numpy as np
+import pytest
+
+import compound_prism_designer as cpd
+from compound_prism_designer import _native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def small_requests(sizes, seed=0):
+    """Requests of config-2 designs with reduced sampling so the test is quick."""
+    rng = np.random.default_rng(seed)
+    _, D, S, _ = cpd.generate_designs(2, 0, 256)
+    reqs = []
+    for n in sizes:
+        idx = rng.integers(0, len(D), n)
+        designs = [D[i] for i in idx]
+        samp = [N.Sampling(int(rng.integers(4, 24)), int(64 * rng.integers(16, 160)), N.MODE_FIXED, 0) for _ in idx]
+        reqs.append((designs, samp))
+    return reqs
+
+
+def test_server_mixed_arrivals_match_standalone_batches():
+    reqs = small_requests([40, 7, 120, 1, 60])
+    ref = [cpd.fitness_batch(d, s)[0] for d, s in reqs]
+    with cpd.Server(step_items=512) as srv:
+        ids = [srv.submit(*reqs[0]), srv.submit(*reqs[1])]
+        time.sleep(0.002)  # the first requests are in flight when the rest arrive
+        ids += [srv.submit(*r) for r in reqs[2:]]
+        got = [srv.wait(i, 60000) for i in ids]
+        stats = srv.stats()
+    for (res, status, lat), want in zip(got, ref):
+        assert (status == 0).all()
+        assert np.array_equal(res, want)
+        assert lat > 0
+    assert stats["completed"] == len(reqs) and stats["steps"] > 1
+
+
+def test_server_admission_by_slots():
+    # 64 design slots hold two 30-design requests at a time: the others queue
+    reqs = small_requests([30, 30, 30, 30], seed=1)
+    with cpd.Server(max_designs=64, step_items=256) as srv:
+        ids = [srv.submit(*r) for r in reqs]
+        got = [srv.wait(i, 60000) for i in ids]
+        # a request that can never fit is refused up front
+        big_d, big_s = small_requests([65], seed=2)[0]
+        with pytest.raises(cpd.CudaError):
+            srv.submit(big_d, big_s)
+        assert srv.stats()["rejected"] == 1
+    for (res, status, _), (d, s) in zip(got, reqs):
+        assert np.array_equal(res, cpd.fitness_batch(d, s)[0])
+
+
+def test_server_reports_invalid_designs_per_request():
+    (designs, samp), = small_requests([5], seed=3)
+    samp[2] = N.Sampling(0, 1024, N.MODE_FIXED, 0)  # empty sampling
+    with cpd.Server() as srv:
+        res, status, _ = srv.wait(srv.submit(designs, samp), 60000)
+    assert status[2] == N.ERR_INVALID_ARGUMENT and np.isnan(res[2]).all()
+    assert (np.delete(status, 2) == 0).all()
