@@ -14,7 +14,6 @@ namespace cpd {
 // the order in which rays and tiles are added.
 static constexpr double kQuantScale = 4194304.0;
 static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 re-trace
-static constexpr uint32_t kMaxTileRays = 16384;
 
 // One launch's work on the device.  Design records live in a pool
 // (designs/traces, indexed by slot); the launch covers n_entries entries,
@@ -22,7 +21,8 @@ static constexpr uint32_t kMaxTileRays = 16384;
 // entry e is launch items [item_begin[e], item_begin[e+1]), which are items
 // item_offset[e] + 0, 1, ... of its design (offset 0 when item_offset is
 // null).  Item i of a design is wavelength sample j = i / n_tiles and ray
-// tile i % n_tiles of tile_rays rays.  Per-(design, wavelength, bin) sums
+// tile i % n_tiles of tile_rays rays (both from its trace record).
+// Per-(design, wavelength, bin) sums
 // live at acc[acc_begin[slot] + j * nbin + b]; the bin of ray k = n_rays (the
 // last ray, which the reference counts once more, src/fitness.rs:20-32) at
 // last_bin[lb_begin[slot] + j].
@@ -42,9 +42,7 @@ struct BatchArgs {
     unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)
     uint32_t n_designs;          // entries
-    uint32_t tile_rays;          // multiple of 64, <= kMaxTileRays
-    uint32_t cols;               // LDS histogram columns, 64 or 32
-    uint32_t nbin_max;
+    uint32_t hist_words;         // max over the entries of nbin * hist_cols
     uint64_t n_items;
 };
 
@@ -63,7 +61,7 @@ struct FinalizeArgs {
     uint32_t n_designs;
 };
 
-size_t trace_lds_bytes(uint32_t nbin_max, uint32_t cols);
+size_t trace_lds_bytes(uint32_t hist_words);
 
 hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64);
 // start/stop (may be null) are recorded immediately around the kernel.

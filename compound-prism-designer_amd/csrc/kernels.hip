@@ -146,9 +146,9 @@ template <bool COUNT>
 __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
     extern __shared__ uint32_t lds[];
     uint32_t *hist = lds;
-    uint32_t *queue = lds + a.nbin_max * a.cols;
+    uint32_t *queue = lds + a.hist_words;
     const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < a.nbin_max * a.cols; i += 64) hist[i] = 0;
+    for (uint32_t i = lane; i < a.hist_words; i += 64) hist[i] = 0;
     __syncthreads();
     unsigned long long ops = 0;
     uint64_t item = blockIdx.x;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         const cpd_design *g = a.designs + d;
         const TraceDesign *td = a.traces + d;
         const uint32_t M = td->n_rays, nbin = td->nbin, P = td->n_glass;
-        const uint32_t n_tiles = (M + a.tile_rays - 1) / a.tile_rays;
+        const uint32_t n_tiles = td->n_tiles, tile_rays = td->tile_rays, cols = td->hist_cols;
         const uint64_t local = item - a.item_begin[e] + (a.item_offset ? a.item_offset[e] : 0);
         const uint32_t j = (uint32_t)(local / n_tiles), tile = (uint32_t)(local % n_tiles);
 
@@ -173,11 +173,11 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.td = td;
         const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
         make_lambda_consts(*g, lambda, c.lc);
-        c.k_begin = tile * a.tile_rays + 1;
-        c.k_end = min(c.k_begin + a.tile_rays, M + 1);
+        c.k_begin = tile * tile_rays + 1;
+        c.k_end = min(c.k_begin + tile_rays, M + 1);
         c.n_rays = M;
         c.nbin = nbin;
-        c.cols = a.cols;
+        c.cols = cols;
         c.hist = hist;
         c.queue = queue;
         c.last_bin = a.last_bin + a.lb_begin[d] + j;
@@ -198,10 +198,10 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
             __syncthreads();
             unsigned long long *acc = a.acc + a.acc_begin[d] + (uint64_t)j * nbin;
             for (uint32_t b = lane; b < nbin; b += 64) {
-                uint32_t *row = hist + b * a.cols;
+                uint32_t *row = hist + b * cols;
                 uint64_t sum = 0;
-                for (uint32_t q = 0; q < a.cols; ++q) {
-                    const uint32_t cc = (q + b) & (a.cols - 1);
+                for (uint32_t q = 0; q < cols; ++q) {
+                    const uint32_t cc = (q + b) & (cols - 1);
                     sum += row[cc];
                     row[cc] = 0;
                 }
@@ -324,8 +324,8 @@ __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, con
 
 }  // namespace
 
-size_t trace_lds_bytes(uint32_t nbin_max, uint32_t cols) {
-    return (size_t)nbin_max * cols * sizeof(uint32_t) + kQueueCap * sizeof(uint32_t);
+size_t trace_lds_bytes(uint32_t hist_words) {
+    return (size_t)hist_words * sizeof(uint32_t) + kQueueCap * sizeof(uint32_t);
 }
 
 hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64) {
@@ -337,7 +337,7 @@ hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64)
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start,
                         hipEvent_t stop) {
     if (grid == 0 || a.n_items == 0) return hipSuccess;
-    const size_t lds = trace_lds_bytes(a.nbin_max, a.cols);
+    const size_t lds = trace_lds_bytes(a.hist_words);
     if (count_ops)
         hipExtLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, start, stop, 0, a);
     else

@@ -54,7 +54,6 @@ struct Context {
     ~Context();
 
     cpd_status ensure_beam_table(uint32_t n);
-    uint32_t choose_tile(uint32_t n_rays) const;
     cpd_status run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid,
                    bool persistent, const double *wavelengths, cpd_fitness *out, double *rows,
                    unsigned long long *ops_out);

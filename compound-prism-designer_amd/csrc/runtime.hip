@@ -89,14 +89,6 @@ cpd_status check_design(const cpd_design &g, const cpd_sampling &s) {
 
 }  // namespace
 
-uint32_t Context::choose_tile(uint32_t n_rays) const {
-    // one wavefront per wavelength when that is at most kMaxTileRays rays;
-    // otherwise tiles of the largest size that keeps tiles equal
-    uint32_t want = std::min<uint32_t>(tile_rays_hint ? tile_rays_hint : kMaxTileRays, kMaxTileRays);
-    uint32_t r = (n_rays + 63) / 64 * 64;
-    return std::max<uint32_t>(64, std::min(r, want));
-}
-
 // Stage a batch on the device and run trace + finalize.  `valid` lists the
 // indices of the designs to run; out/rows receive results for them.
 cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid,
@@ -105,14 +97,8 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     const uint32_t n = (uint32_t)valid.size();
     if (n == 0) return CPD_OK;
     HIP_TRY(hipSetDevice(device));
-    // one tile size for the whole launch
-    uint32_t max_rays = 0, nbin_max = 0;
-    for (uint32_t i : valid) {
-        max_rays = std::max(max_rays, sampling[i].n_rays);
-        nbin_max = std::max(nbin_max, designs[i].bin_count);
-    }
-    const uint32_t tile = choose_tile(max_rays);
-    const uint32_t cols = nbin_max <= 128 ? 64 : 32;
+    uint32_t max_rays = 0;
+    for (uint32_t i : valid) max_rays = std::max(max_rays, sampling[i].n_rays);
     // host staging: designs | traces | item_begin | acc_begin | lb_begin | row_begin
     const size_t off_designs = 0;
     const size_t off_traces = align_up(off_designs + n * sizeof(cpd_design), 256);
@@ -133,17 +119,19 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     uint64_t *hlb = (uint64_t *)(h + off_lb);
     uint64_t *hrow = (uint64_t *)(h + off_rows);
     uint64_t items = 0, acc_n = 0, lb_n = 0;
+    uint32_t hist_words = 0, tile = 0;
     for (uint32_t q = 0; q < n; ++q) {
         const cpd_design &g = designs[valid[q]];
         const cpd_sampling &s = sampling[valid[q]];
         hd[q] = g;
-        make_trace_design(g, s.n_wavelengths, s.n_rays, ht[q]);
+        make_trace_design(g, s.n_wavelengths, s.n_rays, ht[q], tile_rays_hint);
+        hist_words = std::max(hist_words, ht[q].hist_cols * g.bin_count);
+        tile = std::max(tile, ht[q].tile_rays);
         hitem[q] = items;
         hacc[q] = acc_n;
         hlb[q] = lb_n;
         hrow[q] = acc_n;
-        const uint64_t tiles = (s.n_rays + tile - 1) / tile;
-        items += (uint64_t)s.n_wavelengths * tiles;
+        items += (uint64_t)s.n_wavelengths * ht[q].n_tiles;
         acc_n += (uint64_t)s.n_wavelengths * g.bin_count;
         lb_n += s.n_wavelengths;
     }
@@ -179,9 +167,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.ops = want_ops ? dops : nullptr;
     a.wavelengths = n_wl ? (const double *)(dstage + off_wl) : nullptr;
     a.n_designs = n;
-    a.tile_rays = tile;
-    a.cols = cols;
-    a.nbin_max = nbin_max;
+    a.hist_words = hist_words;
     a.n_items = items;
     uint32_t grid;
     if (persistent) {

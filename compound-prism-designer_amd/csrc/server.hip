@@ -90,6 +90,7 @@ struct Request {
     size_t n_total = 0;
     std::vector<cpd_design> designs;  // the valid designs
     std::vector<cpd_sampling> sampling;
+    std::vector<TraceDesign> traces;  // FP32 trace records (tile and histogram shape)
     std::vector<uint32_t> index;      // position of each valid design in the submission
     std::vector<uint64_t> items;      // work items per valid design
     std::vector<cpd_status> status;   // per submitted design
@@ -231,7 +232,7 @@ cpd_status Server::upload(const std::vector<std::shared_ptr<Request>> &fresh) {
         uint64_t acc = r.acc0, lb = r.lb0;
         for (size_t q = 0; q < n; ++q) {
             hd[q] = r.designs[q];
-            make_trace_design(r.designs[q], r.sampling[q].n_wavelengths, r.sampling[q].n_rays, ht[q]);
+            ht[q] = r.traces[q];
             ha[q] = acc;
             hl[q] = lb;
             acc += (uint64_t)r.sampling[q].n_wavelengths * r.designs[q].bin_count;
@@ -291,7 +292,7 @@ cpd_status Server::step() {
     std::vector<Entry> entries;
     std::vector<uint32_t> finishing;  // slots to finalize
     std::vector<size_t> done_req;
-    uint32_t nbin_max = 1;
+    uint32_t hist_words = 1;
     for (size_t i = 0; i < A; ++i) {
         Request &r = *act[i];
         uint64_t left = take[i];
@@ -299,7 +300,7 @@ cpd_status Server::step() {
             const uint64_t avail = r.items[r.cursor] - r.cursor_item;
             const uint64_t t = std::min(avail, left);
             entries.push_back(Entry{(uint32_t)(r.slot0 + r.cursor), r.cursor_item, t});
-            nbin_max = std::max(nbin_max, r.designs[r.cursor].bin_count);
+            hist_words = std::max(hist_words, r.traces[r.cursor].hist_cols * r.traces[r.cursor].nbin);
             left -= t;
             r.cursor_item += t;
             if (r.cursor_item == r.items[r.cursor]) {
@@ -353,9 +354,7 @@ cpd_status Server::step() {
     a.z64 = d_z64.as<double>();
     a.work_counter = d_counter.as<unsigned int>();
     a.n_designs = (uint32_t)E;
-    a.tile_rays = kMaxTileRays;
-    a.cols = nbin_max <= 128 ? 64 : 32;
-    a.nbin_max = nbin_max;
+    a.hist_words = hist_words;
     a.n_items = items;
     SRV_TRY(launch_zero_u32(stream, a.work_counter, 1));
     const uint32_t grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)cu_count * waves_per_cu);
@@ -472,8 +471,9 @@ cpd_status cpd_server_submit(cpd_server *srv, const cpd_design *designs, const c
         r->index.push_back((uint32_t)i);
         r->designs.push_back(designs[i]);
         r->sampling.push_back(sampling[i]);
-        const uint64_t tiles = (sampling[i].n_rays + cpd::kMaxTileRays - 1) / cpd::kMaxTileRays;
-        r->items.push_back((uint64_t)sampling[i].n_wavelengths * tiles);
+        r->traces.emplace_back();
+        cpd::make_trace_design(designs[i], sampling[i].n_wavelengths, sampling[i].n_rays, r->traces.back());
+        r->items.push_back((uint64_t)sampling[i].n_wavelengths * r->traces.back().n_tiles);
         r->total_items += r->items.back();
         r->acc_n += (uint64_t)sampling[i].n_wavelengths * designs[i].bin_count;
         r->lb_n += sampling[i].n_wavelengths;

@@ -222,9 +222,27 @@ CPD_HD uint32_t trace64(const cpd_design &g, double wavelength, double y, double
 // ---------------------------------------------------------------------------
 // FP32 trace record and per-wavelength constants
 // ---------------------------------------------------------------------------
+// Per-wavefront LDS histogram of one work item: `cols` columns per bin (lane
+// l adds into column l % cols), at most kHistWords words for bin counts up to
+// 2048, so the histogram stays near 8 KB and several wavefronts fit a CU.
+static constexpr uint32_t kHistWords = 2048;
+static constexpr uint32_t kMaxTileRays = 16384;
+CPD_HD constexpr uint32_t hist_cols(uint32_t nbin) {
+    uint32_t c = 64;
+    while (c > 1 && c * nbin > kHistWords) c >>= 1;
+    return c;
+}
+// Largest tile whose per-column u32 sums cannot overflow.  A column gathers
+// the FP32-traced rays of its 64/cols lanes (tile/cols at most) plus at most
+// one re-traced ray per lane per queue drain (ceil(tile/64) drains), each
+// worth at most 2^22: 2*tile + 64 < 1024*cols keeps every sum below 2^32.
+CPD_HD constexpr uint32_t max_tile_rays(uint32_t cols) {
+    return ((512u * cols - 33u) & ~63u) < kMaxTileRays ? ((512u * cols - 33u) & ~63u) : kMaxTileRays;
+}
+
 struct alignas(16) TraceDesign {
     uint32_t n_glass, nbin, ar_coated, n_lambda;
-    uint32_t n_rays, reserved0, reserved1, reserved2;
+    uint32_t n_rays, hist_cols, tile_rays, n_tiles;  // LDS columns; rays per work item; items per wavelength
     // entrance face: a beam ray at height y meets it at x = a0 + b0 y; the
     // prism spans heights [0, h], tested as |y - h/2| <= h/2
     float a0, b0, h2, nbin_f;
@@ -253,14 +271,22 @@ struct LambdaConsts {
 };
 
 // Build the FP32 record from the FP64 design (host or device).
-CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n_rays, TraceDesign &t) {
+// tile_cap (a multiple of 64, 0 for none) bounds the rays of one work item.
+CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n_rays, TraceDesign &t,
+                              uint32_t tile_cap = 0) {
 #pragma clang fp contract(off)
     t.n_glass = g.n_glass;
     t.nbin = g.bin_count;
     t.ar_coated = g.ar_coated;
     t.n_lambda = n_lambda;
     t.n_rays = n_rays;
-    t.reserved0 = t.reserved1 = t.reserved2 = 0;
+    t.hist_cols = hist_cols(g.bin_count);
+    uint32_t tile = (n_rays + 63) / 64 * 64;
+    if (tile > max_tile_rays(t.hist_cols)) tile = max_tile_rays(t.hist_cols);
+    if (tile_cap && tile > tile_cap) tile = tile_cap;
+    if (tile < 64) tile = 64;
+    t.tile_rays = tile;
+    t.n_tiles = (n_rays + tile - 1) / tile;
     const double n0x = g.plane_normal[0][0], n0y = g.plane_normal[0][1];
     const double c0 = g.plane_point[0][0] * n0x + g.plane_point[0][1] * n0y;
     t.a0 = (float)(c0 / n0x);
