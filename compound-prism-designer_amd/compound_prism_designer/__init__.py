@@ -25,6 +25,8 @@ from . import _native as N
 from ._native import Design, Fitness, Params, Sampling
 
 __all__ = [
+    "autotune",
+    "Plan",
     "RayTraceError",
     "CudaError",
     "GpuError",
@@ -324,6 +326,10 @@ class Context:
         _check(N.lib.cpd_ctx_last_stats(self._h, ctypes.byref(st)))
         return {f: getattr(st, f) for f, _ in N.RunStats._fields_}
 
+    def set_schedule(self, schedule: Optional["N.Schedule"]) -> None:
+        """Schedule of fitness_batch on this context (None: the default)."""
+        _check(N.lib.cpd_ctx_set_schedule(self._h, ctypes.byref(schedule) if schedule is not None else None))
+
     def close(self):
         if self._h:
             N.lib.cpd_ctx_destroy(self._h)
@@ -462,6 +468,71 @@ def fitness_batch(designs: Sequence[Design], sampling, *, ctx: Optional[Context]
         _check(st)
     res = np.array([[f.size, f.info, f.deviation] for f in out], np.float64).reshape(n, 3)
     return res, np.array(status[:], np.int32)
+
+
+def autotune(designs: Sequence[Design], sampling, *, ctx: Optional[Context] = None):
+    """Time candidate schedules on this batch; returns (Schedule, milliseconds).
+    Results never depend on the schedule (exact integer bin sums)."""
+    n = len(designs)
+    darr = (Design * n)(*designs)
+    sarr = _sampling_array(sampling, n)
+    best, ms = N.Schedule(), ctypes.c_double()
+    _check(N.lib.cpd_autotune(_ctx_handle(ctx), darr, sarr, n, ctypes.byref(best), ctypes.byref(ms)))
+    return best, ms.value
+
+
+class Plan:
+    """A captured generation (cpd_plan_* in include/cpd.h): per slot a fixed
+    sampling and bin count.  run(designs) evaluates a generation of that shape
+    by replaying one HIP graph."""
+
+    def __init__(self, shape: Sequence[Design], sampling, *, schedule: Optional["N.Schedule"] = None,
+                 ctx: Optional[Context] = None):
+        self.n = len(shape)
+        darr = (Design * self.n)(*shape)
+        sarr = _sampling_array(sampling, self.n)
+        h = ctypes.c_void_p()
+        _check(N.lib.cpd_plan_create(_ctx_handle(ctx), darr, sarr, self.n,
+                                     ctypes.byref(schedule) if schedule is not None else None, ctypes.byref(h)))
+        self._h = h
+        self._out = (Fitness * self.n)()
+        self._status = (ctypes.c_int * self.n)()
+
+    def schedule(self) -> "N.Schedule":
+        sc = N.Schedule()
+        _check(N.lib.cpd_plan_schedule(self._h, ctypes.byref(sc)))
+        return sc
+
+    def run_raw(self, designs_array) -> int:
+        """Evaluate a ctypes array of n Design records into the plan's own
+        output arrays; returns the status code (no conversion)."""
+        return N.lib.cpd_plan_run(self._h, designs_array, self._out, self._status)
+
+    def run(self, designs: Sequence[Design]):
+        """Returns (fitness float64 [n, 3], status int32 [n])."""
+        if len(designs) != self.n:
+            raise ValueError(f"plan holds {self.n} designs, got {len(designs)}")
+        darr = designs if isinstance(designs, ctypes.Array) else (Design * self.n)(*designs)
+        _check(self.run_raw(darr))
+        res = np.array([[f.size, f.info, f.deviation] for f in self._out], np.float64).reshape(self.n, 3)
+        return res, np.array(self._status[:], np.int32)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib.cpd_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _sampling_array(sampling, n):

@@ -113,6 +113,19 @@ OPT_PROFILE = 1
 OPT_COUNT_OPS = 2
 OPT_TILE_RAYS = 3
 OPT_WAVES_PER_CU = 4
+OPT_ORDER = 5
+
+ORDER_GIVEN = 0
+ORDER_COST = 1
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("tile_rays", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("order", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"tile_rays": self.tile_rays, "waves_per_cu": self.waves_per_cu,
+                "order": "cost" if self.order == ORDER_COST else "given"}
 
 
 class ServerOpts(ctypes.Structure):
@@ -170,6 +183,12 @@ def _load() -> ctypes.CDLL:
         "cpd_p_dets_l_wavelength": ([vp, P(Design), P(d), sz, u32, u32, P(d)], st),
         "cpd_propagate_rays": ([vp, P(Design), P(ctypes.c_float), P(ctypes.c_float), sz, P(u32),
                                 P(ctypes.c_float)], st),
+        "cpd_autotune": ([vp, P(Design), P(Sampling), sz, P(Schedule), P(d)], st),
+        "cpd_ctx_set_schedule": ([vp, P(Schedule)], st),
+        "cpd_plan_create": ([vp, P(Design), P(Sampling), sz, P(Schedule), P(vp)], st),
+        "cpd_plan_run": ([vp, P(Design), P(Fitness), P(st)], st),
+        "cpd_plan_schedule": ([vp, P(Schedule)], st),
+        "cpd_plan_destroy": ([vp], None),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

@@ -64,10 +64,13 @@ struct FinalizeArgs {
 size_t trace_lds_bytes(uint32_t hist_words);
 
 hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64);
-// start/stop (may be null) are recorded immediately around the kernel.
+// start/stop (may be null) are recorded immediately around the kernel; with
+// neither, the launch is a plain one that stream capture records.
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start = nullptr,
                         hipEvent_t stop = nullptr);
 hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a);
+hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
+                               uint32_t tile_cap, TraceDesign *traces);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n);
 // Single rays: wavelength from its CDF value, beam height from its CDF value.
 hipError_t launch_propagate(hipStream_t s, const cpd_design *g, const TraceDesign *td, const float *wcdf,

@@ -322,7 +322,24 @@ __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, con
     t[i] = r.bin < td->nbin ? r.t : 0.f;
 }
 
+// FP32 trace records of n designs from their FP64 records and per-slot
+// sampling (one thread per design).
+__global__ void build_traces_kernel(const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
+                                    uint32_t tile_cap, TraceDesign *traces) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    make_trace_design(designs[i], sampling[i].n_wavelengths, sampling[i].n_rays, traces[i], tile_cap);
+}
+
 }  // namespace
+
+hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
+                               uint32_t tile_cap, TraceDesign *traces) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(build_traces_kernel, dim3((n + 63) / 64), dim3(64), 0, s, designs, sampling, n, tile_cap,
+                       traces);
+    return hipGetLastError();
+}
 
 size_t trace_lds_bytes(uint32_t hist_words) {
     return (size_t)hist_words * sizeof(uint32_t) + kQueueCap * sizeof(uint32_t);
@@ -338,6 +355,14 @@ hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool c
                         hipEvent_t stop) {
     if (grid == 0 || a.n_items == 0) return hipSuccess;
     const size_t lds = trace_lds_bytes(a.hist_words);
+    if (!start && !stop) {
+        // plain launch: the form a stream capture records as a kernel node
+        if (count_ops)
+            hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, a);
+        else
+            hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(64), lds, s, a);
+        return hipGetLastError();
+    }
     if (count_ops)
         hipExtLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, start, stop, 0, a);
     else

@@ -38,6 +38,7 @@ struct Context {
     char name[256] = {0};
     uint32_t waves_per_cu = 16;   // resident wavefronts per CU of the persistent grid
     uint32_t tile_rays_hint = 0;  // rays per work item (0: the default)
+    uint32_t order = CPD_ORDER_GIVEN;
     bool profile = false;
     bool count_ops = false;
     cpd_run_stats last{};
@@ -59,9 +60,20 @@ struct Context {
                    unsigned long long *ops_out);
     cpd_status propagate(const cpd_design &g, const float *wcdf, const float *rcdf, size_t n, uint32_t *bin,
                          float *t);
+    void apply(const cpd_schedule &s);
+    cpd_schedule schedule() const;
+    cpd_status autotune(const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                        const std::vector<uint32_t> &valid, cpd_schedule *best, double *best_ms);
 };
 
 cpd_status default_context(Context **out);
+cpd_schedule default_schedule();
+cpd_status check_schedule(const cpd_schedule &s);
+// Relative cost of one work item of a design (traced ray-stages per item).
+double item_cost(const cpd_design &g, const cpd_sampling &s, uint32_t tile_cap);
+// Indices of `valid` in the order a schedule runs them.
+void order_designs(const cpd_design *designs, const cpd_sampling *sampling, std::vector<uint32_t> &valid,
+                   uint32_t order, uint32_t tile_cap);
 cpd_status check_design_public(const cpd_design &g, const cpd_sampling &s);
 
 }  // namespace cpd

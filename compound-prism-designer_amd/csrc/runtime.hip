@@ -89,13 +89,120 @@ cpd_status check_design(const cpd_design &g, const cpd_sampling &s) {
 
 }  // namespace
 
+cpd_schedule default_schedule() { return cpd_schedule{0, 16, CPD_ORDER_GIVEN, 0}; }
+
+cpd_status check_schedule(const cpd_schedule &s) {
+    if (s.tile_rays > kMaxTileRays || s.tile_rays % 64)
+        return fail(CPD_ERR_INVALID_ARGUMENT, "tile rays must be a multiple of 64 in [0, 16384]");
+    if (s.waves_per_cu < 1 || s.waves_per_cu > 64) return fail(CPD_ERR_INVALID_ARGUMENT, "waves per CU must be in 1..64");
+    if (s.order != CPD_ORDER_GIVEN && s.order != CPD_ORDER_COST) return fail(CPD_ERR_INVALID_ARGUMENT, "unknown order");
+    return CPD_OK;
+}
+
+void Context::apply(const cpd_schedule &s) {
+    tile_rays_hint = s.tile_rays;
+    waves_per_cu = s.waves_per_cu;
+    order = s.order;
+}
+
+cpd_schedule Context::schedule() const { return cpd_schedule{tile_rays_hint, waves_per_cu, order, 0}; }
+
+// Coordinate search over work order, grid size and tile cap.  Each candidate
+// runs the whole batch twice; its time is the faster run's trace + finalize
+// time (HIP events).  Any schedule gives the same results, so they are
+// discarded.
+cpd_status Context::autotune(const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                             const std::vector<uint32_t> &valid, cpd_schedule *best, double *best_ms) {
+    const cpd_schedule saved = schedule();
+    const bool saved_profile = profile, saved_count = count_ops;
+    const cpd_run_stats saved_last = last;
+    profile = true;
+    count_ops = false;
+    std::vector<cpd_fitness> scratch(n);
+    cpd_status st = CPD_OK;
+    auto time_of = [&](const cpd_schedule &sc, double &ms) -> cpd_status {
+        apply(sc);
+        ms = INFINITY;
+        for (int rep = 0; rep < 2; ++rep) {
+            cpd_status s2 = run(designs, sampling, valid, true, nullptr, scratch.data(), nullptr, nullptr);
+            if (s2 != CPD_OK) return s2;
+            ms = std::min(ms, last.trace_ms + last.finalize_ms);
+        }
+        return CPD_OK;
+    };
+    cpd_schedule cur{0, 12, CPD_ORDER_COST, 0};
+    double cur_ms;
+    st = time_of(cur, cur_ms);
+    const uint32_t orders[] = {CPD_ORDER_GIVEN};
+    const uint32_t waves[] = {4, 8, 16, 24};
+    const uint32_t tiles[] = {8192, 4096, 2048};
+    for (uint32_t o : orders) {
+        if (st != CPD_OK) break;
+        cpd_schedule cand = cur;
+        cand.order = o;
+        double ms;
+        if ((st = time_of(cand, ms)) == CPD_OK && ms < cur_ms) cur = cand, cur_ms = ms;
+    }
+    for (uint32_t w : waves) {
+        if (st != CPD_OK) break;
+        cpd_schedule cand = cur;
+        cand.waves_per_cu = w;
+        double ms;
+        if ((st = time_of(cand, ms)) == CPD_OK && ms < cur_ms) cur = cand, cur_ms = ms;
+    }
+    for (uint32_t t : tiles) {
+        if (st != CPD_OK) break;
+        cpd_schedule cand = cur;
+        cand.tile_rays = t;
+        double ms;
+        if ((st = time_of(cand, ms)) == CPD_OK && ms < cur_ms) cur = cand, cur_ms = ms;
+    }
+    apply(saved);
+    profile = saved_profile;
+    count_ops = saved_count;
+    last = saved_last;
+    if (st != CPD_OK) return st;
+    *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return CPD_OK;
+}
+
+double item_cost(const cpd_design &g, const cpd_sampling &s, uint32_t tile_cap) {
+    const double per_ray = kOpsBeam + kOpsDetector + kOpsHit + kOpsLensHit + kOpsLensRefract +
+                           (double)g.n_glass * (kOpsFaceHit + kOpsFaceRefract);
+    return per_ray * tile_for(g.bin_count, s.n_rays, tile_cap);
+}
+
+void order_designs(const cpd_design *designs, const cpd_sampling *sampling, std::vector<uint32_t> &valid,
+                   uint32_t order, uint32_t tile_cap) {
+    if (order != CPD_ORDER_COST) return;
+    // longest-processing-time first: items of the costliest designs are
+    // fetched first, so the grid ends on the cheapest ones
+    std::vector<std::pair<double, uint32_t>> key;
+    key.reserve(valid.size());
+    for (uint32_t i : valid) key.emplace_back(-item_cost(designs[i], sampling[i], tile_cap), i);
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<double, uint32_t> &a, const std::pair<double, uint32_t> &b) {
+                         return a.first < b.first;
+                     });
+    for (size_t q = 0; q < key.size(); ++q) valid[q] = key[q].second;
+}
+
 // Stage a batch on the device and run trace + finalize.  `valid` lists the
 // indices of the designs to run; out/rows receive results for them.
-cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid,
+cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid_in,
                         bool persistent, const double *wavelengths, cpd_fitness *out, double *rows,
                         unsigned long long *ops_out) {
+    if (valid_in.empty()) return CPD_OK;
+    // per-wavelength rows are written in staging order: keep the given order
+    const bool reorder = order != CPD_ORDER_GIVEN && rows == nullptr;
+    std::vector<uint32_t> ordered;
+    if (reorder) {
+        ordered = valid_in;
+        order_designs(designs, sampling, ordered, order, tile_rays_hint);
+    }
+    const std::vector<uint32_t> &valid = reorder ? ordered : valid_in;
     const uint32_t n = (uint32_t)valid.size();
-    if (n == 0) return CPD_OK;
     HIP_TRY(hipSetDevice(device));
     uint32_t max_rays = 0;
     for (uint32_t i : valid) max_rays = std::max(max_rays, sampling[i].n_rays);
@@ -337,8 +444,39 @@ cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value) {
             if (value < 1 || value > 64) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "waves per CU must be in 1..64");
             c->waves_per_cu = (uint32_t)value;
             return CPD_OK;
+        case CPD_OPT_ORDER:
+            if (value != CPD_ORDER_GIVEN && value != CPD_ORDER_COST)
+                return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown order %lld", (long long)value);
+            c->order = (uint32_t)value;
+            return CPD_OK;
     }
     return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown option %d", option);
+}
+
+cpd_status cpd_ctx_set_schedule(cpd_ctx *ctx, const cpd_schedule *schedule) {
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    const cpd_schedule sc = schedule ? *schedule : cpd::default_schedule();
+    if ((st = cpd::check_schedule(sc)) != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->apply(sc);
+    return CPD_OK;
+}
+
+cpd_status cpd_autotune(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                        cpd_schedule *best, double *best_ms) {
+    if (!designs || !sampling || !best) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n == 0 || n > (1u << 24)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "batch size must be in 1..2^24");
+    std::vector<uint32_t> valid;
+    for (size_t i = 0; i < n; ++i)
+        if (cpd::check_design_public(designs[i], sampling[i]) == CPD_OK) valid.push_back((uint32_t)i);
+    if (valid.empty()) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "no valid design to tune on");
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return c->autotune(designs, sampling, n, valid, best, best_ms);
 }
 
 cpd_status cpd_ctx_last_stats(cpd_ctx *ctx, cpd_run_stats *out) {

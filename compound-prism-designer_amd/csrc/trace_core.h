@@ -240,6 +240,17 @@ CPD_HD constexpr uint32_t max_tile_rays(uint32_t cols) {
     return ((512u * cols - 33u) & ~63u) < kMaxTileRays ? ((512u * cols - 33u) & ~63u) : kMaxTileRays;
 }
 
+// Rays per work item of a design: the whole wavelength sample rounded up to
+// a wavefront multiple, capped by the overflow bound and by tile_cap (a
+// multiple of 64, 0 for none).
+CPD_HD uint32_t tile_for(uint32_t nbin, uint32_t n_rays, uint32_t tile_cap) {
+    uint32_t tile = (n_rays + 63) / 64 * 64;
+    const uint32_t bound = max_tile_rays(hist_cols(nbin));
+    if (tile > bound) tile = bound;
+    if (tile_cap && tile > tile_cap) tile = tile_cap;
+    return tile < 64 ? 64 : tile;
+}
+
 struct alignas(16) TraceDesign {
     uint32_t n_glass, nbin, ar_coated, n_lambda;
     uint32_t n_rays, hist_cols, tile_rays, n_tiles;  // LDS columns; rays per work item; items per wavelength
@@ -281,12 +292,8 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.n_lambda = n_lambda;
     t.n_rays = n_rays;
     t.hist_cols = hist_cols(g.bin_count);
-    uint32_t tile = (n_rays + 63) / 64 * 64;
-    if (tile > max_tile_rays(t.hist_cols)) tile = max_tile_rays(t.hist_cols);
-    if (tile_cap && tile > tile_cap) tile = tile_cap;
-    if (tile < 64) tile = 64;
-    t.tile_rays = tile;
-    t.n_tiles = (n_rays + tile - 1) / tile;
+    t.tile_rays = tile_for(g.bin_count, n_rays, tile_cap);
+    t.n_tiles = (n_rays + t.tile_rays - 1) / t.tile_rays;
     const double n0x = g.plane_normal[0][0], n0y = g.plane_normal[0][1];
     const double c0 = g.plane_point[0][0] * n0x + g.plane_point[0][1] * n0y;
     t.a0 = (float)(c0 / n0x);

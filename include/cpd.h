@@ -224,7 +224,8 @@ enum {
     CPD_OPT_PROFILE = 1,       /* record HIP events around each kernel of a run (0/1) */
     CPD_OPT_COUNT_OPS = 2,     /* count the algorithmic operations a run performs (0/1) */
     CPD_OPT_TILE_RAYS = 3,     /* rays per work item, multiple of 64, <= 16384 (0: automatic) */
-    CPD_OPT_WAVES_PER_CU = 4   /* resident wavefronts per CU of the persistent grid */
+    CPD_OPT_WAVES_PER_CU = 4,  /* resident wavefronts per CU of the persistent grid */
+    CPD_OPT_ORDER = 5          /* work order of a batch, CPD_ORDER_* */
 };
 cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value);
 
@@ -271,6 +272,49 @@ cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const
  * src/cuda_fitness.rs:247-317. */
 cpd_status cpd_propagate_rays(cpd_ctx *ctx, const cpd_design *design, const float *wavelength_cdf,
                               const float *ray_cdf, size_t n, uint32_t *bin_out, float *t_out);
+
+/* ------------------------------------------------------------------------ */
+/* Schedules, autotuning and captured generations                            */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart.  A schedule fixes how a batch is cut into work
+ * items and fed to the persistent grid.  Results never depend on it: bin sums
+ * are exact integers, so any schedule gives bit-identical fitness values. */
+enum {
+    CPD_ORDER_GIVEN = 0,   /* designs run in the order given */
+    CPD_ORDER_COST = 1     /* costliest work items first (longest-processing-time order) */
+};
+typedef struct cpd_schedule {
+    uint32_t tile_rays;     /* rays per work item cap, multiple of 64, <= 16384 (0: largest safe tile) */
+    uint32_t waves_per_cu;  /* resident wavefronts per CU of the persistent grid, 1..64 */
+    uint32_t order;         /* CPD_ORDER_* */
+    uint32_t reserved;
+} cpd_schedule;
+
+/* Times candidate schedules on this batch (each evaluated in full, results
+ * discarded) and returns the fastest in *best and its trace + finalize time
+ * in *best_ms (may be NULL).  The context's own schedule is unchanged. */
+cpd_status cpd_autotune(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling, size_t n,
+                        cpd_schedule *best, double *best_ms);
+/* Schedule of cpd_fitness_batch on this context (NULL: the default). */
+cpd_status cpd_ctx_set_schedule(cpd_ctx *ctx, const cpd_schedule *schedule);
+
+/* A plan is a generation of n designs whose shape is fixed: per slot the
+ * sampling and the bin count.  cpd_plan_create stages the shape on the device
+ * and captures upload, trace, reduction and download of one generation in a
+ * HIP graph; cpd_plan_run copies the generation's design records into pinned
+ * memory, replays the graph and waits.  The FP32 trace records are built on
+ * the device.  A design whose bin count differs from its slot's, or that fails
+ * validation, gets CPD_ERR_INVALID_ARGUMENT in per_design (may be NULL) and
+ * NaN results; the others are evaluated.  Replaces the per-generation loop of
+ * designer.py:267-300 over fitness(), src/fitness.rs:130-147. */
+typedef struct cpd_plan cpd_plan;
+cpd_status cpd_plan_create(cpd_ctx *ctx, const cpd_design *shape, const cpd_sampling *sampling, size_t n,
+                           const cpd_schedule *schedule /* NULL: autotuned on `shape` */, cpd_plan **out);
+cpd_status cpd_plan_run(cpd_plan *plan, const cpd_design *designs, cpd_fitness *out, cpd_status *per_design);
+/* Schedule the plan was captured with. */
+cpd_status cpd_plan_schedule(const cpd_plan *plan, cpd_schedule *out);
+void cpd_plan_destroy(cpd_plan *plan);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: continuous batching of fitness requests                          */
