@@ -82,6 +82,100 @@ def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed):
             "designs_per_s": designs / (t_end - t0), "steps": stats["steps"]}
 
 
+def uneven_run(cpd, N, np, ctx, args):
+    """The uneven batch: autotune the schedule on it, capture it as a plan
+    (one HIP graph per generation) and time generation replays; the kernel's
+    algorithmic rate comes from an op-counted and an event-timed eager run
+    under the same schedule."""
+    n = args.uneven
+    t = time.time()
+    _, D, S, _ = cpd.generate_designs(2, 0, n)
+    darr = (N.Design * n)(*D)
+    sarr = (N.Sampling * n)(*S)
+    rays = sum(s.n_wavelengths * s.n_rays for s in S)
+    log(f"uneven: {n} designs ({rays:.3e} rays) generated in {time.time() - t:.1f}s")
+    t = time.time()
+    best, tuned_ms = cpd.autotune(D, S, ctx=ctx)
+    tune_s = time.time() - t
+    log(f"uneven: autotuned {best.as_dict()} -> {tuned_ms:.2f} ms in {tune_s:.1f}s")
+    plan = cpd.Plan(D, S, schedule=best, ctx=ctx)
+    for _ in range(2):
+        st = plan.run_raw(darr)
+        if st != 0:
+            raise RuntimeError(N.last_error())
+    step = []
+    t0 = time.perf_counter()
+    for _ in range(args.uneven_steps):
+        t1 = time.perf_counter()
+        st = plan.run_raw(darr)
+        step.append((time.perf_counter() - t1) * 1e3)
+        if st != 0:
+            raise RuntimeError(N.last_error())
+    elapsed = time.perf_counter() - t0
+    value = n * args.uneven_steps / elapsed
+    plan_info = np.array([f.info for f in plan._out])
+    log(f"uneven: captured generation {elapsed / args.uneven_steps * 1e3:.2f} ms/step, {value:.0f} designs/s")
+
+    # eager runs under the same schedule: op count, then HIP-event kernel time
+    ctx.set_schedule(best)
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+    ctx.set_option(N.OPT_COUNT_OPS, 1)
+    if N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status) != 0:
+        raise RuntimeError(N.last_error())
+    stats = ctx.last_stats()
+    ctx.set_option(N.OPT_COUNT_OPS, 0)
+    ctx.set_option(N.OPT_PROFILE, 1)
+    kms = []
+    for _ in range(3):
+        N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status)
+        kms.append(ctx.last_stats()["trace_ms"])
+    ctx.set_option(N.OPT_PROFILE, 0)
+    ctx.set_schedule(None)
+    eager_info = np.array([f.info for f in out])
+    kernel_ms = float(np.mean(kms))
+    achieved = stats["ops"] / (kernel_ms * 1e-3) / 1e12
+    log(f"uneven: trace kernel {kernel_ms:.2f} ms, {achieved:.2f} TFLOP/s; items {stats['items']}")
+
+    # plain path on a prefix
+    n_plain = min(args.uneven_plain, n)
+    f1 = N.Fitness()
+    N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(darr[0]), ctypes.byref(sarr[0]), ctypes.byref(f1))
+    t = time.perf_counter()
+    for i in range(n_plain):
+        if N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(darr[i]), ctypes.byref(sarr[i]), ctypes.byref(f1)) != 0:
+            raise RuntimeError(N.last_error())
+    plain_s = time.perf_counter() - t
+    # the prefix's rays per second, expressed in designs of the batch's mean size
+    prefix_rays = sum(S[i].n_wavelengths * S[i].n_rays for i in range(n_plain))
+    plain_value = prefix_rays / plain_s / (rays / n)
+    log(f"uneven: plain {n_plain / plain_s:.0f} designs/s on the prefix, {plain_value:.0f} scaled to the batch mix")
+
+    mi_err = None
+    if args.uneven_check:
+        from oracle import oracle as O
+        # the cheapest designs of the batch, so the FP64 oracle finishes in seconds
+        cost = np.array([S[i].n_wavelengths * S[i].n_rays * (2 + D[i].n_glass) for i in range(n)])
+        pick = [int(i) for i in np.argsort(cost)[: args.uneven_check]]
+        errs = []
+        for i in pick:
+            ref, _ = O.fitness_many([D[i]], S[i].n_wavelengths, S[i].n_rays, precision=64, qrng=O.QRNG_FIXED_POINT)
+            errs.append(abs(ref[0, 1] - plan_info[i]))
+        mi_err = float(max(errs))
+        log(f"uneven: max |dI| vs FP64 oracle over {len(pick)} designs: {mi_err:.3e} bits")
+    plan.close()
+    return {"workload": f"{n} designs: 1-6 prisms, 16-128 bins, 32-256 wavelengths x 4096-65536 rays",
+            "rays": rays, "designs_per_s": value, "ms_per_step": elapsed / args.uneven_steps * 1e3,
+            "step_p50_ms": float(np.percentile(step, 50)), "step_p99_ms": float(np.percentile(step, 99)),
+            "captured": "HIP graph per generation (upload, trace records, trace, entropy, download)",
+            "schedule": best.as_dict(), "autotune_ms": tuned_ms, "autotune_s": tune_s,
+            "kernel_ms": kernel_ms, "ops_per_launch": stats["ops"], "achieved_tflops": achieved,
+            "frac": achieved / PEAK_FP32_TFLOPS, "identical_to_eager": bool(np.array_equal(plan_info, eager_info)),
+            "plain": {"designs_per_s_prefix": n_plain / plain_s, "designs": n_plain,
+                      "designs_per_s_batch_mix": plain_value, "batched_over_plain": value / plain_value},
+            "mi_error_bits": mi_err}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +190,12 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-sweep", action="store_true",
                     help="served runs at 20, 50, 100 and 200 req/s (the served configuration), 2 s of arrivals each")
+    ap.add_argument("--uneven", type=int, default=4096,
+                    help="designs of the uneven batch (1-6 prisms, 16-128 bins, 32-256 wavelengths x 4096-65536 "
+                         "rays), autotuned and replayed as a captured generation (0: skip)")
+    ap.add_argument("--uneven-steps", type=int, default=5)
+    ap.add_argument("--uneven-plain", type=int, default=32, help="uneven designs timed through the plain path")
+    ap.add_argument("--uneven-check", type=int, default=6, help="uneven designs checked against the FP64 oracle")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -216,6 +316,10 @@ def main() -> int:
                          f"fixed-count, one design per thread, {cpu_s:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.2f} designs/s on {threads} threads")
 
+    uneven = None
+    if args.uneven > 0:
+        uneven = uneven_run(cpd, N, np, ctx, args)
+
     serve = None
     if args.serve_rate > 0 or args.serve_sweep:
         t = time.time()
@@ -252,6 +356,7 @@ def main() -> int:
                        "plain_p99": float(np.percentile(lat_sorted, 99)),
                        "batch_p50": float(np.percentile(step_ms, 50)),
                        "batch_p99": float(np.percentile(step_ms, 99))},
+        "uneven": uneven,
         "served": serve,
         "mi_error_bits": mi_err,
         "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
