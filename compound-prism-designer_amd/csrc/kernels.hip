@@ -102,13 +102,13 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
     }
 }
 
-template <int P, bool COUNT>
+template <bool COUNT>
 __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     const uint32_t lane = threadIdx.x;
     const uint32_t col = lane & (c.cols - 1);
     // register copies of the per-item records: the loop stores through
     // last_bin, so loads through c.td would be repeated every iteration
-    const HotRecord<P> hot = to_sgprs(make_hot<P>(*c.td, c.lc));
+    const HotRecord hot = to_sgprs(make_hot(*c.td, c.lc));
     const float *__restrict__ z32 = c.z32;
     uint32_t *__restrict__ hist = c.hist;
     const uint32_t k_end = c.k_end, n_rays = c.n_rays, nbin = c.nbin, cols = c.cols;
@@ -118,7 +118,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         const bool active = k < k_end;
         const float z = active ? z32[k - 1] : 0.f;
         const float y = hot.y_mean - hot.beam_w * z;
-        const Trace32Result r = trace32<P, COUNT>(hot, y);
+        const Trace32Result r = trace32<COUNT>(hot, y);
         if (COUNT && active) ops += r.ops;
         const bool flagged = active && r.guard;
         if (active && !r.guard) {
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         const uint32_t d = a.slot ? a.slot[e] : e;
         const cpd_design *g = a.designs + d;
         const TraceDesign *td = a.traces + d;
-        const uint32_t M = td->n_rays, nbin = td->nbin, P = td->n_glass;
+        const uint32_t M = td->n_rays, nbin = td->nbin;
         const uint32_t n_tiles = td->n_tiles, tile_rays = td->tile_rays, cols = td->hist_cols;
         const uint64_t local = item - a.item_begin[e] + (a.item_offset ? a.item_offset[e] : 0);
         const uint32_t j = (uint32_t)(local / n_tiles), tile = (uint32_t)(local % n_tiles);
@@ -187,14 +187,7 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
             // the entrance face reflects every ray of this wavelength
             if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
         } else {
-            switch (P) {
-                case 1: tile_loop<1, COUNT>(c, ops); break;
-                case 2: tile_loop<2, COUNT>(c, ops); break;
-                case 3: tile_loop<3, COUNT>(c, ops); break;
-                case 4: tile_loop<4, COUNT>(c, ops); break;
-                case 5: tile_loop<5, COUNT>(c, ops); break;
-                default: tile_loop<6, COUNT>(c, ops); break;
-            }
+            tile_loop<COUNT>(c, ops);
             __syncthreads();
             unsigned long long *acc = a.acc + a.acc_begin[d] + (uint64_t)j * nbin;
             for (uint32_t b = lane; b < nbin; b += 64) {
@@ -287,11 +280,6 @@ __global__ void zero_u32_kernel(unsigned int *p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
-template <int P>
-__device__ Trace32Result trace32_p(const TraceDesign &td, const LambdaConsts &lc, float y) {
-    return trace32<P, false>(make_hot<P>(td, lc), y);
-}
-
 __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, const float *wcdf, const float *rcdf,
                                  uint32_t n, uint32_t *bin, float *t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,14 +291,7 @@ __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, con
     Trace32Result r{td->nbin, 0.f, false, 0};
     if (lc.ok) {
         const float y = td->y_mean - td->beam_w * (float)z;
-        switch (td->n_glass) {
-            case 1: r = trace32_p<1>(*td, lc, y); break;
-            case 2: r = trace32_p<2>(*td, lc, y); break;
-            case 3: r = trace32_p<3>(*td, lc, y); break;
-            case 4: r = trace32_p<4>(*td, lc, y); break;
-            case 5: r = trace32_p<5>(*td, lc, y); break;
-            default: r = trace32_p<6>(*td, lc, y); break;
-        }
+        r = trace32<false>(make_hot(*td, lc), y);
     }
     if (!lc.ok || r.guard) {
         double t64;

@@ -168,9 +168,11 @@ cpd_status Context::autotune(const cpd_design *designs, const cpd_sampling *samp
 }
 
 double item_cost(const cpd_design &g, const cpd_sampling &s, uint32_t tile_cap) {
-    const double per_ray = kOpsBeam + kOpsDetector + kOpsHit + kOpsLensHit + kOpsLensRefract +
-                           (double)g.n_glass * (kOpsFaceHit + kOpsFaceRefract);
-    return per_ray * tile_for(g.bin_count, s.n_rays, tile_cap);
+    // per ray the trace costs the same for every prism count (the planar
+    // faces are hoisted per wavelength); per item the wavelength set-up adds
+    // a few hundred FP64 operations per face
+    const double per_ray = kOpsBeam + kOpsChain + kOpsLensHit + kOpsLensRefract + kOpsDetector + kOpsHit;
+    return per_ray * tile_for(g.bin_count, s.n_rays, tile_cap) + 300.0 * g.n_glass;
 }
 
 void order_designs(const cpd_design *designs, const cpd_sampling *sampling, std::vector<uint32_t> &valid,

@@ -1,14 +1,15 @@
 // trace_core.h -- per-ray trace of the 2D compound-prism spectrometer.
 //
 // Two traces live here:
-//  * trace32<P>: the FP32 hot path.  Per-wavelength invariants (refractive
-//    ratios, the entrance-face refraction of the collimated beam) arrive
-//    precomputed in FP64 and rounded once; the exit lens and the detector are
-//    solved relative to the lens' chord midpoint, which avoids the
-//    cancellation of |o - c|^2 - R^2 at lens radii far larger than the prism.
-//    Every comparison whose outcome decides hit/miss or the bin carries an
-//    error margin; a ray that lands inside a margin is flagged for the FP64
-//    trace instead of being counted.
+//  * trace32: the FP32 hot path.  Per-wavelength invariants arrive
+//    precomputed in FP64 and rounded once: the planar faces, which every ray
+//    of the collimated beam crosses with the same direction, collapse to an
+//    interval of accepted beam heights and an affine exit point.  The exit
+//    lens and the detector are solved relative to the lens' chord midpoint,
+//    which avoids the cancellation of |o - c|^2 - R^2 at lens radii far
+//    larger than the prism.  Every comparison whose outcome decides hit/miss
+//    or the bin carries an error margin; a ray that lands inside a margin is
+//    flagged for the FP64 trace instead of being counted.
 //  * trace64: the FP64 trace in the reference's operation order
 //    (src/spectrometer/ray.rs:309-382, 485-499; geometry.rs:124-145,
 //    161-216; spectrometer.rs:265-303), with fused multiply-adds exactly at the
@@ -254,30 +255,36 @@ CPD_HD uint32_t tile_for(uint32_t nbin, uint32_t n_rays, uint32_t tile_cap) {
 struct alignas(16) TraceDesign {
     uint32_t n_glass, nbin, ar_coated, n_lambda;
     uint32_t n_rays, hist_cols, tile_rays, n_tiles;  // LDS columns; rays per work item; items per wavelength
-    // entrance face: a beam ray at height y meets it at x = a0 + b0 y; the
-    // prism spans heights [0, h], tested as |y - h/2| <= h/2
-    float a0, b0, h2, nbin_f;
-    // faces 1..P-1 (slot 0 unused): normal and point.normal
-    float nx[CPD_MAX_PRISMS], ny[CPD_MAX_PRISMS], pn[CPD_MAX_PRISMS];
     // exit lens relative to its chord midpoint m: centre offset cm = c - m,
-    // kappa = |cm|^2 - R^2, 1/R, bound radius^2, error scales
-    float mx, my, cmx, cmy;
-    float kappa, inv_r, max_dist_sq, lens_scale;
-    float bound_scale, reserved3, reserved4, reserved5;
+    // 1/R, bound radius^2, error scales
+    float cmx, cmy, inv_r, max_dist_sq;
+    float lens_scale, bound_scale, nbin_f, reserved0;
     // detector, relative to m: m - P0, normal, direction, |pos - L/2| <= L/2
     float mpx, mpy, dnx, dny;
     float ddx, ddy, min_ci, half_len;
     float slope, inv_slope, intercept, bin_size;
-    float beam_w, y_mean, reserved6, reserved7;
+    float beam_w, y_mean, reserved1, reserved2;
 };
 
+// Per-wavelength constants of one design.  Every ray of the collimated beam
+// enters travelling +x, and refraction at a plane depends only on the
+// direction, so through the planar faces all rays share one direction and
+// one pair of Fresnel factors; they are computed once, in FP64 and in the
+// reference's operation order, and are the reference's values exactly.  The
+// hit point on each face is affine in the beam height y, so the planar chain
+// reduces to an interval of accepted heights and an affine map to the point
+// where a ray leaves the last face.
 struct LambdaConsts {
-    float r[CPD_MAX_PRISMS + 1];     // n_{i-1}/n_i at faces 1..P-1, n_{P-1} at the lens
-    float r2[CPD_MAX_PRISMS + 1];    // r^2
-    float omr2[CPD_MAX_PRISMS + 1];  // 1 - r^2
-    float dx0, dy0, ts0, tp0;        // beam after the entrance face
-    float amp0;                      // angular amplification of the entrance refraction, >= 1
-    int ok;                          // 0 when the entrance face reflects totally
+    float ylo, yhi;            // accepted beam heights (empty when ylo > yhi)
+    float e0x, e0y, e1x, e1y;  // last planar hit point minus the lens chord midpoint: e0 + e1 y
+    float lb0, lb1;            // lens: b(y) = dir.(e - cm) = lb0 + lb1 y
+    float lc0, lc1, lc2;       // lens: |e|^2 - 2 e.cm + |cm|^2 - R^2 = lc0 + lc1 y + lc2 y^2
+    float dx, dy, rdx, rdy;    // direction into the lens, and times the lens ratio r
+    float ts, tp;              // Fresnel transmittances through the planar faces
+    float amp;                 // angular amplification through the planar faces, >= 1
+    float path;                // length scale of the FP32 error budget up to the lens
+    float lr, lr2, lomr2;      // lens ratio r = n_{P-1}, r^2, 1 - r^2
+    int ok;                    // 0 when every ray is lost in the planar faces (TIR, parallel face)
     double lambda;
 };
 
@@ -294,28 +301,14 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.hist_cols = hist_cols(g.bin_count);
     t.tile_rays = tile_for(g.bin_count, n_rays, tile_cap);
     t.n_tiles = (n_rays + t.tile_rays - 1) / t.tile_rays;
-    const double n0x = g.plane_normal[0][0], n0y = g.plane_normal[0][1];
-    const double c0 = g.plane_point[0][0] * n0x + g.plane_point[0][1] * n0y;
-    t.a0 = (float)(c0 / n0x);
-    t.b0 = (float)(-n0y / n0x);
-    t.h2 = (float)(g.height * 0.5);
-    t.nbin_f = (float)g.bin_count;
-    for (int i = 0; i < CPD_MAX_PRISMS; ++i) {
-        t.nx[i] = (float)g.plane_normal[i][0];
-        t.ny[i] = (float)g.plane_normal[i][1];
-        t.pn[i] = (float)(g.plane_point[i][0] * g.plane_normal[i][0] + g.plane_point[i][1] * g.plane_normal[i][1]);
-    }
-    const double cmx = g.arc_center[0] - g.arc_midpt[0], cmy = g.arc_center[1] - g.arc_midpt[1];
-    t.mx = (float)g.arc_midpt[0];
-    t.my = (float)g.arc_midpt[1];
-    t.cmx = (float)cmx;
-    t.cmy = (float)cmy;
-    t.kappa = (float)(cmx * cmx + cmy * cmy - g.arc_radius * g.arc_radius);
+    t.cmx = (float)(g.arc_center[0] - g.arc_midpt[0]);
+    t.cmy = (float)(g.arc_center[1] - g.arc_midpt[1]);
     t.inv_r = (float)(1.0 / g.arc_radius);
     t.max_dist_sq = (float)g.arc_max_dist_sq;
     t.lens_scale = (float)(g.arc_radius + sqrt(g.arc_max_dist_sq));
     t.bound_scale = (float)(4.0 * (sqrt(g.arc_max_dist_sq) + 1.0));
-    t.reserved3 = t.reserved4 = t.reserved5 = 0.f;
+    t.nbin_f = (float)g.bin_count;
+    t.reserved0 = 0.f;
     t.mpx = (float)(g.arc_midpt[0] - g.det_position[0]);
     t.mpy = (float)(g.arc_midpt[1] - g.det_position[1]);
     t.dnx = (float)g.det_normal[0];
@@ -330,45 +323,115 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.bin_size = (float)g.det_bin_size;
     t.beam_w = (float)g.beam_width;
     t.y_mean = (float)g.y_mean;
-    t.reserved6 = t.reserved7 = 0.f;
+    t.reserved1 = t.reserved2 = 0.f;
 }
 
-// Wavelength sample j (1-based) of the R1 sequence and its per-face
-// constants, all in FP64 (src/spectrometer/distribution.rs:32-34,
-// glasscat.rs:17-22, and the entrance refraction of ray.rs:309-367 for the
-// beam direction (1, 0), which every ray shares).
+// Wavelength sample j (1-based) of the R1 sequence (src/spectrometer/
+// distribution.rs:32-34).
 CPD_HD double lambda_of(const cpd_design &g, double u) {
 #pragma clang fp contract(off)
     return g.wavelength_lo + (g.wavelength_hi - g.wavelength_lo) * u;
 }
 
+// Fold the linear condition c + m y >= 0 on the beam height into [lo, hi].
+CPD_HD void clip_heights(double c, double m, double &lo, double &hi) {
+    if (m > 0.0) {
+        const double t = -c / m;
+        if (t > lo) lo = t;
+    } else if (m < 0.0) {
+        const double t = -c / m;
+        if (t < hi) hi = t;
+    } else if (!(c >= 0.0)) {
+        lo = INFINITY;
+        hi = -INFINITY;
+    }
+}
+
+// Angular amplification max(r ci, cr) / cr of a refraction (>= 1), the
+// factor by which it can grow an error of the incoming direction.
+CPD_HD double refraction_gain(double dx, double dy, double nx, double ny, double n1, double n2) {
+    const double ci = -(dx * nx + dy * ny), r = n1 / n2;
+    const double cr2 = 1.0 - r * r * (1.0 - ci * ci);
+    if (!(cr2 > 0.0)) return 1.0;
+    const double cr = sqrt(cr2), a = r * ci / cr;
+    return a > 1.0 ? a : 1.0;
+}
+
+// The planar chain for one wavelength (src/spectrometer/ray.rs:485-499 with
+// geometry.rs:124-145 and ray.rs:309-367 per face), in FP64.  The shared
+// direction and Fresnel factors use refract64, the reference's operation
+// order.  Face i's hit point is o_i(y) = A_i + B_i y and its distance along
+// the shared direction d is (p.n - o_i.n) / (d.n), affine in y as well; the
+// conditions d >= 0 and 0 <= hit y <= h become bounds on y.
 CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts &lc) {
 #pragma clang fp contract(off)
     const int P = (int)g.n_glass;
+    const bool ar = g.ar_coated != 0;
     double n[CPD_MAX_PRISMS];
     for (int i = 0; i < P; ++i) n[i] = glass_n64(g.glass[i], lambda);
     lc.lambda = lambda;
+    const double h = g.height;
+    // entrance face: the ray from (0, y) along +x meets it at
+    // x = ((p - o).n) / n.x = (p.n - y n.y) / n.x, at height y
+    const double n0x = g.plane_normal[0][0], n0y = g.plane_normal[0][1];
+    const double c0 = g.plane_point[0][0] * n0x + g.plane_point[0][1] * n0y;
+    bool ok = n0x != 0.0;
+    double ax = ok ? c0 / n0x : 0.0, bx = ok ? -n0y / n0x : 0.0, ay = 0.0, by = 1.0;
+    double lo = 0.0, hi = h;
+    clip_heights(ax, bx, lo, hi);
+    double path = 1.0 + h + fabs(ax) + fabs(bx) * h;
+    double amp = refraction_gain(1.0, 0.0, n0x, n0y, 1.0, n[0]);
     Ray64 ray{0.0, 0.0, 1.0, 0.0, 1.0, 1.0};
-    lc.ok = refract64(ray, g.plane_normal[0][0], g.plane_normal[0][1], 1.0, n[0], g.ar_coated != 0) == T_OK;
-    lc.dx0 = (float)ray.dx;
-    lc.dy0 = (float)ray.dy;
-    lc.ts0 = (float)ray.ts;
-    lc.tp0 = (float)ray.tp;
-    {
-        double ci = -g.plane_normal[0][0], r = 1.0 / n[0];
-        double cr2 = 1.0 - r * r * (1.0 - ci * ci);
-        double cr = cr2 > 0.0 ? sqrt(cr2) : 0.0;
-        double a = cr > 0.0 ? r * ci / cr : 1e30;
-        lc.amp0 = (float)(a > 1.0 ? a : 1.0);
+    ok = ok && refract64(ray, n0x, n0y, 1.0, n[0], ar) == T_OK;
+    for (int i = 1; i < P && ok; ++i) {
+        const double nx = g.plane_normal[i][0], ny = g.plane_normal[i][1];
+        const double ci = ray.dx * nx + ray.dy * ny;
+        if (ci == 0.0) {
+            ok = false;
+            break;
+        }
+        const double pn = g.plane_point[i][0] * nx + g.plane_point[i][1] * ny;
+        const double alpha = (pn - (ax * nx + ay * ny)) / ci;
+        const double beta = -(bx * nx + by * ny) / ci;
+        clip_heights(alpha, beta, lo, hi);
+        ax = ax + ray.dx * alpha;
+        ay = ay + ray.dy * alpha;
+        bx = bx + ray.dx * beta;
+        by = by + ray.dy * beta;
+        clip_heights(ay, by, lo, hi);
+        clip_heights(h - ay, -by, lo, hi);
+        path += fabs(alpha) + fabs(beta) * h;
+        amp *= refraction_gain(ray.dx, ray.dy, nx, ny, n[i - 1], n[i]);
+        ok = refract64(ray, nx, ny, n[i - 1], n[i], ar) == T_OK;
     }
-    for (int i = 0; i <= CPD_MAX_PRISMS; ++i) {
-        double r = 1.0;
-        if (i >= 1 && i < P) r = n[i - 1] / n[i];
-        if (i == P) r = n[P - 1];
-        lc.r[i] = (float)r;
-        lc.r2[i] = (float)(r * r);
-        lc.omr2[i] = (float)(1.0 - r * r);
-    }
+    lc.ok = ok ? 1 : 0;
+    lc.ylo = (float)lo;
+    lc.yhi = (float)hi;
+    // relative to the lens chord midpoint m; c = m + cm
+    const double e0x = ax - g.arc_midpt[0], e0y = ay - g.arc_midpt[1];
+    const double cmx = g.arc_center[0] - g.arc_midpt[0], cmy = g.arc_center[1] - g.arc_midpt[1];
+    const double kappa = cmx * cmx + cmy * cmy - g.arc_radius * g.arc_radius;
+    lc.e0x = (float)e0x;
+    lc.e0y = (float)e0y;
+    lc.e1x = (float)bx;
+    lc.e1y = (float)by;
+    lc.lb0 = (float)(ray.dx * (e0x - cmx) + ray.dy * (e0y - cmy));
+    lc.lb1 = (float)(ray.dx * bx + ray.dy * by);
+    lc.lc0 = (float)(e0x * e0x + e0y * e0y - 2.0 * (e0x * cmx + e0y * cmy) + kappa);
+    lc.lc1 = (float)(2.0 * (e0x * bx + e0y * by) - 2.0 * (bx * cmx + by * cmy));
+    lc.lc2 = (float)(bx * bx + by * by);
+    const double r = n[P - 1];
+    lc.dx = (float)ray.dx;
+    lc.dy = (float)ray.dy;
+    lc.rdx = (float)(ray.dx * r);
+    lc.rdy = (float)(ray.dy * r);
+    lc.ts = (float)ray.ts;
+    lc.tp = (float)ray.tp;
+    lc.amp = (float)amp;
+    lc.path = (float)(path + fabs(e0x) + fabs(e0y) + (fabs(bx) + fabs(by)) * h);
+    lc.lr = (float)r;
+    lc.lr2 = (float)(r * r);
+    lc.lomr2 = (float)(1.0 - r * r);
 }
 
 // ---------------------------------------------------------------------------
@@ -387,13 +450,15 @@ CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
 static constexpr float kGuardRel = 8.0e-6f;
 static constexpr float kGuardCos = 2.0e-5f;
 
-// Per-stage operation counts of the reference formulas with per-design and
-// per-wavelength invariants hoisted (add/mul/div/sqrt = 1, fma = 2): beam
-// height 2; planar face 42 = intersection 13 + refraction 29; lens 58 =
-// intersection, bounds and normal 26 + refraction 32; detector 24; hit 1.
-// A ray is charged for the stages it reaches.
-static constexpr uint32_t kOpsBeam = 2, kOpsFaceHit = 13, kOpsFaceRefract = 29, kOpsLensHit = 26,
-                          kOpsLensRefract = 32, kOpsDetector = 24, kOpsHit = 1;
+// Algorithmic operations per stage a ray reaches (add/mul/div/sqrt = 1,
+// fma = 2), after hoisting what the beam's rays share: beam height 2; the
+// planar chain 4 (the affine hit point on the last face); lens 54 =
+// intersection, bounds and normal 22 + refraction 32; detector 24; hit 1.
+// The same stages without the planar-chain hoist cost 42 per planar face
+// instead of 4 in total (per ray 85 + 42 P, the reference formulas with
+// only per-design and per-wavelength invariants hoisted).
+static constexpr uint32_t kOpsBeam = 2, kOpsChain = 4, kOpsLensHit = 22, kOpsLensRefract = 32, kOpsDetector = 24,
+                          kOpsHit = 1;
 
 struct Trace32Result {
     uint32_t bin;  // nbin on a miss
@@ -402,91 +467,54 @@ struct Trace32Result {
     uint32_t ops;  // algorithmic operations of the stages the ray reached (COUNT)
 };
 
-// Snell refraction of (dx, dy) at unit normal (nx, ny) with ci = -d.n and
-// ratio r = n1/n2 (r2 = r^2, omr2 = 1 - r^2), plus the Fresnel factors
-// 1 - rs^2 and 1 - rp^2 with rs = (r ci - cr) / (r ci + cr) and
-// rp = (r cr - ci) / (r cr + ci) (the reference's amplitudes divided through
-// by n2) from one reciprocal.  The angular amplification of the refraction,
-// A = r ci / cr, is folded into the margin as amp_num / amp_den.
-struct Refract32 {
-    float vx, vy, fs, fp;
-    float cr_sq;
-    float amp_num, amp_den;
-};
-
-CPD_HD Refract32 refract32(float dx, float dy, float nx, float ny, float ci, float r, float r2, float omr2) {
-    Refract32 o;
-    o.cr_sq = fmaf(r2, ci * ci, omr2);
-    const float cr = fast_sqrt(fmaxf(o.cr_sq, 0.f));
-    const float rci = r * ci;
-    const float k = rci - cr;
-    o.vx = fmaf(nx, k, dx * r);
-    o.vy = fmaf(ny, k, dy * r);
-    const float a = rci + cr;
-    const float b = fmaf(r, cr, ci);
-    const float inv = fast_rcp(a * b);
-    const float as = k * (b * inv);
-    const float ap = fmaf(r, cr, -ci) * (a * inv);
-    o.fs = fmaf(-as, as, 1.f);
-    o.fp = fmaf(-ap, ap, 1.f);
-    o.amp_num = fmaxf(rci, cr);
-    o.amp_den = cr;
-    return o;
-}
-
-// The constants one ray of a P-glass design at one wavelength reads: the
-// hot subset of TraceDesign and LambdaConsts, small enough to live in scalar
-// registers for the whole ray loop.  Faces 1..P-1 are stored at [0, P-2].
-template <int P>
+// The constants one ray reads: the hot subset of TraceDesign and
+// LambdaConsts, small enough to live in scalar registers for the whole ray
+// loop.
 struct alignas(4) HotRecord {
-    static constexpr int F = P > 1 ? P - 1 : 1;
     uint32_t nbin, ar_coated;
-    float a0, b0, h2, y_mean, beam_w, nbin_f;
-    float dx0, dy0, ts0, tp0, amp0;
-    float fnx[F], fny[F], fpn[F], fr[F], fr2[F], fomr2[F];
-    float mx, my, cmx, cmy, kappa, inv_r, max_dist_sq, lens_scale, bound_scale;
-    float lr, lr2, lomr2;
+    float y_mean, beam_w, ylo, yhi;
+    float e0x, e0y, e1x, e1y, lb0, lb1, lc0, lc1, lc2;
+    float dx, dy, rdx, rdy, ts, tp, amp, path;
+    float cmx, cmy, inv_r, max_dist_sq, lens_scale, bound_scale;
+    float lr2, lomr2, lr;
     float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
-    float slope, inv_slope, intercept, bin_size;
+    float slope, inv_slope, intercept, bin_size, nbin_f;
 };
 
-template <int P>
-CPD_HD HotRecord<P> make_hot(const TraceDesign &d, const LambdaConsts &lc) {
-    HotRecord<P> h;
+CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
+    HotRecord h;
     h.nbin = d.nbin;
     h.ar_coated = d.ar_coated;
-    h.a0 = d.a0;
-    h.b0 = d.b0;
-    h.h2 = d.h2;
     h.y_mean = d.y_mean;
     h.beam_w = d.beam_w;
-    h.nbin_f = d.nbin_f;
-    h.dx0 = lc.dx0;
-    h.dy0 = lc.dy0;
-    h.ts0 = lc.ts0;
-    h.tp0 = lc.tp0;
-    h.amp0 = lc.amp0;
-    for (int i = 0; i < HotRecord<P>::F; ++i) {
-        const int f = i + 1 < CPD_MAX_PRISMS ? i + 1 : CPD_MAX_PRISMS - 1;
-        h.fnx[i] = d.nx[f];
-        h.fny[i] = d.ny[f];
-        h.fpn[i] = d.pn[f];
-        h.fr[i] = lc.r[f];
-        h.fr2[i] = lc.r2[f];
-        h.fomr2[i] = lc.omr2[f];
-    }
-    h.mx = d.mx;
-    h.my = d.my;
+    h.ylo = lc.ylo;
+    h.yhi = lc.yhi;
+    h.e0x = lc.e0x;
+    h.e0y = lc.e0y;
+    h.e1x = lc.e1x;
+    h.e1y = lc.e1y;
+    h.lb0 = lc.lb0;
+    h.lb1 = lc.lb1;
+    h.lc0 = lc.lc0;
+    h.lc1 = lc.lc1;
+    h.lc2 = lc.lc2;
+    h.dx = lc.dx;
+    h.dy = lc.dy;
+    h.rdx = lc.rdx;
+    h.rdy = lc.rdy;
+    h.ts = lc.ts;
+    h.tp = lc.tp;
+    h.amp = lc.amp;
+    h.path = lc.path;
     h.cmx = d.cmx;
     h.cmy = d.cmy;
-    h.kappa = d.kappa;
     h.inv_r = d.inv_r;
     h.max_dist_sq = d.max_dist_sq;
     h.lens_scale = d.lens_scale;
     h.bound_scale = d.bound_scale;
-    h.lr = lc.r[P];
-    h.lr2 = lc.r2[P];
-    h.lomr2 = lc.omr2[P];
+    h.lr2 = lc.lr2;
+    h.lomr2 = lc.lomr2;
+    h.lr = lc.lr;
     h.mpx = d.mpx;
     h.mpy = d.mpy;
     h.dnx = d.dnx;
@@ -499,133 +527,91 @@ CPD_HD HotRecord<P> make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     h.inv_slope = d.inv_slope;
     h.intercept = d.intercept;
     h.bin_size = d.bin_size;
+    h.nbin_f = d.nbin_f;
     return h;
 }
 
-template <int P, bool COUNT>
-CPD_HD Trace32Result trace32(const HotRecord<P> &d, float y) {
+// One ray at beam height y.  Decisions are kept as per-lane booleans and
+// combined with bitwise operators, so the whole trace is straight-line code.
+template <bool COUNT>
+CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     Trace32Result res{d.nbin, 0.f, false, 0};
     const bool ar = d.ar_coated != 0;
-    // entrance face: the beam ray (0, y) travelling +x meets it at height y
-    const float ay = y - d.h2;
-    bool alive = fabsf(ay) <= d.h2;
-    bool guard = fabsf(fabsf(ay) - d.h2) <= kGuardRel * (fabsf(y) + 1.f);
-    float ox = fmaf(d.b0, y, d.a0), oy = y;
-    alive = alive && ox >= 0.f;
-    float path = fabsf(ox) + fabsf(y) + 1.f;
-    float num = d.amp0, den = 1.f;
-    float dx = d.dx0, dy = d.dy0, ts = d.ts0, tp = d.tp0;
-    if (COUNT) res.ops = kOpsBeam + kOpsFaceHit + (alive ? kOpsFaceRefract : 0u);
-#pragma unroll
-    for (int i = 1; i < P; ++i) {
-        if (COUNT) res.ops += alive ? kOpsFaceHit : 0u;
-        const float nx = d.fnx[i - 1], ny = d.fny[i - 1];
-        const float ci_p = fmaf(dx, nx, dy * ny);
-        const float dist = (d.fpn[i - 1] - fmaf(ox, nx, oy * ny)) * fast_rcp(ci_p);
-        const float hx = fmaf(dx, dist, ox), hy = fmaf(dy, dist, oy);
-        path += fabsf(dist);
-        const float a = hy - d.h2;
-        guard = guard || (alive && fabsf(fabsf(a) - d.h2) * den <= kGuardRel * path * num);
-        alive = alive && dist >= 0.f && fabsf(a) <= d.h2;
-        if (COUNT) res.ops += alive ? kOpsFaceRefract : 0u;
-        ox = hx;
-        oy = hy;
-        const float ci = -ci_p;
-        const Refract32 f = refract32(dx, dy, nx, ny, ci, d.fr[i - 1], d.fr2[i - 1], d.fomr2[i - 1]);
-        guard = guard || (alive && fabsf(f.cr_sq) <= kGuardCos);
-        alive = alive && f.cr_sq >= 0.f;
-        if (ar) {
-            guard = guard || (alive && fabsf(ci - 0.5f) * den <= kGuardCos * num);
-            const bool coat = ci > 0.5f;
-            ts *= coat ? 0.99f : f.fs;
-            tp *= coat ? 0.99f : f.fp;
-        } else {
-            ts *= f.fs;
-            tp *= f.fp;
-        }
-        num *= f.amp_num;
-        den *= f.amp_den;
-        dx = f.vx;
-        dy = f.vy;
-    }
+    // planar chain: the ray survives the planar faces iff ylo <= y <= yhi
+    const float ymarg = kGuardRel * (fabsf(y) + 1.f);
+    bool alive = (y >= d.ylo) & (y <= d.yhi);
+    bool guard = (fabsf(y - d.ylo) <= ymarg) | (fabsf(y - d.yhi) <= ymarg);
+    const float ex = fmaf(d.e1x, y, d.e0x), ey = fmaf(d.e1y, y, d.e0y);
+    if (COUNT) res.ops = kOpsBeam + kOpsChain + (alive ? kOpsLensHit : 0u);
     // exit lens, relative to its chord midpoint m: e = o - m, c = m + cm.
-    // |o + s dir - c|^2 = R^2  <=>  s^2 + 2 b s + c2 = 0 with
-    // b = dir.(e - cm) and c2 = |e|^2 - 2 e.cm + (|cm|^2 - R^2); the root of
-    // larger magnitude is -(b + sign(b) sqrt(disc)) and the other one is
-    // c2 over it, so neither root cancels.
-    if (COUNT) res.ops += alive ? kOpsLensHit : 0u;
-    float qx, qy;
-    {
-        const float ex = ox - d.mx, ey = oy - d.my;
-        const float b = fmaf(dx, ex - d.cmx, dy * (ey - d.cmy));
-        const float e2 = fmaf(ex, ex, ey * ey);
-        const float ec = fmaf(ex, d.cmx, ey * d.cmy);
-        const float c2 = fmaf(-2.f, ec, e2 + d.kappa);
-        const float disc = fmaf(b, b, -c2);
-        const float lim = kGuardRel * path * num;
-        guard = guard || (alive && fabsf(disc) * den <= lim * 4.f * (fabsf(b) + d.lens_scale));
-        alive = alive && disc > 0.f;
-        const float sd = fast_sqrt(fmaxf(disc, 0.f));
-        const float big = -(b + copysignf(sd, b));
-        const float small = c2 * fast_rcp(big);
-        const bool bpos = !signbit(b);
-        const float d1 = bpos ? big : small, d2 = bpos ? small : big;
-        const float q1x = fmaf(dx, d1, ex), q1y = fmaf(dy, d1, ey);
-        const float q2x = fmaf(dx, d2, ex), q2y = fmaf(dy, d2, ey);
-        const float s1 = fmaf(q1x, q1x, q1y * q1y), s2 = fmaf(q2x, q2x, q2y * q2y);
-        const bool ok1 = d1 >= 0.f && s1 <= d.max_dist_sq;
-        const bool ok2 = d2 >= 0.f && s2 <= d.max_dist_sq;
-        const float blim = lim * d.bound_scale;
-        guard = guard || (alive && (fabsf(s1 - d.max_dist_sq) * den <= blim ||
-                                    (!ok1 && fabsf(s2 - d.max_dist_sq) * den <= blim)));
-        alive = alive && (ok1 || ok2);
-        if (COUNT) res.ops += alive ? kOpsLensRefract : 0u;
-        qx = ok1 ? q1x : q2x;
-        qy = ok1 ? q1y : q2y;
-        path += fabsf(ok1 ? d1 : d2);
-        // normal (c - p) / R = (cm - q) / R
-        const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
-        const float ci = -fmaf(dx, nx, dy * ny);
-        const Refract32 f = refract32(dx, dy, nx, ny, ci, d.lr, d.lr2, d.lomr2);
-        guard = guard || (alive && fabsf(f.cr_sq) <= kGuardCos);
-        alive = alive && f.cr_sq >= 0.f;
-        if (ar) {
-            guard = guard || (alive && fabsf(ci - 0.5f) * den <= kGuardCos * num);
-            const bool coat = ci > 0.5f;
-            ts *= coat ? 0.99f : f.fs;
-            tp *= coat ? 0.99f : f.fp;
-        } else {
-            ts *= f.fs;
-            tp *= f.fp;
-        }
-        num *= f.amp_num;
-        den *= f.amp_den;
-        dx = f.vx;
-        dy = f.vy;
-    }
+    // |e + s dir - cm|^2 = R^2  <=>  s^2 + 2 b s + c = 0; the root of larger
+    // magnitude is -(b + sign(b) sqrt(disc)) and the other one is c over it,
+    // so neither root cancels.
+    const float num0 = d.amp;
+    const float lim = kGuardRel * d.path * num0;
+    const float dx = d.dx, dy = d.dy;
+    const float b = fmaf(d.lb1, y, d.lb0);
+    const float c = fmaf(fmaf(d.lc2, y, d.lc1), y, d.lc0);
+    const float disc = fmaf(b, b, -c);
+    guard |= alive & (fabsf(disc) <= lim * 4.f * (fabsf(b) + d.lens_scale));
+    alive &= disc > 0.f;
+    const float sd = fast_sqrt(fmaxf(disc, 0.f));
+    const float big = -(b + copysignf(sd, b));
+    const float small = c * fast_rcp(big);
+    const bool bpos = !signbit(b);
+    const float d1 = bpos ? big : small, d2 = bpos ? small : big;
+    const float q1x = fmaf(dx, d1, ex), q1y = fmaf(dy, d1, ey);
+    const float q2x = fmaf(dx, d2, ex), q2y = fmaf(dy, d2, ey);
+    const float s1 = fmaf(q1x, q1x, q1y * q1y), s2 = fmaf(q2x, q2x, q2y * q2y);
+    const bool ok1 = (d1 >= 0.f) & (s1 <= d.max_dist_sq);
+    const bool ok2 = (d2 >= 0.f) & (s2 <= d.max_dist_sq);
+    const float blim = lim * d.bound_scale;
+    guard |= alive & ((fabsf(s1 - d.max_dist_sq) <= blim) | (!ok1 & (fabsf(s2 - d.max_dist_sq) <= blim)));
+    alive &= ok1 | ok2;
+    if (COUNT) res.ops += alive ? kOpsLensRefract : 0u;
+    const float qx = ok1 ? q1x : q2x, qy = ok1 ? q1y : q2y;
+    const float path = d.path + fabsf(ok1 ? d1 : d2);
+    // normal (c - p) / R = (cm - q) / R
+    const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
+    const float ci = -fmaf(dx, nx, dy * ny);
+    const float cr_sq = fmaf(d.lr2, ci * ci, d.lomr2);
+    const float cr = fast_sqrt(fmaxf(cr_sq, 0.f));
+    const float rci = d.lr * ci;
+    const float k = rci - cr;
+    const float vx = fmaf(nx, k, d.rdx), vy = fmaf(ny, k, d.rdy);
+    guard |= alive & (fabsf(cr_sq) <= kGuardCos);
+    alive &= cr_sq >= 0.f;
+    // Fresnel: rs = (r ci - cr) / (r ci + cr), rp = (r cr - ci) / (r cr + ci)
+    // (the reference's amplitudes divided through by n2), one reciprocal
+    const float fa = rci + cr, fb = fmaf(d.lr, cr, ci);
+    const float inv = fast_rcp(fa * fb);
+    const float as = k * (fb * inv), ap = fmaf(d.lr, cr, -ci) * (fa * inv);
+    const bool coat = ar & (ci > 0.5f);
+    guard |= alive & ar & (fabsf(ci - 0.5f) <= kGuardCos * num0);
+    const float ts = d.ts * (coat ? 0.99f : fmaf(-as, as, 1.f));
+    const float tp = d.tp * (coat ? 0.99f : fmaf(-ap, ap, 1.f));
+    const float num = num0 * fmaxf(rci, cr), den = cr;
     // detector array: w = o - P0 = (m - P0) + q
     if (COUNT) res.ops += alive ? kOpsDetector : 0u;
     {
-        const float ci = -fmaf(dx, d.dnx, dy * d.dny);
-        guard = guard || (alive && fabsf(ci - d.min_ci) * den <= kGuardCos * num);
-        alive = alive && ci > d.min_ci;
+        const float cd = -fmaf(vx, d.dnx, vy * d.dny);
+        guard |= alive & (fabsf(cd - d.min_ci) * den <= kGuardCos * num);
+        alive &= cd > d.min_ci;
         const float wx = d.mpx + qx, wy = d.mpy + qy;
-        const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(ci);
-        const float pos = fmaf(fmaf(dx, dist, wx), d.ddx, fmaf(dy, dist, wy) * d.ddy);
-        const float lim = kGuardRel * (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * num;
-        const float ap = pos - d.half_len;
-        guard = guard || (alive && fabsf(fabsf(ap) - d.half_len) * den <= lim);
-        alive = alive && fabsf(ap) <= d.half_len;
+        const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(cd);
+        const float pos = fmaf(fmaf(vx, dist, wx), d.ddx, fmaf(vy, dist, wy) * d.ddy);
+        const float plim = kGuardRel * (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * num;
+        const float apos = pos - d.half_len;
+        guard |= alive & (fabsf(fabsf(apos) - d.half_len) * den <= plim);
+        alive &= fabsf(apos) <= d.half_len;
         const float v = pos - d.intercept;
         const float qb = floorf(v * d.inv_slope);
         const float rem = fmaf(-qb, d.slope, v);
         const float edge = fminf(fminf(fabsf(rem), fabsf(d.slope - rem)), fabsf(rem - d.bin_size));
-        guard = guard || (alive && edge * den <= lim);
-        alive = alive && qb >= 0.f && qb < d.nbin_f && rem < d.bin_size;
-        if (alive) {
-            res.bin = (uint32_t)qb;
-            res.t = (ts + tp) * 0.5f;
-        }
+        guard |= alive & (edge * den <= plim);
+        alive &= (qb >= 0.f) & (qb < d.nbin_f) & (rem < d.bin_size);
+        res.bin = alive ? (uint32_t)qb : d.nbin;
+        res.t = alive ? (ts + tp) * 0.5f : 0.f;
         if (COUNT) res.ops += alive ? kOpsHit : 0u;
     }
     res.guard = guard;
