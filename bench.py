@@ -12,9 +12,11 @@ Printed as ONE JSON line:
   plain            designs/s of the plain one-design-per-launch path (the
                    reference's cuda_fitness structure) and the batched/plain ratio
   mi_error_bits    max |info_GPU - info_FP64 oracle| over a checked subset
-  roofline         trace kernel: algorithmic operations per launch / mean
-                   kernel time (HIP events on the context's stream) against
-                   MI355X's 157.3 TFLOP/s FP32 peak
+  roofline         trace kernel: algorithmic operations per launch (stages
+                   reached, planar faces hoisted per wavelength) / mean kernel
+                   time (HIP events on the context's stream) against MI355X's
+                   157.3 TFLOP/s FP32 peak; also the same rays priced with the
+                   reference formulas (85 + 42 P operations per ray)
   cpu_baseline     the FP64 oracle (C++ restatement of the reference's
                    fitness.rs) on the host cores, one design per thread
 """
@@ -33,6 +35,10 @@ sys.path[:0] = [str(ROOT / "compound-prism-designer_amd"), str(ROOT)]
 
 METRIC = "designs/sec (batched vs plain GPU) at ≤1e-4 bit MI error; p50/p99 req latency"
 PEAK_FP32_TFLOPS = 157.3
+# HBM bytes per launch of the batched trace kernel on this workload, from
+# rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy's readings
+# (profiles/trace_kernel_batch1024_p3.md)
+TRACE_TRAFFIC_BYTES = 15.43e6
 
 
 def log(msg: str) -> None:
@@ -135,6 +141,7 @@ def uneven_run(cpd, N, np, ctx, args):
     eager_info = np.array([f.info for f in out])
     kernel_ms = float(np.mean(kms))
     achieved = stats["ops"] / (kernel_ms * 1e-3) / 1e12
+    reference_ops = sum(s.n_wavelengths * s.n_rays * (85 + 42 * d.n_glass) for d, s in zip(D, S))
     log(f"uneven: trace kernel {kernel_ms:.2f} ms, {achieved:.2f} TFLOP/s; items {stats['items']}")
 
     # plain path on a prefix
@@ -170,7 +177,7 @@ def uneven_run(cpd, N, np, ctx, args):
             "captured": "HIP graph per generation (upload, trace records, trace, entropy, download)",
             "schedule": best.as_dict(), "autotune_ms": tuned_ms, "autotune_s": tune_s,
             "kernel_ms": kernel_ms, "ops_per_launch": stats["ops"], "achieved_tflops": achieved,
-            "frac": achieved / PEAK_FP32_TFLOPS, "identical_to_eager": bool(np.array_equal(plan_info, eager_info)),
+            "frac": achieved / PEAK_FP32_TFLOPS, "reference_formula_tflops": reference_ops / (kernel_ms * 1e-3) / 1e12, "identical_to_eager": bool(np.array_equal(plan_info, eager_info)),
             "plain": {"designs_per_s_prefix": n_plain / plain_s, "designs": n_plain,
                       "designs_per_s_batch_mix": plain_value, "batched_over_plain": value / plain_value},
             "mi_error_bits": mi_err}
@@ -271,8 +278,11 @@ def main() -> int:
     ctx.set_option(N.OPT_PROFILE, 0)
     trace_mean = float(np.mean(trace_ms))
     achieved = ops_per_launch / (trace_mean * 1e-3) / 1e12
-    nominal_ops = n * (64 * 16384 * (85 + 42 * 3) + 64 * (12 * 3 + 3 * 32))
-    log(f"trace kernel {trace_mean:.3f} ms, finalize {np.mean(fin_ms):.3f} ms, {achieved:.2f} TFLOP/s")
+    # the same rays priced with the reference formulas (every stage, 85 + 42 P
+    # per ray, only per-design and per-wavelength invariants hoisted)
+    reference_ops = n * 64 * 16384 * (85 + 42 * 3)
+    log(f"trace kernel {trace_mean:.3f} ms, finalize {np.mean(fin_ms):.3f} ms, {achieved:.2f} TFLOP/s "
+        f"({reference_ops / (trace_mean * 1e-3) / 1e12:.2f} in reference-formula operations)")
 
     # plain path: one design per launch, synchronised and copied back
     f1 = N.Fitness()
@@ -361,9 +371,10 @@ def main() -> int:
         "mi_error_bits": mi_err,
         "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "ops_per_launch": ops_per_launch, "nominal_ops_per_launch": nominal_ops,
-                     "kernel_ms": trace_mean},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRACE_TRAFFIC_BYTES,
+                     "ops_per_launch": ops_per_launch, "kernel_ms": trace_mean,
+                     "reference_formula_ops_per_launch": reference_ops,
+                     "reference_formula_tflops": reference_ops / (trace_mean * 1e-3) / 1e12},
         "cpu_baseline": cpu,
         "device": info,
     }
