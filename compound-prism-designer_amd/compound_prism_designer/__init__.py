@@ -26,6 +26,7 @@ from ._native import Design, Fitness, Params, Sampling
 
 __all__ = [
     "autotune",
+    "gpu_fitness_batch",
     "Plan",
     "RayTraceError",
     "CudaError",
@@ -398,26 +399,49 @@ class Spectrometer:
 
     def cpu_fitness(self, *, max_n: int = 16_384, max_m: int = 16_384, ctx: Optional[Context] = None
                     ) -> DesignFitness:
-        """fitness(spec, max_n, max_m) (src/pylib.rs:1008-1026, src/fitness.rs:130-147), evaluated on
-        the GPU: max_n wavelength samples x max_m rays, every sample traced."""
-        return self.fitness(max_n=max_n, max_m=max_m, ctx=ctx)
+        """fitness(spec, max_n, max_m) (src/pylib.rs:1008-1026, src/fitness.rs:130-147) with the
+        reference's SEM early exits: rows of rays traced in FP64 on the GPU, folded exactly as
+        fitness() folds them."""
+        s = Sampling(max_n, max_m, N.MODE_ADAPTIVE, 0)
+        f = Fitness()
+        _check(N.lib.cpd_fitness_gpu(_ctx_handle(ctx), ctypes.byref(self.design()), ctypes.byref(s),
+                                     ctypes.byref(f)))
+        return DesignFitness(f.size, f.info, f.deviation)
+
+    def gpu_fitness(self, seeds, *, max_n: int = 256, nwarp: int = 2, max_eval: int = 16_384,
+                    ctx: Optional[Context] = None) -> Optional[DesignFitness]:
+        """cuda_fitness(spec, seeds, max_n, nwarp, max_eval) (src/pylib.rs:1087-1135,
+        src/cuda_fitness.rs:347-423): None when the bin errors stay above 5e-3 after every seed."""
+        sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
+        f = Fitness()
+        st = N.lib.cpd_gpu_fitness(_ctx_handle(ctx), ctypes.byref(self.design()),
+                                   sd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), sd.size, int(max_n),
+                                   int(nwarp), int(max_eval), ctypes.byref(f))
+        if st == N.ERR_NOT_CONVERGED:
+            return None
+        _check(st)
+        return DesignFitness(f.size, f.info, f.deviation)
 
     def fitness(self, *, max_n: int = 64, max_m: int = 16_384, ctx: Optional[Context] = None) -> DesignFitness:
+        """Fixed-count fitness: every one of max_n wavelength samples x max_m rays traced (the
+        benchmark mode; the reference's estimator with both SEM thresholds at 0)."""
         s = Sampling(max_n, max_m, N.MODE_FIXED, 0)
         f = Fitness()
         _check(N.lib.cpd_fitness_gpu(_ctx_handle(ctx), ctypes.byref(self.design()), ctypes.byref(s),
                                      ctypes.byref(f)))
         return DesignFitness(f.size, f.info, f.deviation)
 
-    def transmission_probability(self, wavelengths, *, max_m: int = 16_384, ctx: Optional[Context] = None
-                                 ) -> np.ndarray:
-        """p(d|l) rows for the given wavelengths: src/pylib.rs:1028-1058."""
+    def transmission_probability(self, wavelengths, *, max_m: int = 16_384, adaptive: bool = True,
+                                 ctx: Optional[Context] = None) -> np.ndarray:
+        """p(d|l) rows for the given wavelengths: src/pylib.rs:1028-1058 (with the reference's SEM
+        early exit per row; adaptive=False traces all max_m rays)."""
         w = np.ascontiguousarray(wavelengths, dtype=np.float64).ravel()
         out = np.empty((w.size, self.detector_array.bin_count), np.float64)
         if w.size:
             _check(N.lib.cpd_p_dets_l_wavelength(
                 _ctx_handle(ctx), ctypes.byref(self.design()), w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                w.size, max_m, N.MODE_FIXED, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+                w.size, max_m, N.MODE_ADAPTIVE if adaptive else N.MODE_FIXED,
+                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
     def ray_trace(self, wavelength: float, inital_y: float) -> np.ndarray:
@@ -466,6 +490,22 @@ def fitness_batch(designs: Sequence[Design], sampling, *, ctx: Optional[Context]
     st = N.lib.cpd_fitness_batch(_ctx_handle(ctx), darr, sarr, n, out, status)
     if st != N.OK and st != N.ERR_INVALID_ARGUMENT:
         _check(st)
+    res = np.array([[f.size, f.info, f.deviation] for f in out], np.float64).reshape(n, 3)
+    return res, np.array(status[:], np.int32)
+
+
+def gpu_fitness_batch(designs: Sequence[Design], seeds, *, max_n: int = 256, nwarp: int = 2,
+                      max_eval: int = 16_384, ctx: Optional[Context] = None):
+    """The reference's GPU estimator (Spectrometer.gpu_fitness) for many designs in shared launches.
+    Returns (fitness float64 [n, 3], status int32 [n]); status ERR_NOT_CONVERGED where gpu_fitness
+    returns None."""
+    n = len(designs)
+    darr = (Design * n)(*designs)
+    sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
+    out = (Fitness * n)()
+    status = (ctypes.c_int * n)()
+    _check(N.lib.cpd_gpu_fitness_batch(_ctx_handle(ctx), darr, n, sd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       sd.size, int(max_n), int(nwarp), int(max_eval), out, status))
     res = np.array([[f.size, f.info, f.deviation] for f in out], np.float64).reshape(n, 3)
     return res, np.array(status[:], np.int32)
 

@@ -183,6 +183,8 @@ def _load() -> ctypes.CDLL:
         "cpd_p_dets_l_wavelength": ([vp, P(Design), P(d), sz, u32, u32, P(d)], st),
         "cpd_propagate_rays": ([vp, P(Design), P(ctypes.c_float), P(ctypes.c_float), sz, P(u32),
                                 P(ctypes.c_float)], st),
+        "cpd_gpu_fitness": ([vp, P(Design), P(d), sz, u32, u32, u32, P(Fitness)], st),
+        "cpd_gpu_fitness_batch": ([vp, P(Design), sz, P(d), sz, u32, u32, u32, P(Fitness), P(st)], st),
         "cpd_autotune": ([vp, P(Design), P(Sampling), sz, P(Schedule), P(d)], st),
         "cpd_ctx_set_schedule": ([vp, P(Schedule)], st),
         "cpd_plan_create": ([vp, P(Design), P(Sampling), sz, P(Schedule), P(vp)], st),

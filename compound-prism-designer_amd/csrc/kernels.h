@@ -63,7 +63,6 @@ struct FinalizeArgs {
 
 size_t trace_lds_bytes(uint32_t hist_words);
 
-hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64);
 // start/stop (may be null) are recorded immediately around the kernel; with
 // neither, the launch is a plain one that stream capture records.
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start = nullptr,

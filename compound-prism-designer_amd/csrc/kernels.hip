@@ -94,7 +94,7 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
     const uint32_t lane = threadIdx.x;
     if (lane < count) {
         const uint32_t k = c.queue[(head + lane) & (kQueueCap - 1)];
-        const double y = c.g->y_mean - c.g->beam_width * c.z64[k - 1];
+        const double y = beam_y64(*c.g, c.z64[k - 1]);
         double t;
         const uint32_t b = trace64(*c.g, c.lc.lambda, y, t);
         if (b < c.nbin) atomicAdd(&c.hist[b * c.cols + (lane & (c.cols - 1))], quant64(t));
@@ -266,15 +266,6 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
     }
 }
 
-__global__ void beam_table_kernel(uint32_t n, float *z32, double *z64) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const double z = norminv64(qmc_u64((uint64_t)i + 1));
-        z64[i] = z;
-        z32[i] = (float)z;
-    }
-}
-
 __global__ void zero_u32_kernel(unsigned int *p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -295,7 +286,7 @@ __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, con
     }
     if (!lc.ok || r.guard) {
         double t64;
-        const uint32_t b = trace64(*g, lambda, g->y_mean - g->beam_width * z, t64);
+        const uint32_t b = trace64(*g, lambda, beam_y64(*g, z), t64);
         r.bin = b;
         r.t = (float)t64;
     }
@@ -324,12 +315,6 @@ hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const c
 
 size_t trace_lds_bytes(uint32_t hist_words) {
     return (size_t)hist_words * sizeof(uint32_t) + kQueueCap * sizeof(uint32_t);
-}
-
-hipError_t launch_beam_table(hipStream_t s, uint32_t n, float *z32, double *z64) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(beam_table_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, z32, z64);
-    return hipGetLastError();
 }
 
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start,

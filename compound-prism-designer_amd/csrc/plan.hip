@@ -80,6 +80,8 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     std::vector<uint32_t> all(n);
     for (uint32_t i = 0; i < n; ++i) {
         cpd_status st = check_design_public(shape[i], sampling[i]);
+        if (st == CPD_OK && sampling[i].mode != CPD_MODE_FIXED)
+            st = fail(CPD_ERR_INVALID_ARGUMENT, "plans run fixed-count sampling");
         if (st != CPD_OK) {
             const std::string why = cpd_last_error();
             return fail(st, "plan shape: design %u is not valid: %s", i, why.c_str());
@@ -147,7 +149,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
                             stream));
     PLAN_TRY(hipMemcpyAsync(fx + off_acc, acc_begin.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
     PLAN_TRY(hipMemcpyAsync(fx + off_lb, lb_begin.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
-    PLAN_TRY(launch_beam_table(stream, max_rays, d_z32.as<float>(), d_z64.as<double>()));
+    if ((st = upload_beam_table(stream, kHalfFx, max_rays, d_z32.as<float>(), d_z64.as<double>())) != CPD_OK) return st;
     PLAN_TRY(hipStreamSynchronize(stream));  // host vectors above go out of scope
 
     BatchArgs a{};

@@ -49,6 +49,8 @@ struct Context {
     DeviceBuffer z32, z64;
     // batch state
     DeviceBuffer stage, acc, last_bin, results, counters, row_buf, prop_buf;
+    // adaptive estimators: row descriptors, design records, seeded beam tables
+    DeviceBuffer row_stage, design_stage, seed_z32, seed_z64;
     HostBuffer host_stage, host_out;
 
     static cpd_status create(int device, Context **out);
@@ -60,6 +62,20 @@ struct Context {
                    unsigned long long *ops_out);
     cpd_status propagate(const cpd_design &g, const float *wcdf, const float *rcdf, size_t n, uint32_t *bin,
                          float *t);
+    // adaptive estimators (adaptive.hip)
+    cpd_status run_adaptive(const cpd_design *designs, const cpd_sampling *sampling,
+                            const std::vector<uint32_t> &valid, cpd_fitness *out);
+    cpd_status rows_adaptive(const cpd_design &g, const double *wavelengths, size_t n_w, uint32_t max_m,
+                             double *out);
+    cpd_status run_gpu_fitness(const cpd_design *designs, const std::vector<uint32_t> &valid, const double *seeds,
+                               size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval, cpd_fitness *out,
+                               cpd_status *per_design);
+    cpd_status run_rows(bool chunked, const cpd_design *designs_dev, const std::vector<uint32_t> &row_design,
+                        const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
+                        const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays, uint32_t nbin_max,
+                        const double *z64_dev, std::vector<double> &rows_host, std::vector<uint64_t> &rays_host);
+    cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
+                             const cpd_design **designs_dev, uint32_t *nbin_max);
     void apply(const cpd_schedule &s);
     cpd_schedule schedule() const;
     cpd_status autotune(const cpd_design *designs, const cpd_sampling *sampling, size_t n,
@@ -68,6 +84,7 @@ struct Context {
 
 cpd_status default_context(Context **out);
 cpd_schedule default_schedule();
+cpd_status upload_beam_table(hipStream_t s, uint64_t seed_fx, uint32_t n, float *d_z32, double *d_z64);
 cpd_status check_schedule(const cpd_schedule &s);
 // Relative cost of one work item of a design (traced ray-stages per item).
 double item_cost(const cpd_design &g, const cpd_sampling &s, uint32_t tile_cap);

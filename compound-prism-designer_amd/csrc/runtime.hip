@@ -61,6 +61,24 @@ void HostBuffer::release() {
 }
 
 // Fill the beam-quantile tables for rays k = 1..n (grow-only).
+// z_k = norminv(u_k) for k = 1..n of the R1 sequence from seed_fx, computed
+// on the host in FP64 (the same code and libm log as every host FP64 path, so
+// the device's beam heights are bit-identical to the host's) and copied to
+// the device; the copy has completed when this returns.
+cpd_status upload_beam_table(hipStream_t s, uint64_t seed_fx, uint32_t n, float *d_z32, double *d_z64) {
+    if (n == 0) return CPD_OK;
+    std::vector<double> z(n);
+    std::vector<float> zf(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        z[i] = norminv64(qmc_seeded(seed_fx, (uint64_t)i + 1));
+        zf[i] = (float)z[i];
+    }
+    HIP_TRY(hipMemcpyAsync(d_z64, z.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_z32, zf.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return CPD_OK;
+}
+
 cpd_status Context::ensure_beam_table(uint32_t n) {
     if (n <= z_n) return CPD_OK;
     uint32_t want = std::max<uint32_t>(n, 65536);
@@ -68,7 +86,7 @@ cpd_status Context::ensure_beam_table(uint32_t n) {
     if (st != CPD_OK) return st;
     st = z64.reserve((size_t)want * sizeof(double), false, stream);
     if (st != CPD_OK) return st;
-    HIP_TRY(launch_beam_table(stream, want, z32.as<float>(), z64.as<double>()));
+    if ((st = upload_beam_table(stream, kHalfFx, want, z32.as<float>(), z64.as<double>())) != CPD_OK) return st;
     z_n = want;
     return CPD_OK;
 }
@@ -82,7 +100,7 @@ cpd_status check_design(const cpd_design &g, const cpd_sampling &s) {
     if (g.bin_count < 1 || g.bin_count > CPD_MAX_BINS) return fail(CPD_ERR_INVALID_ARGUMENT, "bin_count out of range");
     if (s.n_wavelengths < 1 || s.n_rays < 1) return fail(CPD_ERR_INVALID_ARGUMENT, "empty sampling");
     if (s.n_rays > (1u << 30)) return fail(CPD_ERR_INVALID_ARGUMENT, "n_rays above 2^30");
-    if (s.mode != CPD_MODE_FIXED) return fail(CPD_ERR_INVALID_ARGUMENT, "only CPD_MODE_FIXED is implemented on the GPU");
+    if (s.mode != CPD_MODE_FIXED && s.mode != CPD_MODE_ADAPTIVE) return fail(CPD_ERR_INVALID_ARGUMENT, "unknown sampling mode");
     if (!(g.det_slope > 0.0)) return fail(CPD_ERR_INVALID_ARGUMENT, "detector slope must be > 0");
     return CPD_OK;
 }
@@ -346,6 +364,10 @@ Context::~Context() {
     counters.release();
     row_buf.release();
     prop_buf.release();
+    row_stage.release();
+    design_stage.release();
+    seed_z32.release();
+    seed_z64.release();
     host_stage.release();
     host_out.release();
     for (hipEvent_t e : ev)
@@ -499,6 +521,7 @@ cpd_status cpd_fitness_gpu(cpd_ctx *ctx, const cpd_design *design, const cpd_sam
     if ((st = resolve(ctx, &c)) != CPD_OK) return st;
     std::lock_guard<std::mutex> lk(c->mu);
     std::vector<uint32_t> valid{0};
+    if (sampling->mode == CPD_MODE_ADAPTIVE) return c->run_adaptive(design, sampling, valid, out);
     return c->run(design, sampling, valid, false, nullptr, out, nullptr, nullptr);
 }
 
@@ -522,10 +545,57 @@ cpd_status cpd_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, const cpd_
     cpd_status st = resolve(ctx, &c);
     if (st != CPD_OK) return st;
     std::lock_guard<std::mutex> lk(c->mu);
-    st = c->run(designs, sampling, valid, true, nullptr, out, nullptr, nullptr);
+    std::vector<uint32_t> fixed, adaptive;
+    for (uint32_t i : valid) (sampling[i].mode == CPD_MODE_ADAPTIVE ? adaptive : fixed).push_back(i);
+    st = c->run(designs, sampling, fixed, true, nullptr, out, nullptr, nullptr);
+    if (st != CPD_OK) {
+        if (per_design)
+            for (uint32_t i : valid) per_design[i] = st;
+        return st;
+    }
+    st = c->run_adaptive(designs, sampling, adaptive, out);
+    if (st != CPD_OK && per_design)
+        for (uint32_t i : adaptive) per_design[i] = st;
+    return st;
+}
+
+cpd_status cpd_gpu_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, size_t n, const double *seeds,
+                                 size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval,
+                                 cpd_fitness *out, cpd_status *per_design) {
+    if (n == 0) return CPD_OK;
+    if (!designs || !out || (n_seeds && !seeds)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nwarp == 0) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "nwarp must be >= 1");
+    if (n > (1u << 20)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "batch above 2^20 designs");
+    if ((uint64_t)max_n * n > (1ull << 28)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "max_n x designs above 2^28 rows");
+    const cpd_sampling probe{1, 1, CPD_MODE_ADAPTIVE, 0};
+    std::vector<uint32_t> valid;
+    for (size_t i = 0; i < n; ++i) {
+        cpd_status s = cpd::check_design_public(designs[i], probe);
+        if (per_design) per_design[i] = s;
+        if (s == CPD_OK)
+            valid.push_back((uint32_t)i);
+        else
+            out[i].size = out[i].info = out[i].deviation = NAN;
+    }
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    st = c->run_gpu_fitness(designs, valid, seeds, n_seeds, max_n, nwarp, max_eval, out, per_design);
     if (st != CPD_OK && per_design)
         for (uint32_t i : valid) per_design[i] = st;
     return st;
+}
+
+cpd_status cpd_gpu_fitness(cpd_ctx *ctx, const cpd_design *design, const double *seeds, size_t n_seeds,
+                           uint32_t max_n, uint32_t nwarp, uint32_t max_eval, cpd_fitness *out) {
+    if (!design || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    cpd_status per = CPD_OK;
+    cpd_status st = cpd_gpu_fitness_batch(ctx, design, 1, seeds, n_seeds, max_n, nwarp, max_eval, out, &per);
+    if (st != CPD_OK) return st;
+    if (per == CPD_ERR_NOT_CONVERGED)
+        return cpd::fail(CPD_ERR_NOT_CONVERGED, "bin errors above 5e-3 after %zu seeds", n_seeds);
+    return per;
 }
 
 cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const double *wavelengths,
@@ -540,6 +610,7 @@ cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const
     Context *c;
     if ((st = resolve(ctx, &c)) != CPD_OK) return st;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (mode == CPD_MODE_ADAPTIVE) return c->rows_adaptive(*design, wavelengths, n_wavelengths, max_m, out);
     std::vector<uint32_t> valid{0};
     return c->run(design, &s, valid, true, wavelengths, nullptr, out, nullptr);
 }

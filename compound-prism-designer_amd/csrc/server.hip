@@ -210,7 +210,7 @@ cpd_status Server::ensure_tables(uint32_t n) {
     cpd_status st;
     if ((st = d_z32.reserve((size_t)want * sizeof(float), false, stream)) != CPD_OK) return st;
     if ((st = d_z64.reserve((size_t)want * sizeof(double), false, stream)) != CPD_OK) return st;
-    SRV_TRY(launch_beam_table(stream, want, d_z32.as<float>(), d_z64.as<double>()));
+    if ((st = upload_beam_table(stream, kHalfFx, want, d_z32.as<float>(), d_z64.as<double>())) != CPD_OK) return st;
     z_n = want;
     return CPD_OK;
 }
@@ -466,6 +466,10 @@ cpd_status cpd_server_submit(cpd_server *srv, const cpd_design *designs, const c
     r->out.assign(n, cpd_fitness{NAN, NAN, NAN});
     for (size_t i = 0; i < n; ++i) {
         const cpd_status ds = cpd::check_design_public(designs[i], sampling[i]);
+        if (ds == CPD_OK && sampling[i].mode != CPD_MODE_FIXED) {
+            r->status[i] = CPD_ERR_INVALID_ARGUMENT;  // the server runs fixed-count sampling
+            continue;
+        }
         r->status[i] = ds;
         if (ds != CPD_OK) continue;
         r->index.push_back((uint32_t)i);

@@ -43,6 +43,14 @@ static constexpr uint64_t kAlphaFx = 0x9E3779B97F4A7C16ull;  // round(2^64 / phi
 static constexpr uint64_t kHalfFx = 0x8000000000000000ull;
 
 CPD_HD double qmc_u64(uint64_t k) { return (double)((kHalfFx + k * kAlphaFx) >> 11) * 0x1.0p-53; }
+// The same sequence from any seed (a fraction in 64-bit fixed point).
+CPD_HD double qmc_seeded(uint64_t seed_fx, uint64_t k) { return (double)((seed_fx + k * kAlphaFx) >> 11) * 0x1.0p-53; }
+// A seed in [0, 1) (any real: its fractional part) as a 64-bit fraction.
+CPD_HD uint64_t seed_to_fx(double seed) {
+    double f = seed - floor(seed);
+    if (!(f >= 0.0 && f < 1.0)) f = 0.0;
+    return (uint64_t)(f * 0x1.0p+64);  // f <= 1 - 2^-53, so the product is below 2^64
+}
 
 // Standard normal quantile, Shaw-Luu-Brickman rational in v = -ln(2 min(u, 1-u))
 // (arXiv:0901.0638), as src/spectrometer/erf.rs:27-73 uses it.
@@ -62,6 +70,13 @@ CPD_HD double norminv64(double x) {
         q = fma(q, v, Q[i]);
     }
     return copysign(v * p / q, x - 0.5);
+}
+
+// Beam height of a ray from its quantile z: y = y_mean - width z
+// (spectrometer.rs:26-30), a separate multiply and subtract.
+CPD_HD double beam_y64(const cpd_design &g, double z) {
+#pragma clang fp contract(off)
+    return g.y_mean - g.beam_width * z;
 }
 
 // ---------------------------------------------------------------------------

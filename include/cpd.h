@@ -247,8 +247,11 @@ cpd_status cpd_ctx_last_stats(cpd_ctx *ctx, cpd_run_stats *out);
 
 /* Plain one-design launch: the design is copied to the device, traced by one
  * wavefront per wavelength sample, reduced to DesignFitness on the device and
- * copied back before return.  Replaces fitness(), src/fitness.rs:130-147, with
- * the per-design launch structure of src/cuda_fitness.rs:199-245. */
+ * copied back before return.  With CPD_MODE_ADAPTIVE the estimator keeps the
+ * reference's early exits: rows of wavelength samples are traced in FP64 on
+ * the device and folded on the host, exactly as fitness() folds them.
+ * Replaces fitness(), src/fitness.rs:130-147, with the per-design launch
+ * structure of src/cuda_fitness.rs:199-245. */
 cpd_status cpd_fitness_gpu(cpd_ctx *ctx, const cpd_design *design, const cpd_sampling *sampling,
                            cpd_fitness *out);
 
@@ -260,11 +263,31 @@ cpd_status cpd_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, const cpd_
                              size_t n, cpd_fitness *out, cpd_status *per_design);
 
 /* Per-bin transmission probabilities p(d|l) for the given wavelengths with
- * max_m rays each; out is row-major [n_wavelengths][bin_count].  Replaces
+ * max_m rays each; out is row-major [n_wavelengths][bin_count].  With
+ * CPD_MODE_ADAPTIVE a row stops at the reference's SEM early exit, as
+ * Spectrometer.transmission_probability does.  Replaces
  * p_dets_l_wavelength, src/fitness.rs:48-69, as exposed by
  * Spectrometer.transmission_probability, src/pylib.rs:1028-1058. */
 cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const double *wavelengths,
                                    size_t n_wavelengths, uint32_t max_m, uint32_t mode, double *out);
+
+/* The reference's GPU estimator: per seed, max_n / nwarp * nwarp wavelength
+ * rows with u = frac(seed + id / phi), rays u = frac(seed + k / phi) for
+ * k = lane + 32 (i + 1), 32-ray chunks folded per distinct bin with the SEM
+ * <= 5e-3 early exit per row; seeds are added until every bin's mean over
+ * the rows has an SEM <= 5e-3.  Returns CPD_ERR_NOT_CONVERGED (out holds the
+ * estimate after the last seed) where the reference returns None.  Rays are
+ * traced in FP64 and folded in FP64 (the reference folds in FP32 with
+ * fast-math).  Replaces cuda_fitness, src/cuda_fitness.rs:347-423, with the
+ * kernel src/spectrometer/kernel.rs:89-193, as exposed by
+ * Spectrometer.gpu_fitness, src/pylib.rs:1087-1135. */
+cpd_status cpd_gpu_fitness(cpd_ctx *ctx, const cpd_design *design, const double *seeds, size_t n_seeds,
+                           uint32_t max_n, uint32_t nwarp, uint32_t max_eval, cpd_fitness *out);
+/* The same for n designs in shared launches; per_design (may be NULL)
+ * receives CPD_OK, CPD_ERR_NOT_CONVERGED or a validation error. */
+cpd_status cpd_gpu_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, size_t n, const double *seeds,
+                                 size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval,
+                                 cpd_fitness *out, cpd_status *per_design);
 
 /* One ray per (wavelength cdf, ray cdf) pair through the FP32 device trace:
  * bin index (bin_count on a miss) and transmittance (0 on a miss).  Replaces

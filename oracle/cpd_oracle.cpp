@@ -21,6 +21,8 @@
 //   src/spectrometer/vector.rs           2D vector ops
 //   src/spectrometer/utils.rs            euclid_div_rem, plog2p
 //   src/spectrometer/kernel.rs:195-224   propagation_test_kernel (CPU emulation)
+//   src/spectrometer/kernel.rs:89-193    the GPU estimator's per-warp fold (FP64)
+//   src/cuda_fitness.rs:347-423          cuda_fitness's loop over seeds (FP64)
 //
 // Floating-point fidelity: compiled with -ffp-contract=off, so every mul and
 // add rounds separately, and std::fma is used exactly where the reference
@@ -126,6 +128,15 @@ struct Welford {
     }
     // welford.rs:55-58
     bool sem_le_error_threshold(F error_squared) const { return m2 < error_squared * (count * (count - F(1))); }
+    // welford.rs:76-85 Add
+    static Welford add(const Welford &a, const Welford &b) {
+        Welford r;
+        r.count = a.count + b.count;
+        F delta = b.mean - a.mean;
+        r.mean = (a.count * a.mean + b.count * b.mean) / r.count;
+        r.m2 = a.m2 + b.m2 + delta * delta * a.count * b.count / r.count;
+        return r;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -676,6 +687,81 @@ static F mutual_information(const Spec<F> &s, uint64_t max_n, uint64_t max_m, bo
     return h_det_l_w.mean - h_det;
 }
 
+// ---------------------------------------------------------------------------
+// src/spectrometer/kernel.rs:89-193 (one warp = one wavelength row) and
+// src/cuda_fitness.rs:347-423 (the host loop over seeds), in FP64.  The
+// reference folds in F = f32 with fast-math; this restatement folds in FP64,
+// with each 32-lane warp sum taken in the butterfly order 16, 8, 4, 2, 1.
+// ---------------------------------------------------------------------------
+static inline uint64_t seed_fx_of(double seed) {
+    double f = seed - std::floor(seed);
+    if (!(f >= 0.0 && f < 1.0)) f = 0.0;
+    return (uint64_t)(f * 0x1.0p+64);
+}
+
+static inline double butterfly_sum32(const double *v) {
+    double a[32], b[32];
+    for (int l = 0; l < 32; ++l) a[l] = v[l];
+    for (int off = 16; off > 0; off >>= 1) {
+        for (int l = 0; l < 32; ++l) b[l] = a[l] + a[l ^ off];
+        for (int l = 0; l < 32; ++l) a[l] = b[l];
+    }
+    return a[0];
+}
+
+// kernel.rs:89-193 for one wavelength: returns the rays folded (count).
+static uint64_t gpu_row(const Spec<double> &s, double wavelength, uint64_t seed_fx, uint32_t max_eval,
+                        std::vector<Welford<double>> &stats) {
+    const uint32_t nbin = s.det.bin_count;
+    stats.assign(nbin, Welford<double>());
+    const double e2 = 5e-3 * 5e-3;
+    const uint32_t max_evals = max_eval - max_eval % 32;
+    double count = 0.0;
+    for (uint32_t i = 0; 32 * i < max_evals; ++i) {
+        count += 32.0;
+        uint32_t bin[32];
+        double t[32];
+        bool pending[32];
+        bool any = false;
+        for (uint32_t l = 0; l < 32; ++l) {
+            const uint64_t k = l + 32 * (uint64_t)(i + 1);
+            const double u = fx_to_f64(seed_fx + k * ALPHA_FX);  // frac(seed + k alpha)
+            bin[l] = nbin;
+            t[l] = 0.0;
+            uint32_t b;
+            double tt;
+            if (spec_propagate(s, ray_from_start(beam_y(s, u)), wavelength, b, tt) == OK) {
+                bin[l] = b;
+                t[l] = tt;
+            }
+            pending[l] = bin[l] < nbin;
+            any = any || pending[l];
+        }
+        bool finished = any;
+        for (;;) {
+            uint32_t bmin = 0xffffffffu;
+            for (uint32_t l = 0; l < 32; ++l)
+                if (pending[l] && bin[l] < bmin) bmin = bin[l];
+            if (bmin == 0xffffffffu) break;
+            double tb[32], dev[32];
+            for (uint32_t l = 0; l < 32; ++l) tb[l] = bin[l] == bmin ? t[l] : 0.0;
+            const double mean = butterfly_sum32(tb) / 32.0;
+            for (uint32_t l = 0; l < 32; ++l) dev[l] = (tb[l] - mean) * (tb[l] - mean);
+            Welford<double> w;
+            w.count = 32.0;
+            w.mean = mean;
+            w.m2 = butterfly_sum32(dev);
+            stats[bmin] = Welford<double>::add(stats[bmin], w);
+            finished = finished && stats[bmin].sem_le_error_threshold(e2);
+            for (uint32_t l = 0; l < 32; ++l)
+                if (bin[l] == bmin) pending[l] = false;
+        }
+        if (finished) break;
+    }
+    for (auto &w : stats) w.skip(count);
+    return (uint64_t)count;
+}
+
 }  // namespace ora
 
 using namespace ora;
@@ -991,6 +1077,64 @@ uint64_t ora_fitness_many(const cpd_design *designs, const uint32_t *max_n, cons
 }  // extern "C"
 
 extern "C" {
+// One row of the GPU estimator (kernel.rs:89-193): p(d|wavelength) into
+// out[bin_count]; returns the rays folded.
+uint64_t ora_gpu_row(const cpd_design *d, double wavelength, double seed, uint32_t max_eval, double *out) {
+    Spec<double> s{};
+    spec_from_design(*d, s);
+    std::vector<Welford<double>> row;
+    const uint64_t rays = gpu_row(s, wavelength, seed_fx_of(seed), max_eval, row);
+    for (uint32_t b = 0; b < s.det.bin_count; ++b) out[b] = row[b].mean;
+    return rays;
+}
+
+// cuda_fitness (cuda_fitness.rs:347-423) over the given seeds: out[3];
+// *converged = 0 where the reference returns None (out then holds the
+// estimate after the last seed); counts[2] = {rows, rays}.
+int ora_gpu_fitness(const cpd_design *d, const double *seeds, size_t n_seeds, uint32_t max_n, uint32_t nwarp,
+                    uint32_t max_eval, double *out, int *converged, uint64_t *counts) {
+    if (!d || !out || nwarp == 0) return CPD_ERR_INVALID_ARGUMENT;
+    Spec<double> s{};
+    spec_from_design(*d, s);
+    const uint32_t nbin = s.det.bin_count;
+    const uint32_t n_rows = (max_n / nwarp) * nwarp;
+    const double e2 = 5e-3 * 5e-3;
+    std::vector<Welford<double>> p_dets(nbin), row;
+    Welford<double> h_det_l_w;
+    uint64_t rows = 0, rays = 0;
+    int conv = 0;
+    for (size_t si = 0; si < n_seeds && !conv; ++si) {
+        const uint64_t seed_fx = seed_fx_of(seeds[si]);
+        for (uint32_t id = 0; id < n_rows; ++id) {
+            const double u = fx_to_f64(seed_fx + (uint64_t)id * ALPHA_FX);
+            rays += gpu_row(s, sample_wavelength(s, u), seed_fx, max_eval, row);
+            double h = 0.0;
+            for (uint32_t b = 0; b < nbin; ++b) {
+                p_dets[b].next_sample(row[b].mean);
+                h += plog2p(row[b].mean);
+            }
+            h_det_l_w.next_sample(h);
+            ++rows;
+        }
+        bool all = rows > 0;
+        for (uint32_t b = 0; b < nbin; ++b) all = all && p_dets[b].sem_le_error_threshold(e2);
+        conv = all ? 1 : 0;
+    }
+    double h_det = 0.0;
+    for (uint32_t b = 0; b < nbin; ++b) h_det = h_det + plog2p(p_dets[b].mean);
+    double size, dev;
+    size_and_deviation(s, size, dev);
+    out[0] = size;
+    out[1] = h_det_l_w.mean - h_det;
+    out[2] = dev;
+    if (converged) *converged = conv;
+    if (counts) {
+        counts[0] = rows;
+        counts[1] = rays;
+    }
+    return CPD_OK;
+}
+
 // Layout checks for the ctypes mirrors of cpd.h.
 size_t ora_sizeof_params(void) { return sizeof(cpd_params); }
 size_t ora_sizeof_design(void) { return sizeof(cpd_design); }

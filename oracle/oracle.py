@@ -28,6 +28,12 @@ path (src/fitness.rs and src/spectrometer/*, 2D variant).  What pins each part
   detector fit (geometry.rs:495-542)  both end rays land on the array ends
   early-exit stop points              parity unpinned (only the KAT pins the
                                       resulting info to 1%)
+  GPU estimator (kernel.rs:89-193,    a row equals the direct mean of t over the
+  cuda_fitness.rs:347-423)            rays it folded (brute force per ray); the
+                                      KAT's info within the bench's 1e-2
+                                      (benches/benchmark.rs:147-157); its stop
+                                      points are parity unpinned (the reference
+                                      folds in FP32 with fast-math)
 """
 from __future__ import annotations
 
@@ -147,6 +153,9 @@ def lib() -> ctypes.CDLL:
     L.ora_fitness_counts.argtypes = [P(Design), u64, u64, i32, i32, i32, P(d), P(u64)]
     L.ora_fitness_many.argtypes = [P(Design), P(u32), P(u32), sz, i32, i32, i32, i32, P(d)]
     L.ora_fitness_many.restype = u64
+    L.ora_gpu_fitness.argtypes = [P(Design), P(d), sz, u32, u32, u32, P(d), P(i32), P(u64)]
+    L.ora_gpu_row.argtypes = [P(Design), d, d, u32, P(d)]
+    L.ora_gpu_row.restype = u64
     L.ora_sizeof_params.restype = sz
     L.ora_sizeof_design.restype = sz
     _lib = L
@@ -288,6 +297,27 @@ def fitness_counts(design: Design, max_n: int, max_m: int, precision: int = 64, 
     cnt = (ctypes.c_uint64 * 2)()
     lib().ora_fitness_counts(ctypes.byref(design), max_n, max_m, precision, int(adaptive), qrng, out, cnt)
     return (out[0], out[1], out[2]), (cnt[0], cnt[1])
+
+
+def gpu_fitness(design: Design, seeds, max_n: int = 256, nwarp: int = 2, max_eval: int = 16384):
+    """The reference's GPU estimator (kernel.rs:89-193 + cuda_fitness.rs:347-423) in FP64:
+    ((size, info, deviation), converged, (rows, rays))."""
+    design = _as(design, Design)
+    sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
+    out = (ctypes.c_double * 3)()
+    conv = ctypes.c_int()
+    cnt = (ctypes.c_uint64 * 2)()
+    lib().ora_gpu_fitness(ctypes.byref(design), _ptr(sd, ctypes.c_double), sd.size, max_n, nwarp, max_eval, out,
+                          ctypes.byref(conv), cnt)
+    return (out[0], out[1], out[2]), bool(conv.value), (cnt[0], cnt[1])
+
+
+def gpu_row(design: Design, wavelength: float, seed: float, max_eval: int):
+    """One wavelength row of the GPU estimator: (p(d|l) [bin_count], rays folded)."""
+    design = _as(design, Design)
+    out = np.empty(design.bin_count, np.float64)
+    rays = lib().ora_gpu_row(ctypes.byref(design), wavelength, seed, max_eval, _ptr(out, ctypes.c_double))
+    return out, rays
 
 
 def fitness_many(designs, max_n, max_m, precision: int = 64, adaptive: bool = False,

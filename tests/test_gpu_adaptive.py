@@ -1,0 +1,98 @@
+"""The reference's adaptive estimators on the GPU against the FP64 oracle.
+
+Both trace every ray in FP64 in the reference's operation order from the
+same beam quantiles and fold in FP64 in the reference's order, so results
+are compared for equality: fitness() with the SEM early exits
+(src/fitness.rs:5-34, 79-123) and the GPU estimator cuda_fitness
+(src/cuda_fitness.rs:347-423 with src/spectrometer/kernel.rs:89-193)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import compound_prism_designer as cpd
+from compound_prism_designer import _native as N
+from oracle import oracle as O
+import kat
+
+from test_gpu_fitness import kat_design, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_known_design_cpu_fitness_matches_oracle_and_known_answer():
+    d = kat_design()
+    s = N.Sampling(16384, 16384, N.MODE_ADAPTIVE, 0)
+    f = N.Fitness()
+    assert N.lib.cpd_fitness_gpu(None, ctypes.byref(d), ctypes.byref(s), ctypes.byref(f)) == 0, N.last_error()
+    (size, info, dev), _ = O.fitness(to_oracle(d), 16384, 16384, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+    assert (f.size, f.info, f.deviation) == (size, info, dev)
+    # src/fitness.rs:250-257
+    assert abs(f.info - kat.KAT_INFO) <= kat.KAT_INFO_RMAX * kat.KAT_INFO
+    assert abs(f.size - kat.KAT_SIZE) <= kat.KAT_SIZE_RMAX * kat.KAT_SIZE
+
+
+def test_adaptive_batch_matches_oracle():
+    _, D, _, _ = cpd.generate_designs(0, 0, 24)
+    samp = N.Sampling(256, 4096, N.MODE_ADAPTIVE, 0)
+    got, st = cpd.fitness_batch(D, samp)
+    assert (st == 0).all()
+    for i, d in enumerate(D):
+        ref, _ = O.fitness(to_oracle(d), 256, 4096, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+        assert tuple(got[i]) == ref, (i, got[i], ref)
+
+
+def test_adaptive_and_fixed_share_a_batch():
+    _, D, _, _ = cpd.generate_designs(0, 100, 8)
+    samp = [N.Sampling(32, 2048, N.MODE_ADAPTIVE if i % 2 else N.MODE_FIXED, 0) for i in range(8)]
+    got, st = cpd.fitness_batch(D, samp)
+    assert (st == 0).all()
+    for i, d in enumerate(D):
+        alone, _ = cpd.fitness_batch([d], [samp[i]])
+        assert np.array_equal(got[i], alone[0])
+
+
+def test_transmission_probability_adaptive_matches_oracle():
+    d = kat_design()
+    w = np.linspace(0.5, 0.82, 9)
+    out = np.empty((9, 32))
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = N.lib.cpd_p_dets_l_wavelength(None, ctypes.byref(d), w.ctypes.data_as(dp), 9, 16384, N.MODE_ADAPTIVE,
+                                       out.ctypes.data_as(dp))
+    assert st == 0, N.last_error()
+    for j in range(9):
+        ref, _ = O.p_dets_l_wavelength(to_oracle(d), float(w[j]), 16384, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+        assert np.array_equal(out[j], ref), j
+
+
+def test_gpu_fitness_matches_oracle_on_known_design():
+    # benches/benchmark.rs:147-157's call: seeds [0.798713], max_n 256, nwarp 2, max_eval 16384
+    d = kat_design()
+    f = N.Fitness()
+    seeds = np.array([0.798713])
+    st = N.lib.cpd_gpu_fitness(None, ctypes.byref(d), seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1,
+                               256, 2, 16384, ctypes.byref(f))
+    assert st == 0, N.last_error()
+    ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 256, 2, 16384)
+    assert converged
+    assert (f.size, f.info, f.deviation) == ref
+
+
+def test_gpu_fitness_not_converged_is_none_like_the_reference():
+    d = kat_design()
+    seeds = np.array([0.798713, 0.25])
+    got, st = cpd.gpu_fitness_batch([d], seeds, max_n=4, nwarp=2, max_eval=16384)
+    ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 4, 2, 16384)
+    assert not converged and st[0] == N.ERR_NOT_CONVERGED
+    assert tuple(got[0]) == ref
+
+
+def test_gpu_fitness_batch_matches_single_designs():
+    _, D, _, _ = cpd.generate_designs(0, 200, 12)
+    D = list(D) + [kat_design()]
+    seeds = np.array([0.1, 0.7])
+    got, st = cpd.gpu_fitness_batch(D, seeds, max_n=64, nwarp=2, max_eval=4096)
+    for i, d in enumerate(D):
+        ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 64, 2, 4096)
+        assert st[i] == (0 if converged else N.ERR_NOT_CONVERGED)
+        assert tuple(got[i]) == ref, (i, got[i], ref)

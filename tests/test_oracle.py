@@ -328,3 +328,43 @@ def test_detector_positioning_spans_the_extreme_wavelengths():
 def test_layout_matches_header():
     assert O.lib().ora_sizeof_params() == ctypes.sizeof(O.Params)
     assert O.lib().ora_sizeof_design() == ctypes.sizeof(O.Design)
+
+
+# ---------------------------------------------------------------------------
+# the reference's GPU estimator (kernel.rs:89-193, cuda_fitness.rs:347-423)
+# ---------------------------------------------------------------------------
+
+def test_gpu_estimator_known_design_within_bench_tolerance():
+    # benches/benchmark.rs:147-157: cuda_fitness(spec, [0.798713], 256, 2, 16384) converges
+    # and agrees with fitness(spec, 16384, 16384) on info to rmax 1e-2
+    _, d = kat_design()
+    (size, info, dev), converged, (rows, rays) = O.gpu_fitness(d, [0.798713], 256, 2, 16384)
+    (size_c, info_c, dev_c), _ = O.fitness(d, 16384, 16384, 64, adaptive=True)
+    assert converged and rows == 256
+    assert abs(info - info_c) <= 1e-2 * abs(info_c)
+    assert (size, dev) == (size_c, dev_c)
+    assert rays < 256 * 16384  # the per-row early exit fired
+    # too few rows: the bin means over the rows stay imprecise and the reference returns None
+    _, converged, (rows, _) = O.gpu_fitness(d, [0.798713], 4, 2, 16384)
+    assert not converged and rows == 4
+
+
+def test_gpu_row_is_the_mean_of_t_over_its_rays():
+    # a row folds 32-ray chunks into per-bin summaries with the parallel Welford
+    # combination and pads with zeros; its means must equal the plain mean of t
+    # per bin over the rays folded (k = 32 .. 32 + rays - 1, u_k = frac(seed + k/phi))
+    _, d = kat_design()
+    alpha = 0x9E3779B97F4A7C16
+    for lam, seed, max_eval in [(0.61, 0.3, 16384), (0.52, 0.798713, 2000), (0.8, 0.05, 64)]:
+        row, rays = O.gpu_row(d, lam, seed, max_eval)
+        assert rays % 32 == 0 and 32 <= rays <= max_eval - max_eval % 32
+        seed_fx = int((seed - math.floor(seed)) * 2.0 ** 64)
+        ks = np.arange(32, 32 + rays, dtype=np.uint64)
+        u = [(((seed_fx + int(k) * alpha) % 2 ** 64) >> 11) * 2.0 ** -53 for k in ks]
+        y = np.array([d.y_mean - d.beam_width * O.norminv(x) for x in u])
+        bins, ts = O.propagate_many(d, np.full(rays, lam), y, precision=64)
+        want = np.zeros(d.bin_count)
+        hit = bins < d.bin_count
+        np.add.at(want, bins[hit], ts[hit])
+        want /= rays
+        np.testing.assert_allclose(row, want, rtol=1e-12, atol=1e-15)
