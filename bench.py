@@ -45,7 +45,7 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed):
+def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed, step_items=0):
     """Poisson arrivals at `rate` req/s for `duration` s; each request holds
     U{100..2000} designs drawn from the pool.  Returns latency percentiles and
     the served throughput (designs completed / time from first arrival to last
@@ -61,7 +61,7 @@ def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed):
     picks = [rng.integers(0, len(pool_d), n) for n in sizes]
     reqs = [((N.Design * len(ix))(*[pool_d[i] for i in ix]), (N.Sampling * len(ix))(*[pool_s[i] for i in ix]))
             for ix in picks]
-    srv = cpd.Server(ctx)
+    srv = cpd.Server(ctx, step_items=step_items)
     ids = []
     t0 = time.perf_counter()
     for at, (darr, sarr), n in zip(arrivals, reqs, sizes):
@@ -183,6 +183,66 @@ def uneven_run(cpd, N, np, ctx, args):
             "mi_error_bits": mi_err}
 
 
+def adaptive_run(cpd, N, np, ctx, args):
+    """The reference's GPU estimator as designer.py:276 calls it (one random seed, max_n 128,
+    nwarp 2, max_eval 8192): a generation of 3-prism designs in shared launches
+    (cpd_gpu_fitness_batch) vs one call per design (cpd_gpu_fitness), and the FP64 oracle of the same
+    estimator on the host cores; then fitness() with the SEM early exits at its pyo3 defaults
+    (16384 x 16384) batched."""
+    n = args.adaptive
+    _, D, _, _ = cpd.generate_designs(1, 0, n)
+    darr = (N.Design * n)(*D)
+    seeds = np.array([0.37])
+    sp = seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+
+    def batch():
+        if N.lib.cpd_gpu_fitness_batch(ctx.handle, darr, n, sp, 1, 128, 2, 8192, out, status) != 0:
+            raise RuntimeError(N.last_error())
+
+    batch()
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        batch()
+    batched = n * reps / (time.perf_counter() - t)
+    rays = ctx.last_stats()["rays"]
+    conv = sum(1 for i in range(n) if status[i] == 0)
+    f1 = N.Fitness()
+    n_plain = min(64, n)
+    t = time.perf_counter()
+    for i in range(n_plain):
+        st = N.lib.cpd_gpu_fitness(ctx.handle, ctypes.byref(darr[i]), sp, 1, 128, 2, 8192, ctypes.byref(f1))
+        if st not in (0, N.ERR_NOT_CONVERGED):
+            raise RuntimeError(N.last_error())
+    plain = n_plain / (time.perf_counter() - t)
+    log(f"adaptive: gpu_fitness batched {batched:.0f} designs/s ({rays / n:.0f} rays per design, {conv}/{n} "
+        f"converged), per design {plain:.0f} designs/s")
+    from oracle import oracle as O
+    threads = len(os.sched_getaffinity(0))
+    from concurrent.futures import ThreadPoolExecutor
+    n_cpu = min(n, 4 * threads)
+    t = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda d: O.gpu_fitness(d, seeds, 128, 2, 8192), D[:n_cpu]))
+    cpu = n_cpu / (time.perf_counter() - t)
+    log(f"adaptive: FP64 oracle of the same estimator {cpu:.1f} designs/s on {threads} threads")
+    samp = N.Sampling(16384, 16384, N.MODE_ADAPTIVE, 0)
+    sarr = (N.Sampling * n)(*([samp] * n))
+    t = time.perf_counter()
+    if N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status) != 0:
+        raise RuntimeError(N.last_error())
+    fit = n / (time.perf_counter() - t)
+    log(f"adaptive: fitness() with SEM early exits (16384 x 16384) batched {fit:.0f} designs/s")
+    return {"workload": f"{n} 3-prism designs; gpu_fitness(seeds=[0.37], max_n=128, nwarp=2, max_eval=8192) "
+                        "as designer.py calls it",
+            "gpu_fitness_batched_designs_per_s": batched, "gpu_fitness_per_design_designs_per_s": plain,
+            "batched_over_per_design": batched / plain, "rays_per_design": rays / n, "converged": conv,
+            "cpu_oracle_designs_per_s": cpu, "cpu_threads": threads,
+            "fitness_sem_batched_designs_per_s": fit}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,7 +253,7 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="designs checked against the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="req/s of the short served phase (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="req/s of the short served phase (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-sweep", action="store_true",
                     help="served runs at 20, 50, 100 and 200 req/s (the served configuration), 2 s of arrivals each")
@@ -203,6 +263,9 @@ def main() -> int:
     ap.add_argument("--uneven-steps", type=int, default=5)
     ap.add_argument("--uneven-plain", type=int, default=32, help="uneven designs timed through the plain path")
     ap.add_argument("--uneven-check", type=int, default=6, help="uneven designs checked against the FP64 oracle")
+    ap.add_argument("--adaptive", type=int, default=1024,
+                    help="designs of the adaptive phase: the reference GPU estimator as the optimiser calls it "
+                         "(gpu_fitness, max_n 128, max_eval 8192) batched vs per design (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -330,6 +393,10 @@ def main() -> int:
     if args.uneven > 0:
         uneven = uneven_run(cpd, N, np, ctx, args)
 
+    adaptive = None
+    if args.adaptive > 0:
+        adaptive = adaptive_run(cpd, N, np, ctx, args)
+
     serve = None
     if args.serve_rate > 0 or args.serve_sweep:
         t = time.time()
@@ -367,6 +434,7 @@ def main() -> int:
                        "batch_p50": float(np.percentile(step_ms, 50)),
                        "batch_p99": float(np.percentile(step_ms, 99))},
         "uneven": uneven,
+        "adaptive": adaptive,
         "served": serve,
         "mi_error_bits": mi_err,
         "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",

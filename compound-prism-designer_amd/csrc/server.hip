@@ -151,7 +151,9 @@ cpd_status Server::init(Context *ctx, const cpd_server_opts *o) {
     waves_per_cu = ctx->waves_per_cu;
     const uint64_t budget = o && o->memory_budget ? o->memory_budget : (4ull << 30);
     max_designs = o && o->max_designs ? o->max_designs : 65536;
-    step_items = o && o->step_items ? o->step_items : (uint32_t)(4 * cu_count * waves_per_cu);
+    // large steps: per-step host work (shares, finalize, copies) is amortised over
+    // ~10 ms of device work, and a step only holds the work that is queued
+    step_items = o && o->step_items ? o->step_items : (uint32_t)(64 * cu_count * waves_per_cu);
     const uint64_t record_bytes = max_designs * (sizeof(cpd_design) + sizeof(TraceDesign) + 2 * sizeof(uint64_t) +
                                                  sizeof(cpd_fitness));
     if (budget <= record_bytes + (1 << 20))

@@ -354,7 +354,7 @@ void cpd_plan_destroy(cpd_plan *plan);
  * item has run; its results are the same as from cpd_fitness_batch. */
 typedef struct cpd_server_opts {
     uint64_t memory_budget;    /* device bytes for admitted requests (0: 4 GiB) */
-    uint32_t step_items;       /* work items per step (0: 4 per resident wavefront) */
+    uint32_t step_items;       /* work items per step, a cap (0: 64 per resident wavefront) */
     uint32_t max_designs;      /* design slots (0: 65536) */
 } cpd_server_opts;
 
