@@ -43,6 +43,7 @@ struct BatchArgs {
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)
     uint32_t n_designs;          // entries
     uint32_t hist_words;         // max over the entries of nbin * hist_cols
+    uint32_t row_words;          // max over the entries of nbin
     uint64_t n_items;
 };
 
@@ -61,7 +62,9 @@ struct FinalizeArgs {
     uint32_t n_designs;
 };
 
-size_t trace_lds_bytes(uint32_t hist_words);
+// LDS of one trace wavefront: u32 columns | flagged-ray queue | u64 row.
+CPD_HD uint32_t trace_row_offset_words(uint32_t hist_words) { return (hist_words + kQueueCap + 1) & ~1u; }
+size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words);
 
 // start/stop (may be null) are recorded immediately around the kernel; with
 // neither, the launch is a plain one that stream capture records.

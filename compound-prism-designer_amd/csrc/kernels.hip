@@ -83,6 +83,7 @@ struct TileCtx {
     LambdaConsts lc;
     uint32_t k_begin, k_end, n_rays, nbin, cols;
     uint32_t *hist;
+    unsigned long long *row;  // the wavefront's u64 sums per bin
     uint32_t *queue;
     uint32_t *last_bin;
     const float *z32;
@@ -102,6 +103,23 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
     }
 }
 
+// Adds each bin's u32 columns (skewed, conflict-free order) to the u64 row and
+// clears them.  Lane b handles bins b, b + 64, ...
+__device__ void fold_columns(uint32_t *hist, unsigned long long *row, uint32_t nbin, uint32_t cols) {
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nbin; b += 64) {
+        uint32_t *cell = hist + b * cols;
+        uint64_t sum = 0;
+        for (uint32_t q = 0; q < cols; ++q) {
+            const uint32_t cc = (q + b) & (cols - 1);
+            sum += cell[cc];
+            cell[cc] = 0;
+        }
+        row[b] += sum;
+    }
+    __syncthreads();
+}
+
 template <bool COUNT>
 __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     const uint32_t lane = threadIdx.x;
@@ -113,6 +131,8 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     uint32_t *__restrict__ hist = c.hist;
     const uint32_t k_end = c.k_end, n_rays = c.n_rays, nbin = c.nbin, cols = c.cols;
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
+    const uint32_t fold = fold_rays(cols);
+    uint32_t since_fold = 0;
     for (uint32_t base = c.k_begin; base < k_end; base += 64) {
         const uint32_t k = base + lane;
         const bool active = k < k_end;
@@ -135,6 +155,11 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
                 head += 64;
             }
         }
+        since_fold += 64;
+        if (since_fold >= fold) {
+            fold_columns(hist, c.row, nbin, cols);
+            since_fold = 0;
+        }
     }
     if (tail != head) {
         __builtin_amdgcn_wave_barrier();
@@ -147,8 +172,10 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
     extern __shared__ uint32_t lds[];
     uint32_t *hist = lds;
     uint32_t *queue = lds + a.hist_words;
+    unsigned long long *row = (unsigned long long *)(lds + trace_row_offset_words(a.hist_words));
     const uint32_t lane = threadIdx.x;
     for (uint32_t i = lane; i < a.hist_words; i += 64) hist[i] = 0;
+    for (uint32_t i = lane; i < a.row_words; i += 64) row[i] = 0;
     __syncthreads();
     unsigned long long ops = 0;
     uint64_t item = blockIdx.x;
@@ -179,6 +206,7 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.nbin = nbin;
         c.cols = cols;
         c.hist = hist;
+        c.row = row;
         c.queue = queue;
         c.last_bin = a.last_bin + a.lb_begin[d] + j;
         c.z32 = a.z32;
@@ -188,15 +216,17 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
             if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
         } else {
             tile_loop<COUNT>(c, ops);
+            // last fold and flush: row + columns into the global sums
             __syncthreads();
             unsigned long long *acc = a.acc + a.acc_begin[d] + (uint64_t)j * nbin;
             for (uint32_t b = lane; b < nbin; b += 64) {
-                uint32_t *row = hist + b * cols;
-                uint64_t sum = 0;
+                uint32_t *cell = hist + b * cols;
+                uint64_t sum = row[b];
+                row[b] = 0;
                 for (uint32_t q = 0; q < cols; ++q) {
                     const uint32_t cc = (q + b) & (cols - 1);
-                    sum += row[cc];
-                    row[cc] = 0;
+                    sum += cell[cc];
+                    cell[cc] = 0;
                 }
                 if (sum) atomicAdd(acc + b, (unsigned long long)sum);
             }
@@ -313,14 +343,14 @@ hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const c
     return hipGetLastError();
 }
 
-size_t trace_lds_bytes(uint32_t hist_words) {
-    return (size_t)hist_words * sizeof(uint32_t) + kQueueCap * sizeof(uint32_t);
+size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words) {
+    return (size_t)trace_row_offset_words(hist_words) * sizeof(uint32_t) + (size_t)row_words * sizeof(uint64_t);
 }
 
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start,
                         hipEvent_t stop) {
     if (grid == 0 || a.n_items == 0) return hipSuccess;
-    const size_t lds = trace_lds_bytes(a.hist_words);
+    const size_t lds = trace_lds_bytes(a.hist_words, a.row_words);
     if (!start && !stop) {
         // plain launch: the form a stream capture records as a kernel node
         if (count_ops)

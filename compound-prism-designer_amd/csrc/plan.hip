@@ -104,7 +104,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     std::vector<cpd_sampling> samp_q(n);
     std::vector<uint64_t> item_begin(n + 1), acc_begin(n), lb_begin(n);
     uint64_t items = 0, acc_n = 0, lb_n = 0;
-    uint32_t hist_words = 0, max_rays = 0;
+    uint32_t hist_words = 0, row_words = 0, max_rays = 0;
     for (uint32_t q = 0; q < n; ++q) {
         const uint32_t i = perm[q];
         TraceDesign t;
@@ -117,6 +117,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
         acc_n += (uint64_t)sampling[i].n_wavelengths * shape[i].bin_count;
         lb_n += sampling[i].n_wavelengths;
         hist_words = std::max(hist_words, t.hist_cols * t.nbin);
+        row_words = std::max(row_words, t.nbin);
         max_rays = std::max(max_rays, sampling[i].n_rays);
     }
     item_begin[n] = items;
@@ -165,6 +166,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     a.work_counter = d_counter.as<unsigned int>();
     a.n_designs = n;
     a.hist_words = hist_words;
+    a.row_words = row_words;
     a.n_items = items;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)ctx->cu_count * sched.waves_per_cu);
     FinalizeArgs f{};

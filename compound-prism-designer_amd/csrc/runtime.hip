@@ -111,7 +111,7 @@ cpd_schedule default_schedule() { return cpd_schedule{0, 16, CPD_ORDER_GIVEN, 0}
 
 cpd_status check_schedule(const cpd_schedule &s) {
     if (s.tile_rays > kMaxTileRays || s.tile_rays % 64)
-        return fail(CPD_ERR_INVALID_ARGUMENT, "tile rays must be a multiple of 64 in [0, 16384]");
+        return fail(CPD_ERR_INVALID_ARGUMENT, "tile rays must be a multiple of 64 in [0, 2^20]");
     if (s.waves_per_cu < 1 || s.waves_per_cu > 64) return fail(CPD_ERR_INVALID_ARGUMENT, "waves per CU must be in 1..64");
     if (s.order != CPD_ORDER_GIVEN && s.order != CPD_ORDER_COST) return fail(CPD_ERR_INVALID_ARGUMENT, "unknown order");
     return CPD_OK;
@@ -153,7 +153,7 @@ cpd_status Context::autotune(const cpd_design *designs, const cpd_sampling *samp
     st = time_of(cur, cur_ms);
     const uint32_t orders[] = {CPD_ORDER_GIVEN};
     const uint32_t waves[] = {4, 8, 16, 24};
-    const uint32_t tiles[] = {8192, 4096, 2048};
+    const uint32_t tiles[] = {16384, 8192, 4096};
     for (uint32_t o : orders) {
         if (st != CPD_OK) break;
         cpd_schedule cand = cur;
@@ -246,13 +246,14 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     uint64_t *hlb = (uint64_t *)(h + off_lb);
     uint64_t *hrow = (uint64_t *)(h + off_rows);
     uint64_t items = 0, acc_n = 0, lb_n = 0;
-    uint32_t hist_words = 0, tile = 0;
+    uint32_t hist_words = 0, row_words = 0, tile = 0;
     for (uint32_t q = 0; q < n; ++q) {
         const cpd_design &g = designs[valid[q]];
         const cpd_sampling &s = sampling[valid[q]];
         hd[q] = g;
         make_trace_design(g, s.n_wavelengths, s.n_rays, ht[q], tile_rays_hint);
         hist_words = std::max(hist_words, ht[q].hist_cols * g.bin_count);
+        row_words = std::max(row_words, g.bin_count);
         tile = std::max(tile, ht[q].tile_rays);
         hitem[q] = items;
         hacc[q] = acc_n;
@@ -295,6 +296,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.wavelengths = n_wl ? (const double *)(dstage + off_wl) : nullptr;
     a.n_designs = n;
     a.hist_words = hist_words;
+    a.row_words = row_words;
     a.n_items = items;
     uint32_t grid;
     if (persistent) {
@@ -461,7 +463,7 @@ cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value) {
         case CPD_OPT_COUNT_OPS: c->count_ops = value != 0; return CPD_OK;
         case CPD_OPT_TILE_RAYS:
             if (value < 0 || value > (int64_t)cpd::kMaxTileRays || value % 64)
-                return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "tile rays must be a multiple of 64 in [0, 16384]");
+                return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "tile rays must be a multiple of 64 in [0, 2^20]");
             c->tile_rays_hint = (uint32_t)value;
             return CPD_OK;
         case CPD_OPT_WAVES_PER_CU:

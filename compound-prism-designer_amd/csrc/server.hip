@@ -294,7 +294,7 @@ cpd_status Server::step() {
     std::vector<Entry> entries;
     std::vector<uint32_t> finishing;  // slots to finalize
     std::vector<size_t> done_req;
-    uint32_t hist_words = 1;
+    uint32_t hist_words = 1, row_words = 1;
     for (size_t i = 0; i < A; ++i) {
         Request &r = *act[i];
         uint64_t left = take[i];
@@ -303,6 +303,7 @@ cpd_status Server::step() {
             const uint64_t t = std::min(avail, left);
             entries.push_back(Entry{(uint32_t)(r.slot0 + r.cursor), r.cursor_item, t});
             hist_words = std::max(hist_words, r.traces[r.cursor].hist_cols * r.traces[r.cursor].nbin);
+            row_words = std::max(row_words, r.traces[r.cursor].nbin);
             left -= t;
             r.cursor_item += t;
             if (r.cursor_item == r.items[r.cursor]) {
@@ -357,6 +358,7 @@ cpd_status Server::step() {
     a.work_counter = d_counter.as<unsigned int>();
     a.n_designs = (uint32_t)E;
     a.hist_words = hist_words;
+    a.row_words = row_words;
     a.n_items = items;
     SRV_TRY(launch_zero_u32(stream, a.work_counter, 1));
     const uint32_t grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)cu_count * waves_per_cu);

@@ -242,28 +242,31 @@ CPD_HD uint32_t trace64(const cpd_design &g, double wavelength, double y, double
 // l adds into column l % cols), at most kHistWords words for bin counts up to
 // 2048, so the histogram stays near 8 KB and several wavefronts fit a CU.
 static constexpr uint32_t kHistWords = 2048;
-static constexpr uint32_t kMaxTileRays = 16384;
+// Rays per work item: at most kMaxTileRays; kDefaultTileRays unless a
+// schedule caps it.
+static constexpr uint32_t kMaxTileRays = 1u << 20;
+static constexpr uint32_t kDefaultTileRays = 65536;
 CPD_HD constexpr uint32_t hist_cols(uint32_t nbin) {
     uint32_t c = 64;
     while (c > 1 && c * nbin > kHistWords) c >>= 1;
     return c;
 }
-// Largest tile whose per-column u32 sums cannot overflow.  A column gathers
-// the FP32-traced rays of its 64/cols lanes (tile/cols at most) plus at most
-// one re-traced ray per lane per queue drain (ceil(tile/64) drains), each
-// worth at most 2^22: 2*tile + 64 < 1024*cols keeps every sum below 2^32.
-CPD_HD constexpr uint32_t max_tile_rays(uint32_t cols) {
-    return ((512u * cols - 33u) & ~63u) < kMaxTileRays ? ((512u * cols - 33u) & ~63u) : kMaxTileRays;
-}
+// Rays between two folds of the u32 columns into the wavefront's u64 row.
+// Over a stretch of S rays a column gathers at most S / cols FP32-traced rays
+// of its 64 / cols lanes plus one re-traced ray per lane per queue drain; the
+// queue carries < 128 entries in, so there are at most S / 64 + 2 drains.
+// Each ray adds at most 2^22, so (2 S + 128) / cols < 1024 keeps every
+// column below 2^32.
+CPD_HD constexpr uint32_t fold_rays(uint32_t cols) { return (512u * cols - 65u) & ~63u; }
 
 // Rays per work item of a design: the whole wavelength sample rounded up to
-// a wavefront multiple, capped by the overflow bound and by tile_cap (a
-// multiple of 64, 0 for none).
+// a wavefront multiple, capped by tile_cap (a multiple of 64; 0 for the
+// default cap).
 CPD_HD uint32_t tile_for(uint32_t nbin, uint32_t n_rays, uint32_t tile_cap) {
+    (void)nbin;
     uint32_t tile = (n_rays + 63) / 64 * 64;
-    const uint32_t bound = max_tile_rays(hist_cols(nbin));
-    if (tile > bound) tile = bound;
-    if (tile_cap && tile > tile_cap) tile = tile_cap;
+    const uint32_t cap = tile_cap ? tile_cap : kDefaultTileRays;
+    if (tile > cap) tile = cap;
     return tile < 64 ? 64 : tile;
 }
 

@@ -223,7 +223,7 @@ cpd_status cpd_ctx_info(cpd_ctx *ctx, char *name, size_t name_len, int *cu_count
 enum {
     CPD_OPT_PROFILE = 1,       /* record HIP events around each kernel of a run (0/1) */
     CPD_OPT_COUNT_OPS = 2,     /* count the algorithmic operations a run performs (0/1) */
-    CPD_OPT_TILE_RAYS = 3,     /* rays per work item, multiple of 64, <= 16384 (0: automatic) */
+    CPD_OPT_TILE_RAYS = 3,     /* rays per work item, multiple of 64, <= 2^20 (0: 65536) */
     CPD_OPT_WAVES_PER_CU = 4,  /* resident wavefronts per CU of the persistent grid */
     CPD_OPT_ORDER = 5          /* work order of a batch, CPD_ORDER_* */
 };
@@ -308,7 +308,7 @@ enum {
     CPD_ORDER_COST = 1     /* costliest work items first (longest-processing-time order) */
 };
 typedef struct cpd_schedule {
-    uint32_t tile_rays;     /* rays per work item cap, multiple of 64, <= 16384 (0: largest safe tile) */
+    uint32_t tile_rays;     /* rays per work item cap, multiple of 64, <= 2^20 (0: 65536) */
     uint32_t waves_per_cu;  /* resident wavefronts per CU of the persistent grid, 1..64 */
     uint32_t order;         /* CPD_ORDER_* */
     uint32_t reserved;

@@ -71,7 +71,9 @@ def test_kat_known_answer_on_gpu():
     assert abs(gpu[2] - kat.KAT_DEVIATION) <= kat.KAT_DEVIATION_RMAX * kat.KAT_DEVIATION
 
 
-@pytest.mark.parametrize("config,n,nl,nr", [(0, 24, 16, 4096), (2, 24, 8, 2048)])
+# (4, 6, 4, 65536): 6 prisms and 256 bins, so a 65536-ray item folds its u32
+# LDS columns into the u64 row 16 times
+@pytest.mark.parametrize("config,n,nl,nr", [(0, 24, 16, 4096), (2, 24, 8, 2048), (4, 6, 4, 65536)])
 def test_generated_batch_matches_oracle(config, n, nl, nr):
     designs = [cpd.generate_design(config, i)[1] for i in range(n)]
     samp = [N.Sampling(nl, nr, N.MODE_FIXED, 0) for _ in range(n)]

@@ -35,11 +35,26 @@ def test_schedules_give_identical_results():
     ctx.set_schedule(None)
 
 
+def test_item_size_does_not_change_results_with_folds():
+    # 256 bins: a whole-wavelength item of 65536 rays folds its columns 16 times;
+    # 1024- and 4096-ray items fold never or once
+    _, D, _, _ = cpd.generate_designs(4, 0, 6)
+    S = N.Sampling(4, 65536, N.MODE_FIXED, 0)
+    ctx = cpd.Context(0)
+    ref, st = cpd.fitness_batch(D, S, ctx=ctx)
+    assert (st == 0).all()
+    for tile in (1024, 4096, 16384):
+        ctx.set_schedule(N.Schedule(tile, 16, N.ORDER_GIVEN, 0))
+        got, _ = cpd.fitness_batch(D, S, ctx=ctx)
+        assert np.array_equal(got, ref), tile
+    ctx.set_schedule(None)
+
+
 def test_autotune_returns_a_candidate_schedule():
     D, S = uneven(32, seed=1)
     best, ms = cpd.autotune(D, S)
     assert ms > 0
-    assert best.tile_rays in (0, 8192, 4096, 2048)
+    assert best.tile_rays in (0, 16384, 8192, 4096)
     assert best.waves_per_cu in (4, 8, 12, 16, 24)
     assert best.order in (N.ORDER_GIVEN, N.ORDER_COST)
 
