@@ -243,6 +243,57 @@ def adaptive_run(cpd, N, np, ctx, args):
             "fitness_sem_batched_designs_per_s": fit}
 
 
+def stress_run(cpd, N, np, ctx, args):
+    """Precision stress: designs of the 6-prism, 256-bin distribution with 1024 wavelengths x 2^20 rays,
+    evaluated once under each accumulation policy (CPD_OPT_ACCUMULATE).  The FP64 oracle (fixed count,
+    same sample indices) runs on the subset of the designs with the largest information, one design per
+    host thread."""
+    n = args.stress_designs
+    t = time.time()
+    _, D, S, _ = cpd.generate_designs(4, 0, n)
+    darr = (N.Design * n)(*D)
+    sarr = (N.Sampling * n)(*S)
+    rays = sum(s.n_wavelengths * s.n_rays for s in S)
+    log(f"stress: {n} designs ({rays:.3e} rays) generated in {time.time() - t:.1f}s")
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+    names = {N.ACC_EXACT: "exact fixed-point + FP64 compensated entropy", N.ACC_F32: "FP32 naive",
+             N.ACC_F32C: "FP32 compensated", N.ACC_F64: "FP64"}
+    res = {}
+    ctx.set_option(N.OPT_PROFILE, 1)
+    for acc in (N.ACC_EXACT, N.ACC_F32, N.ACC_F32C, N.ACC_F64):
+        ctx.set_option(N.OPT_ACCUMULATE, acc)
+        t = time.perf_counter()
+        if N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status) != 0:
+            raise RuntimeError(N.last_error())
+        wall = time.perf_counter() - t
+        st = ctx.last_stats()
+        res[acc] = {"policy": names[acc], "seconds": wall, "designs_per_s": n / wall,
+                    "trace_ms": st["trace_ms"], "finalize_ms": st["finalize_ms"],
+                    "info": np.array([f.info for f in out])}
+        log(f"stress: {names[acc]}: {wall:.2f} s ({st['trace_ms']:.0f} ms trace)")
+    ctx.set_option(N.OPT_ACCUMULATE, N.ACC_EXACT)
+    ctx.set_option(N.OPT_PROFILE, 0)
+    from oracle import oracle as O
+    pick = [int(i) for i in np.argsort(-res[N.ACC_EXACT]["info"])[: args.stress_check]]
+    threads = len(os.sched_getaffinity(0))
+    t = time.perf_counter()
+    ref, _ = O.fitness_many([D[i] for i in pick], 1024, 1 << 20, precision=64, qrng=O.QRNG_FIXED_POINT,
+                            threads=min(threads, len(pick)))
+    cpu_s = time.perf_counter() - t
+    log(f"stress: FP64 oracle on {len(pick)} designs in {cpu_s:.0f} s")
+    variants = []
+    for acc, r in res.items():
+        err = np.abs(r["info"][pick] - ref[:, 1])
+        variants.append({"policy": r["policy"], "designs_per_s": r["designs_per_s"], "seconds": r["seconds"],
+                         "trace_ms": r["trace_ms"], "finalize_ms": r["finalize_ms"],
+                         "mi_error_bits_max": float(err.max()), "mi_error_bits_mean": float(err.mean())})
+        log(f"stress: {r['policy']}: max |dI| {err.max():.3e} bits, mean {err.mean():.3e}")
+    return {"workload": f"{n} designs: 6 prisms, 256 bins, 1024 wavelengths x 2^20 rays, FP32 trace",
+            "rays": rays, "oracle_subset": pick, "oracle_info": [float(x) for x in ref[:, 1]],
+            "oracle_seconds": cpu_s, "oracle_threads": min(threads, len(pick)), "variants": variants}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +317,11 @@ def main() -> int:
     ap.add_argument("--adaptive", type=int, default=1024,
                     help="designs of the adaptive phase: the reference GPU estimator as the optimiser calls it "
                          "(gpu_fitness, max_n 128, max_eval 8192) batched vs per design (0: skip)")
+    ap.add_argument("--stress", action="store_true",
+                    help="only the precision stress: 512 designs (6 prisms, 256 bins, 1024 wavelengths x 2^20 rays) "
+                         "under each accumulation policy, MI error against the FP64 oracle on a subset")
+    ap.add_argument("--stress-designs", type=int, default=512)
+    ap.add_argument("--stress-check", type=int, default=8, help="designs of the FP64 oracle subset")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (counter calibration for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -292,6 +348,9 @@ def main() -> int:
     ctx = cpd.Context(0)
     info = ctx.info()
     log(f"device {info}")
+    if args.stress:
+        print(json.dumps({"stress": stress_run(cpd, N, np, ctx, args), "device": info}), flush=True)
+        return 0
     n = args.designs
     t0 = time.time()
     gen = [cpd.generate_design(1, i) for i in range(n)]

@@ -114,6 +114,12 @@ OPT_COUNT_OPS = 2
 OPT_TILE_RAYS = 3
 OPT_WAVES_PER_CU = 4
 OPT_ORDER = 5
+OPT_ACCUMULATE = 6
+
+ACC_EXACT = 0
+ACC_F32 = 1
+ACC_F32C = 2
+ACC_F64 = 3
 
 ORDER_GIVEN = 0
 ORDER_COST = 1

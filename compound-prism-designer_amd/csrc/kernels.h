@@ -62,15 +62,18 @@ struct FinalizeArgs {
     uint32_t n_designs;
 };
 
-// LDS of one trace wavefront: u32 columns | flagged-ray queue | u64 row.
-CPD_HD uint32_t trace_row_offset_words(uint32_t hist_words) { return (hist_words + kQueueCap + 1) & ~1u; }
-size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words);
+// LDS of one trace wavefront: columns | flagged-ray queue | row; sizes of the
+// accumulation policy `acc` (CPD_ACC_*).
+CPD_HD uint32_t trace_hist_end_words(uint32_t hist_words, uint32_t cell_bytes) {
+    return (hist_words * (cell_bytes / 4) + 1) & ~1u;
+}
+size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc);
 
 // start/stop (may be null) are recorded immediately around the kernel; with
 // neither, the launch is a plain one that stream capture records.
-hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start = nullptr,
-                        hipEvent_t stop = nullptr);
-hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a);
+hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, int acc = CPD_ACC_EXACT,
+                        hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a, int acc = CPD_ACC_EXACT);
 hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
                                uint32_t tile_cap, TraceDesign *traces);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n);

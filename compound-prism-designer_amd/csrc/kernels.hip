@@ -60,30 +60,73 @@ __device__ __forceinline__ uint32_t find_design(const uint64_t *begin, uint32_t 
     return lo;
 }
 
-// Neumaier-compensated FP64 sum.
-struct CSum {
-    double s = 0.0, c = 0.0;
-    __device__ void add(double x) {
+// Neumaier-compensated sum (FP64, or FP32 for the compensated FP32 policy).
+template <class T>
+struct CSumT {
+    T s = T(0), c = T(0);
+    __device__ void add(T x) {
 #pragma clang fp contract(off)
-        double t = s + x;
+        T t = s + x;
         if (fabs(s) >= fabs(x))
             c += (s - t) + x;
         else
             c += (x - t) + s;
         s = t;
     }
-    __device__ double value() const { return s + c; }
+    __device__ T value() const { return s + c; }
 };
+using CSum = CSumT<double>;
 
 __device__ __forceinline__ double plog2p(double p) { return p > 0.0 ? p * log2(p) : 0.0; }
+__device__ __forceinline__ float plog2pf(float p) { return p > 0.f ? p * log2f(p) : 0.f; }
+
+// How a wavefront accumulates transmittance per bin (CPD_OPT_ACCUMULATE).
+//  * exact (default): round(t 2^22) into u32 LDS columns, folded into a u64
+//    LDS row before a column can overflow, flushed with u64 atomics: exact
+//    integer sums, independent of order; FP64 compensated entropies.
+//  * f32: FP32 LDS columns and FP32 global atomics, FP32 entropies summed
+//    plainly -- the naive FP32 estimator.
+//  * f32c: FP32 columns folded every 1024 rays into a Neumaier-compensated FP32
+//    row; a work item is a whole wavelength, so the row is stored, not added;
+//    FP32 compensated entropies.
+//  * f64: FP64 LDS columns and FP64 global atomics, FP64 compensated entropies.
+template <int A> struct Acc;
+template <> struct Acc<CPD_ACC_EXACT> {
+    using Cell = uint32_t;
+    static constexpr uint32_t kRowBytes = 8;
+    static __device__ Cell of32(float t) { return quant32(t); }
+    static __device__ Cell of64(double t) { return quant64(t); }
+    static __device__ uint32_t fold_every(uint32_t cols) { return fold_rays(cols); }
+};
+template <> struct Acc<CPD_ACC_F32> {
+    using Cell = float;
+    static constexpr uint32_t kRowBytes = 4;
+    static __device__ Cell of32(float t) { return t; }
+    static __device__ Cell of64(double t) { return (float)t; }
+    static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
+};
+template <> struct Acc<CPD_ACC_F32C> {
+    using Cell = float;
+    static constexpr uint32_t kRowBytes = 8;
+    static __device__ Cell of32(float t) { return t; }
+    static __device__ Cell of64(double t) { return (float)t; }
+    static __device__ uint32_t fold_every(uint32_t) { return 1024; }
+};
+template <> struct Acc<CPD_ACC_F64> {
+    using Cell = double;
+    static constexpr uint32_t kRowBytes = 8;
+    static __device__ Cell of32(float t) { return (double)t; }
+    static __device__ Cell of64(double t) { return t; }
+    static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
+};
 
 struct TileCtx {
     const cpd_design *g;
     const TraceDesign *td;
     LambdaConsts lc;
     uint32_t k_begin, k_end, n_rays, nbin, cols;
-    uint32_t *hist;
-    unsigned long long *row;  // the wavefront's u64 sums per bin
+    void *hist;      // [bin][column] cells of the accumulation policy
+    void *row;       // the wavefront's row per bin
     uint32_t *queue;
     uint32_t *last_bin;
     const float *z32;
@@ -91,47 +134,68 @@ struct TileCtx {
 };
 
 // FP64 re-trace of `count` (<= 64) queued rays starting at ring slot `head`.
+template <int A>
 __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
+    using Cell = typename Acc<A>::Cell;
     const uint32_t lane = threadIdx.x;
     if (lane < count) {
         const uint32_t k = c.queue[(head + lane) & (kQueueCap - 1)];
         const double y = beam_y64(*c.g, c.z64[k - 1]);
         double t;
         const uint32_t b = trace64(*c.g, c.lc.lambda, y, t);
-        if (b < c.nbin) atomicAdd(&c.hist[b * c.cols + (lane & (c.cols - 1))], quant64(t));
+        if (b < c.nbin) atomicAdd((Cell *)c.hist + b * c.cols + (lane & (c.cols - 1)), Acc<A>::of64(t));
         if (k == c.n_rays) *c.last_bin = b;
     }
 }
 
-// Adds each bin's u32 columns (skewed, conflict-free order) to the u64 row and
-// clears them.  Lane b handles bins b, b + 64, ...
-__device__ void fold_columns(uint32_t *hist, unsigned long long *row, uint32_t nbin, uint32_t cols) {
+// Adds each bin's columns (skewed, conflict-free order) to the row and clears
+// them.  Lane b handles bins b, b + 64, ...
+template <int A>
+__device__ void fold_columns(void *hist_v, void *row_v, uint32_t nbin, uint32_t cols) {
+    using Cell = typename Acc<A>::Cell;
+    Cell *hist = (Cell *)hist_v;
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < nbin; b += 64) {
-        uint32_t *cell = hist + b * cols;
-        uint64_t sum = 0;
-        for (uint32_t q = 0; q < cols; ++q) {
-            const uint32_t cc = (q + b) & (cols - 1);
-            sum += cell[cc];
-            cell[cc] = 0;
+        Cell *cell = hist + b * cols;
+        if (A == CPD_ACC_EXACT) {
+            uint64_t sum = 0;
+            for (uint32_t q = 0; q < cols; ++q) {
+                const uint32_t cc = (q + b) & (cols - 1);
+                sum += (uint64_t)cell[cc];
+                cell[cc] = 0;
+            }
+            ((unsigned long long *)row_v)[b] += sum;
+        } else if (A == CPD_ACC_F32C) {
+            float sum = 0.f;
+            for (uint32_t q = 0; q < cols; ++q) {
+                const uint32_t cc = (q + b) & (cols - 1);
+                sum += (float)cell[cc];
+                cell[cc] = 0;
+            }
+            CSumT<float> r;
+            r.s = ((float *)row_v)[2 * b];
+            r.c = ((float *)row_v)[2 * b + 1];
+            r.add(sum);
+            ((float *)row_v)[2 * b] = r.s;
+            ((float *)row_v)[2 * b + 1] = r.c;
         }
-        row[b] += sum;
     }
     __syncthreads();
 }
 
-template <bool COUNT>
+template <bool COUNT, int A>
 __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
+    using Cell = typename Acc<A>::Cell;
     const uint32_t lane = threadIdx.x;
     const uint32_t col = lane & (c.cols - 1);
     // register copies of the per-item records: the loop stores through
     // last_bin, so loads through c.td would be repeated every iteration
     const HotRecord hot = to_sgprs(make_hot(*c.td, c.lc));
     const float *__restrict__ z32 = c.z32;
-    uint32_t *__restrict__ hist = c.hist;
+    Cell *__restrict__ hist = (Cell *)c.hist;
     const uint32_t k_end = c.k_end, n_rays = c.n_rays, nbin = c.nbin, cols = c.cols;
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
-    const uint32_t fold = fold_rays(cols);
+    const uint32_t fold = Acc<A>::fold_every(cols);
     uint32_t since_fold = 0;
     for (uint32_t base = c.k_begin; base < k_end; base += 64) {
         const uint32_t k = base + lane;
@@ -142,7 +206,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         if (COUNT && active) ops += r.ops;
         const bool flagged = active && r.guard;
         if (active && !r.guard) {
-            if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], quant32(r.t));
+            if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
             if (k == n_rays) *c.last_bin = r.bin;
         }
         const uint64_t mask = __ballot(flagged);
@@ -151,31 +215,88 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
             tail += __popcll(mask);
             if (tail - head >= 64) {
                 __builtin_amdgcn_wave_barrier();
-                drain_queue(c, head, 64);
+                drain_queue<A>(c, head, 64);
                 head += 64;
             }
         }
         since_fold += 64;
         if (since_fold >= fold) {
-            fold_columns(hist, c.row, nbin, cols);
+            fold_columns<A>(c.hist, c.row, nbin, cols);
             since_fold = 0;
         }
     }
     if (tail != head) {
         __builtin_amdgcn_wave_barrier();
-        drain_queue(c, head, tail - head);
+        drain_queue<A>(c, head, tail - head);
     }
 }
 
-template <bool COUNT>
+// Last fold of an item and flush of its row into the global sums of
+// (design, wavelength j).
+template <int A>
+__device__ void flush_row(void *hist_v, void *row_v, uint32_t nbin, uint32_t cols, unsigned long long *acc_base,
+                          uint64_t idx0) {
+    using Cell = typename Acc<A>::Cell;
+    Cell *hist = (Cell *)hist_v;
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nbin; b += 64) {
+        Cell *cell = hist + b * cols;
+        if (A == CPD_ACC_EXACT) {
+            unsigned long long *row = (unsigned long long *)row_v;
+            uint64_t sum = row[b];
+            row[b] = 0;
+            for (uint32_t q = 0; q < cols; ++q) {
+                const uint32_t cc = (q + b) & (cols - 1);
+                sum += (uint64_t)cell[cc];
+                cell[cc] = 0;
+            }
+            if (sum) atomicAdd(acc_base + idx0 + b, (unsigned long long)sum);
+        } else if (A == CPD_ACC_F32) {
+            float sum = 0.f;
+            for (uint32_t q = 0; q < cols; ++q) {
+                const uint32_t cc = (q + b) & (cols - 1);
+                sum += (float)cell[cc];
+                cell[cc] = 0;
+            }
+            if (sum != 0.f) atomicAdd((float *)(acc_base + idx0 + b), sum);
+        } else if (A == CPD_ACC_F32C) {
+            float sum = 0.f;
+            for (uint32_t q = 0; q < cols; ++q) {
+                const uint32_t cc = (q + b) & (cols - 1);
+                sum += (float)cell[cc];
+                cell[cc] = 0;
+            }
+            float *row = (float *)row_v;
+            CSumT<float> r;
+            r.s = row[2 * b];
+            r.c = row[2 * b + 1];
+            r.add(sum);
+            row[2 * b] = row[2 * b + 1] = 0.f;
+            *(float *)(acc_base + idx0 + b) = r.value();  // the only item of this wavelength
+        } else {
+            double sum = 0.0;
+            for (uint32_t q = 0; q < cols; ++q) {
+                const uint32_t cc = (q + b) & (cols - 1);
+                sum += (double)cell[cc];
+                cell[cc] = 0;
+            }
+            if (sum != 0.0) atomicAdd((double *)(acc_base + idx0 + b), sum);
+        }
+    }
+    __syncthreads();
+}
+
+template <bool COUNT, int A>
 __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
-    extern __shared__ uint32_t lds[];
-    uint32_t *hist = lds;
-    uint32_t *queue = lds + a.hist_words;
-    unsigned long long *row = (unsigned long long *)(lds + trace_row_offset_words(a.hist_words));
+    using Cell = typename Acc<A>::Cell;
+    extern __shared__ __align__(8) uint32_t lds[];
+    Cell *hist = (Cell *)lds;
+    const uint32_t hist_end = trace_hist_end_words(a.hist_words, sizeof(Cell));
+    uint32_t *queue = lds + hist_end;
+    uint32_t *row32 = lds + ((hist_end + kQueueCap + 1) & ~1u);
     const uint32_t lane = threadIdx.x;
     for (uint32_t i = lane; i < a.hist_words; i += 64) hist[i] = 0;
-    for (uint32_t i = lane; i < a.row_words; i += 64) row[i] = 0;
+    for (uint32_t i = lane; i < a.row_words * (Acc<A>::kRowBytes / 4); i += 64) row32[i] = 0;
     __syncthreads();
     unsigned long long ops = 0;
     uint64_t item = blockIdx.x;
@@ -206,31 +327,17 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.nbin = nbin;
         c.cols = cols;
         c.hist = hist;
-        c.row = row;
+        c.row = row32;
         c.queue = queue;
         c.last_bin = a.last_bin + a.lb_begin[d] + j;
         c.z32 = a.z32;
         c.z64 = a.z64;
         if (!c.lc.ok) {
-            // the entrance face reflects every ray of this wavelength
+            // every ray of this wavelength is lost in the planar faces
             if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
         } else {
-            tile_loop<COUNT>(c, ops);
-            // last fold and flush: row + columns into the global sums
-            __syncthreads();
-            unsigned long long *acc = a.acc + a.acc_begin[d] + (uint64_t)j * nbin;
-            for (uint32_t b = lane; b < nbin; b += 64) {
-                uint32_t *cell = hist + b * cols;
-                uint64_t sum = row[b];
-                row[b] = 0;
-                for (uint32_t q = 0; q < cols; ++q) {
-                    const uint32_t cc = (q + b) & (cols - 1);
-                    sum += cell[cc];
-                    cell[cc] = 0;
-                }
-                if (sum) atomicAdd(acc + b, (unsigned long long)sum);
-            }
-            __syncthreads();
+            tile_loop<COUNT, A>(c, ops);
+            flush_row<A>(hist, row32, nbin, cols, a.acc, a.acc_begin[d] + (uint64_t)j * nbin);
         }
         if (!a.work_counter) break;
     }
@@ -240,6 +347,15 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
     }
 }
 
+// p(d|l) of one accumulator cell (the bin of the last ray divides by M + 1).
+template <int A>
+__device__ double cell_p(const unsigned long long *acc, uint64_t idx, double div) {
+    if (A == CPD_ACC_EXACT) return (double)acc[idx] * (1.0 / kQuantScale) / div;
+    if (A == CPD_ACC_F64) return *(const double *)(acc + idx) / div;
+    return (double)(*(const float *)(acc + idx) / (float)div);
+}
+
+template <int A>
 __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
     const uint32_t e = blockIdx.x;
     const uint32_t d = a.slot ? a.slot[e] : e;
@@ -250,38 +366,91 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
     unsigned long long *acc = a.acc + a.acc_begin[d];
     uint32_t *lb = a.last_bin + a.lb_begin[d];
     double *rows = a.rows ? a.rows + a.row_begin[d] : nullptr;
-    const double inv_scale = 1.0 / kQuantScale;
-    CSum h_cond, h_det;
-    for (uint32_t b = lane; b < nbin; b += 64) {
-        CSum pb;
-        for (uint32_t j = 0; j < N; ++j) {
-            const uint64_t s = acc[(uint64_t)j * nbin + b];
-            const double div = lb[j] == b ? (double)M + 1.0 : (double)M;
-            const double p = (double)s * inv_scale / div;
-            h_cond.add(plog2p(p));
-            pb.add(p);
-            if (rows) rows[(uint64_t)j * nbin + b] = p;
-            acc[(uint64_t)j * nbin + b] = 0;
+    double info;
+    if (A == CPD_ACC_EXACT || A == CPD_ACC_F64) {
+        CSum h_cond, h_det;
+        for (uint32_t b = lane; b < nbin; b += 64) {
+            CSum pb;
+            for (uint32_t j = 0; j < N; ++j) {
+                const uint64_t idx = (uint64_t)j * nbin + b;
+                const double p = cell_p<A>(acc, idx, lb[j] == b ? (double)M + 1.0 : (double)M);
+                h_cond.add(plog2p(p));
+                pb.add(p);
+                if (rows) rows[idx] = p;
+                acc[idx] = 0;
+            }
+            h_det.add(plog2p(pb.value() / (double)N));
         }
-        h_det.add(plog2p(pb.value() / (double)N));
-    }
-    // fixed-order butterfly over the wavefront on (sum, compensation) pairs
-    double hs = h_cond.s, hc = h_cond.c, ds = h_det.s, dc = h_det.c;
-    for (int off = 32; off > 0; off >>= 1) {
-        CSum x;
-        x.s = hs;
-        x.c = hc;
-        x.add(__shfl_xor(hs, off));
-        x.c += __shfl_xor(hc, off);
-        hs = x.s;
-        hc = x.c;
-        CSum y;
-        y.s = ds;
-        y.c = dc;
-        y.add(__shfl_xor(ds, off));
-        y.c += __shfl_xor(dc, off);
-        ds = y.s;
-        dc = y.c;
+        // fixed-order butterfly over the wavefront on (sum, compensation) pairs
+        double hs = h_cond.s, hc = h_cond.c, ds = h_det.s, dc = h_det.c;
+        for (int off = 32; off > 0; off >>= 1) {
+            CSum x;
+            x.s = hs;
+            x.c = hc;
+            x.add(__shfl_xor(hs, off));
+            x.c += __shfl_xor(hc, off);
+            hs = x.s;
+            hc = x.c;
+            CSum y;
+            y.s = ds;
+            y.c = dc;
+            y.add(__shfl_xor(ds, off));
+            y.c += __shfl_xor(dc, off);
+            ds = y.s;
+            dc = y.c;
+        }
+        info = (hs + hc) / (double)N - (ds + dc);
+    } else if (A == CPD_ACC_F32) {
+        float h_cond = 0.f, h_det = 0.f;
+        for (uint32_t b = lane; b < nbin; b += 64) {
+            float pb = 0.f;
+            for (uint32_t j = 0; j < N; ++j) {
+                const uint64_t idx = (uint64_t)j * nbin + b;
+                const float p = (float)cell_p<A>(acc, idx, lb[j] == b ? (double)M + 1.0 : (double)M);
+                h_cond += plog2pf(p);
+                pb += p;
+                if (rows) rows[idx] = p;
+                acc[idx] = 0;
+            }
+            h_det += plog2pf(pb / (float)N);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            h_cond += __shfl_xor(h_cond, off);
+            h_det += __shfl_xor(h_det, off);
+        }
+        info = (double)(h_cond / (float)N - h_det);
+    } else {
+        CSumT<float> h_cond, h_det;
+        for (uint32_t b = lane; b < nbin; b += 64) {
+            CSumT<float> pb;
+            for (uint32_t j = 0; j < N; ++j) {
+                const uint64_t idx = (uint64_t)j * nbin + b;
+                const float p = (float)cell_p<A>(acc, idx, lb[j] == b ? (double)M + 1.0 : (double)M);
+                h_cond.add(plog2pf(p));
+                pb.add(p);
+                if (rows) rows[idx] = p;
+                acc[idx] = 0;
+            }
+            h_det.add(plog2pf(pb.value() / (float)N));
+        }
+        float hs = h_cond.s, hc = h_cond.c, ds = h_det.s, dc = h_det.c;
+        for (int off = 32; off > 0; off >>= 1) {
+            CSumT<float> x;
+            x.s = hs;
+            x.c = hc;
+            x.add(__shfl_xor(hs, off));
+            x.c += __shfl_xor(hc, off);
+            hs = x.s;
+            hc = x.c;
+            CSumT<float> y;
+            y.s = ds;
+            y.c = dc;
+            y.add(__shfl_xor(ds, off));
+            y.c += __shfl_xor(dc, off);
+            ds = y.s;
+            dc = y.c;
+        }
+        info = (double)((hs + hc) / (float)N - (ds + dc));
     }
     __syncthreads();
     for (uint32_t j = lane; j < N; j += 64) lb[j] = nbin;
@@ -290,7 +459,7 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
         size_and_deviation64(g, size, deviation);
         cpd_fitness f;
         f.size = size;
-        f.info = (hs + hc) / (double)N - (ds + dc);
+        f.info = info;
         f.deviation = deviation;
         a.out[a.out_stride_slot ? d : e] = f;
     }
@@ -343,32 +512,52 @@ hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const c
     return hipGetLastError();
 }
 
-size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words) {
-    return (size_t)trace_row_offset_words(hist_words) * sizeof(uint32_t) + (size_t)row_words * sizeof(uint64_t);
-}
 
-hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, hipEvent_t start,
-                        hipEvent_t stop) {
-    if (grid == 0 || a.n_items == 0) return hipSuccess;
-    const size_t lds = trace_lds_bytes(a.hist_words, a.row_words);
-    if (!start && !stop) {
+namespace {
+template <bool COUNT, int A>
+hipError_t launch_trace_t(hipStream_t s, const BatchArgs &a, uint32_t grid, hipEvent_t start, hipEvent_t stop) {
+    const size_t lds = trace_lds_bytes(a.hist_words, a.row_words, A);
+    if (!start && !stop)
         // plain launch: the form a stream capture records as a kernel node
-        if (count_ops)
-            hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, a);
-        else
-            hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(64), lds, s, a);
-        return hipGetLastError();
-    }
-    if (count_ops)
-        hipExtLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(64), lds, s, start, stop, 0, a);
+        hipLaunchKernelGGL((trace_kernel<COUNT, A>), dim3(grid), dim3(64), lds, s, a);
     else
-        hipExtLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(64), lds, s, start, stop, 0, a);
+        hipExtLaunchKernelGGL((trace_kernel<COUNT, A>), dim3(grid), dim3(64), lds, s, start, stop, 0, a);
     return hipGetLastError();
 }
+template <bool COUNT>
+hipError_t launch_trace_c(hipStream_t s, const BatchArgs &a, uint32_t grid, int acc, hipEvent_t start,
+                          hipEvent_t stop) {
+    switch (acc) {
+        case CPD_ACC_F32: return launch_trace_t<COUNT, CPD_ACC_F32>(s, a, grid, start, stop);
+        case CPD_ACC_F32C: return launch_trace_t<COUNT, CPD_ACC_F32C>(s, a, grid, start, stop);
+        case CPD_ACC_F64: return launch_trace_t<COUNT, CPD_ACC_F64>(s, a, grid, start, stop);
+        default: return launch_trace_t<COUNT, CPD_ACC_EXACT>(s, a, grid, start, stop);
+    }
+}
+}  // namespace
 
-hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a) {
+size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc) {
+    const uint32_t cell = acc == CPD_ACC_F64 ? 8 : 4;
+    const uint32_t row = acc == CPD_ACC_F32 ? 4 : 8;
+    const uint32_t end = trace_hist_end_words(hist_words, cell);
+    return (size_t)((end + kQueueCap + 1) & ~1u) * 4 + (size_t)row_words * row;
+}
+
+hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, int acc, hipEvent_t start,
+                        hipEvent_t stop) {
+    if (grid == 0 || a.n_items == 0) return hipSuccess;
+    return count_ops ? launch_trace_c<true>(s, a, grid, acc, start, stop)
+                     : launch_trace_c<false>(s, a, grid, acc, start, stop);
+}
+
+hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a, int acc) {
     if (a.n_designs == 0) return hipSuccess;
-    hipLaunchKernelGGL(finalize_kernel, dim3(a.n_designs), dim3(64), 0, s, a);
+    switch (acc) {
+        case CPD_ACC_F32: hipLaunchKernelGGL(finalize_kernel<CPD_ACC_F32>, dim3(a.n_designs), dim3(64), 0, s, a); break;
+        case CPD_ACC_F32C: hipLaunchKernelGGL(finalize_kernel<CPD_ACC_F32C>, dim3(a.n_designs), dim3(64), 0, s, a); break;
+        case CPD_ACC_F64: hipLaunchKernelGGL(finalize_kernel<CPD_ACC_F64>, dim3(a.n_designs), dim3(64), 0, s, a); break;
+        default: hipLaunchKernelGGL(finalize_kernel<CPD_ACC_EXACT>, dim3(a.n_designs), dim3(64), 0, s, a); break;
+    }
     return hipGetLastError();
 }
 

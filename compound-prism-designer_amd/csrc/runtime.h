@@ -39,6 +39,7 @@ struct Context {
     uint32_t waves_per_cu = 16;   // resident wavefronts per CU of the persistent grid
     uint32_t tile_rays_hint = 0;  // rays per work item (0: the default)
     uint32_t order = CPD_ORDER_GIVEN;
+    int acc_kind = CPD_ACC_EXACT;  // accumulation policy of fixed-count batches
     bool profile = false;
     bool count_ops = false;
     cpd_run_stats last{};

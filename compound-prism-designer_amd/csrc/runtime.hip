@@ -226,6 +226,8 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     HIP_TRY(hipSetDevice(device));
     uint32_t max_rays = 0;
     for (uint32_t i : valid) max_rays = std::max(max_rays, sampling[i].n_rays);
+    if (acc_kind == CPD_ACC_F32C && max_rays > kMaxTileRays)
+        return fail(CPD_ERR_INVALID_ARGUMENT, "the compensated FP32 accumulation takes at most 2^20 rays per wavelength");
     // host staging: designs | traces | item_begin | acc_begin | lb_begin | row_begin
     const size_t off_designs = 0;
     const size_t off_traces = align_up(off_designs + n * sizeof(cpd_design), 256);
@@ -251,7 +253,8 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
         const cpd_design &g = designs[valid[q]];
         const cpd_sampling &s = sampling[valid[q]];
         hd[q] = g;
-        make_trace_design(g, s.n_wavelengths, s.n_rays, ht[q], tile_rays_hint);
+        // the compensated FP32 policy stores a wavelength's row from one item
+        make_trace_design(g, s.n_wavelengths, s.n_rays, ht[q], acc_kind == CPD_ACC_F32C ? kMaxTileRays : tile_rays_hint);
         hist_words = std::max(hist_words, ht[q].hist_cols * g.bin_count);
         row_words = std::max(row_words, g.bin_count);
         tile = std::max(tile, ht[q].tile_rays);
@@ -308,7 +311,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
         a.work_counter = nullptr;
         grid = (uint32_t)items;
     }
-    HIP_TRY(launch_trace(stream, a, grid, want_ops, profile ? ev[0] : nullptr, profile ? ev[1] : nullptr));
+    HIP_TRY(launch_trace(stream, a, grid, want_ops, acc_kind, profile ? ev[0] : nullptr, profile ? ev[1] : nullptr));
 
     FinalizeArgs f{};
     f.designs = a.designs;
@@ -321,7 +324,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     f.rows = rows ? row_buf.as<double>() : nullptr;
     f.row_begin = (const uint64_t *)(dstage + off_rows);
     f.n_designs = n;
-    HIP_TRY(launch_finalize(stream, f));
+    HIP_TRY(launch_finalize(stream, f, acc_kind));
     if (profile) HIP_TRY(hipEventRecord(ev[2], stream));
     if (out) {
         st = host_out.reserve(n * sizeof(cpd_fitness));
@@ -469,6 +472,11 @@ cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value) {
         case CPD_OPT_WAVES_PER_CU:
             if (value < 1 || value > 64) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "waves per CU must be in 1..64");
             c->waves_per_cu = (uint32_t)value;
+            return CPD_OK;
+        case CPD_OPT_ACCUMULATE:
+            if (value < CPD_ACC_EXACT || value > CPD_ACC_F64)
+                return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown accumulation %lld", (long long)value);
+            c->acc_kind = (int)value;
             return CPD_OK;
         case CPD_OPT_ORDER:
             if (value != CPD_ORDER_GIVEN && value != CPD_ORDER_COST)

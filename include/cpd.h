@@ -225,8 +225,17 @@ enum {
     CPD_OPT_COUNT_OPS = 2,     /* count the algorithmic operations a run performs (0/1) */
     CPD_OPT_TILE_RAYS = 3,     /* rays per work item, multiple of 64, <= 2^20 (0: 65536) */
     CPD_OPT_WAVES_PER_CU = 4,  /* resident wavefronts per CU of the persistent grid */
-    CPD_OPT_ORDER = 5          /* work order of a batch, CPD_ORDER_* */
+    CPD_OPT_ORDER = 5,         /* work order of a batch, CPD_ORDER_* */
+    CPD_OPT_ACCUMULATE = 6     /* per-bin accumulation of fixed-count batches, CPD_ACC_* */
 };
+/* Accumulation of the per-bin transmittance (a precision study; results of
+ * the default are exact and order-independent):
+ * CPD_ACC_EXACT  round(t 2^22) in integers, FP64 compensated entropies (default);
+ * CPD_ACC_F32    FP32 sums and atomics, FP32 entropies summed plainly;
+ * CPD_ACC_F32C   FP32 with Neumaier compensation of partial sums and entropies
+ *                (one work item per wavelength, so n_rays <= 2^20);
+ * CPD_ACC_F64    FP64 sums and atomics, FP64 compensated entropies. */
+enum { CPD_ACC_EXACT = 0, CPD_ACC_F32 = 1, CPD_ACC_F32C = 2, CPD_ACC_F64 = 3 };
 cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value);
 
 /* Statistics of the last fitness run on a context. */

@@ -130,3 +130,18 @@ def test_transmission_probability_matches_oracle():
     for j in range(7):
         ref, _ = O.p_dets_l_wavelength(to_oracle(d), float(w[j]), 4096, qrng=O.QRNG_FIXED_POINT)
         np.testing.assert_allclose(out[j], ref, atol=2e-6)
+
+
+@pytest.mark.parametrize("acc", [N.ACC_EXACT, N.ACC_F32, N.ACC_F32C, N.ACC_F64])
+def test_accumulation_policies_match_oracle(acc):
+    # 6 prisms and 256 bins, 4 wavelengths x 65536 rays: every policy estimates
+    # the same information; at this size all are within INFO_TOL of the FP64 oracle
+    _, D, _, _ = cpd.generate_designs(4, 0, 6)
+    samp = N.Sampling(4, 65536, N.MODE_FIXED, 0)
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ACCUMULATE, acc)
+    got, st = cpd.fitness_batch(D, samp, ctx=ctx)
+    assert (st == 0).all()
+    for i, d in enumerate(D):
+        ref, _ = O.fitness(to_oracle(d), 4, 65536, 64, adaptive=False, qrng=O.QRNG_FIXED_POINT)
+        check_close(got[i], np.array(ref), f"policy {acc} design {i}")
