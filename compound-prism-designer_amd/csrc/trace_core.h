@@ -458,9 +458,12 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
 #if defined(__HIP_DEVICE_COMPILE__)
 CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// Whether any lane of the wavefront holds v (one lane on the host).
+CPD_HD bool wave_any(bool v) { return __ballot(v) != 0; }
 #else
 CPD_HD float fast_rcp(float x) { return 1.0f / x; }
 CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
+CPD_HD bool wave_any(bool v) { return v; }
 #endif
 
 // Error budget of an FP32 length per unit of path travelled, and of an FP32
@@ -615,6 +618,9 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         const float cd = -fmaf(vx, d.dnx, vy * d.dny);
         guard |= alive & (fabsf(cd - d.min_ci) * den <= kGuardCos * num);
         alive &= cd > d.min_ci;
+        // most wavefronts lose every ray here (the acceptance cosine): skip
+        // the rest of the stage when no lane has a ray left or near a decision
+        if (!wave_any(alive | guard)) return res;
         const float wx = d.mpx + qx, wy = d.mpy + qy;
         const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(cd);
         const float pos = fmaf(fmaf(vx, dist, wx), d.ddx, fmaf(vy, dist, wy) * d.ddy);
