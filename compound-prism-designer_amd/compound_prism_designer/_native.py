@@ -115,6 +115,7 @@ OPT_TILE_RAYS = 3
 OPT_WAVES_PER_CU = 4
 OPT_ORDER = 5
 OPT_ACCUMULATE = 6
+OPT_ROW_TRACE = 7
 
 ACC_EXACT = 0
 ACC_F32 = 1

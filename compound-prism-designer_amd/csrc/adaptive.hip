@@ -17,10 +17,11 @@
 //    small enough error.  Seeds are added until every bin's mean over the
 //    rows has one; otherwise the result is "not converged".
 //
-// Both trace every ray with trace64, the FP64 trace in the reference's
-// operation order, from beam quantiles built on the host, and fold in FP64 in
-// the reference's operation order, so a row is the FP64 oracle's row bit for
-// bit.  A wavefront owns one row; the sequential fold runs on wave-uniform
+// Rays are traced with trace64, the FP64 trace in the reference's operation
+// order, from beam quantiles built on the host, or (CPD_OPT_ROW_TRACE 32, the
+// default of the GPU estimator) with the FP32 trace and its FP64 re-trace of
+// flagged rays.  The fold is FP64 in the reference's operation order, so with
+// the FP64 trace a row is the FP64 oracle's row bit for bit.  A wavefront owns one row; the sequential fold runs on wave-uniform
 // values (each lane computes the same numbers) over summaries kept in LDS.
 // The rows of one design are independent, so the host launches blocks of
 // rows speculatively and scans them in order, discarding rows past the stop.
@@ -29,7 +30,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "cpd.h"
@@ -59,6 +63,7 @@ struct RowArgs {
     double *out;                 // p(d|lambda) rows
     uint64_t *row_rays;          // [n_rows] rays folded into the row
     const double *z64;           // beam quantiles, index k - 1
+    const float *z32;            // the same in FP32
     uint32_t n_rows;
     uint32_t max_rays;           // rays cap (chunked: a multiple of 32)
     uint32_t nbin_max;
@@ -93,7 +98,45 @@ __device__ __forceinline__ void store_w(double *lds, uint32_t nbin_max, uint32_t
     lds[2 * nbin_max + b] = w.m2;
 }
 
+// One ray of a row: FP64 in the reference's operation order, or the FP32
+// trace with its FP64 re-trace of flagged rays (the decisions are the FP64
+// ones; t is rounded to FP32).  k is the ray's 1-based index.
+template <bool FP32>
+__device__ __forceinline__ uint32_t row_ray(const RowArgs &a, const cpd_design &g, double lambda,
+                                            const HotRecord &hot, bool active, uint32_t k, double &t) {
+    t = 0.0;
+    if (!FP32) return active ? trace64(g, lambda, beam_y64(g, a.z64[k - 1]), t) : g.bin_count;
+    const float z = active ? a.z32[k - 1] : 0.f;
+    const Trace32Result r = trace32<false>(hot, hot.y_mean - hot.beam_w * z);
+    uint32_t bin = r.bin;
+    t = r.t;
+    const bool redo = active && r.guard;
+    if (wave_any(redo) && redo) bin = trace64(g, lambda, beam_y64(g, a.z64[k - 1]), t);
+    if (!active) {
+        bin = g.bin_count;
+        t = 0.0;
+    }
+    return bin;
+}
+
+// Per-row constants of the FP32 trace: the planar chain of this wavelength.
+template <bool FP32>
+__device__ __forceinline__ HotRecord row_hot(const cpd_design &g, double lambda, bool &ok) {
+    HotRecord hot{};
+    ok = true;
+    if (FP32) {
+        TraceDesign td;
+        make_trace_design(g, 1, 64, td);
+        LambdaConsts lc;
+        make_lambda_consts(g, lambda, lc);
+        ok = lc.ok != 0;
+        hot = make_hot(td, lc);
+    }
+    return hot;
+}
+
 // fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).
+template <bool FP32>
 __global__ __launch_bounds__(64) void sem_rows_kernel(RowArgs a) {
 #pragma clang fp contract(off)
     extern __shared__ double lds[];
@@ -105,30 +148,38 @@ __global__ __launch_bounds__(64) void sem_rows_kernel(RowArgs a) {
     const double lambda = a.row_lambda[r];
     const uint32_t cap = a.row_cap[r];
     const double max_err_sq = kMaxErr * kMaxErr;
+    bool chain_ok;
+    const HotRecord hot = row_hot<FP32>(g, lambda, chain_ok);
     double count = 0.0;
     bool stop = false;
     for (uint32_t base = 1; base <= cap && !stop; base += 64) {
         const uint32_t k = base + lane;
-        double t = 0.0;
-        uint32_t bin = nbin;
-        if (k <= cap) bin = trace64(g, lambda, beam_y64(g, a.z64[k - 1]), t);
+        double t;
+        const uint32_t bin = chain_ok ? row_ray<FP32>(a, g, lambda, hot, k <= cap, k, t) : nbin;
+        if (!chain_ok) t = 0.0;
         const uint32_t n_here = min(64u, cap - base + 1);
-        for (uint32_t l = 0; l < n_here; ++l) {
+        // visit only the lanes whose ray hit a bin, in ray order; count still
+        // advances by one per ray, as in the reference's loop
+        uint64_t hits = __ballot(lane < n_here && bin < nbin);
+        uint32_t consumed = 0;
+        while (hits) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(hits);
+            hits &= hits - 1;
+            count += (double)(l + 1 - consumed);
+            consumed = l + 1;
             const uint32_t b = __builtin_amdgcn_readlane(bin, l);
-            count += 1.0;
-            if (b < nbin) {
-                W64 w = load_w(lds, a.nbin_max, b);
-                w_skip(w, count);
-                w_next(w, readlane_f64(t, l));
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) store_w(lds, a.nbin_max, b, w);
-                __builtin_amdgcn_wave_barrier();
-                if (w_sem_le(w, max_err_sq)) {
-                    stop = true;
-                    break;
-                }
+            W64 w = load_w(lds, a.nbin_max, b);
+            w_skip(w, count);
+            w_next(w, readlane_f64(t, l));
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) store_w(lds, a.nbin_max, b, w);
+            __builtin_amdgcn_wave_barrier();
+            if (w_sem_le(w, max_err_sq)) {
+                stop = true;
+                break;
             }
         }
+        if (!stop) count += (double)(n_here - consumed);
     }
     __syncthreads();
     for (uint32_t b = lane; b < nbin; b += 64) {
@@ -141,6 +192,7 @@ __global__ __launch_bounds__(64) void sem_rows_kernel(RowArgs a) {
 
 // kernel.rs:89-193: 32-ray chunks; a wavefront traces two chunks at once
 // (lanes 0-31 and 32-63) and folds them in order.
+template <bool FP32>
 __global__ __launch_bounds__(64) void chunk_rows_kernel(RowArgs a) {
 #pragma clang fp contract(off)
     extern __shared__ double lds[];
@@ -153,16 +205,16 @@ __global__ __launch_bounds__(64) void chunk_rows_kernel(RowArgs a) {
     const double max_err_sq = kMaxErr * kMaxErr;
     const uint32_t iters = a.max_rays / 32;
     const uint32_t half = lane >> 5, l = lane & 31;
+    bool chain_ok;
+    const HotRecord hot = row_hot<FP32>(g, lambda, chain_ok);
     double count = 0.0;
     bool stop = false;
     for (uint32_t i = 0; i < iters && !stop; i += 2) {
         const uint32_t c = i + half;
-        double t = 0.0;
-        uint32_t bin = nbin;
-        if (c < iters) {
-            const uint32_t k = l + 32 * (c + 1);
-            bin = trace64(g, lambda, beam_y64(g, a.z64[k - 1]), t);
-        }
+        double t;
+        const uint32_t k = l + 32 * (c + 1);
+        const uint32_t bin = chain_ok ? row_ray<FP32>(a, g, lambda, hot, c < iters, k, t) : nbin;
+        if (!chain_ok) t = 0.0;
         for (uint32_t h = 0; h < 2 && i + h < iters; ++h) {
             count += 32.0;
             const bool mine = half == h;
@@ -235,15 +287,33 @@ struct OuterFold {
     }
 };
 
+// Runs body(i) for i in [0, n) on up to 16 host threads.
+template <class F>
+void parallel_for(size_t n, F body) {
+    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    const size_t threads = std::min(hw, n / 64 + 1);
+    if (threads <= 1) {
+        for (size_t i = 0; i < n; ++i) body(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) body(i);
+        });
+    for (auto &th : pool) th.join();
+}
+
 }  // namespace
 
 // Launch one block of rows and copy the rows back.  rows_host receives
 // n_rows x bin_count values at row_out offsets.
-cpd_status Context::run_rows(bool chunked, const cpd_design *designs_dev, const std::vector<uint32_t> &row_design,
-                             const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
-                             const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays,
-                             uint32_t nbin_max, const double *z64_dev, std::vector<double> &rows_host,
-                             std::vector<uint64_t> &rays_host) {
+cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_dev,
+                             const std::vector<uint32_t> &row_design, const std::vector<double> &row_lambda,
+                             const std::vector<uint64_t> &row_out, const std::vector<uint32_t> &row_cap,
+                             uint64_t out_n, uint32_t max_rays, uint32_t nbin_max, const double *z64_dev,
+                             const float *z32_dev, std::vector<double> &rows_host, std::vector<uint64_t> &rays_host) {
     const uint32_t n_rows = (uint32_t)row_design.size();
     if (n_rows == 0) return CPD_OK;
     const size_t off_lam = (n_rows * sizeof(uint32_t) + 255) / 256 * 256;
@@ -270,20 +340,32 @@ cpd_status Context::run_rows(bool chunked, const cpd_design *designs_dev, const 
     a.out = row_buf.as<double>();
     a.row_rays = (uint64_t *)(row_buf.as<char>() + (out_n * sizeof(double) + 255) / 256 * 256);
     a.z64 = z64_dev;
+    a.z32 = z32_dev;
     a.n_rows = n_rows;
     a.max_rays = max_rays;
     a.nbin_max = nbin_max;
     const size_t lds = (size_t)3 * nbin_max * sizeof(double);
-    if (chunked)
-        hipLaunchKernelGGL(chunk_rows_kernel, dim3(n_rows), dim3(64), lds, stream, a);
+    if (profile) AD_TRY(hipEventRecord(ev[0], stream));
+    if (chunked && fp32)
+        hipLaunchKernelGGL(chunk_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
+    else if (chunked)
+        hipLaunchKernelGGL(chunk_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
+    else if (fp32)
+        hipLaunchKernelGGL(sem_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
     else
-        hipLaunchKernelGGL(sem_rows_kernel, dim3(n_rows), dim3(64), lds, stream, a);
+        hipLaunchKernelGGL(sem_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
     AD_TRY(hipGetLastError());
+    if (profile) AD_TRY(hipEventRecord(ev[1], stream));
     rows_host.resize(out_n);
     rays_host.resize(n_rows);
     AD_TRY(hipMemcpyAsync(rows_host.data(), a.out, out_n * sizeof(double), hipMemcpyDeviceToHost, stream));
     AD_TRY(hipMemcpyAsync(rays_host.data(), a.row_rays, n_rows * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     AD_TRY(hipStreamSynchronize(stream));
+    if (profile) {
+        float ms = 0.f;
+        AD_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        row_kernel_ms += ms;
+    }
     return CPD_OK;
 }
 
@@ -318,6 +400,8 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
     const cpd_design *ddev;
     uint32_t nbin_max;
     if ((st = stage_designs(designs, valid, &ddev, &nbin_max)) != CPD_OK) return st;
+    row_kernel_ms = 0.0;
+    double fold_ms = 0.0;
     std::vector<OuterFold> fold(n);
     std::vector<uint64_t> next_j(n, 1), block(n, 4);
     for (size_t q = 0; q < n; ++q) fold[q].reset(designs[valid[q]].bin_count);
@@ -345,10 +429,11 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
             }
         }
         if (row_design.empty()) break;
-        if ((st = run_rows(false, ddev, row_design, row_lambda, row_out, row_cap, out_n, max_m, nbin_max,
-                           z64.as<double>(), rows, rays)) != CPD_OK)
+        if ((st = run_rows(false, sem_fp32(), ddev, row_design, row_lambda, row_out, row_cap, out_n, max_m,
+                           nbin_max, z64.as<double>(), z32.as<float>(), rows, rays)) != CPD_OK)
             return st;
         // scan each design's rows in order
+        const auto f0 = std::chrono::steady_clock::now();
         for (size_t r = 0; r < row_design.size();) {
             const uint32_t q = row_design[r];
             size_t e = r;
@@ -363,6 +448,7 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
             block[q] = std::min<uint64_t>(block[q] * 4, 1024);
             r = e;
         }
+        fold_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
     }
     for (size_t q = 0; q < n; ++q) {
         const cpd_design &g = designs[valid[q]];
@@ -372,6 +458,8 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
     }
     last = cpd_run_stats{};
     last.rays = total_rays;
+    last.trace_ms = row_kernel_ms;  // with CPD_OPT_PROFILE
+    last.finalize_ms = fold_ms;     // the host fold over rows
     return CPD_OK;
 }
 
@@ -390,8 +478,8 @@ cpd_status Context::rows_adaptive(const cpd_design &g, const double *wavelengths
     std::vector<double> row_lambda(wavelengths, wavelengths + n_w), rows;
     std::vector<uint64_t> row_out(n_w), rays;
     for (size_t j = 0; j < n_w; ++j) row_out[j] = j * g.bin_count;
-    if ((st = run_rows(false, ddev, row_design, row_lambda, row_out, row_cap, n_w * g.bin_count, max_m, nbin_max,
-                       z64.as<double>(), rows, rays)) != CPD_OK)
+    if ((st = run_rows(false, sem_fp32(), ddev, row_design, row_lambda, row_out, row_cap, n_w * g.bin_count,
+                       max_m, nbin_max, z64.as<double>(), z32.as<float>(), rows, rays)) != CPD_OK)
         return st;
     memcpy(out, rows.data(), n_w * g.bin_count * sizeof(double));
     return CPD_OK;
@@ -413,6 +501,8 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
     const cpd_design *ddev;
     uint32_t nbin_max;
     if ((st = stage_designs(designs, valid, &ddev, &nbin_max)) != CPD_OK) return st;
+    row_kernel_ms = 0.0;
+    double fold_ms = 0.0;
     std::vector<OuterFold> fold(n);
     for (size_t q = 0; q < n; ++q) fold[q].reset(designs[valid[q]].bin_count);
     std::vector<uint32_t> row_design;
@@ -439,16 +529,21 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
         if ((st = upload_beam_table(stream, seed_fx, table_n, seed_z32.as<float>(), seed_z64.as<double>())) !=
             CPD_OK)
             return st;
-        if ((st = run_rows(true, ddev, row_design, row_lambda, row_out, std::vector<uint32_t>(), out_n, max_evals,
-                           nbin_max, seed_z64.as<double>(), rows, rays)) != CPD_OK)
+        if ((st = run_rows(true, chunk_fp32(), ddev, row_design, row_lambda, row_out, std::vector<uint32_t>(), out_n,
+                           max_evals, nbin_max, seed_z64.as<double>(), seed_z32.as<float>(), rows, rays)) != CPD_OK)
             return st;
-        for (size_t r = 0; r < row_design.size(); ++r) {
-            const uint32_t q = row_design[r];
-            total_rays += rays[r];
-            fold[q].fold(rows.data() + row_out[r]);
-        }
-        for (size_t q = 0; q < n; ++q)
-            if (!fold[q].done && fold[q].rows && fold[q].converged()) fold[q].done = true;
+        const auto f0 = std::chrono::steady_clock::now();
+        for (size_t r = 0; r < row_design.size(); ++r) total_rays += rays[r];
+        // each design's rows are contiguous and in id order; designs fold in parallel
+        std::vector<size_t> starts;
+        for (size_t r = 0; r < row_design.size(); r += n_rows_per) starts.push_back(r);
+        parallel_for(starts.size(), [&](size_t i) {
+            const size_t r0 = starts[i];
+            OuterFold &f = fold[row_design[r0]];
+            for (size_t r = r0; r < r0 + n_rows_per; ++r) f.fold(rows.data() + row_out[r]);
+            if (f.rows && f.converged()) f.done = true;
+        });
+        fold_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
     }
     for (size_t q = 0; q < n; ++q) {
         const cpd_design &g = designs[valid[q]];
@@ -459,6 +554,8 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
     }
     last = cpd_run_stats{};
     last.rays = total_rays;
+    last.trace_ms = row_kernel_ms;  // with CPD_OPT_PROFILE
+    last.finalize_ms = fold_ms;     // the host fold over rows
     return CPD_OK;
 }
 

@@ -40,6 +40,10 @@ struct Context {
     uint32_t tile_rays_hint = 0;  // rays per work item (0: the default)
     uint32_t order = CPD_ORDER_GIVEN;
     int acc_kind = CPD_ACC_EXACT;  // accumulation policy of fixed-count batches
+    int row_trace = 0;             // adaptive rows: 0 per-estimator default, 32 or 64
+    bool sem_fp32() const { return row_trace == 32; }
+    bool chunk_fp32() const { return row_trace != 64; }
+    double row_kernel_ms = 0.0;    // row kernels of the last adaptive run (CPD_OPT_PROFILE)
     bool profile = false;
     bool count_ops = false;
     cpd_run_stats last{};
@@ -71,10 +75,11 @@ struct Context {
     cpd_status run_gpu_fitness(const cpd_design *designs, const std::vector<uint32_t> &valid, const double *seeds,
                                size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval, cpd_fitness *out,
                                cpd_status *per_design);
-    cpd_status run_rows(bool chunked, const cpd_design *designs_dev, const std::vector<uint32_t> &row_design,
+    cpd_status run_rows(bool chunked, bool fp32, const cpd_design *designs_dev, const std::vector<uint32_t> &row_design,
                         const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
                         const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays, uint32_t nbin_max,
-                        const double *z64_dev, std::vector<double> &rows_host, std::vector<uint64_t> &rays_host);
+                        const double *z64_dev, const float *z32_dev, std::vector<double> &rows_host,
+                        std::vector<uint64_t> &rays_host);
     cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
                              const cpd_design **designs_dev, uint32_t *nbin_max);
     void apply(const cpd_schedule &s);

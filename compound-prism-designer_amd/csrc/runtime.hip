@@ -478,6 +478,11 @@ cpd_status cpd_ctx_set_option(cpd_ctx *ctx, int option, int64_t value) {
                 return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown accumulation %lld", (long long)value);
             c->acc_kind = (int)value;
             return CPD_OK;
+        case CPD_OPT_ROW_TRACE:
+            if (value != 0 && value != 32 && value != 64)
+                return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "row trace precision must be 0, 32 or 64");
+            c->row_trace = (int)value;
+            return CPD_OK;
         case CPD_OPT_ORDER:
             if (value != CPD_ORDER_GIVEN && value != CPD_ORDER_COST)
                 return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "unknown order %lld", (long long)value);

@@ -226,7 +226,11 @@ enum {
     CPD_OPT_TILE_RAYS = 3,     /* rays per work item, multiple of 64, <= 2^20 (0: 65536) */
     CPD_OPT_WAVES_PER_CU = 4,  /* resident wavefronts per CU of the persistent grid */
     CPD_OPT_ORDER = 5,         /* work order of a batch, CPD_ORDER_* */
-    CPD_OPT_ACCUMULATE = 6     /* per-bin accumulation of fixed-count batches, CPD_ACC_* */
+    CPD_OPT_ACCUMULATE = 6,    /* per-bin accumulation of fixed-count batches, CPD_ACC_* */
+    CPD_OPT_ROW_TRACE = 7      /* ray trace of the adaptive estimators: 64 = FP64 in the reference's
+                                  operation order for every ray; 32 = the FP32 trace with its FP64
+                                  re-trace of flagged rays; 0 = 64 for CPD_MODE_ADAPTIVE rows and
+                                  32 for cpd_gpu_fitness (the reference's GPU estimator is FP32) */
 };
 /* Accumulation of the per-bin transmittance (a precision study; results of
  * the default are exact and order-independent):

@@ -1,10 +1,14 @@
-"""The reference's adaptive estimators on the GPU against the FP64 oracle.
+"""The reference's adaptive estimators on the GPU against the FP64 oracle:
+fitness() with the SEM early exits (src/fitness.rs:5-34, 79-123) and the GPU
+estimator cuda_fitness (src/cuda_fitness.rs:347-423 with
+src/spectrometer/kernel.rs:89-193).
 
-Both trace every ray in FP64 in the reference's operation order from the
-same beam quantiles and fold in FP64 in the reference's order, so results
-are compared for equality: fitness() with the SEM early exits
-(src/fitness.rs:5-34, 79-123) and the GPU estimator cuda_fitness
-(src/cuda_fitness.rs:347-423 with src/spectrometer/kernel.rs:89-193)."""
+With CPD_OPT_ROW_TRACE = 64 every ray is traced in FP64 in the reference's
+operation order from the same beam quantiles and folded in FP64 in the
+reference's order, so results are compared for equality.  With the FP32
+trace (the GPU estimator's default) decisions stay the FP64 ones and each
+ray's transmittance is rounded to FP32: information is compared within
+INFO_TOL = 1e-4 bits, size and deviation for equality."""
 import ctypes
 
 import numpy as np
@@ -15,7 +19,7 @@ from compound_prism_designer import _native as N
 from oracle import oracle as O
 import kat
 
-from test_gpu_fitness import kat_design, to_oracle
+from test_gpu_fitness import INFO_TOL, kat_design, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -65,34 +69,69 @@ def test_transmission_probability_adaptive_matches_oracle():
         assert np.array_equal(out[j], ref), j
 
 
-def test_gpu_fitness_matches_oracle_on_known_design():
+def fp64_rows_context():
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ROW_TRACE, 64)
+    return ctx
+
+
+@pytest.mark.parametrize("trace", [64, 32])
+def test_gpu_fitness_matches_oracle_on_known_design(trace):
     # benches/benchmark.rs:147-157's call: seeds [0.798713], max_n 256, nwarp 2, max_eval 16384
     d = kat_design()
     f = N.Fitness()
     seeds = np.array([0.798713])
-    st = N.lib.cpd_gpu_fitness(None, ctypes.byref(d), seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1,
-                               256, 2, 16384, ctypes.byref(f))
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ROW_TRACE, trace)
+    st = N.lib.cpd_gpu_fitness(ctx.handle, ctypes.byref(d), seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               1, 256, 2, 16384, ctypes.byref(f))
     assert st == 0, N.last_error()
     ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 256, 2, 16384)
     assert converged
-    assert (f.size, f.info, f.deviation) == ref
+    if trace == 64:
+        assert (f.size, f.info, f.deviation) == ref
+    else:
+        assert (f.size, f.deviation) == (ref[0], ref[2]) and abs(f.info - ref[1]) <= INFO_TOL
 
 
 def test_gpu_fitness_not_converged_is_none_like_the_reference():
     d = kat_design()
     seeds = np.array([0.798713, 0.25])
-    got, st = cpd.gpu_fitness_batch([d], seeds, max_n=4, nwarp=2, max_eval=16384)
+    got, st = cpd.gpu_fitness_batch([d], seeds, max_n=4, nwarp=2, max_eval=16384, ctx=fp64_rows_context())
     ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 4, 2, 16384)
     assert not converged and st[0] == N.ERR_NOT_CONVERGED
     assert tuple(got[0]) == ref
+    spec_like, st32 = cpd.gpu_fitness_batch([d], seeds, max_n=4, nwarp=2, max_eval=16384)
+    assert st32[0] == N.ERR_NOT_CONVERGED
 
 
-def test_gpu_fitness_batch_matches_single_designs():
+@pytest.mark.parametrize("trace", [64, 32])
+def test_gpu_fitness_batch_matches_single_designs(trace):
     _, D, _, _ = cpd.generate_designs(0, 200, 12)
     D = list(D) + [kat_design()]
     seeds = np.array([0.1, 0.7])
-    got, st = cpd.gpu_fitness_batch(D, seeds, max_n=64, nwarp=2, max_eval=4096)
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ROW_TRACE, trace)
+    got, st = cpd.gpu_fitness_batch(D, seeds, max_n=64, nwarp=2, max_eval=4096, ctx=ctx)
+    agree = 0
     for i, d in enumerate(D):
         ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 64, 2, 4096)
-        assert st[i] == (0 if converged else N.ERR_NOT_CONVERGED)
-        assert tuple(got[i]) == ref, (i, got[i], ref)
+        if trace == 64:
+            assert st[i] == (0 if converged else N.ERR_NOT_CONVERGED)
+            assert tuple(got[i]) == ref, (i, got[i], ref)
+        else:
+            agree += st[i] == (0 if converged else N.ERR_NOT_CONVERGED)
+            assert abs(got[i][1] - ref[1]) <= INFO_TOL, (i, got[i], ref)
+    # a design whose bin statistics sit on the SEM threshold may converge on one side only
+    assert trace == 64 or agree >= len(D) - 1
+
+
+def test_fp32_rows_fitness_close_to_oracle():
+    _, D, _, _ = cpd.generate_designs(0, 0, 12)
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ROW_TRACE, 32)
+    got, st = cpd.fitness_batch(D, N.Sampling(256, 4096, N.MODE_ADAPTIVE, 0), ctx=ctx)
+    assert (st == 0).all()
+    for i, d in enumerate(D):
+        ref, _ = O.fitness(to_oracle(d), 256, 4096, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+        assert abs(got[i][1] - ref[1]) <= INFO_TOL, (i, got[i], ref)
