@@ -9,9 +9,9 @@
 
 namespace cpd {
 
-// Fixed-point scale of the per-bin transmittance sums: a ray adds
-// round(t * 2^22) to its bin, so sums are exact integers and independent of
-// the order in which rays and tiles are added.
+// Fixed-point scale of the per-bin transmittance sums of the default (exact)
+// accumulation: a ray adds round(t * 2^22) to its bin, so sums are exact
+// integers and independent of the order in which rays and tiles are added.
 static constexpr double kQuantScale = 4194304.0;
 static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 re-trace
 
@@ -22,9 +22,10 @@ static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 r
 // item_offset[e] + 0, 1, ... of its design (offset 0 when item_offset is
 // null).  Item i of a design is wavelength sample j = i / n_tiles and ray
 // tile i % n_tiles of tile_rays rays (both from its trace record).
-// Per-(design, wavelength, bin) sums
-// live at acc[acc_begin[slot] + j * nbin + b]; the bin of ray k = n_rays (the
-// last ray, which the reference counts once more, src/fitness.rs:20-32) at
+// Per-(design, wavelength, bin) sums live in 8-byte cells at
+// acc[acc_begin[slot] + j * nbin + b] (u64, or the FP32 / FP64 value of the
+// other accumulation policies); the bin of ray k = n_rays (the last ray,
+// which the reference counts once more, src/fitness.rs:20-32) at
 // last_bin[lb_begin[slot] + j].
 struct BatchArgs {
     const cpd_design *designs;

@@ -1,24 +1,29 @@
 // kernels.hip -- gfx950 kernels of the compound-prism fitness evaluator.
 //
 // trace_kernel: one 64-lane wavefront per work item (design, wavelength
-// sample, tile of rays).  Lane l traces rays k = k0 + l, k0 + 64 + l, ... of
-// the tile in FP32 (trace32) and adds round(t * 2^22) to its own column of a
-// [bin][column] LDS histogram, so no two lanes ever contend for a bin.  Rays
-// whose FP32 outcome lies within the error margin of a hit/miss or bin
-// decision go to a per-wave LDS queue and are re-traced 64 at a time in FP64
-// (trace64, the reference's operation order).  At the end of an item the
-// columns are summed (skewed so the 64 lanes hit 32 distinct banks) into
-// 64-bit integer sums, which are added to the design's per-(wavelength, bin)
-// accumulators with integer atomics -- exact, so the result is the same
-// whatever the tile order or the batch a design shares the GPU with.
+// sample, tile of rays), taken from an atomic counter by a persistent grid.
+// Per item the wavefront builds the wavelength's constants in FP64 (the
+// planar faces collapse to an interval of beam heights and an affine exit
+// point), copies them into scalar registers, and lane l traces rays
+// k = k0 + l, k0 + 64 + l, ... of the tile in FP32 (trace32), adding each
+// hit to its own column of a [bin][column] LDS histogram.  Rays whose FP32
+// outcome lies within the error margin of a hit/miss or bin decision go to a
+// per-wave LDS queue and are re-traced 64 at a time in FP64 (trace64, the
+// reference's operation order).  With the default accumulation a ray adds
+// round(t * 2^22) to a u32 column; the columns are folded into a u64 row
+// before they can overflow (skewed, so the lanes hit distinct banks), and
+// the row is added to the design's per-(wavelength, bin) sums with integer
+// atomics -- exact, so the result is the same whatever the tile order or the
+// batch a design shares the GPU with.  The FP32 and FP64 accumulation
+// policies of the precision study replace the cells and the atomics.
 //
-// finalize_kernel: one wavefront per design turns the integer sums into
-// p(d|l) = sum / (2^22 M) (the bin of the last ray divides by M + 1, as the
+// finalize_kernel: one wavefront per design turns the sums into
+// p(d|l) = sum / M (the bin of the last ray divides by M + 1, as the
 // reference's Welford padding does, src/fitness.rs:20-32), and reduces
-// H(D|L) = mean_l sum_d p log2 p and H(D) = sum_d pbar log2 pbar in FP64 with
+// H(D|L) = mean_l sum_d p log2 p and H(D) = sum_d pbar log2 pbar with
 // Neumaier-compensated sums and a fixed reduction order
-// (src/fitness.rs:79-123).  It clears the accumulators it read, so the next
-// launch starts from zero without a memset.
+// (src/fitness.rs:79-123).  It clears the sums it read, so the next launch
+// starts from zero without a memset.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
