@@ -255,6 +255,14 @@ class CompoundPrism:
         a = self.angles[-1]
         return Vector2D(d.arc_midpt[0], d.arc_midpt[1]), UnitVector2D(math.cos(a), math.sin(a))
 
+    def surfaces(self):
+        """Drawing geometry (src/pylib.rs:795-808); drawing is not part of this build."""
+        raise NotImplementedError("CompoundPrism.surfaces (drawing) is not part of the MI355X evaluator")
+
+    def polygons(self):
+        """matplotlib Paths of the prisms (src/pylib.rs:578-630); drawing is not part of this build."""
+        raise NotImplementedError("CompoundPrism.polygons (drawing) is not part of the MI355X evaluator")
+
     def __getnewargs__(self):
         return (self.glasses, self.angles, self.lengths, self.curvature, self.height, self.width, self.ar_coated)
 
