@@ -430,7 +430,7 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
         }
         if (row_design.empty()) break;
         if ((st = run_rows(false, sem_fp32(), ddev, row_design, row_lambda, row_out, row_cap, out_n, max_m,
-                           nbin_max, z64.as<double>(), z32.as<float>(), rows, rays)) != CPD_OK)
+                           nbin_max, beam.z64.as<double>(), beam.z32.as<float>(), rows, rays)) != CPD_OK)
             return st;
         // scan each design's rows in order
         const auto f0 = std::chrono::steady_clock::now();
@@ -479,7 +479,7 @@ cpd_status Context::rows_adaptive(const cpd_design &g, const double *wavelengths
     std::vector<uint64_t> row_out(n_w), rays;
     for (size_t j = 0; j < n_w; ++j) row_out[j] = j * g.bin_count;
     if ((st = run_rows(false, sem_fp32(), ddev, row_design, row_lambda, row_out, row_cap, n_w * g.bin_count,
-                       max_m, nbin_max, z64.as<double>(), z32.as<float>(), rows, rays)) != CPD_OK)
+                       max_m, nbin_max, beam.z64.as<double>(), beam.z32.as<float>(), rows, rays)) != CPD_OK)
         return st;
     memcpy(out, rows.data(), n_w * g.bin_count * sizeof(double));
     return CPD_OK;

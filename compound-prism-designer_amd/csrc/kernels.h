@@ -39,6 +39,9 @@ struct BatchArgs {
     uint32_t *last_bin;
     const float *z32;            // beam quantiles z_k = norminv(u_k), k = 1..n, FP32
     const double *z64;           // the same in FP64
+    const float *zs32;           // banded: within blocks of kBandRays rays sorted by z
+    const double *zs64;
+    const uint32_t *inv;         // 1-based banded position of ray k (index k - 1)
     unsigned long long *ops;     // algorithmic operation count (may be null)
     unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)

@@ -134,8 +134,12 @@ struct TileCtx {
     void *row;       // the wavefront's row per bin
     uint32_t *queue;
     uint32_t *last_bin;
-    const float *z32;
+    const float *z32;    // ray order
     const double *z64;
+    const float *zs32;   // banded order
+    const double *zs64;
+    uint32_t full_end;   // positions 1..full_end are banded, the rest in ray order
+    uint32_t last_pos;   // position of ray k = n_rays
 };
 
 // FP64 re-trace of `count` (<= 64) queued rays starting at ring slot `head`.
@@ -144,12 +148,12 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
     using Cell = typename Acc<A>::Cell;
     const uint32_t lane = threadIdx.x;
     if (lane < count) {
-        const uint32_t k = c.queue[(head + lane) & (kQueueCap - 1)];
-        const double y = beam_y64(*c.g, c.z64[k - 1]);
+        const uint32_t i = c.queue[(head + lane) & (kQueueCap - 1)];  // position
+        const double y = beam_y64(*c.g, i <= c.full_end ? c.zs64[i - 1] : c.z64[i - 1]);
         double t;
         const uint32_t b = trace64(*c.g, c.lc.lambda, y, t);
         if (b < c.nbin) atomicAdd((Cell *)c.hist + b * c.cols + (lane & (c.cols - 1)), Acc<A>::of64(t));
-        if (k == c.n_rays) *c.last_bin = b;
+        if (i == c.last_pos) *c.last_bin = b;
     }
 }
 
@@ -196,23 +200,24 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     // register copies of the per-item records: the loop stores through
     // last_bin, so loads through c.td would be repeated every iteration
     const HotRecord hot = to_sgprs(make_hot(*c.td, c.lc));
-    const float *__restrict__ z32 = c.z32;
     Cell *__restrict__ hist = (Cell *)c.hist;
-    const uint32_t k_end = c.k_end, n_rays = c.n_rays, nbin = c.nbin, cols = c.cols;
+    const uint32_t k_end = c.k_end, nbin = c.nbin, cols = c.cols, full_end = c.full_end, last_pos = c.last_pos;
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
     const uint32_t fold = Acc<A>::fold_every(cols);
     uint32_t since_fold = 0;
     for (uint32_t base = c.k_begin; base < k_end; base += 64) {
-        const uint32_t k = base + lane;
+        // positions base .. base + 63 are all banded or all in ray order
+        const float *__restrict__ zp = base + 63 <= full_end ? c.zs32 : c.z32;
+        const uint32_t k = base + lane;  // position in the tables
         const bool active = k < k_end;
-        const float z = active ? z32[k - 1] : 0.f;
+        const float z = active ? zp[k - 1] : 0.f;
         const float y = hot.y_mean - hot.beam_w * z;
         const Trace32Result r = trace32<COUNT>(hot, y);
         if (COUNT && active) ops += r.ops;
         const bool flagged = active && r.guard;
         if (active && !r.guard) {
             if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
-            if (k == n_rays) *c.last_bin = r.bin;
+            if (k == last_pos) *c.last_bin = r.bin;
         }
         const uint64_t mask = __ballot(flagged);
         if (mask) {
@@ -337,6 +342,10 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.last_bin = a.last_bin + a.lb_begin[d] + j;
         c.z32 = a.z32;
         c.z64 = a.z64;
+        c.zs32 = a.zs32;
+        c.zs64 = a.zs64;
+        c.full_end = M / kBandRays * kBandRays;
+        c.last_pos = M <= c.full_end ? a.inv[M - 1] : M;
         if (!c.lc.ok) {
             // every ray of this wavelength is lost in the planar faces
             if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;

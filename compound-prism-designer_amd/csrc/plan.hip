@@ -42,7 +42,8 @@ struct Plan {
     std::vector<cpd_design> shape;      // per slot: a valid record of the slot's shape
     std::vector<cpd_sampling> sampling;  // per slot
     std::vector<uint32_t> perm;         // staging position -> slot
-    DeviceBuffer d_fixed, d_designs, d_traces, d_acc, d_last_bin, d_results, d_counter, d_z32, d_z64;
+    DeviceBuffer d_fixed, d_designs, d_traces, d_acc, d_last_bin, d_results, d_counter;
+    BeamTables beam;
     HostBuffer h_designs, h_out;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -65,8 +66,7 @@ Plan::~Plan() {
     d_last_bin.release();
     d_results.release();
     d_counter.release();
-    d_z32.release();
-    d_z64.release();
+    beam.release();
     h_designs.release();
     h_out.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -140,8 +140,6 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     if ((st = d_last_bin.reserve(lb_n * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
     if ((st = d_results.reserve(n * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
     if ((st = d_counter.reserve(256, true, stream)) != CPD_OK) return st;
-    if ((st = d_z32.reserve((size_t)max_rays * sizeof(float), false, stream)) != CPD_OK) return st;
-    if ((st = d_z64.reserve((size_t)max_rays * sizeof(double), false, stream)) != CPD_OK) return st;
     if ((st = h_designs.reserve(n * sizeof(cpd_design))) != CPD_OK) return st;
     if ((st = h_out.reserve(n * sizeof(cpd_fitness))) != CPD_OK) return st;
     char *fx = d_fixed.as<char>();
@@ -150,7 +148,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
                             stream));
     PLAN_TRY(hipMemcpyAsync(fx + off_acc, acc_begin.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
     PLAN_TRY(hipMemcpyAsync(fx + off_lb, lb_begin.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
-    if ((st = upload_beam_table(stream, kHalfFx, max_rays, d_z32.as<float>(), d_z64.as<double>())) != CPD_OK) return st;
+    if ((st = beam.ensure(max_rays, stream)) != CPD_OK) return st;
     PLAN_TRY(hipStreamSynchronize(stream));  // host vectors above go out of scope
 
     BatchArgs a{};
@@ -161,8 +159,11 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     a.lb_begin = (const uint64_t *)(fx + off_lb);
     a.acc = d_acc.as<unsigned long long>();
     a.last_bin = d_last_bin.as<uint32_t>();
-    a.z32 = d_z32.as<float>();
-    a.z64 = d_z64.as<double>();
+    a.z32 = beam.z32.as<float>();
+    a.z64 = beam.z64.as<double>();
+    a.zs32 = beam.zs32.as<float>();
+    a.zs64 = beam.zs64.as<double>();
+    a.inv = beam.inv.as<uint32_t>();
     a.work_counter = d_counter.as<unsigned int>();
     a.n_designs = n;
     a.hist_words = hist_words;

@@ -29,6 +29,19 @@ struct HostBuffer {
     void release();
 };
 
+// Beam quantiles z_k = norminv(u_k), k = 1..n, of the R1 sequence from 0.5:
+// in ray order (FP32, FP64), and banded: within each block of kBandRays rays
+// sorted by z (so by beam height), with inv[k - 1] the 1-based position of
+// ray k in the banded tables.  A wavefront that traces 64 consecutive banded
+// positions covers a narrow band of heights, so whole wavefronts miss
+// together; the per-bin sums do not depend on the order rays are added in.
+struct BeamTables {
+    uint32_t n = 0;
+    DeviceBuffer z32, z64, zs32, zs64, inv;
+    cpd_status ensure(uint32_t want, hipStream_t s);  // grow-only, at least 65536 rays
+    void release();
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -49,9 +62,7 @@ struct Context {
     cpd_run_stats last{};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
-    // beam quantile tables z_k, k = 1..z_n
-    uint32_t z_n = 0;
-    DeviceBuffer z32, z64;
+    BeamTables beam;
     // batch state
     DeviceBuffer stage, acc, last_bin, results, counters, row_buf, prop_buf;
     // adaptive estimators: row descriptors, design records, seeded beam tables

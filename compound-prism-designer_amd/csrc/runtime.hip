@@ -79,17 +79,51 @@ cpd_status upload_beam_table(hipStream_t s, uint64_t seed_fx, uint32_t n, float 
     return CPD_OK;
 }
 
-cpd_status Context::ensure_beam_table(uint32_t n) {
-    if (n <= z_n) return CPD_OK;
-    uint32_t want = std::max<uint32_t>(n, 65536);
-    cpd_status st = z32.reserve((size_t)want * sizeof(float), false, stream);
-    if (st != CPD_OK) return st;
-    st = z64.reserve((size_t)want * sizeof(double), false, stream);
-    if (st != CPD_OK) return st;
-    if ((st = upload_beam_table(stream, kHalfFx, want, z32.as<float>(), z64.as<double>())) != CPD_OK) return st;
-    z_n = want;
+cpd_status BeamTables::ensure(uint32_t want, hipStream_t s) {
+    if (want <= n) return CPD_OK;
+    const uint32_t m = std::max<uint32_t>(want, 65536);
+    cpd_status st;
+    if ((st = z32.reserve((size_t)m * sizeof(float), false, s)) != CPD_OK) return st;
+    if ((st = z64.reserve((size_t)m * sizeof(double), false, s)) != CPD_OK) return st;
+    if ((st = zs32.reserve((size_t)m * sizeof(float), false, s)) != CPD_OK) return st;
+    if ((st = zs64.reserve((size_t)m * sizeof(double), false, s)) != CPD_OK) return st;
+    if ((st = inv.reserve((size_t)m * sizeof(uint32_t), false, s)) != CPD_OK) return st;
+    std::vector<double> z(m), zs(m);
+    std::vector<float> zf(m), zsf(m);
+    std::vector<uint32_t> order(m), pos(m);
+    for (uint32_t i = 0; i < m; ++i) {
+        z[i] = norminv64(qmc_seeded(kHalfFx, (uint64_t)i + 1));
+        zf[i] = (float)z[i];
+        order[i] = i;
+    }
+    for (uint32_t b = 0; b + kBandRays <= m; b += kBandRays)
+        std::sort(order.begin() + b, order.begin() + b + kBandRays,
+                  [&](uint32_t x, uint32_t y) { return z[x] < z[y] || (z[x] == z[y] && x < y); });
+    for (uint32_t i = 0; i < m; ++i) {
+        zs[i] = z[order[i]];
+        zsf[i] = zf[order[i]];
+        pos[order[i]] = i + 1;
+    }
+    HIP_TRY(hipMemcpyAsync(z64.ptr, z.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(z32.ptr, zf.data(), (size_t)m * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(zs64.ptr, zs.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(zs32.ptr, zsf.data(), (size_t)m * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(inv.ptr, pos.data(), (size_t)m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    n = m;
     return CPD_OK;
 }
+
+void BeamTables::release() {
+    z32.release();
+    z64.release();
+    zs32.release();
+    zs64.release();
+    inv.release();
+    n = 0;
+}
+
+cpd_status Context::ensure_beam_table(uint32_t n) { return beam.ensure(n, stream); }
 
 namespace {
 
@@ -293,8 +327,11 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.lb_begin = (const uint64_t *)(dstage + off_lb);
     a.acc = acc.as<unsigned long long>();
     a.last_bin = last_bin.as<uint32_t>();
-    a.z32 = z32.as<float>();
-    a.z64 = z64.as<double>();
+    a.z32 = beam.z32.as<float>();
+    a.z64 = beam.z64.as<double>();
+    a.zs32 = beam.zs32.as<float>();
+    a.zs64 = beam.zs64.as<double>();
+    a.inv = beam.inv.as<uint32_t>();
     a.ops = want_ops ? dops : nullptr;
     a.wavelengths = n_wl ? (const double *)(dstage + off_wl) : nullptr;
     a.n_designs = n;
@@ -360,8 +397,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
 Context::~Context() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    z32.release();
-    z64.release();
+    beam.release();
     stage.release();
     acc.release();
     last_bin.release();

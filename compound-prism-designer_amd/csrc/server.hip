@@ -119,10 +119,9 @@ struct Server {
     uint32_t step_items = 0;
     uint32_t waves_per_cu = 16;
     uint64_t max_designs = 0, acc_cap = 0, lb_cap = 0;
-    DeviceBuffer d_designs, d_traces, d_acc_begin, d_lb_begin, d_acc, d_last_bin, d_results, d_z32, d_z64, d_step,
-        d_counter;
+    DeviceBuffer d_designs, d_traces, d_acc_begin, d_lb_begin, d_acc, d_last_bin, d_results, d_step, d_counter;
+    BeamTables beam;
     HostBuffer h_stage, h_results;
-    uint32_t z_n = 0;
     RangeAlloc slots, accs, lbs;
 
     std::mutex mu;
@@ -206,16 +205,7 @@ void Server::release(Request &r) {
     lbs.release(r.lb0, r.lb_n);
 }
 
-cpd_status Server::ensure_tables(uint32_t n) {
-    if (n <= z_n) return CPD_OK;
-    const uint32_t want = std::max<uint32_t>(n, 65536);
-    cpd_status st;
-    if ((st = d_z32.reserve((size_t)want * sizeof(float), false, stream)) != CPD_OK) return st;
-    if ((st = d_z64.reserve((size_t)want * sizeof(double), false, stream)) != CPD_OK) return st;
-    if ((st = upload_beam_table(stream, kHalfFx, want, d_z32.as<float>(), d_z64.as<double>())) != CPD_OK) return st;
-    z_n = want;
-    return CPD_OK;
-}
+cpd_status Server::ensure_tables(uint32_t n) { return beam.ensure(n, stream); }
 
 // Copy the records of newly admitted requests into their pool slots.
 cpd_status Server::upload(const std::vector<std::shared_ptr<Request>> &fresh) {
@@ -353,8 +343,11 @@ cpd_status Server::step() {
     a.lb_begin = d_lb_begin.as<uint64_t>();
     a.acc = d_acc.as<unsigned long long>();
     a.last_bin = d_last_bin.as<uint32_t>();
-    a.z32 = d_z32.as<float>();
-    a.z64 = d_z64.as<double>();
+    a.z32 = beam.z32.as<float>();
+    a.z64 = beam.z64.as<double>();
+    a.zs32 = beam.zs32.as<float>();
+    a.zs64 = beam.zs64.as<double>();
+    a.inv = beam.inv.as<uint32_t>();
     a.work_counter = d_counter.as<unsigned int>();
     a.n_designs = (uint32_t)E;
     a.hist_words = hist_words;
@@ -570,8 +563,9 @@ void cpd_server_destroy(cpd_server *srv) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (cpd::DeviceBuffer *b : {&s->d_designs, &s->d_traces, &s->d_acc_begin, &s->d_lb_begin, &s->d_acc,
-                                 &s->d_last_bin, &s->d_results, &s->d_z32, &s->d_z64, &s->d_step, &s->d_counter})
+                                 &s->d_last_bin, &s->d_results, &s->d_step, &s->d_counter})
         b->release();
+    s->beam.release();
     s->h_stage.release();
     s->h_results.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);

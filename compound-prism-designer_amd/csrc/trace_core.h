@@ -246,6 +246,9 @@ static constexpr uint32_t kHistWords = 2048;
 // schedule caps it.
 static constexpr uint32_t kMaxTileRays = 1u << 20;
 static constexpr uint32_t kDefaultTileRays = 65536;
+// Rays 1..kBandRays, kBandRays + 1..2 kBandRays, ... form the blocks within
+// which the banded beam tables are sorted by height.
+static constexpr uint32_t kBandRays = 4096;
 CPD_HD constexpr uint32_t hist_cols(uint32_t nbin) {
     uint32_t c = 64;
     while (c > 1 && c * nbin > kHistWords) c >>= 1;
@@ -564,6 +567,8 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     bool guard = (fabsf(y - d.ylo) <= ymarg) | (fabsf(y - d.yhi) <= ymarg);
     const float ex = fmaf(d.e1x, y, d.e0x), ey = fmaf(d.e1y, y, d.e0y);
     if (COUNT) res.ops = kOpsBeam + kOpsChain + (alive ? kOpsLensHit : 0u);
+    // wavefronts of banded rays often lie wholly outside the accepted heights
+    if (!wave_any(alive | guard)) return res;
     // exit lens, relative to its chord midpoint m: e = o - m, c = m + cm.
     // |e + s dir - cm|^2 = R^2  <=>  s^2 + 2 b s + c = 0; the root of larger
     // magnitude is -(b + sign(b) sqrt(disc)) and the other one is c over it,
