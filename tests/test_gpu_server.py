@@ -63,3 +63,48 @@ def test_server_reports_invalid_designs_per_request():
         res, status, _ = srv.wait(srv.submit(designs, samp), 60000)
     assert status[2] == N.ERR_INVALID_ARGUMENT and np.isnan(res[2]).all()
     assert (np.delete(status, 2) == 0).all()
+
+
+def test_server_concurrent_submitters_match_standalone_batches():
+    # four threads submit 32 requests of 1..48 designs between them while the
+    # server runs; every result equals the standalone batch of its designs
+    import threading
+    reqs = small_requests(list(np.random.default_rng(7).integers(1, 49, 32)), seed=7)
+    ref = [cpd.fitness_batch(d, s)[0] for d, s in reqs]
+    got = [None] * len(reqs)
+    with cpd.Server(step_items=4096) as srv:
+        def worker(w):
+            for i in range(w, len(reqs), 4):
+                rid = srv.submit(*reqs[i])
+                got[i] = srv.wait(rid, 60000)
+        threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        stats = srv.stats()
+    for (res, status, _), want in zip(got, ref):
+        assert (status == 0).all()
+        assert np.array_equal(res, want)
+    assert stats["completed"] == len(reqs)
+
+
+def test_two_contexts_in_parallel_threads():
+    import threading
+    _, D, S, _ = cpd.generate_designs(2, 300, 64)
+    samp = [N.Sampling(s.n_wavelengths // 8 + 1, s.n_rays // 16 // 64 * 64 + 64, N.MODE_FIXED, 0) for s in S]
+    ref = cpd.fitness_batch(D, samp)[0]
+    out = {}
+
+    def run(tag, lo, hi):
+        ctx = cpd.Context(0)
+        for _ in range(3):
+            out[tag] = cpd.fitness_batch(D[lo:hi], samp[lo:hi], ctx=ctx)[0]
+        ctx.close()
+
+    threads = [threading.Thread(target=run, args=("a", 0, 32)), threading.Thread(target=run, args=("b", 32, 64))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert np.array_equal(np.vstack([out["a"], out["b"]]), ref)
