@@ -53,6 +53,21 @@ __device__ __forceinline__ T to_sgprs(const T &v) {
     return out;
 }
 
+// A wave-uniform value held in a vector register.  A vector instruction can
+// read only one scalar register, so an operand that meets another uniform
+// operand in one instruction (fma(e1, y, e0), for instance) would otherwise
+// be copied into a vector register at every use; the empty asm makes the
+// compiler treat the value as varying per lane, which keeps the single copy
+// made here in a vector register for the whole loop.  It also takes the
+// value out of the scalar register file, which the ray loop otherwise fills
+// until it spills.
+template <class T>
+__device__ __forceinline__ T in_vgpr(T v) {
+    static_assert(sizeof(T) == 4, "32-bit value");
+    asm("; uniform value held in %0" : "+v"(v));
+    return v;
+}
+
 __device__ __forceinline__ uint32_t find_design(const uint64_t *begin, uint32_t n, uint64_t item) {
     uint32_t lo = 0, hi = n;  // begin[lo] <= item < begin[hi]
     while (hi - lo > 1) {
@@ -199,7 +214,28 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     const uint32_t col = lane & (c.cols - 1);
     // register copies of the per-item records: the loop stores through
     // last_bin, so loads through c.td would be repeated every iteration
-    const HotRecord hot = to_sgprs(make_hot(*c.td, c.lc));
+    HotRecord hot = to_sgprs(make_hot(*c.td, c.lc));
+    // operands paired with another uniform operand, the miss bin, and the
+    // detector stage's constants (which would otherwise be spilled)
+    hot.y_mean = in_vgpr(hot.y_mean);
+    hot.e0x = in_vgpr(hot.e0x);
+    hot.e0y = in_vgpr(hot.e0y);
+    hot.lb0 = in_vgpr(hot.lb0);
+    hot.lc1 = in_vgpr(hot.lc1);
+    hot.lomr2 = in_vgpr(hot.lomr2);
+    hot.nbin = in_vgpr(hot.nbin);
+    hot.ts = in_vgpr(hot.ts);
+    hot.tp = in_vgpr(hot.tp);
+    hot.mpx = in_vgpr(hot.mpx);
+    hot.mpy = in_vgpr(hot.mpy);
+    hot.ddx = in_vgpr(hot.ddx);
+    hot.ddy = in_vgpr(hot.ddy);
+    hot.half_len = in_vgpr(hot.half_len);
+    hot.slope = in_vgpr(hot.slope);
+    hot.inv_slope = in_vgpr(hot.inv_slope);
+    hot.intercept = in_vgpr(hot.intercept);
+    hot.bin_size = in_vgpr(hot.bin_size);
+    hot.nbin_f = in_vgpr(hot.nbin_f);
     Cell *__restrict__ hist = (Cell *)c.hist;
     const uint32_t k_end = c.k_end, nbin = c.nbin, cols = c.cols, full_end = c.full_end, last_pos = c.last_pos;
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
@@ -210,7 +246,9 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         const float *__restrict__ zp = base + 63 <= full_end ? c.zs32 : c.z32;
         const uint32_t k = base + lane;  // position in the tables
         const bool active = k < k_end;
-        const float z = active ? zp[k - 1] : 0.f;
+        // a 32-bit byte offset from a scalar base: one load, no 64-bit address
+        // arithmetic per lane (the tables hold at most 2^30 rays)
+        const float z = active ? *(const float *)((const char *)zp + ((k - 1u) << 2)) : 0.f;
         const float y = hot.y_mean - hot.beam_w * z;
         const Trace32Result r = trace32<COUNT>(hot, y);
         if (COUNT && active) ops += r.ops;
@@ -219,7 +257,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
             if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
             if (k == last_pos) *c.last_bin = r.bin;
         }
-        const uint64_t mask = __ballot(flagged);
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
         if (mask) {
             if (flagged) c.queue[(tail + __popcll(mask & lanemask_lt())) & (kQueueCap - 1)] = k;
             tail += __popcll(mask);

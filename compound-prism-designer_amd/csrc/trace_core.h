@@ -462,7 +462,7 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
 CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // Whether any lane of the wavefront holds v (one lane on the host).
-CPD_HD bool wave_any(bool v) { return __ballot(v) != 0; }
+CPD_HD bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
 #else
 CPD_HD float fast_rcp(float x) { return 1.0f / x; }
 CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
