@@ -71,6 +71,14 @@ struct FinalizeArgs {
 CPD_HD uint32_t trace_hist_end_words(uint32_t hist_words, uint32_t cell_bytes) {
     return (hist_words * (cell_bytes / 4) + 1) & ~1u;
 }
+// Byte offset of the wavefront's chain record (Chain64), which follows the
+// histogram, the queue and the row.
+CPD_HD uint32_t trace_chain_offset(uint32_t hist_words, uint32_t row_words, int acc) {
+    const uint32_t cell = acc == CPD_ACC_F64 ? 8 : 4;
+    const uint32_t row = acc == CPD_ACC_F32 ? 4 : 8;
+    const uint32_t end = trace_hist_end_words(hist_words, cell);
+    return (((end + kQueueCap + 1) & ~1u) * 4 + row_words * row + 7) & ~7u;
+}
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc);
 
 // start/stop (may be null) are recorded immediately around the kernel; with

@@ -155,6 +155,7 @@ struct TileCtx {
     const double *zs64;
     uint32_t full_end;   // positions 1..full_end are banded, the rest in ray order
     uint32_t last_pos;   // position of ray k = n_rays
+    const Chain64 *chain;  // the wavelength's planar faces in FP64 (LDS)
 };
 
 // FP64 re-trace of `count` (<= 64) queued rays starting at ring slot `head`.
@@ -166,7 +167,7 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
         const uint32_t i = c.queue[(head + lane) & (kQueueCap - 1)];  // position
         const double y = beam_y64(*c.g, i <= c.full_end ? c.zs64[i - 1] : c.z64[i - 1]);
         double t;
-        const uint32_t b = trace64(*c.g, c.lc.lambda, y, t);
+        const uint32_t b = trace64_chain(*c.g, *c.chain, y, t);
         if (b < c.nbin) atomicAdd((Cell *)c.hist + b * c.cols + (lane & (c.cols - 1)), Acc<A>::of64(t));
         if (i == c.last_pos) *c.last_bin = b;
     }
@@ -342,6 +343,7 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
     const uint32_t hist_end = trace_hist_end_words(a.hist_words, sizeof(Cell));
     uint32_t *queue = lds + hist_end;
     uint32_t *row32 = lds + ((hist_end + kQueueCap + 1) & ~1u);
+    Chain64 *chain = (Chain64 *)((char *)lds + trace_chain_offset(a.hist_words, a.row_words, A));
     const uint32_t lane = threadIdx.x;
     for (uint32_t i = lane; i < a.hist_words; i += 64) hist[i] = 0;
     for (uint32_t i = lane; i < a.row_words * (Acc<A>::kRowBytes / 4); i += 64) row32[i] = 0;
@@ -368,7 +370,11 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.g = g;
         c.td = td;
         const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
-        make_lambda_consts(*g, lambda, c.lc);
+        Chain64 ch;
+        make_lambda_consts(*g, lambda, c.lc, &ch);
+        if (lane == 0) *chain = ch;
+        __syncthreads();  // the queue drains of this item read the record
+        c.chain = chain;
         c.k_begin = tile * tile_rays + 1;
         c.k_end = min(c.k_begin + tile_rays, M + 1);
         c.n_rays = M;
@@ -589,10 +595,7 @@ hipError_t launch_trace_c(hipStream_t s, const BatchArgs &a, uint32_t grid, int 
 }  // namespace
 
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc) {
-    const uint32_t cell = acc == CPD_ACC_F64 ? 8 : 4;
-    const uint32_t row = acc == CPD_ACC_F32 ? 4 : 8;
-    const uint32_t end = trace_hist_end_words(hist_words, cell);
-    return (size_t)((end + kQueueCap + 1) & ~1u) * 4 + (size_t)row_words * row;
+    return (size_t)trace_chain_offset(hist_words, row_words, acc) + sizeof(Chain64);
 }
 
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, int acc, hipEvent_t start,

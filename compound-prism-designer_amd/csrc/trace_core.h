@@ -309,6 +309,17 @@ struct LambdaConsts {
     double lambda;
 };
 
+// The planar faces of one wavelength in FP64, as the per-ray trace meets
+// them: the direction arriving at face i and its cosine with the face normal
+// (identical for every ray of the collimated beam, since refraction reads
+// only the direction), and the direction and transmittances with which rays
+// leave the last face towards the lens.  trace64_chain re-traces single rays
+// with it.
+struct Chain64 {
+    double dx[CPD_MAX_PRISMS], dy[CPD_MAX_PRISMS], ci[CPD_MAX_PRISMS];
+    double ldx, ldy, ts, tp, n_last;
+};
+
 // Build the FP32 record from the FP64 design (host or device).
 // tile_cap (a multiple of 64, 0 for none) bounds the rays of one work item.
 CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n_rays, TraceDesign &t,
@@ -384,7 +395,7 @@ CPD_HD double refraction_gain(double dx, double dy, double nx, double ny, double
 // order.  Face i's hit point is o_i(y) = A_i + B_i y and its distance along
 // the shared direction d is (p.n - o_i.n) / (d.n), affine in y as well; the
 // conditions d >= 0 and 0 <= hit y <= h become bounds on y.
-CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts &lc) {
+CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts &lc, Chain64 *chain = nullptr) {
 #pragma clang fp contract(off)
     const int P = (int)g.n_glass;
     const bool ar = g.ar_coated != 0;
@@ -403,6 +414,11 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     double path = 1.0 + h + fabs(ax) + fabs(bx) * h;
     double amp = refraction_gain(1.0, 0.0, n0x, n0y, 1.0, n[0]);
     Ray64 ray{0.0, 0.0, 1.0, 0.0, 1.0, 1.0};
+    if (chain) {
+        chain->dx[0] = ray.dx;
+        chain->dy[0] = ray.dy;
+        chain->ci[0] = ray.dx * n0x + ray.dy * n0y;
+    }
     ok = ok && refract64(ray, n0x, n0y, 1.0, n[0], ar) == T_OK;
     for (int i = 1; i < P && ok; ++i) {
         const double nx = g.plane_normal[i][0], ny = g.plane_normal[i][1];
@@ -410,6 +426,11 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
         if (ci == 0.0) {
             ok = false;
             break;
+        }
+        if (chain) {
+            chain->dx[i] = ray.dx;
+            chain->dy[i] = ray.dy;
+            chain->ci[i] = ci;
         }
         const double pn = g.plane_point[i][0] * nx + g.plane_point[i][1] * ny;
         const double alpha = (pn - (ax * nx + ay * ny)) / ci;
@@ -426,6 +447,13 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
         ok = refract64(ray, nx, ny, n[i - 1], n[i], ar) == T_OK;
     }
     lc.ok = ok ? 1 : 0;
+    if (chain) {
+        chain->ldx = ray.dx;
+        chain->ldy = ray.dy;
+        chain->ts = ray.ts;
+        chain->tp = ray.tp;
+        chain->n_last = n[P - 1];
+    }
     lc.ylo = (float)lo;
     lc.yhi = (float)hi;
     // relative to the lens chord midpoint m; c = m + cm
@@ -453,6 +481,34 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.lr = (float)r;
     lc.lr2 = (float)(r * r);
     lc.lomr2 = (float)(1.0 - r * r);
+}
+
+// trace64 of one ray with the planar faces' directions and transmittances
+// taken from the wavelength's chain record (built by make_lambda_consts for a
+// wavelength whose record has ok set): each face costs the ray only its hit
+// point, computed with the operations of plane64 on the same values, and the
+// lens and detector stages are trace64's, so the bin and transmittance are
+// trace64's bit for bit.
+CPD_HD uint32_t trace64_chain(const cpd_design &g, const Chain64 &ch, double y, double &t) {
+#pragma clang fp contract(off)
+    t = 0.0;
+    double ox = 0.0, oy = y;
+    for (uint32_t i = 0; i < g.n_glass; ++i) {
+        const double *p = g.plane_point[i], *n = g.plane_normal[i];
+        const double d = ((p[0] - ox) * n[0] + (p[1] - oy) * n[1]) / ch.ci[i];
+        if (!(d >= 0.0)) return g.bin_count;
+        const double hx = fma(ch.dx[i], d, ox), hy = fma(ch.dy[i], d, oy);
+        if (!(0.0 <= hy && hy <= g.height)) return g.bin_count;
+        ox = hx;
+        oy = hy;
+    }
+    Ray64 ray{ox, oy, ch.ldx, ch.ldy, ch.ts, ch.tp};
+    if (arc64(ray, g, ch.n_last, g.ar_coated != 0) != T_OK) return g.bin_count;
+    uint32_t bin = g.bin_count;
+    double tt = 0.0;
+    if (detector64(ray, g, bin, tt) != T_OK) return g.bin_count;
+    t = tt;
+    return bin;
 }
 
 // ---------------------------------------------------------------------------
