@@ -224,6 +224,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     hot.lb0 = in_vgpr(hot.lb0);
     hot.lc1 = in_vgpr(hot.lc1);
     hot.lomr2 = in_vgpr(hot.lomr2);
+    hot.nd0 = in_vgpr(hot.nd0);
     hot.nbin = in_vgpr(hot.nbin);
     hot.ts = in_vgpr(hot.ts);
     hot.tp = in_vgpr(hot.tp);
@@ -370,9 +371,8 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.g = g;
         c.td = td;
         const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
-        Chain64 ch;
-        make_lambda_consts(*g, lambda, c.lc, &ch);
-        if (lane == 0) *chain = ch;
+        // every lane computes the same constants; lane 0 stores the chain record
+        make_lambda_consts(*g, lambda, c.lc, lane == 0 ? chain : nullptr);
         __syncthreads();  // the queue drains of this item read the record
         c.chain = chain;
         c.k_begin = tile * tile_rays + 1;

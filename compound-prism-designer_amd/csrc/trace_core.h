@@ -305,6 +305,7 @@ struct LambdaConsts {
     float amp;                 // angular amplification through the planar faces, >= 1
     float path;                // length scale of the FP32 error budget up to the lens
     float lr, lr2, lomr2;      // lens ratio r = n_{P-1}, r^2, 1 - r^2
+    float nd0, nd1, ddn, rddn; // detector normal dn: (cm - e).dn = nd0 + nd1 y, dir.dn, r dir.dn
     int ok;                    // 0 when every ray is lost in the planar faces (TIR, parallel face)
     double lambda;
 };
@@ -481,6 +482,11 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.lr = (float)r;
     lc.lr2 = (float)(r * r);
     lc.lomr2 = (float)(1.0 - r * r);
+    const double dnx = g.det_normal[0], dny = g.det_normal[1];
+    lc.nd0 = (float)((cmx - e0x) * dnx + (cmy - e0y) * dny);
+    lc.nd1 = (float)(-(bx * dnx + by * dny));
+    lc.ddn = (float)(ray.dx * dnx + ray.dy * dny);
+    lc.rddn = (float)(r * (ray.dx * dnx + ray.dy * dny));
 }
 
 // trace64 of one ray with the planar faces' directions and transmittances
@@ -556,7 +562,7 @@ struct alignas(4) HotRecord {
     float e0x, e0y, e1x, e1y, lb0, lb1, lc0, lc1, lc2;
     float dx, dy, rdx, rdy, ts, tp, amp, path;
     float cmx, cmy, inv_r, max_dist_sq, lens_scale, bound_scale;
-    float lr2, lomr2, lr;
+    float lr2, lomr2, lr, nd0, nd1, ddn, rddn;
     float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
     float slope, inv_slope, intercept, bin_size, nbin_f;
 };
@@ -595,6 +601,10 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     h.lr2 = lc.lr2;
     h.lomr2 = lc.lomr2;
     h.lr = lc.lr;
+    h.nd0 = lc.nd0;
+    h.nd1 = lc.nd1;
+    h.ddn = lc.ddn;
+    h.rddn = lc.rddn;
     h.mpx = d.mpx;
     h.mpy = d.mpy;
     h.dnx = d.dnx;
@@ -644,44 +654,54 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     const float d1 = bpos ? big : small, d2 = bpos ? small : big;
     const float q1x = fmaf(dx, d1, ex), q1y = fmaf(dy, d1, ey);
     const float q2x = fmaf(dx, d2, ex), q2y = fmaf(dy, d2, ey);
-    const float s1 = fmaf(q1x, q1x, q1y * q1y), s2 = fmaf(q2x, q2x, q2y * q2y);
-    const bool ok1 = (d1 >= 0.f) & (s1 <= d.max_dist_sq);
-    const bool ok2 = (d2 >= 0.f) & (s2 <= d.max_dist_sq);
+    // |q|^2 - max_dist_sq, with the bound inside the fused products
+    const float s1 = fmaf(q1x, q1x, fmaf(q1y, q1y, -d.max_dist_sq));
+    const float s2 = fmaf(q2x, q2x, fmaf(q2y, q2y, -d.max_dist_sq));
+    const bool ok1 = (d1 >= 0.f) & (s1 <= 0.f);
+    const bool ok2 = (d2 >= 0.f) & (s2 <= 0.f);
     const float blim = lim * d.bound_scale;
-    guard |= alive & ((fabsf(s1 - d.max_dist_sq) <= blim) | (!ok1 & (fabsf(s2 - d.max_dist_sq) <= blim)));
+    guard |= alive & ((fabsf(s1) <= blim) | (!ok1 & (fabsf(s2) <= blim)));
     alive &= ok1 | ok2;
     if (COUNT) res.ops += alive ? kOpsLensRefract : 0u;
-    const float qx = ok1 ? q1x : q2x, qy = ok1 ? q1y : q2y;
-    const float path = d.path + fabsf(ok1 ? d1 : d2);
-    // normal (c - p) / R = (cm - q) / R
-    const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
-    const float ci = -fmaf(dx, nx, dy * ny);
+    const float tq = ok1 ? d1 : d2;  // the hit point is q = e + tq dir
+    // Incidence cosine -dir.n, n = (c - p) / R = (cm - q) / R: it equals
+    // (b + tq) / R, which is -sqrt(disc) / R at the root -b - sqrt(disc) and
+    // +sqrt(disc) / R at the other; its error, that of sqrt(disc) over R, is
+    // the one the normal carries through q.
+    const float sdr = sd * d.inv_r;
+    const float ci = ok1 ? -sdr : sdr;
     const float cr_sq = fmaf(d.lr2, ci * ci, d.lomr2);
     const float cr = fast_sqrt(fmaxf(cr_sq, 0.f));
     const float rci = d.lr * ci;
     const float k = rci - cr;
-    const float vx = fmaf(nx, k, d.rdx), vy = fmaf(ny, k, d.rdy);
     guard |= alive & (fabsf(cr_sq) <= kGuardCos);
     alive &= cr_sq >= 0.f;
-    // Fresnel: rs = (r ci - cr) / (r ci + cr), rp = (r cr - ci) / (r cr + ci)
-    // (the reference's amplitudes divided through by n2), one reciprocal
-    const float fa = rci + cr, fb = fmaf(d.lr, cr, ci);
-    const float inv = fast_rcp(fa * fb);
-    const float as = k * (fb * inv), ap = fmaf(d.lr, cr, -ci) * (fa * inv);
-    const bool coat = ar & (ci > 0.5f);
     guard |= alive & ar & (fabsf(ci - 0.5f) <= kGuardCos * num0);
-    const float ts = d.ts * (coat ? 0.99f : fmaf(-as, as, 1.f));
-    const float tp = d.tp * (coat ? 0.99f : fmaf(-ap, ap, 1.f));
     const float num = num0 * fmaxf(rci, cr), den = cr;
     // detector array: w = o - P0 = (m - P0) + q
     if (COUNT) res.ops += alive ? kOpsDetector : 0u;
     {
-        const float cd = -fmaf(vx, d.dnx, vy * d.dny);
+        // acceptance cosine -v.dn of v = r dir + k n: -(r dir.dn + k n.dn),
+        // n.dn = ((cm - e).dn - tq dir.dn) / R and (cm - e).dn = nd0 + nd1 y
+        const float ndn = fmaf(-tq, d.ddn, fmaf(d.nd1, y, d.nd0)) * d.inv_r;
+        const float cd = -fmaf(k, ndn, d.rddn);
         guard |= alive & (fabsf(cd - d.min_ci) * den <= kGuardCos * num);
         alive &= cd > d.min_ci;
         // most wavefronts lose every ray here (the acceptance cosine): skip
         // the rest of the stage when no lane has a ray left or near a decision
         if (!wave_any(alive | guard)) return res;
+        const float qx = ok1 ? q1x : q2x, qy = ok1 ? q1y : q2y;
+        const float path = d.path + fabsf(tq);
+        const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
+        const float vx = fmaf(nx, k, d.rdx), vy = fmaf(ny, k, d.rdy);
+        // Fresnel: rs = (r ci - cr) / (r ci + cr), rp = (r cr - ci) / (r cr + ci)
+        // (the reference's amplitudes divided through by n2), one reciprocal
+        const float fa = rci + cr, fb = fmaf(d.lr, cr, ci);
+        const float inv = fast_rcp(fa * fb);
+        const float as = k * (fb * inv), ap = fmaf(d.lr, cr, -ci) * (fa * inv);
+        const bool coat = ar & (ci > 0.5f);
+        const float ts = d.ts * (coat ? 0.99f : fmaf(-as, as, 1.f));
+        const float tp = d.tp * (coat ? 0.99f : fmaf(-ap, ap, 1.f));
         const float wx = d.mpx + qx, wy = d.mpy + qy;
         const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(cd);
         const float pos = fmaf(fmaf(vx, dist, wx), d.ddx, fmaf(vy, dist, wy) * d.ddy);

@@ -132,6 +132,31 @@ def test_transmission_probability_matches_oracle():
         np.testing.assert_allclose(out[j], ref, atol=2e-6)
 
 
+@pytest.mark.parametrize("config", [1, 2, 4])
+def test_bin_probabilities_of_generated_designs_match_oracle(config):
+    # Per-(wavelength, bin) probabilities of the generated designs that reach
+    # the detector, 4 wavelengths x 16384 rays each.  Rays whose FP32 outcome
+    # lies within the error margin of a bin edge (a few per mille) are re-traced
+    # in FP64, so every ray must land in the oracle's bin: one ray in another bin
+    # moves two cells by t / 16384 (about 3e-5 for t = 0.5), far above the 2e-6
+    # that FP32 transmittances leave.
+    _, D, _, _ = cpd.generate_designs(config, 0, 12)
+    checked = 0
+    dp = ctypes.POINTER(ctypes.c_double)
+    for d in D:
+        w = np.linspace(d.wavelength_lo, d.wavelength_hi, 6)[1:-1].copy()
+        ref = np.array([O.p_dets_l_wavelength(to_oracle(d), float(x), 16384, qrng=O.QRNG_FIXED_POINT)[0] for x in w])
+        if ref.sum() == 0.0:
+            continue
+        out = np.empty((4, d.bin_count))
+        st = N.lib.cpd_p_dets_l_wavelength(None, ctypes.byref(d), w.ctypes.data_as(dp), 4, 16384, N.MODE_FIXED,
+                                           out.ctypes.data_as(dp))
+        assert st == 0, N.last_error()
+        np.testing.assert_allclose(out, ref, atol=2e-6, err_msg=f"config {config}")
+        checked += 1
+    assert checked >= 2
+
+
 @pytest.mark.parametrize("acc", [N.ACC_EXACT, N.ACC_F32, N.ACC_F32C, N.ACC_F64])
 def test_accumulation_policies_match_oracle(acc):
     # 6 prisms and 256 bins, 4 wavelengths x 65536 rays: every policy estimates
