@@ -38,7 +38,7 @@ PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the batched trace kernel on this workload, from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy's readings
 # (profiles/trace_kernel_batch1024_p3.md)
-TRACE_TRAFFIC_BYTES = 15.49e6
+TRACE_TRAFFIC_BYTES = 15.48e6
 
 
 def log(msg: str) -> None:
