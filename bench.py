@@ -95,7 +95,7 @@ def uneven_run(cpd, N, np, ctx, args):
     under the same schedule."""
     n = args.uneven
     t = time.time()
-    _, D, S, _ = cpd.generate_designs(2, 0, n)
+    P, D, S, _ = cpd.generate_designs(2, 0, n)
     darr = (N.Design * n)(*D)
     sarr = (N.Sampling * n)(*S)
     rays = sum(s.n_wavelengths * s.n_rays for s in S)
@@ -121,6 +121,40 @@ def uneven_run(cpd, N, np, ctx, args):
     value = n * args.uneven_steps / elapsed
     plan_info = np.array([f.info for f in plan._out])
     log(f"uneven: captured generation {elapsed / args.uneven_steps * 1e3:.2f} ms/step, {value:.0f} designs/s")
+
+    # the same generation as parameter vectors: records built on the device
+    # inside the captured graph (cpd_plan_run_params), against building them
+    # on one host thread (cpd_position_detector_array + cpd_build_design)
+    parr = (N.Params * n)(*P)
+    pst = (ctypes.c_int * n)()
+    for _ in range(2):
+        if N.lib.cpd_plan_run_params(plan._h, parr, 1.0, plan._out, pst) != 0:
+            raise RuntimeError(N.last_error())
+    t0 = time.perf_counter()
+    for _ in range(args.uneven_steps):
+        if N.lib.cpd_plan_run_params(plan._h, parr, 1.0, plan._out, pst) != 0:
+            raise RuntimeError(N.last_error())
+    p_elapsed = time.perf_counter() - t0
+    p_info = np.array([f.info for f in plan._out])
+    p_value = n * args.uneven_steps / p_elapsed
+    pos = (ctypes.c_double * 2)()
+    fl = ctypes.c_uint32()
+    rec = N.Design()
+    q = N.Params()
+    t = time.perf_counter()
+    for i in range(n):
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(parr[i]), ctypes.sizeof(q))
+        if N.lib.cpd_position_detector_array(ctypes.byref(q), 1.0, pos, ctypes.byref(fl)) == 0:
+            q.det_position[0], q.det_position[1], q.flipped = pos[0], pos[1], fl.value
+            N.lib.cpd_build_design(ctypes.byref(q), ctypes.byref(rec))
+    host_build_ms = (time.perf_counter() - t) * 1e3
+    params_run = {"designs_per_s": p_value, "ms_per_step": p_elapsed / args.uneven_steps * 1e3,
+                  "failed": int(sum(1 for v in pst if v != 0)),
+                  "max_abs_dinfo_vs_host_built": float(np.max(np.abs(p_info - plan_info))),
+                  "host_construction_ms_one_thread": host_build_ms}
+    log(f"uneven: generation from parameter vectors {params_run['ms_per_step']:.2f} ms/step, {p_value:.0f} designs/s "
+        f"(|dI| vs host-built records <= {params_run['max_abs_dinfo_vs_host_built']:.1e}); host construction "
+        f"of the same records {host_build_ms:.1f} ms on one thread")
 
     # eager runs under the same schedule: op count, then HIP-event kernel time
     ctx.set_schedule(best)
@@ -175,6 +209,7 @@ def uneven_run(cpd, N, np, ctx, args):
             "rays": rays, "designs_per_s": value, "ms_per_step": elapsed / args.uneven_steps * 1e3,
             "step_p50_ms": float(np.percentile(step, 50)), "step_p99_ms": float(np.percentile(step, 99)),
             "captured": "HIP graph per generation (upload, trace records, trace, entropy, download)",
+            "from_parameters": params_run,
             "schedule": best.as_dict(), "autotune_ms": tuned_ms, "autotune_s": tune_s,
             "kernel_ms": kernel_ms, "ops_per_launch": stats["ops"], "achieved_tflops": achieved,
             "frac": achieved / PEAK_FP32_TFLOPS, "reference_formula_tflops": reference_ops / (kernel_ms * 1e-3) / 1e12, "identical_to_eager": bool(np.array_equal(plan_info, eager_info)),

@@ -26,6 +26,7 @@ from ._native import Design, Fitness, Params, Sampling
 
 __all__ = [
     "autotune",
+    "build_designs",
     "gpu_fitness_batch",
     "Plan",
     "RayTraceError",
@@ -565,6 +566,18 @@ class Plan:
         res = np.array([[f.size, f.info, f.deviation] for f in self._out], np.float64).reshape(self.n, 3)
         return res, np.array(self._status[:], np.int32)
 
+    def run_params(self, params: Sequence["N.Params"], acceptance: float = 1.0):
+        """One generation from parameter vectors, built into designs on the GPU
+        (cpd_plan_run_params): acceptance > 0 positions each detector as
+        position_detector_array does, 0 takes params' position as given.
+        Returns (fitness float64 [n, 3], status int32 [n])."""
+        if len(params) != self.n:
+            raise ValueError(f"plan holds {self.n} designs, got {len(params)}")
+        parr = params if isinstance(params, ctypes.Array) else (N.Params * self.n)(*params)
+        _check(N.lib.cpd_plan_run_params(self._h, parr, float(acceptance), self._out, self._status))
+        res = np.array([[f.size, f.info, f.deviation] for f in self._out], np.float64).reshape(self.n, 3)
+        return res, np.array(self._status[:], np.int32)
+
     def close(self):
         if getattr(self, "_h", None):
             N.lib.cpd_plan_destroy(self._h)
@@ -581,6 +594,22 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def build_designs(params: Sequence["N.Params"], acceptance: float = 1.0, *, ctx: Optional[Context] = None):
+    """Design records built on the GPU from parameter vectors
+    (cpd_build_designs): acceptance > 0 positions each detector as
+    position_detector_array does, 0 takes params' position as given.  Returns
+    (list of Design, status int32 [n]); a design is meaningful where its status
+    is 0."""
+    n = len(params)
+    if n == 0:
+        return [], np.zeros(0, np.int32)
+    parr = params if isinstance(params, ctypes.Array) else (N.Params * n)(*params)
+    out = (Design * n)()
+    st = (ctypes.c_int * n)()
+    _check(N.lib.cpd_build_designs(_ctx_handle(ctx), parr, n, float(acceptance), out, st))
+    return list(out), np.array(st[:], np.int32)
 
 
 def _sampling_array(sampling, n):

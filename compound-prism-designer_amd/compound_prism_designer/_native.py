@@ -196,6 +196,8 @@ def _load() -> ctypes.CDLL:
         "cpd_ctx_set_schedule": ([vp, P(Schedule)], st),
         "cpd_plan_create": ([vp, P(Design), P(Sampling), sz, P(Schedule), P(vp)], st),
         "cpd_plan_run": ([vp, P(Design), P(Fitness), P(st)], st),
+        "cpd_plan_run_params": ([vp, P(Params), d, P(Fitness), P(st)], st),
+        "cpd_build_designs": ([vp, P(Params), sz, d, P(Design), P(st)], st),
         "cpd_plan_schedule": ([vp, P(Schedule)], st),
         "cpd_plan_destroy": ([vp], None),
     }

@@ -1,6 +1,7 @@
-// design.cpp -- host-side (FP64) construction of 2D compound-prism
-// spectrometers, detector positioning, Sellmeier glass fits and the synthetic
-// benchmark workloads.  None of this touches the GPU.
+// design.cpp -- the host entry points of design construction (the FP64
+// construction itself is construct.h, shared with the device), Sellmeier
+// glass fits and the synthetic benchmark workloads.  None of this touches the
+// GPU.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "cpd.h"
+#include "construct.h"
 #include "cpd_internal.h"
 #include "trace_core.h"
 
@@ -18,150 +20,16 @@ namespace {
 
 thread_local std::string g_last_error;
 
-struct Vec2 {
-    double x, y;
-};
-
-// Unit vector at angle theta, rotated by 180 degrees: (-cos, -sin)
-// (src/spectrometer/vector.rs:336-371, angled_xy().rot_180_xy()).
-Vec2 face_normal(double theta) {
-    double c = cos(theta), s = sin(theta);
-    return Vec2{-c, -s};
-}
-
-bool sign_positive(double v) { return !signbit(v); }
-
 cpd_status check_params(const cpd_params *p) {
     if (!p) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "params is NULL");
-    if (p->n_glass < 1 || p->n_glass > CPD_MAX_PRISMS)
-        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "n_glass must be in 1..6, got %u", p->n_glass);
-    if (!(p->height > 0.0)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "height must be > 0");
-    if (!(p->curvature != 0.0) || !isfinite(p->curvature))
-        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "curvature must be finite and non-zero");
-    if (!(p->wavelength_hi >= p->wavelength_lo))
-        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "lower wavelength bound > upper bound");
-    return CPD_OK;
-}
-
-// The compound prism part of a design: face planes from the angles and
-// separation lengths (create_joined_trapezoids, geometry.rs:430-489), the
-// exit lens from the last plane (CurvedPlane::from_hyperplane,
-// geometry.rs:280-315).
-void build_prism(const cpd_params &p, cpd_design &d) {
-#pragma clang fp contract(off)
-    const uint32_t P = p.n_glass;
-    d.n_glass = P;
-    d.ar_coated = p.ar_coated ? 1u : 0u;
-    for (uint32_t i = 0; i < P; ++i)
-        for (int j = 0; j < CPD_GLASS_COEFFS; ++j) d.glass[i][j] = p.glass[i][j];
-    d.height = p.height;
-
-    const double h2 = p.height * 0.5;
-    // entrance face: mid-height point at x = |tan(theta0)| h/2
-    Vec2 n = face_normal(p.angles[0]);
-    double x0 = fabs(n.y / n.x) * h2;
-    d.plane_point[0][0] = x0;
-    d.plane_point[0][1] = h2;
-    d.plane_normal[0][0] = n.x;
-    d.plane_normal[0][1] = n.y;
-    bool prev_up = sign_positive(n.y);
-    double prev_half = x0;  // half of the face's x-extent over the height
-    double x = x0;
-    Vec2 exit_point{0, 0}, exit_normal{0, 0};
-    // faces 1..P: face i sits lengths[i-1] beyond the previous one, measured
-    // between the facing edges of the two trapezoids
-    for (uint32_t i = 1; i <= P; ++i) {
-        Vec2 ni = face_normal(p.angles[i]);
-        bool up = sign_positive(ni.y);
-        double half = fabs(ni.y / ni.x) * h2;
-        double gap = (up != prev_up) ? prev_half + half : fabs(prev_half - half);
-        x += p.lengths[i - 1] + gap;
-        prev_up = up;
-        prev_half = half;
-        if (i < P) {
-            d.plane_point[i][0] = x;
-            d.plane_point[i][1] = h2;
-            d.plane_normal[i][0] = ni.x;
-            d.plane_normal[i][1] = ni.y;
-        } else {
-            exit_point = Vec2{x, h2};
-            exit_normal = ni;
-        }
+    switch (cpd::check_prism_params(*p)) {
+        case cpd::PARAM_N_GLASS:
+            return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "n_glass must be in 1..6, got %u", p->n_glass);
+        case cpd::PARAM_HEIGHT: return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "height must be > 0");
+        case cpd::PARAM_CURVATURE: return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "curvature must be finite and non-zero");
+        case cpd::PARAM_WAVELENGTHS: return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "lower wavelength bound > upper bound");
+        default: return CPD_OK;
     }
-    // exit lens: chord = |h / n.x|, R = chord / (2 |k|), centre on the normal
-    // through the chord midpoint at the apothem, bounds centred on the arc's
-    // midpoint with radius^2 = sagitta^2 + chord^2 / 4
-    const double k = p.curvature;
-    double chord = fabs(p.height / exit_normal.x);
-    double radius = chord * 0.5 / fabs(k);
-    double apothem = sqrt(radius * radius - chord * chord * 0.25);
-    double sagitta = radius - apothem;
-    double sa = copysign(apothem, k), ss = -copysign(sagitta, k);
-    d.arc_center[0] = fma(exit_normal.x, sa, exit_point.x);
-    d.arc_center[1] = fma(exit_normal.y, sa, exit_point.y);
-    d.arc_radius = radius;
-    d.arc_midpt[0] = fma(exit_normal.x, ss, exit_point.x);
-    d.arc_midpt[1] = fma(exit_normal.y, ss, exit_point.y);
-    d.arc_max_dist_sq = sagitta * sagitta + chord * chord * 0.25;
-
-    d.beam_width = p.beam_width;
-    d.y_mean = p.y_mean;
-    d.wavelength_lo = p.wavelength_lo;
-    d.wavelength_hi = p.wavelength_hi;
-}
-
-// LinearDetectorArray::new (spectrometer.rs:183-219) with
-// min_ci = cos(max_incident_angle) (pylib.rs:906-920).
-void build_detector(const cpd_params &p, cpd_design &d) {
-    d.bin_count = p.bin_count;
-    d.det_bin_size = p.bin_size;
-    d.det_slope = p.linear_slope;
-    d.det_intercept = p.linear_intercept;
-    d.det_min_ci = cos(p.max_incident_angle);
-    d.det_length = p.det_length;
-    Vec2 n = face_normal(p.det_angle);
-    d.det_normal[0] = n.x;
-    d.det_normal[1] = n.y;
-    // direction: normal rotated +90 degrees when flipped, -90 otherwise
-    if (p.flipped) {
-        d.det_direction[0] = -n.y;
-        d.det_direction[1] = n.x;
-    } else {
-        d.det_direction[0] = n.y;
-        d.det_direction[1] = -n.x;
-    }
-    d.det_position[0] = p.det_position[0];
-    d.det_position[1] = p.det_position[1];
-}
-
-// fit_ray_difference_surface (geometry.rs:495-542): the array of length L at
-// angle a spans the two exit rays of the extreme wavelengths.
-bool fit_detector(const cpd::Ray64 &lo, const cpd::Ray64 &hi, double length, double angle, double pos[2],
-                  bool &flipped) {
-#pragma clang fp contract(off)
-    // array direction (-sin a, cos a) scaled by its length
-    double sx = -sin(angle) * length, sy = cos(angle) * length;
-    double a = hi.dx, b = -lo.dx, c = hi.dy, dd = -lo.dy;
-    double det = a * dd - b * c;
-    if (det == 0.0) return false;
-    double m10 = -c / det, m11 = a / det;
-    // distance along the lower ray to the array end that the upper ray meets
-    double tx = sx - hi.ox + lo.ox, ty = sy - hi.oy + lo.oy;
-    double s = m10 * tx + m11 * ty;
-    if (s > 0.0) {
-        pos[0] = fma(lo.dx, s, lo.ox);
-        pos[1] = fma(lo.dy, s, lo.oy);
-        flipped = false;
-        return true;
-    }
-    tx = -sx - hi.ox + lo.ox;
-    ty = -sy - hi.oy + lo.oy;
-    s = m10 * tx + m11 * ty;
-    if (s < 0.0) return false;
-    pos[0] = fma(lo.dx, s, lo.ox);
-    pos[1] = fma(lo.dy, s, lo.oy);
-    flipped = true;
-    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -278,27 +146,10 @@ const char *cpd_last_error(void) { return g_last_error.c_str(); }
 
 cpd_status cpd_position_detector_array(const cpd_params *params, double acceptance, double position_out[2],
                                        uint32_t *flipped_out) {
-#pragma clang fp contract(off)
     cpd_status st = check_params(params);
     if (st != CPD_OK) return st;
     if (!position_out || !flipped_out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL output");
-    cpd_design d{};
-    build_prism(*params, d);
-    // the two extreme accepted wavelengths through the prism from the beam's mean height
-    double half_out = fmax(1.0 - acceptance, 0.0) * 0.5;
-    double w_lo = cpd::lambda_of(d, half_out);
-    double w_hi = cpd::lambda_of(d, 1.0 - half_out);
-    cpd::Ray64 lo{0.0, d.y_mean, 1.0, 0.0, 1.0, 1.0}, hi = lo;
-    int e = cpd::prism64(lo, d, w_lo);
-    if (e != CPD_OK) return (cpd_status)e;
-    e = cpd::prism64(hi, d, w_hi);
-    if (e != CPD_OK) return (cpd_status)e;
-    if ((lo.ts + lo.tp) * 0.5 <= 1e-3 || (hi.ts + hi.tp) * 0.5 <= 1e-3) return CPD_ERR_ANGULAR_RESPONSE_TOO_WEAK;
-    bool flipped = false;
-    if (!fit_detector(lo, hi, params->det_length, params->det_angle, position_out, flipped))
-        return CPD_ERR_NO_SURFACE_INTERSECTION;
-    *flipped_out = flipped ? 1u : 0u;
-    return CPD_OK;
+    return cpd::position_detector(*params, acceptance, position_out, *flipped_out);
 }
 
 cpd_status cpd_build_design(const cpd_params *params, cpd_design *out) {
@@ -308,11 +159,7 @@ cpd_status cpd_build_design(const cpd_params *params, cpd_design *out) {
     if (params->bin_count < 1 || params->bin_count > CPD_MAX_BINS)
         return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "bin_count must be in 1..%d", CPD_MAX_BINS);
     if (!(params->linear_slope > 0.0)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "linear_slope must be > 0");
-    cpd_design d{};
-    build_prism(*params, d);
-    build_detector(*params, d);
-    *out = d;
-    return CPD_OK;
+    return cpd::construct_design(*params, 0.0, *out);
 }
 
 cpd_status cpd_size_and_deviation(const cpd_design *g, double *size, double *deviation) {

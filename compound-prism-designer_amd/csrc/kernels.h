@@ -92,5 +92,12 @@ hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n);
 // Single rays: wavelength from its CDF value, beam height from its CDF value.
 hipError_t launch_propagate(hipStream_t s, const cpd_design *g, const TraceDesign *td, const float *wcdf,
                             const float *rcdf, uint32_t n, uint32_t *bin, float *t);
+// Designs built on the device from their parameters (construct_design):
+// out[i] and status[i] for i < n, the detector positioned when *acceptance
+// (a device scalar) is > 0.  With standin non-null, a design that fails, or
+// whose bin count differs from standin[i]'s, is replaced by standin[i], so a
+// launch over the outputs stays well formed.
+hipError_t launch_construct(hipStream_t s, const cpd_params *params, uint32_t n, const double *acceptance,
+                            const cpd_design *standin, cpd_design *out, int *status);
 
 }  // namespace cpd

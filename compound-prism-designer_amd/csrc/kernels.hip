@@ -27,6 +27,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include "construct.h"
 #include "kernels.h"
 
 namespace cpd {
@@ -561,6 +562,31 @@ __global__ void build_traces_kernel(const cpd_design *designs, const cpd_samplin
 }
 
 }  // namespace
+
+namespace {
+// One thread per design (construct.h, FP64).
+__global__ __launch_bounds__(64) void construct_kernel(const cpd_params *params, uint32_t n, const double *acceptance,
+                                                       const cpd_design *standin, cpd_design *out, int *status) {
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    cpd_design d;
+    cpd_status st = construct_design(params[i], *acceptance, d);
+    if (standin) {
+        if (st == CPD_OK && d.bin_count != standin[i].bin_count) st = CPD_ERR_INVALID_ARGUMENT;
+        if (st != CPD_OK) d = standin[i];
+    }
+    out[i] = d;
+    status[i] = st;
+}
+}  // namespace
+
+hipError_t launch_construct(hipStream_t s, const cpd_params *params, uint32_t n, const double *acceptance,
+                            const cpd_design *standin, cpd_design *out, int *status) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(construct_kernel, dim3((n + 63) / 64), dim3(64), 0, s, params, n, acceptance, standin, out,
+                       status);
+    return hipGetLastError();
+}
 
 hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
                                uint32_t tile_cap, TraceDesign *traces) {

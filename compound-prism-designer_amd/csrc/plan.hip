@@ -9,6 +9,10 @@
 // and the download of the results) is captured once in a HIP graph that each
 // generation replays.  No memset node is captured: the work counter is
 // cleared by a kernel and the reduction leaves the accumulators at zero.
+// A generation can also arrive as parameter vectors (cpd_plan_run_params):
+// a second graph uploads them and builds the FP64 design records on the
+// device (construct_kernel) before the same evaluation, so the host does no
+// per-design work at all.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -47,10 +51,24 @@ struct Plan {
     HostBuffer h_designs, h_out;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the evaluation every generation runs, set up by init
+    BatchArgs args{};
+    FinalizeArgs fin{};
+    uint32_t grid = 0;
+    const cpd_sampling *d_sampling = nullptr;
+    // generations from parameter vectors: [acceptance | params] uploaded in
+    // one copy, the slots' shape records as stand-ins, per-slot statuses
+    DeviceBuffer d_params, d_shape, d_status;
+    HostBuffer h_params, h_status;
+    hipGraph_t graph_p = nullptr;
+    hipGraphExec_t exec_p = nullptr;
 
     cpd_status init(Context *ctx, const cpd_design *shape_in, const cpd_sampling *samp, size_t count,
                     const cpd_schedule *schedule);
+    hipError_t capture_evaluation();
+    cpd_status capture_params();
     cpd_status run(const cpd_design *designs, cpd_fitness *out, cpd_status *per_design);
+    cpd_status run_params(const cpd_params *params, double acceptance, cpd_fitness *out, cpd_status *per_design);
     ~Plan();
 };
 
@@ -59,6 +77,13 @@ Plan::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (exec_p) (void)hipGraphExecDestroy(exec_p);
+    if (graph_p) (void)hipGraphDestroy(graph_p);
+    d_params.release();
+    d_shape.release();
+    d_status.release();
+    h_params.release();
+    h_status.release();
     d_fixed.release();
     d_designs.release();
     d_traces.release();
@@ -151,7 +176,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     if ((st = beam.ensure(max_rays, stream)) != CPD_OK) return st;
     PLAN_TRY(hipStreamSynchronize(stream));  // host vectors above go out of scope
 
-    BatchArgs a{};
+    BatchArgs &a = args;
     a.designs = d_designs.as<cpd_design>();
     a.traces = d_traces.as<TraceDesign>();
     a.item_begin = (const uint64_t *)(fx + off_items);
@@ -169,8 +194,9 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     a.hist_words = hist_words;
     a.row_words = row_words;
     a.n_items = items;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)ctx->cu_count * sched.waves_per_cu);
-    FinalizeArgs f{};
+    grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)ctx->cu_count * sched.waves_per_cu);
+    d_sampling = (const cpd_sampling *)(fx + off_samp);
+    FinalizeArgs &f = fin;
     f.designs = a.designs;
     f.traces = a.traces;
     f.acc_begin = a.acc_begin;
@@ -182,20 +208,83 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
 
     PLAN_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipMemcpyAsync(d_designs.ptr, h_designs.ptr, n * sizeof(cpd_design), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-        e = launch_build_traces(stream, a.designs, (const cpd_sampling *)(fx + off_samp), n, sched.tile_rays,
-                                d_traces.as<TraceDesign>());
-    if (e == hipSuccess) e = launch_zero_u32(stream, a.work_counter, 1);
-    if (e == hipSuccess) e = launch_trace(stream, a, grid, false);
-    if (e == hipSuccess) e = launch_finalize(stream, f);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(h_out.ptr, d_results.ptr, n * sizeof(cpd_fitness), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = capture_evaluation();
     hipGraph_t g = nullptr;
     const hipError_t e_end = hipStreamEndCapture(stream, &g);
     graph = g;
     if (e != hipSuccess) return fail(CPD_ERR_DEVICE, "plan capture: %s", hipGetErrorString(e));
     if (e_end != hipSuccess) return fail(CPD_ERR_DEVICE, "plan capture: %s", hipGetErrorString(e_end));
     PLAN_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return CPD_OK;
+}
+
+// Trace records, cleared work counter, trace, reduction and download of the
+// results, recorded on the stream being captured.
+hipError_t Plan::capture_evaluation() {
+    hipError_t e = launch_build_traces(stream, args.designs, d_sampling, n, sched.tile_rays, d_traces.as<TraceDesign>());
+    if (e == hipSuccess) e = launch_zero_u32(stream, args.work_counter, 1);
+    if (e == hipSuccess) e = launch_trace(stream, args, grid, false);
+    if (e == hipSuccess) e = launch_finalize(stream, fin);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_out.ptr, d_results.ptr, n * sizeof(cpd_fitness), hipMemcpyDeviceToHost, stream);
+    return e;
+}
+
+// The parameter-vector graph, captured on first use.
+cpd_status Plan::capture_params() {
+    const size_t off_p = 256;
+    const size_t bytes = off_p + (size_t)n * sizeof(cpd_params);
+    cpd_status st;
+    if ((st = d_params.reserve(bytes, false, stream)) != CPD_OK) return st;
+    if ((st = h_params.reserve(bytes)) != CPD_OK) return st;
+    if ((st = d_shape.reserve((size_t)n * sizeof(cpd_design), false, stream)) != CPD_OK) return st;
+    if ((st = d_status.reserve((size_t)n * sizeof(int), false, stream)) != CPD_OK) return st;
+    if ((st = h_status.reserve((size_t)n * sizeof(int))) != CPD_OK) return st;
+    std::vector<cpd_design> standin(n);
+    for (uint32_t q = 0; q < n; ++q) standin[q] = shape[perm[q]];
+    PLAN_TRY(hipMemcpyAsync(d_shape.ptr, standin.data(), (size_t)n * sizeof(cpd_design), hipMemcpyHostToDevice, stream));
+    PLAN_TRY(hipStreamSynchronize(stream));
+    char *dp = (char *)d_params.ptr;
+    PLAN_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(dp, h_params.ptr, bytes, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = launch_construct(stream, (const cpd_params *)(dp + off_p), n, (const double *)dp,
+                             d_shape.as<cpd_design>(), d_designs.as<cpd_design>(), d_status.as<int>());
+    if (e == hipSuccess) e = capture_evaluation();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_status.ptr, d_status.ptr, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e_end = hipStreamEndCapture(stream, &g);
+    graph_p = g;
+    if (e != hipSuccess) return fail(CPD_ERR_DEVICE, "plan capture: %s", hipGetErrorString(e));
+    if (e_end != hipSuccess) return fail(CPD_ERR_DEVICE, "plan capture: %s", hipGetErrorString(e_end));
+    PLAN_TRY(hipGraphInstantiate(&exec_p, graph_p, nullptr, nullptr, 0));
+    return CPD_OK;
+}
+
+cpd_status Plan::run_params(const cpd_params *params, double acceptance, cpd_fitness *out, cpd_status *per_design) {
+    std::lock_guard<std::mutex> lk(mu);
+    PLAN_TRY(hipSetDevice(device));
+    if (!exec_p) {
+        cpd_status st = capture_params();
+        if (st != CPD_OK) return st;
+    }
+    char *h = (char *)h_params.ptr;
+    memcpy(h, &acceptance, sizeof acceptance);
+    cpd_params *hp = (cpd_params *)(h + 256);
+    for (uint32_t q = 0; q < n; ++q) hp[q] = params[perm[q]];
+    PLAN_TRY(hipGraphLaunch(exec_p, stream));
+    PLAN_TRY(hipStreamSynchronize(stream));
+    const cpd_fitness *r = (const cpd_fitness *)h_out.ptr;
+    const int *stq = (const int *)h_status.ptr;
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t i = perm[q];
+        if (per_design) per_design[i] = (cpd_status)stq[q];
+        if (stq[q] != CPD_OK)
+            out[i].size = out[i].info = out[i].deviation = NAN;
+        else
+            out[i] = r[q];
+    }
     return CPD_OK;
 }
 
@@ -249,6 +338,14 @@ cpd_status cpd_plan_create(cpd_ctx *ctx, const cpd_design *shape, const cpd_samp
 cpd_status cpd_plan_run(cpd_plan *plan, const cpd_design *designs, cpd_fitness *out, cpd_status *per_design) {
     if (!plan || !designs || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
     return reinterpret_cast<cpd::Plan *>(plan)->run(designs, out, per_design);
+}
+
+cpd_status cpd_plan_run_params(cpd_plan *plan, const cpd_params *params, double acceptance, cpd_fitness *out,
+                               cpd_status *per_design) {
+    if (!plan || !params || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!(acceptance >= 0.0 && acceptance <= 1.0))
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "acceptance must be in [0, 1]");
+    return reinterpret_cast<cpd::Plan *>(plan)->run_params(params, acceptance, out, per_design);
 }
 
 cpd_status cpd_plan_schedule(const cpd_plan *plan, cpd_schedule *out) {

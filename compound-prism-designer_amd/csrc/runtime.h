@@ -64,7 +64,7 @@ struct Context {
 
     BeamTables beam;
     // batch state
-    DeviceBuffer stage, acc, last_bin, results, counters, row_buf, prop_buf;
+    DeviceBuffer stage, acc, last_bin, results, counters, row_buf, prop_buf, build_buf;
     // adaptive estimators: row descriptors, design records, seeded beam tables
     DeviceBuffer row_stage, design_stage, seed_z32, seed_z64;
     HostBuffer host_stage, host_out;
@@ -76,6 +76,8 @@ struct Context {
     cpd_status run(const cpd_design *designs, const cpd_sampling *sampling, const std::vector<uint32_t> &valid,
                    bool persistent, const double *wavelengths, cpd_fitness *out, double *rows,
                    unsigned long long *ops_out);
+    cpd_status build_designs(const cpd_params *params, size_t n, double acceptance, cpd_design *out,
+                             cpd_status *status);
     cpd_status propagate(const cpd_design &g, const float *wcdf, const float *rcdf, size_t n, uint32_t *bin,
                          float *t);
     // adaptive estimators (adaptive.hip)

@@ -405,6 +405,7 @@ Context::~Context() {
     counters.release();
     row_buf.release();
     prop_buf.release();
+    build_buf.release();
     row_stage.release();
     design_stage.release();
     seed_z32.release();
@@ -681,11 +682,50 @@ cpd_status cpd_propagate_rays(cpd_ctx *ctx, const cpd_design *design, const floa
     return c->propagate(*design, wavelength_cdf, ray_cdf, n, bin_out, t_out);
 }
 
+cpd_status cpd_build_designs(cpd_ctx *ctx, const cpd_params *params, size_t n, double acceptance,
+                             cpd_design *designs_out, cpd_status *per_design) {
+    if (n == 0) return CPD_OK;
+    if (!params || !designs_out || !per_design) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n > (1u << 24)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "more than 2^24 designs");
+    if (!(acceptance >= 0.0 && acceptance <= 1.0))
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "acceptance must be in [0, 1]");
+    Context *c;
+    cpd_status st = resolve(ctx, &c);
+    if (st != CPD_OK) return st;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return c->build_designs(params, n, acceptance, designs_out, per_design);
+}
+
 }  // extern "C"
 
 namespace cpd {
 
 cpd_status check_design_public(const cpd_design &g, const cpd_sampling &s) { return check_design(g, s); }
+
+// Upload of the parameters (one copy from the pinned stage), one thread per
+// design, download of the records and statuses.
+cpd_status Context::build_designs(const cpd_params *params, size_t n, double acceptance, cpd_design *out,
+                                  cpd_status *status) {
+    HIP_TRY(hipSetDevice(device));
+    const size_t off_p = 256, off_d = align_up(off_p + n * sizeof(cpd_params), 256);
+    const size_t off_s = align_up(off_d + n * sizeof(cpd_design), 256);
+    const size_t total = align_up(off_s + n * sizeof(int), 256);
+    cpd_status st = build_buf.reserve(total, false, stream);
+    if (st != CPD_OK) return st;
+    if ((st = host_stage.reserve(off_d)) != CPD_OK) return st;
+    char *h = (char *)host_stage.ptr;
+    memcpy(h, &acceptance, sizeof acceptance);
+    memcpy(h + off_p, params, n * sizeof(cpd_params));
+    char *d = (char *)build_buf.ptr;
+    HIP_TRY(hipMemcpyAsync(d, h, off_d, hipMemcpyHostToDevice, stream));
+    HIP_TRY(launch_construct(stream, (const cpd_params *)(d + off_p), (uint32_t)n, (const double *)d, nullptr,
+                             (cpd_design *)(d + off_d), (int *)(d + off_s)));
+    HIP_TRY(hipMemcpyAsync(out, d + off_d, n * sizeof(cpd_design), hipMemcpyDeviceToHost, stream));
+    static_assert(sizeof(cpd_status) == sizeof(int), "status is an int");
+    HIP_TRY(hipMemcpyAsync(status, d + off_s, n * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return CPD_OK;
+}
 
 cpd_status Context::propagate(const cpd_design &g, const float *wcdf, const float *rcdf, size_t n, uint32_t *bin,
                               float *t) {

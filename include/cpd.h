@@ -302,6 +302,18 @@ cpd_status cpd_gpu_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, size_t
                                  size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval,
                                  cpd_fitness *out, cpd_status *per_design);
 
+/* n designs built on the GPU from their parameters: cpd_build_design, and
+ * first, when acceptance > 0, cpd_position_detector_array with that
+ * acceptance (params' det_position and flipped are then not read; with
+ * acceptance 0 they are used as given, designer.py:236-243's explicit
+ * position).  The same FP64 construction code as the host entry points; only
+ * the device's cos and sin may differ from the host's in the last bit.
+ * per_design[i] receives design i's status (the host entry points' codes,
+ * without their messages); designs_out[i] is meaningful where it is CPD_OK.
+ * acceptance in [0, 1].  designer.py:203-265 per individual. */
+cpd_status cpd_build_designs(cpd_ctx *ctx, const cpd_params *params, size_t n, double acceptance,
+                             cpd_design *designs_out, cpd_status *per_design);
+
 /* One ray per (wavelength cdf, ray cdf) pair through the FP32 device trace:
  * bin index (bin_count on a miss) and transmittance (0 on a miss).  Replaces
  * propagation_test_kernel, src/spectrometer/kernel.rs:195-224, launched by
@@ -348,6 +360,17 @@ typedef struct cpd_plan cpd_plan;
 cpd_status cpd_plan_create(cpd_ctx *ctx, const cpd_design *shape, const cpd_sampling *sampling, size_t n,
                            const cpd_schedule *schedule /* NULL: autotuned on `shape` */, cpd_plan **out);
 cpd_status cpd_plan_run(cpd_plan *plan, const cpd_design *designs, cpd_fitness *out, cpd_status *per_design);
+/* One generation from parameter vectors: the parameters are uploaded and the
+ * design records built on the device (as cpd_build_designs, with the same
+ * acceptance rule) inside a second captured graph, then evaluated as
+ * cpd_plan_run does, so the host does no per-design work.  This is
+ * designer.py:203-265 (create_spectrometer: CompoundPrism, then
+ * position_detector_array or the given position, then DetectorArray) and
+ * :267-300 (fitness per individual) for a whole population.  per_design[i]
+ * (may be NULL) is the construction status, or CPD_ERR_INVALID_ARGUMENT for a
+ * bin count other than the slot's; such designs get NaN results. */
+cpd_status cpd_plan_run_params(cpd_plan *plan, const cpd_params *params, double acceptance, cpd_fitness *out,
+                               cpd_status *per_design);
 /* Schedule the plan was captured with. */
 cpd_status cpd_plan_schedule(const cpd_plan *plan, cpd_schedule *out);
 void cpd_plan_destroy(cpd_plan *plan);
