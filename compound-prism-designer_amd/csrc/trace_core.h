@@ -304,7 +304,8 @@ struct LambdaConsts {
     float ts, tp;              // Fresnel transmittances through the planar faces
     float amp;                 // angular amplification through the planar faces, >= 1
     float path;                // length scale of the FP32 error budget up to the lens
-    float lr, lr2, lomr2;      // lens ratio r = n_{P-1}, r^2, 1 - r^2
+    float lr, lomr2;           // lens ratio r = n_{P-1}, 1 - r^2
+    float lr2r;                // r^2 / R^2 (R the lens radius)
     float nd0, nd1, ddn, rddn; // detector normal dn: (cm - e).dn = nd0 + nd1 y, dir.dn, r dir.dn
     int ok;                    // 0 when every ray is lost in the planar faces (TIR, parallel face)
     double lambda;
@@ -480,7 +481,7 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.amp = (float)amp;
     lc.path = (float)(path + fabs(e0x) + fabs(e0y) + (fabs(bx) + fabs(by)) * h);
     lc.lr = (float)r;
-    lc.lr2 = (float)(r * r);
+    lc.lr2r = (float)(r * r / (g.arc_radius * g.arc_radius));
     lc.lomr2 = (float)(1.0 - r * r);
     const double dnx = g.det_normal[0], dny = g.det_normal[1];
     lc.nd0 = (float)((cmx - e0x) * dnx + (cmy - e0y) * dny);
@@ -531,6 +532,14 @@ CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
 CPD_HD bool wave_any(bool v) { return v; }
 #endif
 
+// A float whose value does not matter (clang: an arbitrary but defined value,
+// so no register is set for it).
+#if defined(__clang__)
+#define kAnyFloat __builtin_nondeterministic_value(0.f)
+#else
+#define kAnyFloat 0.f
+#endif
+
 // Error budget of an FP32 length per unit of path travelled, and of an FP32
 // cosine, before angular amplification.
 static constexpr float kGuardRel = 8.0e-6f;
@@ -558,11 +567,11 @@ struct Trace32Result {
 // loop.
 struct alignas(4) HotRecord {
     uint32_t nbin, ar_coated;
-    float y_mean, beam_w, ylo, yhi;
+    float y_mean, beam_w, ylo, yhi, ymarg;
     float e0x, e0y, e1x, e1y, lb0, lb1, lc0, lc1, lc2;
     float dx, dy, rdx, rdy, ts, tp, amp, path;
     float cmx, cmy, inv_r, max_dist_sq, lens_scale, bound_scale;
-    float lr2, lomr2, lr, nd0, nd1, ddn, rddn;
+    float lr2r, lomr2, lr, nd0, nd1, ddn, rddn;
     float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
     float slope, inv_slope, intercept, bin_size, nbin_f;
 };
@@ -575,6 +584,11 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     h.beam_w = d.beam_w;
     h.ylo = lc.ylo;
     h.yhi = lc.yhi;
+    // One height margin for the wavelength.  A ray within the per-height
+    // margin kGuardRel (|y| + 1) of a bound y_b has |y| <= (|y_b| +
+    // kGuardRel) / (1 - kGuardRel), so kGuardRel (max |y_b| + 1) /
+    // (1 - kGuardRel), rounded up, covers every such ray.
+    h.ymarg = kGuardRel * (fmaxf(fabsf(lc.ylo), fabsf(lc.yhi)) + 1.f) * (1.f + 2.f * kGuardRel) * 1.000001f;
     h.e0x = lc.e0x;
     h.e0y = lc.e0y;
     h.e1x = lc.e1x;
@@ -598,7 +612,7 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     h.max_dist_sq = d.max_dist_sq;
     h.lens_scale = d.lens_scale;
     h.bound_scale = d.bound_scale;
-    h.lr2 = lc.lr2;
+    h.lr2r = lc.lr2r;
     h.lomr2 = lc.lomr2;
     h.lr = lc.lr;
     h.nd0 = lc.nd0;
@@ -625,12 +639,13 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
 // combined with bitwise operators, so the whole trace is straight-line code.
 template <bool COUNT>
 CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
-    Trace32Result res{d.nbin, 0.f, false, 0};
+    // t is read only with a bin below nbin, which only the end of the trace
+    // sets: the miss paths leave it without a value to materialise
+    Trace32Result res{d.nbin, kAnyFloat, false, 0};
     const bool ar = d.ar_coated != 0;
     // planar chain: the ray survives the planar faces iff ylo <= y <= yhi
-    const float ymarg = kGuardRel * (fabsf(y) + 1.f);
     bool alive = (y >= d.ylo) & (y <= d.yhi);
-    bool guard = (fabsf(y - d.ylo) <= ymarg) | (fabsf(y - d.yhi) <= ymarg);
+    bool guard = fminf(fabsf(y - d.ylo), fabsf(y - d.yhi)) <= d.ymarg;
     const float ex = fmaf(d.e1x, y, d.e0x), ey = fmaf(d.e1y, y, d.e0y);
     if (COUNT) res.ops = kOpsBeam + kOpsChain + (alive ? kOpsLensHit : 0u);
     // wavefronts of banded rays often lie wholly outside the accepted heights
@@ -670,7 +685,8 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     // the one the normal carries through q.
     const float sdr = sd * d.inv_r;
     const float ci = ok1 ? -sdr : sdr;
-    const float cr_sq = fmaf(d.lr2, ci * ci, d.lomr2);
+    // 1 - r^2 (1 - ci^2) with ci^2 = disc / R^2
+    const float cr_sq = fmaf(d.lr2r, disc, d.lomr2);
     const float cr = fast_sqrt(fmaxf(cr_sq, 0.f));
     const float rci = d.lr * ci;
     const float k = rci - cr;
