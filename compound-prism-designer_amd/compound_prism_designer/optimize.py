@@ -1,0 +1,281 @@
+"""Generation-level driver for the designer's optimisation loop.
+
+The reference optimises with pymoo, one individual per `_evaluate` call
+(designer.py:115-300: CompoundPrismSpectrometerProblem builds a Spectrometer
+from each parameter vector and calls gpu_fitness on it).  Here a whole
+population is evaluated per call: its parameter vectors become `cpd_params`
+records, and one captured generation (cpd_plan_run_params) builds the designs
+on the GPU, positions their detectors and evaluates them.
+
+pymoo is not part of this build, so `evolve` is a small elitist
+multi-objective loop of its own (non-dominated sorting with crowding
+distance, simulated binary crossover, polynomial mutation, the designer's
+feasibility constraint); any other optimiser can call
+`SpectrometerProblem.evaluate` on its populations instead.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import Context, Glass, Plan, _check, build_designs
+from . import _native as N
+
+
+def sellmeier_glass(name: str, b: Sequence[float], c: Sequence[float], wavelength_range=(0.5, 0.82),
+                    npts: int = 100) -> Glass:
+    """A Sellmeier1 glass fitted to the Horner polynomial the kernels evaluate
+    (cpd_fit_sellmeier1; AbstractGlass.into_glass, glasscat.py:114-127)."""
+    bb = (ctypes.c_double * 3)(*b)
+    cc = (ctypes.c_double * 3)(*c)
+    out = (ctypes.c_double * 6)()
+    _check(N.lib.cpd_fit_sellmeier1(bb, cc, float(wavelength_range[0]), float(wavelength_range[1]), npts, out,
+                                    None))
+    return Glass(name, list(out))
+
+
+def synthetic_catalog(n: int, seed: int = 0, wavelength_range=(0.5, 0.82)) -> List[Glass]:
+    """n Sellmeier1 glasses with B ~ U[0.3, 1.5], C ~ U[0.001, 0.2] um^2 (the
+    benchmark generator's ranges), standing in for a glass catalogue file."""
+    rng = np.random.default_rng(seed)
+    return [sellmeier_glass(f"G{i:03d}", rng.uniform(0.3, 1.5, 3), rng.uniform(0.001, 0.2, 3), wavelength_range)
+            for i in range(n)]
+
+
+@dataclass
+class SpectrometerProblem:
+    """The designer's problem (designer.py:115-201), Gaussian beam, detector
+    positioned automatically.  Variables are normalised to [0, 1]: glass
+    indices (nglass), face angles (nglass + 1, mapped to [-pi/2, pi/2]),
+    tanh of the separation lengths (nglass; length = artanh(t) max_height / 4,
+    designer.py:217), curvature ([1e-5, 1]), height ([1e-4, 1] max_height),
+    normalised beam height ([0, 1]) and detector angle ([-pi, pi])."""
+
+    glasses: Sequence[Glass]
+    nglass: int
+    max_height: float = 25.0
+    prism_width: float = 7.0
+    ar_coated: bool = True
+    beam_width: float = 3.2
+    wavelength_range: tuple = (0.5, 0.82)
+    bin_count: int = 32
+    bin_size: float = 0.8
+    linear_slope: float = 1.0
+    linear_intercept: float = 0.1
+    detector_length: float = 32.0
+    max_incident_angle: float = math.radians(45.0)
+    acceptance: float = 1.0
+    n_wavelengths: int = 64
+    n_rays: int = 16384
+    optimize_size: bool = True
+    optimize_deviation: bool = True
+    ctx: Optional[Context] = None
+    _plan: Optional[Plan] = field(default=None, init=False, repr=False)
+
+    @property
+    def n_var(self) -> int:
+        return 3 * self.nglass + 1 + 4
+
+    @property
+    def n_obj(self) -> int:
+        return 1 + int(self.optimize_size) + int(self.optimize_deviation)
+
+    def params(self, X: np.ndarray) -> List["N.Params"]:
+        """The cpd_params records of a population X [pop, n_var] (create_spectrometer,
+        designer.py:203-265, with the detector left to positioning)."""
+        X = np.clip(np.asarray(X, np.float64), 0.0, 1.0)
+        g, P = len(self.glasses), self.nglass
+        out = []
+        for x in X:
+            p = N.Params()
+            idx = np.minimum((x[:P] * g).astype(np.int64), g - 1)
+            angles = -math.pi / 2 + math.pi * x[P:2 * P + 1]
+            tanh_len = np.minimum(x[2 * P + 1:3 * P + 1], 1.0 - 1e-12)
+            curvature, height, nym, det_angle = x[3 * P + 1:]
+            p.n_glass = P
+            p.ar_coated = int(self.ar_coated)
+            for i in range(P):
+                for j, c in enumerate(self.glasses[int(idx[i])].glass):
+                    p.glass[i][j] = c
+            for i in range(P + 1):
+                p.angles[i] = angles[i]
+            for i in range(P):
+                p.lengths[i] = math.atanh(tanh_len[i]) * self.max_height / 4
+            p.curvature = 1e-5 + (1.0 - 1e-5) * curvature
+            p.height = self.max_height * (1e-4 + (1.0 - 1e-4) * height)
+            p.width = self.prism_width
+            p.bin_count = self.bin_count
+            p.bin_size = self.bin_size
+            p.linear_slope = self.linear_slope
+            p.linear_intercept = self.linear_intercept
+            p.det_length = self.detector_length
+            p.max_incident_angle = self.max_incident_angle
+            p.det_angle = -math.pi + 2 * math.pi * det_angle
+            p.beam_width = self.beam_width
+            p.y_mean = p.height * nym
+            p.wavelength_lo, p.wavelength_hi = self.wavelength_range
+            out.append(p)
+        return out
+
+    def _plan_for(self, params: List["N.Params"]) -> Plan:
+        n = len(params)
+        if self._plan is not None and self._plan.n == n:
+            return self._plan
+        if self._plan is not None:
+            self._plan.close()
+        # every slot has the same shape (sampling, bin count): any buildable
+        # individual is a valid stand-in record for all of them; random
+        # parameter vectors mostly fail (total internal reflection), so draw
+        # more when the population has none
+        designs, st = build_designs(params, self.acceptance, ctx=self.ctx)
+        rng = np.random.default_rng(0)
+        for _ in range(64):
+            if (st == 0).any():
+                break
+            designs, st = build_designs(self.params(rng.random((256, self.n_var))), self.acceptance, ctx=self.ctx)
+        ok = np.flatnonzero(st == 0)
+        if ok.size == 0:
+            raise RuntimeError("no buildable design found for the plan's stand-in record")
+        shape = [designs[int(ok[0])]] * n
+        self._plan = Plan(shape, (self.n_wavelengths, self.n_rays), ctx=self.ctx)
+        return self._plan
+
+    def evaluate(self, X: np.ndarray):
+        """Objectives F [pop, n_obj] (size, log2(nbin) - info, deviation, as
+        designer.py:267-300 orders them), constraint G [pop] (0 feasible),
+        the raw fitness [pop, 3] and the per-design status."""
+        params = self.params(X)
+        fit, st = self._plan_for(params).run_params(params, self.acceptance)
+        max_size = self.max_height * 40 if self.optimize_size else np.inf
+        max_info = math.log2(self.bin_count)
+        n = len(params)
+        F = np.empty((n, self.n_obj))
+        G = np.ones(n)
+        for i in range(n):
+            size, info, dev = fit[i]
+            good = st[i] == 0 and np.isfinite(info) and max_info - info > 0
+            row = []
+            if good:
+                if self.optimize_size:
+                    row.append(size)
+                row.append(max_info - info)
+                if self.optimize_deviation:
+                    row.append(dev)
+                G[i] = 0.0 if (self.max_height / 2 < size < max_size) else 1.0
+            else:
+                if self.optimize_size:
+                    row.append(max_size * 10)
+                row.append(max_info)
+                if self.optimize_deviation:
+                    row.append(1.0)
+            F[i] = row
+        return F, G, fit, st
+
+    def close(self):
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
+
+
+def nondominated_ranks(F: np.ndarray) -> np.ndarray:
+    """Front index of each row of F (minimisation; 0 = non-dominated)."""
+    n = len(F)
+    le = (F[:, None, :] <= F[None, :, :]).all(-1)
+    lt = (F[:, None, :] < F[None, :, :]).any(-1)
+    dom = le & lt  # dom[i, j]: i dominates j
+    count = dom.sum(0)
+    rank = np.full(n, -1)
+    front = np.flatnonzero(count == 0)
+    r = 0
+    while front.size:
+        rank[front] = r
+        count = count - dom[front].sum(0)
+        count[rank >= 0] = -1
+        front = np.flatnonzero(count == 0)
+        r += 1
+    return rank
+
+
+def crowding(F: np.ndarray) -> np.ndarray:
+    n, m = F.shape
+    d = np.zeros(n)
+    if n <= 2:
+        return np.full(n, np.inf)
+    for k in range(m):
+        o = np.argsort(F[:, k], kind="stable")
+        span = F[o[-1], k] - F[o[0], k]
+        d[o[0]] = d[o[-1]] = np.inf
+        if span > 0:
+            d[o[1:-1]] += (F[o[2:], k] - F[o[:-2], k]) / span
+    return d
+
+
+def survive(F: np.ndarray, G: np.ndarray, k: int) -> np.ndarray:
+    """Indices of k survivors: feasible before infeasible (each group ranked on
+    its own, so an infeasible row never dominates a feasible one), then by
+    front, then by crowding distance (wider first)."""
+    rank = np.zeros(len(F), np.int64)
+    crowd = np.zeros(len(F))
+    for g in np.unique(G):
+        grp = np.flatnonzero(G == g)
+        rank[grp] = nondominated_ranks(F[grp])
+        for r in np.unique(rank[grp]):
+            idx = grp[rank[grp] == r]
+            crowd[idx] = crowding(F[idx])
+    order = np.lexsort((-crowd, rank, G))
+    return order[:k]
+
+
+def _sbx(rng, a, b, eta):
+    u = rng.random(a.shape)
+    beta = np.where(u <= 0.5, (2 * u) ** (1 / (eta + 1)), (1 / (2 * (1 - u))) ** (1 / (eta + 1)))
+    c1 = 0.5 * ((1 + beta) * a + (1 - beta) * b)
+    c2 = 0.5 * ((1 - beta) * a + (1 + beta) * b)
+    swap = rng.random(a.shape) < 0.5
+    return np.clip(np.where(swap, c2, c1), 0, 1)
+
+
+def _mutate(rng, x, eta, prob):
+    u = rng.random(x.shape)
+    delta = np.where(u < 0.5, (2 * u) ** (1 / (eta + 1)) - 1, 1 - (2 * (1 - u)) ** (1 / (eta + 1)))
+    m = rng.random(x.shape) < prob
+    return np.clip(np.where(m, x + delta * np.where(u < 0.5, x, 1 - x), x), 0, 1)
+
+
+def evolve(problem: SpectrometerProblem, pop_size: int = 64, generations: int = 10, seed: int = 0,
+           crossover_eta: float = 4.0, mutation_eta: float = 2.0, mutation_prob: Optional[float] = None,
+           X0: Optional[np.ndarray] = None):
+    """Elitist (mu + mu) multi-objective loop; every generation of pop_size
+    offspring is one cpd_plan_run_params call.  The distribution indices and
+    the per-variable mutation probability default to design_config.toml's
+    optimiser section (4, 2 and 0.05).  Returns the final population (X, F, G)
+    and a per-generation history."""
+    rng = np.random.default_rng(seed)
+    prob = mutation_prob if mutation_prob is not None else 0.05
+    X = rng.random((pop_size, problem.n_var)) if X0 is None else np.asarray(X0, np.float64)
+    t = time.perf_counter()
+    F, G, _, st = problem.evaluate(X)
+    history = [{"generation": 0, "eval_s": time.perf_counter() - t, "feasible": int((G == 0).sum()),
+                "built": int((st == 0).sum()), "best": F[G == 0].min(0).tolist() if (G == 0).any() else None}]
+    for gen in range(1, generations + 1):
+        # binary tournaments on (feasibility, front)
+        rank = nondominated_ranks(F) + 1000 * G
+        a, b = rng.integers(0, pop_size, (2, pop_size))
+        parents = np.where(rank[a] <= rank[b], a, b)
+        mates = np.roll(parents, 1)
+        Y = _mutate(rng, _sbx(rng, X[parents], X[mates], crossover_eta), mutation_eta, prob)
+        t = time.perf_counter()
+        FY, GY, _, stY = problem.evaluate(Y)
+        eval_s = time.perf_counter() - t
+        XX, FF, GG = np.vstack([X, Y]), np.vstack([F, FY]), np.concatenate([G, GY])
+        keep = survive(FF, GG, pop_size)
+        X, F, G = XX[keep], FF[keep], GG[keep]
+        history.append({"generation": gen, "eval_s": eval_s, "feasible": int((G == 0).sum()),
+                        "built": int((stY == 0).sum()),
+                        "best": F[G == 0].min(0).tolist() if (G == 0).any() else None})
+    return X, F, G, history
