@@ -563,8 +563,7 @@ class Plan:
             raise ValueError(f"plan holds {self.n} designs, got {len(designs)}")
         darr = designs if isinstance(designs, ctypes.Array) else (Design * self.n)(*designs)
         _check(self.run_raw(darr))
-        res = np.array([[f.size, f.info, f.deviation] for f in self._out], np.float64).reshape(self.n, 3)
-        return res, np.array(self._status[:], np.int32)
+        return self._results()
 
     def run_params(self, params: Sequence["N.Params"], acceptance: float = 1.0):
         """One generation from parameter vectors, built into designs on the GPU
@@ -575,8 +574,11 @@ class Plan:
             raise ValueError(f"plan holds {self.n} designs, got {len(params)}")
         parr = params if isinstance(params, ctypes.Array) else (N.Params * self.n)(*params)
         _check(N.lib.cpd_plan_run_params(self._h, parr, float(acceptance), self._out, self._status))
-        res = np.array([[f.size, f.info, f.deviation] for f in self._out], np.float64).reshape(self.n, 3)
-        return res, np.array(self._status[:], np.int32)
+        return self._results()
+
+    def _results(self):
+        res = np.frombuffer(self._out, np.float64).reshape(self.n, 3).copy()
+        return res, np.frombuffer(self._status, np.int32).copy()
 
     def close(self):
         if getattr(self, "_h", None):

@@ -47,6 +47,17 @@ def synthetic_catalog(n: int, seed: int = 0, wavelength_range=(0.5, 0.82)) -> Li
             for i in range(n)]
 
 
+_PARAMS_DTYPE = None
+
+
+def _params_dtype():
+    """numpy view type of cpd_params records (built once: it is slow to derive)."""
+    global _PARAMS_DTYPE
+    if _PARAMS_DTYPE is None:
+        _PARAMS_DTYPE = np.ctypeslib.as_array((N.Params * 1)()).dtype
+    return _PARAMS_DTYPE
+
+
 @dataclass
 class SpectrometerProblem:
     """The designer's problem (designer.py:115-201), Gaussian beam, detector
@@ -85,42 +96,35 @@ class SpectrometerProblem:
     def n_obj(self) -> int:
         return 1 + int(self.optimize_size) + int(self.optimize_deviation)
 
-    def params(self, X: np.ndarray) -> List["N.Params"]:
-        """The cpd_params records of a population X [pop, n_var] (create_spectrometer,
-        designer.py:203-265, with the detector left to positioning)."""
-        X = np.clip(np.asarray(X, np.float64), 0.0, 1.0)
-        g, P = len(self.glasses), self.nglass
-        out = []
-        for x in X:
-            p = N.Params()
-            idx = np.minimum((x[:P] * g).astype(np.int64), g - 1)
-            angles = -math.pi / 2 + math.pi * x[P:2 * P + 1]
-            tanh_len = np.minimum(x[2 * P + 1:3 * P + 1], 1.0 - 1e-12)
-            curvature, height, nym, det_angle = x[3 * P + 1:]
-            p.n_glass = P
-            p.ar_coated = int(self.ar_coated)
-            for i in range(P):
-                for j, c in enumerate(self.glasses[int(idx[i])].glass):
-                    p.glass[i][j] = c
-            for i in range(P + 1):
-                p.angles[i] = angles[i]
-            for i in range(P):
-                p.lengths[i] = math.atanh(tanh_len[i]) * self.max_height / 4
-            p.curvature = 1e-5 + (1.0 - 1e-5) * curvature
-            p.height = self.max_height * (1e-4 + (1.0 - 1e-4) * height)
-            p.width = self.prism_width
-            p.bin_count = self.bin_count
-            p.bin_size = self.bin_size
-            p.linear_slope = self.linear_slope
-            p.linear_intercept = self.linear_intercept
-            p.det_length = self.detector_length
-            p.max_incident_angle = self.max_incident_angle
-            p.det_angle = -math.pi + 2 * math.pi * det_angle
-            p.beam_width = self.beam_width
-            p.y_mean = p.height * nym
-            p.wavelength_lo, p.wavelength_hi = self.wavelength_range
-            out.append(p)
-        return out
+    def params(self, X: np.ndarray):
+        """The cpd_params records of a population X [pop, n_var] as a ctypes
+        array (create_spectrometer, designer.py:203-265, with the detector left
+        to positioning), filled through a numpy view of the records."""
+        X = np.clip(np.atleast_2d(np.asarray(X, np.float64)), 0.0, 1.0)
+        n, g, P = len(X), len(self.glasses), self.nglass
+        arr = (N.Params * n)()
+        a = np.frombuffer(arr, dtype=_params_dtype())
+        coeffs = np.array([gl.glass for gl in self.glasses], np.float64)
+        idx = np.minimum((X[:, :P] * g).astype(np.int64), g - 1)
+        a["n_glass"] = P
+        a["ar_coated"] = int(self.ar_coated)
+        a["glass"][:, :P, :] = coeffs[idx]
+        a["angles"][:, :P + 1] = -math.pi / 2 + math.pi * X[:, P:2 * P + 1]
+        a["lengths"][:, :P] = np.arctanh(np.minimum(X[:, 2 * P + 1:3 * P + 1], 1.0 - 1e-12)) * self.max_height / 4
+        a["curvature"] = 1e-5 + (1.0 - 1e-5) * X[:, 3 * P + 1]
+        a["height"] = self.max_height * (1e-4 + (1.0 - 1e-4) * X[:, 3 * P + 2])
+        a["y_mean"] = a["height"] * X[:, 3 * P + 3]
+        a["det_angle"] = -math.pi + 2 * math.pi * X[:, 3 * P + 4]
+        a["width"] = self.prism_width
+        a["bin_count"] = self.bin_count
+        a["bin_size"] = self.bin_size
+        a["linear_slope"] = self.linear_slope
+        a["linear_intercept"] = self.linear_intercept
+        a["det_length"] = self.detector_length
+        a["max_incident_angle"] = self.max_incident_angle
+        a["beam_width"] = self.beam_width
+        a["wavelength_lo"], a["wavelength_hi"] = self.wavelength_range
+        return arr
 
     def _plan_for(self, params: List["N.Params"]) -> Plan:
         n = len(params)
@@ -153,27 +157,18 @@ class SpectrometerProblem:
         fit, st = self._plan_for(params).run_params(params, self.acceptance)
         max_size = self.max_height * 40 if self.optimize_size else np.inf
         max_info = math.log2(self.bin_count)
-        n = len(params)
-        F = np.empty((n, self.n_obj))
-        G = np.ones(n)
-        for i in range(n):
-            size, info, dev = fit[i]
-            good = st[i] == 0 and np.isfinite(info) and max_info - info > 0
-            row = []
-            if good:
-                if self.optimize_size:
-                    row.append(size)
-                row.append(max_info - info)
-                if self.optimize_deviation:
-                    row.append(dev)
-                G[i] = 0.0 if (self.max_height / 2 < size < max_size) else 1.0
-            else:
-                if self.optimize_size:
-                    row.append(max_size * 10)
-                row.append(max_info)
-                if self.optimize_deviation:
-                    row.append(1.0)
-            F[i] = row
+        size, info, dev = fit[:, 0], fit[:, 1], fit[:, 2]
+        with np.errstate(invalid="ignore"):
+            good = (st == 0) & np.isfinite(info) & (max_info - info > 0)
+            feasible = good & (self.max_height / 2 < size) & (size < max_size)
+        cols = []
+        if self.optimize_size:
+            cols.append(np.where(good, size, max_size * 10))
+        cols.append(np.where(good, max_info - info, max_info))
+        if self.optimize_deviation:
+            cols.append(np.where(good, dev, 1.0))
+        F = np.stack(cols, axis=1)
+        G = np.where(feasible, 0.0, 1.0)
         return F, G, fit, st
 
     def close(self):

@@ -87,7 +87,7 @@ hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool c
                         hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a, int acc = CPD_ACC_EXACT);
 hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
-                               uint32_t tile_cap, TraceDesign *traces);
+                               uint32_t tile_cap, TraceDesign *traces, const int *status = nullptr);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int *p, uint32_t n);
 // Single rays: wavelength from its CDF value, beam height from its CDF value.
 hipError_t launch_propagate(hipStream_t s, const cpd_design *g, const TraceDesign *td, const float *wcdf,

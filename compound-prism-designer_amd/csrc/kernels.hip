@@ -391,8 +391,9 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.zs64 = a.zs64;
         c.full_end = M / kBandRays * kBandRays;
         c.last_pos = M <= c.full_end ? a.inv[M - 1] : M;
-        if (!c.lc.ok) {
-            // every ray of this wavelength is lost in the planar faces
+        if (!c.lc.ok || td->skip) {
+            // every ray of this wavelength is lost in the planar faces, or the
+            // design is a stand-in for one that failed construction
             if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
         } else {
             tile_loop<COUNT, A>(c, ops);
@@ -555,10 +556,13 @@ __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, con
 // FP32 trace records of n designs from their FP64 records and per-slot
 // sampling (one thread per design).
 __global__ void build_traces_kernel(const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
-                                    uint32_t tile_cap, TraceDesign *traces) {
+                                    uint32_t tile_cap, TraceDesign *traces, const int *status) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    make_trace_design(designs[i], sampling[i].n_wavelengths, sampling[i].n_rays, traces[i], tile_cap);
+    TraceDesign t;
+    make_trace_design(designs[i], sampling[i].n_wavelengths, sampling[i].n_rays, t, tile_cap);
+    if (status && status[i] != CPD_OK) t.skip = 1;
+    traces[i] = t;
 }
 
 }  // namespace
@@ -589,10 +593,10 @@ hipError_t launch_construct(hipStream_t s, const cpd_params *params, uint32_t n,
 }
 
 hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const cpd_sampling *sampling, uint32_t n,
-                               uint32_t tile_cap, TraceDesign *traces) {
+                               uint32_t tile_cap, TraceDesign *traces, const int *status) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(build_traces_kernel, dim3((n + 63) / 64), dim3(64), 0, s, designs, sampling, n, tile_cap,
-                       traces);
+                       traces, status);
     return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ struct Plan {
 
     cpd_status init(Context *ctx, const cpd_design *shape_in, const cpd_sampling *samp, size_t count,
                     const cpd_schedule *schedule);
-    hipError_t capture_evaluation();
+    hipError_t capture_evaluation(const int *status = nullptr);
     cpd_status capture_params();
     cpd_status run(const cpd_design *designs, cpd_fitness *out, cpd_status *per_design);
     cpd_status run_params(const cpd_params *params, double acceptance, cpd_fitness *out, cpd_status *per_design);
@@ -219,9 +219,11 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
 }
 
 // Trace records, cleared work counter, trace, reduction and download of the
-// results, recorded on the stream being captured.
-hipError_t Plan::capture_evaluation() {
-    hipError_t e = launch_build_traces(stream, args.designs, d_sampling, n, sched.tile_rays, d_traces.as<TraceDesign>());
+// results, recorded on the stream being captured.  With status, the work
+// items of designs whose status is not CPD_OK trace nothing.
+hipError_t Plan::capture_evaluation(const int *status) {
+    hipError_t e = launch_build_traces(stream, args.designs, d_sampling, n, sched.tile_rays, d_traces.as<TraceDesign>(),
+                                       status);
     if (e == hipSuccess) e = launch_zero_u32(stream, args.work_counter, 1);
     if (e == hipSuccess) e = launch_trace(stream, args, grid, false);
     if (e == hipSuccess) e = launch_finalize(stream, fin);
@@ -250,7 +252,7 @@ cpd_status Plan::capture_params() {
     if (e == hipSuccess)
         e = launch_construct(stream, (const cpd_params *)(dp + off_p), n, (const double *)dp,
                              d_shape.as<cpd_design>(), d_designs.as<cpd_design>(), d_status.as<int>());
-    if (e == hipSuccess) e = capture_evaluation();
+    if (e == hipSuccess) e = capture_evaluation(d_status.as<int>());
     if (e == hipSuccess)
         e = hipMemcpyAsync(h_status.ptr, d_status.ptr, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, stream);
     hipGraph_t g = nullptr;

@@ -279,7 +279,8 @@ struct alignas(16) TraceDesign {
     // exit lens relative to its chord midpoint m: centre offset cm = c - m,
     // 1/R, bound radius^2, error scales
     float cmx, cmy, inv_r, max_dist_sq;
-    float lens_scale, bound_scale, nbin_f, reserved0;
+    float lens_scale, bound_scale, nbin_f;
+    uint32_t skip;  // nonzero: the design failed construction; its work items trace nothing
     // detector, relative to m: m - P0, normal, direction, |pos - L/2| <= L/2
     float mpx, mpy, dnx, dny;
     float ddx, ddy, min_ci, half_len;
@@ -342,7 +343,7 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.lens_scale = (float)(g.arc_radius + sqrt(g.arc_max_dist_sq));
     t.bound_scale = (float)(4.0 * (sqrt(g.arc_max_dist_sq) + 1.0));
     t.nbin_f = (float)g.bin_count;
-    t.reserved0 = 0.f;
+    t.skip = 0;
     t.mpx = (float)(g.arc_midpt[0] - g.det_position[0]);
     t.mpy = (float)(g.arc_midpt[1] - g.det_position[1]);
     t.dnx = (float)g.det_normal[0];
