@@ -307,13 +307,14 @@ void parallel_for(size_t n, F body) {
 
 }  // namespace
 
-// Launch one block of rows and copy the rows back.  rows_host receives
-// n_rows x bin_count values at row_out offsets.
+// Launch one block of rows and copy the rows back.  rows_host points at
+// n_rows x bin_count values at row_out offsets, rays_host at the rows' ray
+// counts, both in the context's pinned buffers until its next run_rows.
 cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_dev,
                              const std::vector<uint32_t> &row_design, const std::vector<double> &row_lambda,
                              const std::vector<uint64_t> &row_out, const std::vector<uint32_t> &row_cap,
                              uint64_t out_n, uint32_t max_rays, uint32_t nbin_max, const double *z64_dev,
-                             const float *z32_dev, std::vector<double> &rows_host, std::vector<uint64_t> &rays_host) {
+                             const float *z32_dev, const double *&rows_host, const uint64_t *&rays_host) {
     const uint32_t n_rows = (uint32_t)row_design.size();
     if (n_rows == 0) return CPD_OK;
     const size_t off_lam = (n_rows * sizeof(uint32_t) + 255) / 256 * 256;
@@ -324,13 +325,17 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
     if ((st = row_stage.reserve(bytes, false, stream)) != CPD_OK) return st;
     if ((st = row_buf.reserve(out_n * sizeof(double) + n_rows * sizeof(uint64_t) + 256, false, stream)) != CPD_OK)
         return st;
-    std::vector<char> h(bytes);
-    memcpy(h.data(), row_design.data(), n_rows * sizeof(uint32_t));
-    memcpy(h.data() + off_lam, row_lambda.data(), n_rows * sizeof(double));
-    memcpy(h.data() + off_out, row_out.data(), n_rows * sizeof(uint64_t));
-    if (!row_cap.empty()) memcpy(h.data() + off_cap, row_cap.data(), n_rows * sizeof(uint32_t));
+    // pinned staging both ways: the rows' per-bin means are tens of MB
+    const size_t off_rays = (out_n * sizeof(double) + 255) / 256 * 256;
+    if ((st = row_args_pin.reserve(bytes)) != CPD_OK) return st;
+    if ((st = rows_pin.reserve(off_rays + n_rows * sizeof(uint64_t))) != CPD_OK) return st;
+    char *h = (char *)row_args_pin.ptr;
+    memcpy(h, row_design.data(), n_rows * sizeof(uint32_t));
+    memcpy(h + off_lam, row_lambda.data(), n_rows * sizeof(double));
+    memcpy(h + off_out, row_out.data(), n_rows * sizeof(uint64_t));
+    if (!row_cap.empty()) memcpy(h + off_cap, row_cap.data(), n_rows * sizeof(uint32_t));
     char *d = row_stage.as<char>();
-    AD_TRY(hipMemcpyAsync(d, h.data(), bytes, hipMemcpyHostToDevice, stream));
+    AD_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
     RowArgs a{};
     a.designs = designs_dev;
     a.row_design = (const uint32_t *)d;
@@ -356,10 +361,11 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
         hipLaunchKernelGGL(sem_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
     AD_TRY(hipGetLastError());
     if (profile) AD_TRY(hipEventRecord(ev[1], stream));
-    rows_host.resize(out_n);
-    rays_host.resize(n_rows);
-    AD_TRY(hipMemcpyAsync(rows_host.data(), a.out, out_n * sizeof(double), hipMemcpyDeviceToHost, stream));
-    AD_TRY(hipMemcpyAsync(rays_host.data(), a.row_rays, n_rows * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    char *hr = (char *)rows_pin.ptr;
+    AD_TRY(hipMemcpyAsync(hr, a.out, out_n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    AD_TRY(hipMemcpyAsync(hr + off_rays, a.row_rays, n_rows * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    rows_host = (const double *)hr;
+    rays_host = (const uint64_t *)(hr + off_rays);
     AD_TRY(hipStreamSynchronize(stream));
     if (profile) {
         float ms = 0.f;
@@ -406,8 +412,10 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
     std::vector<uint64_t> next_j(n, 1), block(n, 4);
     for (size_t q = 0; q < n; ++q) fold[q].reset(designs[valid[q]].bin_count);
     std::vector<uint32_t> row_design, row_cap;
-    std::vector<double> row_lambda, rows;
-    std::vector<uint64_t> row_out, rays;
+    std::vector<double> row_lambda;
+    std::vector<uint64_t> row_out;
+    const double *rows = nullptr;
+    const uint64_t *rays = nullptr;
     uint64_t total_rays = 0;
     for (;;) {
         row_design.clear();
@@ -442,7 +450,7 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
             for (size_t x = r; x < e && !fold[q].done; ++x) {
                 total_rays += rays[x];
                 fold[q].rays += rays[x];
-                if (fold[q].fold(rows.data() + row_out[x]) || fold[q].rows >= s.n_wavelengths) fold[q].done = true;
+                if (fold[q].fold(rows + row_out[x]) || fold[q].rows >= s.n_wavelengths) fold[q].done = true;
             }
             next_j[q] += e - r;
             block[q] = std::min<uint64_t>(block[q] * 4, 1024);
@@ -475,13 +483,15 @@ cpd_status Context::rows_adaptive(const cpd_design &g, const double *wavelengths
     std::vector<uint32_t> one{0};
     if ((st = stage_designs(&g, one, &ddev, &nbin_max)) != CPD_OK) return st;
     std::vector<uint32_t> row_design(n_w, 0), row_cap(n_w, max_m);
-    std::vector<double> row_lambda(wavelengths, wavelengths + n_w), rows;
-    std::vector<uint64_t> row_out(n_w), rays;
+    std::vector<double> row_lambda(wavelengths, wavelengths + n_w);
+    std::vector<uint64_t> row_out(n_w);
+    const double *rows = nullptr;
+    const uint64_t *rays = nullptr;
     for (size_t j = 0; j < n_w; ++j) row_out[j] = j * g.bin_count;
     if ((st = run_rows(false, sem_fp32(), ddev, row_design, row_lambda, row_out, row_cap, n_w * g.bin_count,
                        max_m, nbin_max, beam.z64.as<double>(), beam.z32.as<float>(), rows, rays)) != CPD_OK)
         return st;
-    memcpy(out, rows.data(), n_w * g.bin_count * sizeof(double));
+    memcpy(out, rows, n_w * g.bin_count * sizeof(double));
     return CPD_OK;
 }
 
@@ -506,8 +516,10 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
     std::vector<OuterFold> fold(n);
     for (size_t q = 0; q < n; ++q) fold[q].reset(designs[valid[q]].bin_count);
     std::vector<uint32_t> row_design;
-    std::vector<double> row_lambda, rows;
-    std::vector<uint64_t> row_out, rays;
+    std::vector<double> row_lambda;
+    std::vector<uint64_t> row_out;
+    const double *rows = nullptr;
+    const uint64_t *rays = nullptr;
     uint64_t total_rays = 0;
     for (size_t si = 0; si < n_seeds; ++si) {
         const uint64_t seed_fx = seed_to_fx(seeds[si]);
@@ -540,7 +552,7 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
         parallel_for(starts.size(), [&](size_t i) {
             const size_t r0 = starts[i];
             OuterFold &f = fold[row_design[r0]];
-            for (size_t r = r0; r < r0 + n_rows_per; ++r) f.fold(rows.data() + row_out[r]);
+            for (size_t r = r0; r < r0 + n_rows_per; ++r) f.fold(rows + row_out[r]);
             if (f.rows && f.converged()) f.done = true;
         });
         fold_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();

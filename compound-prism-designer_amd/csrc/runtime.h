@@ -68,6 +68,7 @@ struct Context {
     // adaptive estimators: row descriptors, design records, seeded beam tables
     DeviceBuffer row_stage, design_stage, seed_z32, seed_z64;
     HostBuffer host_stage, host_out;
+    HostBuffer row_args_pin, rows_pin;  // adaptive rows: descriptors up, per-bin means and ray counts down
 
     static cpd_status create(int device, Context **out);
     ~Context();
@@ -91,8 +92,8 @@ struct Context {
     cpd_status run_rows(bool chunked, bool fp32, const cpd_design *designs_dev, const std::vector<uint32_t> &row_design,
                         const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
                         const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays, uint32_t nbin_max,
-                        const double *z64_dev, const float *z32_dev, std::vector<double> &rows_host,
-                        std::vector<uint64_t> &rays_host);
+                        const double *z64_dev, const float *z32_dev, const double *&rows_host,
+                        const uint64_t *&rays_host);
     cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
                              const cpd_design **designs_dev, uint32_t *nbin_max);
     void apply(const cpd_schedule &s);

@@ -411,6 +411,8 @@ Context::~Context() {
     seed_z32.release();
     seed_z64.release();
     host_stage.release();
+    row_args_pin.release();
+    rows_pin.release();
     host_out.release();
     for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
