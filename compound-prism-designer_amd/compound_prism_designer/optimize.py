@@ -146,7 +146,10 @@ class SpectrometerProblem:
         if ok.size == 0:
             raise RuntimeError("no buildable design found for the plan's stand-in record")
         shape = [designs[int(ok[0])]] * n
-        self._plan = Plan(shape, (self.n_wavelengths, self.n_rays), ctx=self.ctx)
+        # the default schedule: tuning on n copies of one stand-in says nothing
+        # about the populations to come
+        self._plan = Plan(shape, (self.n_wavelengths, self.n_rays), ctx=self.ctx,
+                          schedule=N.Schedule(0, 16, N.ORDER_GIVEN, 0))
         return self._plan
 
     def evaluate(self, X: np.ndarray):
