@@ -569,9 +569,11 @@ struct Trace32Result {
 struct alignas(4) HotRecord {
     uint32_t nbin, ar_coated;
     float y_mean, beam_w, ylo, yhi, ymarg;
+    float disc_lim;  // margin of the lens discriminant for every accepted height
+    float cd_lim;    // margin of the acceptance cosine, times cos(theta_t), for every ray
     float e0x, e0y, e1x, e1y, lb0, lb1, lc0, lc1, lc2;
     float dx, dy, rdx, rdy, ts, tp, amp, path;
-    float cmx, cmy, inv_r, max_dist_sq, lens_scale, bound_scale;
+    float cmx, cmy, inv_r, max_dist_sq, bound_scale;
     float lr2r, lomr2, lr, nd0, nd1, ddn, rddn;
     float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
     float slope, inv_slope, intercept, bin_size, nbin_f;
@@ -590,6 +592,15 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     // kGuardRel) / (1 - kGuardRel), so kGuardRel (max |y_b| + 1) /
     // (1 - kGuardRel), rounded up, covers every such ray.
     h.ymarg = kGuardRel * (fmaxf(fabsf(lc.ylo), fabsf(lc.yhi)) + 1.f) * (1.f + 2.f * kGuardRel) * 1.000001f;
+    // b(y) = lb0 + lb1 y is affine, so its largest magnitude over the accepted
+    // heights bounds |b| for every ray the discriminant decides; the margin
+    // 4 kGuardRel path A (|b| + lens_scale) at that bound covers each ray's.
+    const float bmax = fmaxf(fabsf(fmaf(lc.lb1, lc.ylo, lc.lb0)), fabsf(fmaf(lc.lb1, lc.yhi, lc.lb0))) * 1.000001f;
+    h.disc_lim = kGuardRel * lc.path * lc.amp * 4.f * (bmax + d.lens_scale) * 1.000001f;
+    // the acceptance cosine's margin kGuardCos A max(r ci, cr) / cr: with
+    // ci, cr <= 1, A max(r, 1) bounds the numerator for every ray (FP32 ci
+    // and cr can exceed 1 by ~1e-6, hence the extra 1e-4)
+    h.cd_lim = kGuardCos * lc.amp * fmaxf(lc.lr, 1.f) * 1.0001f;
     h.e0x = lc.e0x;
     h.e0y = lc.e0y;
     h.e1x = lc.e1x;
@@ -611,7 +622,6 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     h.cmy = d.cmy;
     h.inv_r = d.inv_r;
     h.max_dist_sq = d.max_dist_sq;
-    h.lens_scale = d.lens_scale;
     h.bound_scale = d.bound_scale;
     h.lr2r = lc.lr2r;
     h.lomr2 = lc.lomr2;
@@ -661,7 +671,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     const float b = fmaf(d.lb1, y, d.lb0);
     const float c = fmaf(fmaf(d.lc2, y, d.lc1), y, d.lc0);
     const float disc = fmaf(b, b, -c);
-    guard |= alive & (fabsf(disc) <= lim * 4.f * (fabsf(b) + d.lens_scale));
+    guard |= alive & (fabsf(disc) <= d.disc_lim);
     alive &= disc > 0.f;
     const float sd = fast_sqrt(fmaxf(disc, 0.f));
     const float big = -(b + copysignf(sd, b));
@@ -694,7 +704,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     guard |= alive & (fabsf(cr_sq) <= kGuardCos);
     alive &= cr_sq >= 0.f;
     guard |= alive & ar & (fabsf(ci - 0.5f) <= kGuardCos * num0);
-    const float num = num0 * fmaxf(rci, cr), den = cr;
+    const float den = cr;
     // detector array: w = o - P0 = (m - P0) + q
     if (COUNT) res.ops += alive ? kOpsDetector : 0u;
     {
@@ -702,7 +712,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         // n.dn = ((cm - e).dn - tq dir.dn) / R and (cm - e).dn = nd0 + nd1 y
         const float ndn = fmaf(-tq, d.ddn, fmaf(d.nd1, y, d.nd0)) * d.inv_r;
         const float cd = -fmaf(k, ndn, d.rddn);
-        guard |= alive & (fabsf(cd - d.min_ci) * den <= kGuardCos * num);
+        guard |= alive & (fabsf(cd - d.min_ci) * den <= d.cd_lim);
         alive &= cd > d.min_ci;
         // most wavefronts lose every ray here (the acceptance cosine): skip
         // the rest of the stage when no lane has a ray left or near a decision
@@ -722,6 +732,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         const float wx = d.mpx + qx, wy = d.mpy + qy;
         const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(cd);
         const float pos = fmaf(fmaf(vx, dist, wx), d.ddx, fmaf(vy, dist, wy) * d.ddy);
+        const float num = num0 * fmaxf(rci, cr);
         const float plim = kGuardRel * (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * num;
         const float apos = pos - d.half_len;
         guard |= alive & (fabsf(fabsf(apos) - d.half_len) * den <= plim);
