@@ -4,12 +4,15 @@
 // sample, tile of rays), taken from an atomic counter by a persistent grid.
 // Per item the wavefront builds the wavelength's constants in FP64 (the
 // planar faces collapse to an interval of beam heights and an affine exit
-// point), copies them into scalar registers, and lane l traces rays
-// k = k0 + l, k0 + 64 + l, ... of the tile in FP32 (trace32), adding each
-// hit to its own column of a [bin][column] LDS histogram.  Rays whose FP32
-// outcome lies within the error margin of a hit/miss or bin decision go to a
-// per-wave LDS queue and are re-traced 64 at a time in FP64 (trace64, the
-// reference's operation order).  With the default accumulation a ray adds
+// point; their FP64 directions and transmittances go to an LDS record), holds
+// them in registers (scalar, or vector where an operand would otherwise be
+// copied per use), and lane l traces positions k0 + l, k0 + 64 + l, ... of
+// the tile in FP32 (trace32) from the banded beam table, adding each hit to
+// its own column of a [bin][column] LDS histogram.  Rays whose FP32 outcome
+// lies within the error margin of a hit/miss or bin decision go to a
+// per-wave LDS queue and are re-traced 64 at a time in FP64 (trace64_chain,
+// the reference's operation order).  Items of designs that failed
+// construction (a plan's stand-ins) trace nothing.  With the default accumulation a ray adds
 // round(t * 2^22) to a u32 column; the columns are folded into a u64 row
 // before they can overflow (skewed, so the lanes hit distinct banks), and
 // the row is added to the design's per-(wavelength, bin) sums with integer

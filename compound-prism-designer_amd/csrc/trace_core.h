@@ -9,12 +9,17 @@
 //    which avoids the cancellation of |o - c|^2 - R^2 at lens radii far
 //    larger than the prism.  Every comparison whose outcome decides hit/miss
 //    or the bin carries an error margin; a ray that lands inside a margin is
-//    flagged for the FP64 trace instead of being counted.
+//    flagged for the FP64 trace instead of being counted.  Wave-uniform
+//    exits follow the planar-chain test and the detector's acceptance test,
+//    which a ray passes or fails before its hit point and direction at the
+//    lens are formed.
 //  * trace64: the FP64 trace in the reference's operation order
 //    (src/spectrometer/ray.rs:309-382, 485-499; geometry.rs:124-145,
 //    161-216; spectrometer.rs:265-303), with fused multiply-adds exactly at the
-//    reference's mul_add sites.  Flagged rays are re-traced with it, so their
-//    bin decision is the FP64 one.
+//    reference's mul_add sites; trace64_chain is the same trace with the
+//    planar faces' directions and transmittances taken from the wavelength's
+//    Chain64 record.  Flagged rays are re-traced with it, so their bin
+//    decision is the FP64 one.
 //
 // The code is compiled for the device (HIP) and, for detector positioning and
 // construction, for the host.
