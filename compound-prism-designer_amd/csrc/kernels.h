@@ -44,6 +44,7 @@ struct BatchArgs {
     const uint32_t *inv;         // 1-based banded position of ray k (index k - 1)
     unsigned long long *ops;     // algorithmic operation count (may be null)
     unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
+    uint32_t items_per_fetch;    // consecutive items one counter fetch claims (0: 1)
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)
     uint32_t n_designs;          // entries
     uint32_t hist_words;         // max over the entries of nbin * hist_cols
@@ -71,6 +72,22 @@ struct FinalizeArgs {
 CPD_HD uint32_t trace_hist_end_words(uint32_t hist_words, uint32_t cell_bytes) {
     return (hist_words * (cell_bytes / 4) + 1) & ~1u;
 }
+// Items one fetch of the persistent grid's work counter claims: enough that a
+// fetch covers ~16384 rays on average (at most 16), because short rows
+// otherwise serialise on the counter; one for long rows, for the balance;
+// at most items / (4 grid), so every wavefront still fetches several times;
+// and never so many that the u32 counter could wrap.
+inline uint32_t items_per_fetch(uint64_t rays, uint64_t items, uint32_t grid) {
+    if (items == 0 || grid == 0) return 1;
+    const uint64_t per = rays / items;
+    uint64_t g = per >= 16384 ? 1 : 16384 / (per ? per : 1);
+    if (g > 16) g = 16;
+    const uint64_t balance = items / (4 * (uint64_t)grid);
+    if (g > balance) g = balance ? balance : 1;
+    while (g > 1 && items + (uint64_t)grid * g >= (1ull << 32)) g >>= 1;
+    return (uint32_t)g;
+}
+
 // Byte offset of the wavefront's chain record (Chain64), which follows the
 // histogram, the queue and the row.
 CPD_HD uint32_t trace_chain_offset(uint32_t hist_words, uint32_t row_words, int acc) {

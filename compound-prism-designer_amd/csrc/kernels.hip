@@ -355,11 +355,18 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
     __syncthreads();
     unsigned long long ops = 0;
     uint64_t item = blockIdx.x;
+    // items claimed by the wavefront's last counter fetch: [next, end)
+    const uint32_t per_fetch = a.items_per_fetch > 1 ? a.items_per_fetch : 1u;
+    uint64_t next = 0, end = 0;
     for (;;) {
         if (a.work_counter) {
-            uint32_t v = 0;
-            if (lane == 0) v = atomicAdd(a.work_counter, 1u);
-            item = (uint64_t)__builtin_amdgcn_readfirstlane(v);
+            if (next == end) {
+                uint32_t v = 0;
+                if (lane == 0) v = atomicAdd(a.work_counter, per_fetch);
+                next = (uint64_t)__builtin_amdgcn_readfirstlane(v);
+                end = next + per_fetch;
+            }
+            item = next++;  // claims ascend, so the first item past n_items ends the loop
         }
         if (item >= a.n_items) break;
         const uint32_t e = find_design(a.item_begin, a.n_designs, item);

@@ -128,7 +128,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     // shape-only layout, in staging order
     std::vector<cpd_sampling> samp_q(n);
     std::vector<uint64_t> item_begin(n + 1), acc_begin(n), lb_begin(n);
-    uint64_t items = 0, acc_n = 0, lb_n = 0;
+    uint64_t items = 0, acc_n = 0, lb_n = 0, rays = 0;
     uint32_t hist_words = 0, row_words = 0, max_rays = 0;
     for (uint32_t q = 0; q < n; ++q) {
         const uint32_t i = perm[q];
@@ -139,6 +139,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
         acc_begin[q] = acc_n;
         lb_begin[q] = lb_n;
         items += (uint64_t)sampling[i].n_wavelengths * t.n_tiles;
+        rays += (uint64_t)sampling[i].n_wavelengths * sampling[i].n_rays;
         acc_n += (uint64_t)sampling[i].n_wavelengths * shape[i].bin_count;
         lb_n += sampling[i].n_wavelengths;
         hist_words = std::max(hist_words, t.hist_cols * t.nbin);
@@ -195,6 +196,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     a.row_words = row_words;
     a.n_items = items;
     grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)ctx->cu_count * sched.waves_per_cu);
+    a.items_per_fetch = items_per_fetch(rays, items, grid);
     d_sampling = (const cpd_sampling *)(fx + off_samp);
     FinalizeArgs &f = fin;
     f.designs = a.designs;

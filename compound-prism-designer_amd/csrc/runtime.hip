@@ -281,7 +281,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     uint64_t *hacc = (uint64_t *)(h + off_acc);
     uint64_t *hlb = (uint64_t *)(h + off_lb);
     uint64_t *hrow = (uint64_t *)(h + off_rows);
-    uint64_t items = 0, acc_n = 0, lb_n = 0;
+    uint64_t items = 0, acc_n = 0, lb_n = 0, rays = 0;
     uint32_t hist_words = 0, row_words = 0, tile = 0;
     for (uint32_t q = 0; q < n; ++q) {
         const cpd_design &g = designs[valid[q]];
@@ -297,6 +297,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
         hlb[q] = lb_n;
         hrow[q] = acc_n;
         items += (uint64_t)s.n_wavelengths * ht[q].n_tiles;
+        rays += (uint64_t)s.n_wavelengths * s.n_rays;
         acc_n += (uint64_t)s.n_wavelengths * g.bin_count;
         lb_n += s.n_wavelengths;
     }
@@ -344,6 +345,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
         a.work_counter = dwork;
         const uint64_t resident = (uint64_t)cu_count * waves_per_cu;
         grid = (uint32_t)std::min<uint64_t>(items, resident);
+        a.items_per_fetch = items_per_fetch(rays, items, grid);
     } else {
         a.work_counter = nullptr;
         grid = (uint32_t)items;

@@ -98,6 +98,18 @@ def test_batch_result_independent_of_companions():
         assert np.array_equal(sub[q], batch[i])
 
 
+def test_short_rows_claimed_several_at_a_time():
+    # 1024 designs x 64 wavelengths of 1024 rays: each fetch of the persistent
+    # grid's work counter claims several consecutive work items (rows this
+    # short would otherwise serialise on the counter); the results are the
+    # one-launch-per-design results bit for bit
+    _, D, _, _ = cpd.generate_designs(1, 0, 1024)
+    batch, st = cpd.fitness_batch(D, (64, 1024))
+    assert (st == 0).all()
+    for i in range(0, 1024, 97):
+        assert np.array_equal(gpu_fitness(D[i], 64, 1024), batch[i]), i
+
+
 def test_propagation_matches_oracle_per_ray():
     # src/cuda_fitness.rs:533-607: 1024 seeded (wavelength cdf, ray cdf) pairs on the known design
     d = kat_design()
