@@ -89,6 +89,24 @@ __device__ __forceinline__ double half_sum(double v) {
     return v;
 }
 
+// half_sum's total evaluated by one lane from the 32 values v(0..31), in the
+// order every lane of half_sum adds in (each butterfly step adds the same two
+// partial sums on both partners): s16(i) = v(i) + v(i + 16), s8(i) = s16(i)
+// + s16(i + 8), s4(i) = s8(i) + s8(i + 4), then s2 and s1 alike, so the
+// result is half_sum's bit for bit.  Each s4 is formed from its own eight
+// values behind a compiler barrier, which keeps eight of the 32 live at once.
+template <class V>
+__device__ __forceinline__ double butterfly_sum(V v) {
+#pragma clang fp contract(off)
+    double s4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        asm volatile("" ::: "memory");
+        s4[i] = ((v(i) + v(i + 16)) + (v(i + 8) + v(i + 24))) + ((v(i + 4) + v(i + 20)) + (v(i + 12) + v(i + 28)));
+    }
+    return (s4[0] + s4[2]) + (s4[1] + s4[3]);
+}
+
 __device__ __forceinline__ W64 load_w(const double *lds, uint32_t nbin_max, uint32_t b) {
     return W64{lds[b], lds[nbin_max + b], lds[2 * nbin_max + b]};
 }
@@ -119,16 +137,18 @@ __device__ __forceinline__ uint32_t row_ray(const RowArgs &a, const cpd_design &
     return bin;
 }
 
-// Per-row constants of the FP32 trace: the planar chain of this wavelength.
+// Per-row constants of the FP32 trace: the planar chain of this wavelength
+// (and its FP64 record, where chain is given).
 template <bool FP32>
-__device__ __forceinline__ HotRecord row_hot(const cpd_design &g, double lambda, bool &ok) {
+__device__ __forceinline__ HotRecord row_hot(const cpd_design &g, double lambda, bool &ok,
+                                             Chain64 *chain = nullptr) {
     HotRecord hot{};
     ok = true;
     if (FP32) {
         TraceDesign td;
         make_trace_design(g, 1, 64, td);
         LambdaConsts lc;
-        make_lambda_consts(g, lambda, lc);
+        make_lambda_consts(g, lambda, lc, chain);
         ok = lc.ok != 0;
         hot = make_hot(td, lc);
     }
@@ -190,54 +210,156 @@ __global__ __launch_bounds__(64) void sem_rows_kernel(RowArgs a) {
     if (lane == 0) a.row_rays[r] = (uint64_t)count;
 }
 
-// kernel.rs:89-193: 32-ray chunks; a wavefront traces two chunks at once
-// (lanes 0-31 and 32-63) and folds them in order.
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    return (1ull << __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u))) - 1ull;
+}
+
+// Passes of 64 rays (two chunks) traced ahead of the fold.
+constexpr uint32_t kAhead = 4;
+
+// LDS of chunk_rows_kernel after the 3 nbin_max Welford doubles: a half's
+// (transmittance, bin) pairs, the traced passes' transmittances, bins and
+// re-trace queue, and the wavelength's FP64 chain record.
+__host__ __device__ constexpr size_t chunk_lds_bytes(uint32_t nbin_max) {
+    return (size_t)3 * nbin_max * sizeof(double) + 32 * 16 + kAhead * 64 * (sizeof(double) + 2 * sizeof(uint32_t)) +
+           sizeof(Chain64);
+}
+
+// kernel.rs:89-193: 32-ray chunks folded in order, each chunk's
+// (32, mean, m2) over its 32 lanes into every bin it hit, with the SEM test
+// after each chunk.  A wavefront traces kAhead passes of two chunks (lanes
+// 0-31 and 32-63) before folding them: the FP32 trace's flagged rays of all
+// the passes are queued and re-traced together in FP64 through the
+// wavelength's chain record, and rays traced beyond the chunk where the row
+// stops are not folded.  In the fold, the two lowest bins of a chunk are
+// folded one at a time with half-wave butterflies; when more remain, every
+// lane that hit one of them evaluates the same butterfly sums for its own
+// bin from LDS, so a chunk spread over many bins costs one pass instead of
+// one per bin.
 template <bool FP32>
 __global__ __launch_bounds__(64) void chunk_rows_kernel(RowArgs a) {
 #pragma clang fp contract(off)
     extern __shared__ double lds[];
+    double2 *tpair = (double2 *)(lds + 3 * a.nbin_max);  // bin ~0u: nothing left to fold
+    double *tv = (double *)(tpair + 32);                 // [kAhead * 64] pass p, lane at p * 64 + lane
+    uint32_t *bv = (uint32_t *)(tv + kAhead * 64);
+    uint32_t *queue = bv + kAhead * 64;
+    Chain64 *chain = (Chain64 *)(queue + kAhead * 64);
     const uint32_t r = blockIdx.x, lane = threadIdx.x;
     const cpd_design &g = a.designs[a.row_design[r]];
     const uint32_t nbin = g.bin_count;
     for (uint32_t b = lane; b < nbin; b += 64) store_w(lds, a.nbin_max, b, W64{0.0, 0.0, 0.0});
-    __syncthreads();
     const double lambda = a.row_lambda[r];
     const double max_err_sq = kMaxErr * kMaxErr;
     const uint32_t iters = a.max_rays / 32;
     const uint32_t half = lane >> 5, l = lane & 31;
     bool chain_ok;
-    const HotRecord hot = row_hot<FP32>(g, lambda, chain_ok);
+    const HotRecord hot = row_hot<FP32>(g, lambda, chain_ok, lane == 0 ? chain : nullptr);
+    __syncthreads();
     double count = 0.0;
     bool stop = false;
-    for (uint32_t i = 0; i < iters && !stop; i += 2) {
-        const uint32_t c = i + half;
-        double t;
-        const uint32_t k = l + 32 * (c + 1);
-        const uint32_t bin = chain_ok ? row_ray<FP32>(a, g, lambda, hot, c < iters, k, t) : nbin;
-        if (!chain_ok) t = 0.0;
-        for (uint32_t h = 0; h < 2 && i + h < iters; ++h) {
-            count += 32.0;
-            const bool mine = half == h;
-            bool pending = mine && bin < nbin;
-            bool finished = __ballot(pending) != 0;
-            while (__ballot(pending) != 0) {
-                const uint32_t bmin = wave_min_u32(pending ? bin : 0xffffffffu);
-                const double tb = (mine && bin == bmin) ? t : 0.0;
-                const double wmean = readlane_f64(half_sum(tb), 32 * h) / 32.0;
-                const double dev = mine ? tb - wmean : 0.0;
-                const double wm2 = readlane_f64(half_sum(dev * dev), 32 * h);
-                const W64 w = w_add(load_w(lds, a.nbin_max, bmin), W64{32.0, wmean, wm2});
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) store_w(lds, a.nbin_max, bmin, w);
-                __builtin_amdgcn_wave_barrier();
-                finished = finished && w_sem_le(w, max_err_sq);
-                if (bin == bmin) pending = false;
+    for (uint32_t i0 = 0; i0 < iters && !stop; i0 += 2 * kAhead) {
+        const uint32_t passes = min(kAhead, (iters - i0 + 1) / 2);
+        // trace: ray k = l + 32 (c + 1) of chunk c = i0 + 2 p + half
+        uint32_t queued = 0;
+        for (uint32_t p = 0; p < passes; ++p) {
+            const uint32_t c = i0 + 2 * p + half;
+            const bool active = chain_ok && c < iters;
+            const uint32_t k = l + 32 * (c + 1);
+            double t = 0.0;
+            uint32_t bin = nbin;
+            bool redo = false;
+            if (FP32) {
+                const float z = active ? a.z32[k - 1] : 0.f;
+                const Trace32Result tr = trace32<false>(hot, hot.y_mean - hot.beam_w * z);
+                redo = active && tr.guard;
+                if (active && !tr.guard) {
+                    bin = tr.bin;
+                    t = tr.bin < nbin ? (double)tr.t : 0.0;
+                }
+            } else if (active) {
+                bin = trace64(g, lambda, beam_y64(g, a.z64[k - 1]), t);
             }
-            if (finished) {
-                stop = true;
-                break;
+            const uint32_t slot = p * 64 + lane;
+            tv[slot] = t;
+            bv[slot] = bin;
+            if (FP32) {
+                const uint64_t mask = __builtin_amdgcn_ballot_w64(redo);
+                if (redo) queue[queued + __popcll(mask & lanemask_lt())] = slot;
+                queued += __popcll(mask);
             }
         }
+        // FP64 re-trace of the queued rays, 64 at a time
+        if (FP32) {
+            for (uint32_t q0 = 0; q0 < queued; q0 += 64) {
+                __builtin_amdgcn_wave_barrier();
+                if (q0 + lane < queued) {
+                    const uint32_t slot = queue[q0 + lane];
+                    const uint32_t c = i0 + 2 * (slot >> 6) + ((slot >> 5) & 1);
+                    const uint32_t k = (slot & 31) + 32 * (c + 1);
+                    double t;
+                    const uint32_t bin = trace64_chain(g, *chain, beam_y64(g, a.z64[k - 1]), t);
+                    tv[slot] = bin < nbin ? t : 0.0;
+                    bv[slot] = bin;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // fold the passes' chunks in order
+        for (uint32_t p = 0; p < passes && !stop; ++p) {
+            const uint32_t i = i0 + 2 * p;
+            const double t = tv[p * 64 + lane];
+            const uint32_t bin = bv[p * 64 + lane];
+            for (uint32_t h = 0; h < 2 && i + h < iters; ++h) {
+                count += 32.0;
+                const bool mine = half == h;
+                bool pending = mine && bin < nbin;
+                bool finished = __ballot(pending) != 0;
+                for (int serial = 0; serial < 2 && __ballot(pending) != 0; ++serial) {
+                    const uint32_t bmin = wave_min_u32(pending ? bin : 0xffffffffu);
+                    const double tb = (mine && bin == bmin) ? t : 0.0;
+                    const double wmean = readlane_f64(half_sum(tb), 32 * h) / 32.0;
+                    const double dev = mine ? tb - wmean : 0.0;
+                    const double wm2 = readlane_f64(half_sum(dev * dev), 32 * h);
+                    const W64 w = w_add(load_w(lds, a.nbin_max, bmin), W64{32.0, wmean, wm2});
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) store_w(lds, a.nbin_max, bmin, w);
+                    __builtin_amdgcn_wave_barrier();
+                    finished = finished && w_sem_le(w, max_err_sq);
+                    if (bin == bmin) pending = false;
+                }
+                if (__ballot(pending) != 0) {
+                    // the remaining bins at once; lanes that share a bin
+                    // compute and store the same record
+                    __builtin_amdgcn_wave_barrier();
+                    if (mine) tpair[l] = make_double2(t, __longlong_as_double(pending ? bin : ~0u));
+                    __builtin_amdgcn_wave_barrier();
+                    bool ok = true;
+                    if (pending) {
+                        auto v = [&](int j) {
+                            const double2 pr = tpair[j];
+                            return (uint32_t)__double_as_longlong(pr.y) == bin ? pr.x : 0.0;
+                        };
+                        const double wmean = butterfly_sum(v) / 32.0;
+                        const double wm2 = butterfly_sum([&](int j) {
+                            const double dev = v(j) - wmean;
+                            return dev * dev;
+                        });
+                        const W64 w = w_add(load_w(lds, a.nbin_max, bin), W64{32.0, wmean, wm2});
+                        store_w(lds, a.nbin_max, bin, w);
+                        ok = w_sem_le(w, max_err_sq);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    finished = finished && __ballot(pending && !ok) == 0;
+                    pending = false;
+                }
+                if (finished) {
+                    stop = true;
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
     for (uint32_t b = lane; b < nbin; b += 64) {
@@ -349,7 +471,7 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
     a.n_rows = n_rows;
     a.max_rays = max_rays;
     a.nbin_max = nbin_max;
-    const size_t lds = (size_t)3 * nbin_max * sizeof(double);
+    const size_t lds = chunked ? chunk_lds_bytes(nbin_max) : (size_t)3 * nbin_max * sizeof(double);
     if (profile) AD_TRY(hipEventRecord(ev[0], stream));
     if (chunked && fp32)
         hipLaunchKernelGGL(chunk_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
