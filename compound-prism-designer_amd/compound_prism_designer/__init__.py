@@ -1,4 +1,4 @@
-"""compound_prism_designer -- MI355X fitness evaluator for 2D compound-prism spectrometers.
+"""compound_prism_designer -- MI355X fitness evaluator for compound-prism spectrometers.
 
 The Python surface mirrors the reference's pyo3 module (src/pylib.rs,
 compound_prism_designer/compound_prism_designer.pyi): Glass, DesignFitness,
@@ -9,6 +9,10 @@ fitness and probability evaluation runs on the GPU through libcpd.so
 (include/cpd.h).  There is no CPU fitness path: the methods that the
 reference ran on the CPU (cpu_fitness, transmission_probability) evaluate the
 same estimator on the GPU.
+
+A CompoundPrism given a curvature pair ((ip, it), (fp, ft)) is the
+reference's 3D variant (toric entrance and exit faces); its spectrometers take
+a FiberBeam or a GaussianBeam and run on the cpd3_* entry points.
 
 Beyond the reference: fitness_batch() evaluates many designs in one
 persistent launch, and Context selects a GPU.
@@ -22,7 +26,7 @@ from typing import Iterable, Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from ._native import Design, Fitness, Params, Sampling
+from ._native import Design, Design3, Fitness, Params, Params3, Sampling
 
 __all__ = [
     "autotune",
@@ -48,6 +52,9 @@ __all__ = [
     "Server",
     "generate_design",
     "generate_designs",
+    "fitness_batch3",
+    "gpu_fitness_batch3",
+    "generate_designs3",
 ]
 
 
@@ -180,7 +187,9 @@ class GaussianBeam:
 
 
 class FiberBeam:
-    """Fiber beam of the 3D toric variant (src/pylib.rs:380-430); not evaluated by this build."""
+    """Multi-mode fiber beam (src/pylib.rs:380-430, spectrometer.rs:70-149): a uniform disc of
+    the core radius around center_y, directions uniform over the acceptance cone of the
+    numerical aperture.  3D spectrometers only (the reference's 2D fiber beam is unimplemented)."""
 
     def __init__(self, core_radius: float, numerical_aperture: float, center_y: float):
         self.core_radius = float(core_radius)
@@ -212,17 +221,24 @@ class CompoundPrism:
             raise ValueError("len(glasses) + 1 != len(angles)")
         if len(glasses) != len(lengths):
             raise ValueError("len(glasses) != len(lengths)")
-        if isinstance(curvature, (tuple, list)):
-            raise NotImplementedError("the 3D toric compound prism is not part of this build")
         self.glasses = glasses
         self.angles = angles
         self.lengths = lengths
-        self.curvature = float(curvature)
+        if isinstance(curvature, (tuple, list)):
+            # DimensionedCurvature::Dim3 (src/pylib.rs:651-655): ((signed normalised poloidal,
+            # normalised toroidal) of the entrance face, the same of the exit face)
+            (ip, it), (fp, ft) = curvature
+            self.curvature = ((float(ip), float(it)), (float(fp), float(ft)))
+            self.dim = 3
+        else:
+            self.curvature = float(curvature)
+            self.dim = 2
         self.height = float(height)
         self.width = float(width)
         self.ar_coated = bool(ar_coated)
 
-    def _fill(self, p: Params) -> None:
+    def _fill(self, p) -> None:
+        """Prism fields of a Params (2D) or Params3 (3D)."""
         p.n_glass = len(self.glasses)
         p.ar_coated = int(self.ar_coated)
         for i, g in enumerate(self.glasses):
@@ -232,21 +248,35 @@ class CompoundPrism:
             p.angles[i] = a
         for i, v in enumerate(self.lengths):
             p.lengths[i] = v
-        p.curvature = self.curvature
+        if self.dim == 3:
+            (p.initial_curvature[0], p.initial_curvature[1]), (p.final_curvature[0], p.final_curvature[1]) = \
+                self.curvature
+        else:
+            p.curvature = self.curvature
         p.height = self.height
         p.width = self.width
 
-    def _design(self) -> Design:
-        p = Params()
+    def _design(self):
+        p = Params3() if self.dim == 3 else Params()
         self._fill(p)
         p.bin_count = 1
         p.linear_slope = 1.0
         p.bin_size = 1.0
-        d = Design()
-        _check(N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)))
+        if self.dim == 3:
+            d = Design3()
+            _check(N.lib.cpd3_build_design(ctypes.byref(p), ctypes.byref(d)))
+        else:
+            d = Design()
+            _check(N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)))
         return d
 
     def exit_ray(self, y: float, wavelength: float):
+        """The ray leaving the prism for a start ray at height y along x (its xy part in 3D):
+        src/pylib.rs:763-768."""
+        if self.dim == 3:
+            out = (ctypes.c_double * 6)()
+            _check(N.lib.cpd3_exit_ray(ctypes.byref(self._design()), y, wavelength, out))
+            return Vector2D(out[0], out[1]), UnitVector2D(out[3], out[4])
         out = (ctypes.c_double * 4)()
         _check(N.lib.cpd_exit_ray(ctypes.byref(self._design()), y, wavelength, out))
         return Vector2D(out[0], out[1]), UnitVector2D(out[2], out[3])
@@ -254,6 +284,13 @@ class CompoundPrism:
     def final_midpt_and_normal(self):
         d = self._design()
         a = self.angles[-1]
+        if self.dim == 3:
+            # the arc's midpoint: the face's mid-height point moved by the sagitta against the
+            # bound normal (ToricLens drawing, toric.rs:294-321)
+            t = d.lens[1]
+            mid = (t.bound_center[0] + t.bound_normal[0] * t.bound_height,
+                   t.bound_center[1] + t.bound_normal[1] * t.bound_height)
+            return Vector2D(*mid), UnitVector2D(math.cos(a), math.sin(a))
         return Vector2D(d.arc_midpt[0], d.arc_midpt[1]), UnitVector2D(math.cos(a), math.sin(a))
 
     def surfaces(self):
@@ -270,21 +307,42 @@ class CompoundPrism:
 
 def _beam_fields(beam) -> tuple:
     if not isinstance(beam, GaussianBeam):
+        # FiberBeam in 2D: unimplemented in the reference (spectrometer.rs:96-106)
         raise NotImplementedError("No kernel compiled for this Spectrometer")
     return beam.width, beam.y_mean
+
+
+def _beam_fields3(p: Params3, beam) -> None:
+    if isinstance(beam, FiberBeam):
+        p.beam_kind = N.BEAM_FIBER
+        p.beam_width, p.y_mean, p.numerical_aperture = beam.core_radius, beam.center_y, beam.numerical_aperture
+    elif isinstance(beam, GaussianBeam):
+        p.beam_kind = N.BEAM_GAUSSIAN
+        p.beam_width, p.y_mean = beam.width, beam.y_mean
+    else:
+        raise TypeError("beam must be a GaussianBeam or a FiberBeam")
 
 
 def position_detector_array(length: float, angle: float, compound_prism: CompoundPrism,
                             wavelengths: UniformWavelengthDistribution, beam, acceptance: float):
     """src/pylib.rs:814-838: ((x, y), flipped) of the detector array."""
+    pos = (ctypes.c_double * 2)()
+    flipped = ctypes.c_uint32()
+    if compound_prism.dim == 3:
+        p3 = Params3()
+        compound_prism._fill(p3)
+        _beam_fields3(p3, beam)
+        p3.wavelength_lo, p3.wavelength_hi = wavelengths.bounds
+        p3.det_length = length
+        p3.det_angle = angle
+        _check(N.lib.cpd3_position_detector_array(ctypes.byref(p3), acceptance, pos, ctypes.byref(flipped)))
+        return (pos[0], pos[1]), bool(flipped.value)
     p = Params()
     compound_prism._fill(p)
     p.beam_width, p.y_mean = _beam_fields(beam)
     p.wavelength_lo, p.wavelength_hi = wavelengths.bounds
     p.det_length = length
     p.det_angle = angle
-    pos = (ctypes.c_double * 2)()
-    flipped = ctypes.c_uint32()
     _check(N.lib.cpd_position_detector_array(ctypes.byref(p), acceptance, pos, ctypes.byref(flipped)))
     return (pos[0], pos[1]), bool(flipped.value)
 
@@ -369,8 +427,13 @@ class Spectrometer:
     def __getnewargs__(self):
         return (self.compound_prism, self.detector_array, self.wavelengths, self.beam)
 
-    def params(self) -> Params:
-        p = Params()
+    @property
+    def dim(self) -> int:
+        return self.compound_prism.dim
+
+    def params(self):
+        """Params (2D) or Params3 (3D) of the spectrometer."""
+        p = Params3() if self.dim == 3 else Params()
         self.compound_prism._fill(p)
         da = self.detector_array
         p.bin_count = da.bin_count
@@ -382,11 +445,19 @@ class Spectrometer:
         p.max_incident_angle = da.max_incident_angle
         p.det_angle = da.angle
         p.det_position[0], p.det_position[1] = da.position
-        p.beam_width, p.y_mean = _beam_fields(self.beam)
+        if self.dim == 3:
+            _beam_fields3(p, self.beam)
+        else:
+            p.beam_width, p.y_mean = _beam_fields(self.beam)
         p.wavelength_lo, p.wavelength_hi = self.wavelengths.bounds
         return p
 
-    def design(self) -> Design:
+    def design(self):
+        """The constructed record: Design (2D) or Design3 (3D)."""
+        if self.dim == 3:
+            d = Design3()
+            _check(N.lib.cpd3_build_design(ctypes.byref(self.params()), ctypes.byref(d)))
+            return d
         d = Design()
         _check(N.lib.cpd_build_design(ctypes.byref(self.params()), ctypes.byref(d)))
         return d
@@ -403,14 +474,26 @@ class Spectrometer:
 
     def size_and_deviation(self):
         s, dv = ctypes.c_double(), ctypes.c_double()
-        _check(N.lib.cpd_size_and_deviation(ctypes.byref(self.design()), ctypes.byref(s), ctypes.byref(dv)))
+        fn = N.lib.cpd3_size_and_deviation if self.dim == 3 else N.lib.cpd_size_and_deviation
+        _check(fn(ctypes.byref(self.design()), ctypes.byref(s), ctypes.byref(dv)))
         return s.value, dv.value
+
+    def _fitness3(self, max_n: int, max_m: int, mode: int, ctx) -> DesignFitness:
+        s = Sampling(max_n, max_m, mode, 0)
+        f = Fitness()
+        per = ctypes.c_int()
+        _check(N.lib.cpd3_fitness_batch(_ctx_handle(ctx), ctypes.byref(self.design()), ctypes.byref(s), 1,
+                                        ctypes.byref(f), ctypes.byref(per)))
+        _check(per.value)
+        return DesignFitness(f.size, f.info, f.deviation)
 
     def cpu_fitness(self, *, max_n: int = 16_384, max_m: int = 16_384, ctx: Optional[Context] = None
                     ) -> DesignFitness:
         """fitness(spec, max_n, max_m) (src/pylib.rs:1008-1026, src/fitness.rs:130-147) with the
         reference's SEM early exits: rows of rays traced in FP64 on the GPU, folded exactly as
         fitness() folds them."""
+        if self.dim == 3:
+            return self._fitness3(max_n, max_m, N.MODE_ADAPTIVE, ctx)
         s = Sampling(max_n, max_m, N.MODE_ADAPTIVE, 0)
         f = Fitness()
         _check(N.lib.cpd_fitness_gpu(_ctx_handle(ctx), ctypes.byref(self.design()), ctypes.byref(s),
@@ -423,6 +506,18 @@ class Spectrometer:
         src/cuda_fitness.rs:347-423): None when the bin errors stay above 5e-3 after every seed."""
         sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
         f = Fitness()
+        if self.dim == 3:
+            if not isinstance(self.beam, FiberBeam):
+                raise NotImplementedError("No kernel compiled for this Spectrometer")
+            per = ctypes.c_int()
+            _check(N.lib.cpd3_gpu_fitness_batch(_ctx_handle(ctx), ctypes.byref(self.design()), 1,
+                                                sd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), sd.size,
+                                                int(max_n), int(nwarp), int(max_eval), ctypes.byref(f),
+                                                ctypes.byref(per)))
+            if per.value == N.ERR_NOT_CONVERGED:
+                return None
+            _check(per.value)
+            return DesignFitness(f.size, f.info, f.deviation)
         st = N.lib.cpd_gpu_fitness(_ctx_handle(ctx), ctypes.byref(self.design()),
                                    sd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), sd.size, int(max_n),
                                    int(nwarp), int(max_eval), ctypes.byref(f))
@@ -434,6 +529,8 @@ class Spectrometer:
     def fitness(self, *, max_n: int = 64, max_m: int = 16_384, ctx: Optional[Context] = None) -> DesignFitness:
         """Fixed-count fitness: every one of max_n wavelength samples x max_m rays traced (the
         benchmark mode; the reference's estimator with both SEM thresholds at 0)."""
+        if self.dim == 3:
+            return self._fitness3(max_n, max_m, N.MODE_FIXED, ctx)
         s = Sampling(max_n, max_m, N.MODE_FIXED, 0)
         f = Fitness()
         _check(N.lib.cpd_fitness_gpu(_ctx_handle(ctx), ctypes.byref(self.design()), ctypes.byref(s),
@@ -446,7 +543,12 @@ class Spectrometer:
         early exit per row; adaptive=False traces all max_m rays)."""
         w = np.ascontiguousarray(wavelengths, dtype=np.float64).ravel()
         out = np.empty((w.size, self.detector_array.bin_count), np.float64)
-        if w.size:
+        if w.size and self.dim == 3:
+            _check(N.lib.cpd3_p_dets_l_wavelength(
+                _ctx_handle(ctx), ctypes.byref(self.design()), w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                w.size, max_m, N.MODE_ADAPTIVE if adaptive else N.MODE_FIXED,
+                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        elif w.size:
             _check(N.lib.cpd_p_dets_l_wavelength(
                 _ctx_handle(ctx), ctypes.byref(self.design()), w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                 w.size, max_m, N.MODE_ADAPTIVE if adaptive else N.MODE_FIXED,
@@ -454,7 +556,9 @@ class Spectrometer:
         return out
 
     def ray_trace(self, wavelength: float, inital_y: float) -> np.ndarray:
-        """Propagation path rows (x, y, ux, uy): src/pylib.rs:1060-1083."""
+        """Propagation path rows (x, y, ux, uy): src/pylib.rs:1060-1083 (2D spectrometers)."""
+        if self.dim == 3:
+            raise NotImplementedError("ray_trace draws 2D paths; use CompoundPrism.exit_ray for a 3D prism")
         out = np.zeros((N.MAX_PRISMS + 3, 4), np.float64)
         n = ctypes.c_uint32()
         _check(N.lib.cpd_ray_trace(ctypes.byref(self.design()), wavelength, inital_y,
@@ -462,7 +566,23 @@ class Spectrometer:
         return out[: n.value].copy()
 
     def propagate_rays(self, wavelength_cdf, ray_cdf, *, ctx: Optional[Context] = None):
-        """Per-ray bin and transmittance (the reference's propagation test kernel)."""
+        """Per-ray bin and transmittance (the reference's propagation test kernel).  3D: ray_cdf
+        holds 4 values per ray (fiber: rho, theta, deflection, rotation; Gaussian: the first 2),
+        traced in FP64."""
+        if self.dim == 3:
+            w = np.ascontiguousarray(wavelength_cdf, dtype=np.float64).ravel()
+            r = np.ascontiguousarray(ray_cdf, dtype=np.float64).reshape(-1, 4)
+            if w.size != r.shape[0]:
+                raise ValueError("wavelength_cdf and ray_cdf differ in length")
+            bins = np.empty(w.size, np.uint32)
+            ts = np.empty(w.size, np.float64)
+            if w.size:
+                dp = ctypes.POINTER(ctypes.c_double)
+                _check(N.lib.cpd3_propagate_rays(_ctx_handle(ctx), ctypes.byref(self.design()), w.ctypes.data_as(dp),
+                                                 r.ctypes.data_as(dp), w.size,
+                                                 bins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                 ts.ctypes.data_as(dp)))
+            return bins, ts
         w = np.ascontiguousarray(wavelength_cdf, dtype=np.float32).ravel()
         r = np.ascontiguousarray(ray_cdf, dtype=np.float32).ravel()
         if w.size != r.size:
@@ -710,3 +830,44 @@ def generate_designs(config: int, first: int, n: int, threads: int = 0):
     _check(N.lib.cpd_generate_designs(config, first, n, threads, P, D, S, att))
     return list(P), list(D), list(S), np.array(att[:], np.uint32)
 
+
+
+def generate_designs3(first: int, n: int, threads: int = 0):
+    """Synthetic 3D designs first..first+n-1 (3 prisms, toric faces, fiber beam, 32 bins; see
+    cpd3_generate_design): returns (Params3 list, Design3 list, Sampling list)."""
+    P, D, S = (Params3 * n)(), (Design3 * n)(), (Sampling * n)()
+    _check(N.lib.cpd3_generate_designs(first, n, threads, P, D, S))
+    return list(P), list(D), list(S)
+
+
+def fitness_batch3(designs: Sequence[Design3], sampling, *, ctx: Optional[Context] = None):
+    """fitness() of many 3D designs on the GPU (cpd3_fitness_batch): sampling is one Sampling, an
+    (n_wavelengths, n_rays) pair (fixed counts) or a sequence of Sampling; CPD_MODE_ADAPTIVE keeps
+    the reference's early exits.  Returns (fitness float64 [n, 3], status int32 [n])."""
+    n = len(designs)
+    darr = designs if isinstance(designs, ctypes.Array) else (Design3 * n)(*designs)
+    sarr = _sampling_array(sampling, n)
+    out = (Fitness * n)()
+    status = (ctypes.c_int * n)()
+    st = N.lib.cpd3_fitness_batch(_ctx_handle(ctx), darr, sarr, n, out, status)
+    if st != N.OK and st != N.ERR_INVALID_ARGUMENT:
+        _check(st)
+    res = np.array([[f.size, f.info, f.deviation] for f in out], np.float64).reshape(n, 3)
+    return res, np.array(status[:], np.int32)
+
+
+def gpu_fitness_batch3(designs: Sequence[Design3], seeds, *, max_n: int = 256, nwarp: int = 2,
+                       max_eval: int = 16_384, ctx: Optional[Context] = None):
+    """The reference's GPU estimator for fiber-beam 3D designs (cpd3_gpu_fitness_batch).
+    Returns (fitness float64 [n, 3], status int32 [n])."""
+    n = len(designs)
+    darr = designs if isinstance(designs, ctypes.Array) else (Design3 * n)(*designs)
+    sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
+    out = (Fitness * n)()
+    status = (ctypes.c_int * n)()
+    st = N.lib.cpd3_gpu_fitness_batch(_ctx_handle(ctx), darr, n, sd.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      sd.size, int(max_n), int(nwarp), int(max_eval), out, status)
+    if st != N.OK and st != N.ERR_INVALID_ARGUMENT:
+        _check(st)
+    res = np.array([[f.size, f.info, f.deviation] for f in out], np.float64).reshape(n, 3)
+    return res, np.array(status[:], np.int32)

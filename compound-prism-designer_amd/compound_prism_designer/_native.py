@@ -32,6 +32,9 @@ ERR_CAPACITY = 35
 MODE_FIXED = 0
 MODE_ADAPTIVE = 1
 
+BEAM_GAUSSIAN = 0
+BEAM_FIBER = 1
+
 
 class Params(ctypes.Structure):
     _fields_ = [
@@ -83,6 +86,80 @@ class Design(ctypes.Structure):
         ("det_bin_size", ctypes.c_double),
         ("beam_width", ctypes.c_double),
         ("y_mean", ctypes.c_double),
+        ("wavelength_lo", ctypes.c_double),
+        ("wavelength_hi", ctypes.c_double),
+    ]
+
+
+class Params3(ctypes.Structure):
+    """cpd_params3: a 3D spectrometer's parameters (toric lenses, fiber or Gaussian beam)."""
+
+    _fields_ = [
+        ("n_glass", ctypes.c_uint32),
+        ("ar_coated", ctypes.c_uint32),
+        ("glass", (ctypes.c_double * COEFFS) * MAX_PRISMS),
+        ("angles", ctypes.c_double * (MAX_PRISMS + 1)),
+        ("lengths", ctypes.c_double * MAX_PRISMS),
+        ("initial_curvature", ctypes.c_double * 2),
+        ("final_curvature", ctypes.c_double * 2),
+        ("height", ctypes.c_double),
+        ("width", ctypes.c_double),
+        ("bin_count", ctypes.c_uint32),
+        ("flipped", ctypes.c_uint32),
+        ("bin_size", ctypes.c_double),
+        ("linear_slope", ctypes.c_double),
+        ("linear_intercept", ctypes.c_double),
+        ("det_length", ctypes.c_double),
+        ("max_incident_angle", ctypes.c_double),
+        ("det_angle", ctypes.c_double),
+        ("det_position", ctypes.c_double * 2),
+        ("beam_kind", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("beam_width", ctypes.c_double),
+        ("y_mean", ctypes.c_double),
+        ("numerical_aperture", ctypes.c_double),
+        ("wavelength_lo", ctypes.c_double),
+        ("wavelength_hi", ctypes.c_double),
+    ]
+
+
+class Torus(ctypes.Structure):
+    _fields_ = [
+        ("center", ctypes.c_double * 2),
+        ("toroidal_normal", ctypes.c_double * 2),
+        ("toroidal_radius", ctypes.c_double),
+        ("poloidal_radius", ctypes.c_double),
+        ("bound_center", ctypes.c_double * 2),
+        ("bound_normal", ctypes.c_double * 2),
+        ("bound_height", ctypes.c_double),
+    ]
+
+
+class Design3(ctypes.Structure):
+    """cpd_design3: a constructed 3D spectrometer."""
+
+    _fields_ = [
+        ("n_glass", ctypes.c_uint32),
+        ("bin_count", ctypes.c_uint32),
+        ("ar_coated", ctypes.c_uint32),
+        ("beam_kind", ctypes.c_uint32),
+        ("glass", (ctypes.c_double * COEFFS) * MAX_PRISMS),
+        ("lens", Torus * 2),
+        ("plane_point", (ctypes.c_double * 2) * MAX_PRISMS),
+        ("plane_normal", (ctypes.c_double * 2) * MAX_PRISMS),
+        ("height", ctypes.c_double),
+        ("half_width", ctypes.c_double),
+        ("det_position", ctypes.c_double * 2),
+        ("det_direction", ctypes.c_double * 2),
+        ("det_normal", ctypes.c_double * 2),
+        ("det_min_ci", ctypes.c_double),
+        ("det_length", ctypes.c_double),
+        ("det_slope", ctypes.c_double),
+        ("det_intercept", ctypes.c_double),
+        ("det_bin_size", ctypes.c_double),
+        ("beam_width", ctypes.c_double),
+        ("y_mean", ctypes.c_double),
+        ("fiber_min_cos", ctypes.c_double),
         ("wavelength_lo", ctypes.c_double),
         ("wavelength_hi", ctypes.c_double),
     ]
@@ -200,6 +277,16 @@ def _load() -> ctypes.CDLL:
         "cpd_build_designs": ([vp, P(Params), sz, d, P(Design), P(st)], st),
         "cpd_plan_schedule": ([vp, P(Schedule)], st),
         "cpd_plan_destroy": ([vp], None),
+        "cpd3_build_design": ([P(Params3), P(Design3)], st),
+        "cpd3_position_detector_array": ([P(Params3), d, P(d), P(u32)], st),
+        "cpd3_size_and_deviation": ([P(Design3), P(d), P(d)], st),
+        "cpd3_exit_ray": ([P(Design3), d, d, P(d)], st),
+        "cpd3_generate_design": ([u64, P(Params3), P(Sampling), P(u32)], st),
+        "cpd3_generate_designs": ([u64, sz, u32, P(Params3), P(Design3), P(Sampling)], st),
+        "cpd3_fitness_batch": ([vp, P(Design3), P(Sampling), sz, P(Fitness), P(st)], st),
+        "cpd3_p_dets_l_wavelength": ([vp, P(Design3), P(d), sz, u32, u32, P(d)], st),
+        "cpd3_gpu_fitness_batch": ([vp, P(Design3), sz, P(d), sz, u32, u32, u32, P(Fitness), P(st)], st),
+        "cpd3_propagate_rays": ([vp, P(Design3), P(d), P(d), sz, P(u32), P(d)], st),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

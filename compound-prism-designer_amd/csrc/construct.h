@@ -59,6 +59,45 @@ CPD_HD int check_design_params(const cpd_params &p) {
     return PARAM_OK;
 }
 
+// Face planes of the joined trapezoids (create_joined_trapezoids,
+// geometry.rs:430-489): faces 0..P-1 into pt/nm (mid-height points and
+// normals), the last face into exit_pt/exit_nm.
+CPD_HD void trapezoid_faces(double height, const double *angles, const double *lengths, uint32_t P,
+                            double pt[][2], double nm[][2], Vec2 &exit_pt, Vec2 &exit_nm) {
+#pragma clang fp contract(off)
+    const double h2 = height * 0.5;
+    // entrance face: mid-height point at x = |tan(theta0)| h/2
+    Vec2 n = face_normal(angles[0]);
+    double x0 = fabs(n.y / n.x) * h2;
+    pt[0][0] = x0;
+    pt[0][1] = h2;
+    nm[0][0] = n.x;
+    nm[0][1] = n.y;
+    bool prev_up = sign_positive(n.y);
+    double prev_half = x0;  // half of the face's x-extent over the height
+    double x = x0;
+    // faces 1..P: face i sits lengths[i-1] beyond the previous one, measured
+    // between the facing edges of the two trapezoids
+    for (uint32_t i = 1; i <= P; ++i) {
+        Vec2 ni = face_normal(angles[i]);
+        bool up = sign_positive(ni.y);
+        double half = fabs(ni.y / ni.x) * h2;
+        double gap = (up != prev_up) ? prev_half + half : fabs(prev_half - half);
+        x += lengths[i - 1] + gap;
+        prev_up = up;
+        prev_half = half;
+        if (i < P) {
+            pt[i][0] = x;
+            pt[i][1] = h2;
+            nm[i][0] = ni.x;
+            nm[i][1] = ni.y;
+        } else {
+            exit_pt = Vec2{x, h2};
+            exit_nm = ni;
+        }
+    }
+}
+
 // The compound prism, the exit lens and the beam of a design.
 CPD_HD void build_prism(const cpd_params &p, cpd_design &d) {
 #pragma clang fp contract(off)
@@ -68,39 +107,8 @@ CPD_HD void build_prism(const cpd_params &p, cpd_design &d) {
     for (uint32_t i = 0; i < P; ++i)
         for (int j = 0; j < CPD_GLASS_COEFFS; ++j) d.glass[i][j] = p.glass[i][j];
     d.height = p.height;
-
-    const double h2 = p.height * 0.5;
-    // entrance face: mid-height point at x = |tan(theta0)| h/2
-    Vec2 n = face_normal(p.angles[0]);
-    double x0 = fabs(n.y / n.x) * h2;
-    d.plane_point[0][0] = x0;
-    d.plane_point[0][1] = h2;
-    d.plane_normal[0][0] = n.x;
-    d.plane_normal[0][1] = n.y;
-    bool prev_up = sign_positive(n.y);
-    double prev_half = x0;  // half of the face's x-extent over the height
-    double x = x0;
     Vec2 exit_point{0, 0}, exit_normal{0, 0};
-    // faces 1..P: face i sits lengths[i-1] beyond the previous one, measured
-    // between the facing edges of the two trapezoids
-    for (uint32_t i = 1; i <= P; ++i) {
-        Vec2 ni = face_normal(p.angles[i]);
-        bool up = sign_positive(ni.y);
-        double half = fabs(ni.y / ni.x) * h2;
-        double gap = (up != prev_up) ? prev_half + half : fabs(prev_half - half);
-        x += p.lengths[i - 1] + gap;
-        prev_up = up;
-        prev_half = half;
-        if (i < P) {
-            d.plane_point[i][0] = x;
-            d.plane_point[i][1] = h2;
-            d.plane_normal[i][0] = ni.x;
-            d.plane_normal[i][1] = ni.y;
-        } else {
-            exit_point = Vec2{x, h2};
-            exit_normal = ni;
-        }
-    }
+    trapezoid_faces(p.height, p.angles, p.lengths, P, d.plane_point, d.plane_normal, exit_point, exit_normal);
     // exit lens: chord = |h / n.x|, R = chord / (2 |k|), centre on the normal
     // through the chord midpoint at the apothem, bounds centred on the arc's
     // midpoint with radius^2 = sagitta^2 + chord^2 / 4

@@ -1,0 +1,296 @@
+// rows_fold.h -- the two adaptive row estimators as device templates over a
+// ray tracer, shared by the 2D rows (adaptive.hip) and the 3D rows
+// (rows3.hip).
+//
+// A wavefront owns one row (a design at one wavelength) and keeps the row's
+// per-bin Welford summaries (count, mean, m2) in LDS.
+//  * sem_row: p_dets_l_wavelength with the early exit of
+//    src/fitness.rs:5-34 -- rays k = 1, 2, ... traced 64 at a time and folded
+//    one at a time, in order, into the summary of the bin each hit, padded
+//    with zeros (skip); the row stops as soon as the bin just updated has a
+//    standard error of the mean below 5e-3.
+//  * chunk_row: the reference's GPU estimator, src/spectrometer/kernel.rs:89-193
+//    -- 32-ray chunks folded in order, each chunk's (32, mean, m2) over its
+//    32 lanes into every bin it hit, lowest bin first, with the SEM test after
+//    each chunk.
+//
+// A tracer Tr supplies, for the row it was built for:
+//   uint32_t nbin;                 bins of the design
+//   bool live;                     false when no ray can reach the detector
+//   uint32_t trace(k, active, t, redo)   ray k's bin (nbin on a miss or when
+//                                  inactive) and transmittance (0 then); with
+//                                  kRetrace it may flag the ray instead (redo)
+//   uint32_t retrace(k, t)         the exact re-trace of a flagged ray
+//   static constexpr bool kRetrace;
+//   static constexpr size_t kExtraLds;   LDS bytes the tracer keeps after
+//                                  chunk_lds_bytes (a chain record, say)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "trace_core.h"
+#include "welford.h"
+
+namespace cpd {
+
+// Rows of one launch: which design and wavelength each block traces, where
+// its bin_count means go, and the rays it may fold.
+struct RowSlots {
+    const uint32_t *row_design;  // [n_rows] index into the launch's designs
+    const double *row_lambda;    // [n_rows] its wavelength
+    const uint64_t *row_out;     // [n_rows] offset of its bin_count outputs
+    const uint32_t *row_cap;     // [n_rows] rays cap of each row (sem rows)
+    double *out;                 // p(d|lambda) rows
+    uint64_t *row_rays;          // [n_rows] rays folded into the row
+    double max_err_sq;           // SEM^2 threshold of the early exits (0: none)
+    uint32_t n_rows;
+    uint32_t max_rays;           // rays cap (chunked: a multiple of 32)
+    uint32_t nbin_max;
+};
+
+__device__ __forceinline__ double readlane_f64(double v, uint32_t l) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), l);
+    return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
+__device__ __forceinline__ uint64_t lanemask_below() {
+    return (1ull << __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u))) - 1ull;
+}
+
+// Sum over the 32 lanes of each half-wave, butterfly order 16, 8, 4, 2, 1
+// (every lane of a half ends with the same total).
+__device__ __forceinline__ double half_sum(double v) {
+#pragma clang fp contract(off)
+    for (int off = 16; off > 0; off >>= 1) v = v + __shfl_xor(v, off);
+    return v;
+}
+
+// half_sum's total evaluated by one lane from the 32 values v(0..31), in the
+// order every lane of half_sum adds in (each butterfly step adds the same two
+// partial sums on both partners): s16(i) = v(i) + v(i + 16), s8(i) = s16(i)
+// + s16(i + 8), s4(i) = s8(i) + s8(i + 4), then s2 and s1 alike, so the
+// result is half_sum's bit for bit.  Each s4 is formed from its own eight
+// values behind a compiler barrier, which keeps eight of the 32 live at once.
+template <class V>
+__device__ __forceinline__ double butterfly_sum(V v) {
+#pragma clang fp contract(off)
+    double s4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        asm volatile("" ::: "memory");
+        s4[i] = ((v(i) + v(i + 16)) + (v(i + 8) + v(i + 24))) + ((v(i + 4) + v(i + 20)) + (v(i + 12) + v(i + 28)));
+    }
+    return (s4[0] + s4[2]) + (s4[1] + s4[3]);
+}
+
+__device__ __forceinline__ W64 load_w(const double *lds, uint32_t nbin_max, uint32_t b) {
+    return W64{lds[b], lds[nbin_max + b], lds[2 * nbin_max + b]};
+}
+__device__ __forceinline__ void store_w(double *lds, uint32_t nbin_max, uint32_t b, const W64 &w) {
+    lds[b] = w.count;
+    lds[nbin_max + b] = w.mean;
+    lds[2 * nbin_max + b] = w.m2;
+}
+
+// Clears the row's summaries (before the tracer is built; the caller
+// synchronises after).
+__device__ __forceinline__ void clear_row(double *lds, uint32_t nbin_max, uint32_t nbin) {
+    for (uint32_t b = threadIdx.x; b < nbin; b += 64) store_w(lds, nbin_max, b, W64{0.0, 0.0, 0.0});
+}
+
+// Pads every summary to the rays folded and writes the row's means.
+__device__ __forceinline__ void finish_row(const RowSlots &a, uint32_t r, const double *lds, uint32_t nbin,
+                                           double count) {
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nbin; b += 64) {
+        W64 w = load_w(lds, a.nbin_max, b);
+        w_skip(w, count);
+        a.out[a.row_out[r] + b] = w.mean;
+    }
+    if (threadIdx.x == 0) a.row_rays[r] = (uint64_t)count;
+}
+
+// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).
+template <class Tr>
+__device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
+#pragma clang fp contract(off)
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nbin = tr.nbin;
+    const uint32_t cap = a.row_cap[r];
+    const double max_err_sq = a.max_err_sq;
+    double count = 0.0;
+    bool stop = false;
+    for (uint32_t base = 1; base <= cap && !stop; base += 64) {
+        const uint32_t k = base + lane;
+        double t = 0.0;
+        bool redo = false;
+        uint32_t bin = tr.live ? tr.trace(k, k <= cap, t, redo) : nbin;
+        if (Tr::kRetrace && wave_any(redo) && redo) bin = tr.retrace(k, t);
+        if (!tr.live) t = 0.0;
+        const uint32_t n_here = min(64u, cap - base + 1);
+        // visit only the lanes whose ray hit a bin, in ray order; count still
+        // advances by one per ray, as in the reference's loop
+        uint64_t hits = __ballot(lane < n_here && bin < nbin);
+        uint32_t consumed = 0;
+        while (hits) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(hits);
+            hits &= hits - 1;
+            count += (double)(l + 1 - consumed);
+            consumed = l + 1;
+            const uint32_t b = __builtin_amdgcn_readlane(bin, l);
+            W64 w = load_w(lds, a.nbin_max, b);
+            w_skip(w, count);
+            w_next(w, readlane_f64(t, l));
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) store_w(lds, a.nbin_max, b, w);
+            __builtin_amdgcn_wave_barrier();
+            if (w_sem_le(w, max_err_sq)) {
+                stop = true;
+                break;
+            }
+        }
+        if (!stop) count += (double)(n_here - consumed);
+    }
+    finish_row(a, r, lds, nbin, count);
+}
+
+// Passes of 64 rays (two chunks) traced ahead of the fold.
+constexpr uint32_t kAhead = 4;
+
+// LDS of chunk_row after the 3 nbin_max Welford doubles: a half's
+// (transmittance, bin) pairs, and the traced passes' transmittances, bins and
+// re-trace queue; the tracer's own bytes follow.
+__host__ __device__ constexpr size_t chunk_lds_bytes(uint32_t nbin_max) {
+    return (size_t)3 * nbin_max * sizeof(double) + 32 * 16 + kAhead * 64 * (sizeof(double) + 2 * sizeof(uint32_t));
+}
+
+// kernel.rs:89-193: a wavefront traces kAhead passes of two chunks (lanes
+// 0-31 and 32-63) before folding them in order; flagged rays of all the
+// passes are queued and re-traced together, and rays traced beyond the chunk
+// where the row stops are not folded.  In the fold, the two lowest bins of a
+// chunk are folded one at a time with half-wave butterflies; when more
+// remain, every lane that hit one of them evaluates the same butterfly sums
+// for its own bin from LDS, so a chunk spread over many bins costs one pass
+// instead of one per bin.
+template <class Tr>
+__device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
+#pragma clang fp contract(off)
+    double2 *tpair = (double2 *)(lds + 3 * a.nbin_max);  // bin ~0u: nothing left to fold
+    double *tv = (double *)(tpair + 32);                 // [kAhead * 64] pass p, lane at p * 64 + lane
+    uint32_t *bv = (uint32_t *)(tv + kAhead * 64);
+    uint32_t *queue = bv + kAhead * 64;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nbin = tr.nbin;
+    const double max_err_sq = a.max_err_sq;
+    const uint32_t iters = a.max_rays / 32;
+    const uint32_t half = lane >> 5, l = lane & 31;
+    double count = 0.0;
+    bool stop = false;
+    for (uint32_t i0 = 0; i0 < iters && !stop; i0 += 2 * kAhead) {
+        const uint32_t passes = min(kAhead, (iters - i0 + 1) / 2);
+        // trace: ray k = l + 32 (c + 1) of chunk c = i0 + 2 p + half
+        uint32_t queued = 0;
+        for (uint32_t p = 0; p < passes; ++p) {
+            const uint32_t c = i0 + 2 * p + half;
+            const bool active = tr.live && c < iters;
+            const uint32_t k = l + 32 * (c + 1);
+            double t = 0.0;
+            bool redo = false;
+            uint32_t bin = tr.live ? tr.trace(k, active, t, redo) : nbin;
+            if (redo || !(bin < nbin)) {
+                bin = redo ? bin : nbin;
+                t = 0.0;
+            }
+            const uint32_t slot = p * 64 + lane;
+            tv[slot] = t;
+            bv[slot] = bin;
+            if (Tr::kRetrace) {
+                const uint64_t mask = __builtin_amdgcn_ballot_w64(redo);
+                if (redo) queue[queued + __popcll(mask & lanemask_below())] = slot;
+                queued += __popcll(mask);
+            }
+        }
+        // exact re-trace of the queued rays, 64 at a time
+        if (Tr::kRetrace) {
+            for (uint32_t q0 = 0; q0 < queued; q0 += 64) {
+                __builtin_amdgcn_wave_barrier();
+                if (q0 + lane < queued) {
+                    const uint32_t slot = queue[q0 + lane];
+                    const uint32_t c = i0 + 2 * (slot >> 6) + ((slot >> 5) & 1);
+                    const uint32_t k = (slot & 31) + 32 * (c + 1);
+                    double t;
+                    const uint32_t bin = tr.retrace(k, t);
+                    tv[slot] = bin < nbin ? t : 0.0;
+                    bv[slot] = bin;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // fold the passes' chunks in order
+        for (uint32_t p = 0; p < passes && !stop; ++p) {
+            const uint32_t i = i0 + 2 * p;
+            const double t = tv[p * 64 + lane];
+            const uint32_t bin = bv[p * 64 + lane];
+            for (uint32_t h = 0; h < 2 && i + h < iters; ++h) {
+                count += 32.0;
+                const bool mine = half == h;
+                bool pending = mine && bin < nbin;
+                bool finished = __ballot(pending) != 0;
+                for (int serial = 0; serial < 2 && __ballot(pending) != 0; ++serial) {
+                    const uint32_t bmin = wave_min_u32(pending ? bin : 0xffffffffu);
+                    const double tb = (mine && bin == bmin) ? t : 0.0;
+                    const double wmean = readlane_f64(half_sum(tb), 32 * h) / 32.0;
+                    const double dev = mine ? tb - wmean : 0.0;
+                    const double wm2 = readlane_f64(half_sum(dev * dev), 32 * h);
+                    const W64 w = w_add(load_w(lds, a.nbin_max, bmin), W64{32.0, wmean, wm2});
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) store_w(lds, a.nbin_max, bmin, w);
+                    __builtin_amdgcn_wave_barrier();
+                    finished = finished && w_sem_le(w, max_err_sq);
+                    if (bin == bmin) pending = false;
+                }
+                if (__ballot(pending) != 0) {
+                    // the remaining bins at once; lanes that share a bin
+                    // compute and store the same record
+                    __builtin_amdgcn_wave_barrier();
+                    if (mine) tpair[l] = make_double2(t, __longlong_as_double(pending ? bin : ~0u));
+                    __builtin_amdgcn_wave_barrier();
+                    bool ok = true;
+                    if (pending) {
+                        auto v = [&](int j) {
+                            const double2 pr = tpair[j];
+                            return (uint32_t)__double_as_longlong(pr.y) == bin ? pr.x : 0.0;
+                        };
+                        const double wmean = butterfly_sum(v) / 32.0;
+                        const double wm2 = butterfly_sum([&](int j) {
+                            const double dev = v(j) - wmean;
+                            return dev * dev;
+                        });
+                        const W64 w = w_add(load_w(lds, a.nbin_max, bin), W64{32.0, wmean, wm2});
+                        store_w(lds, a.nbin_max, bin, w);
+                        ok = w_sem_le(w, max_err_sq);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    finished = finished && __ballot(pending && !ok) == 0;
+                    pending = false;
+                }
+                if (finished) {
+                    stop = true;
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    finish_row(a, r, lds, nbin, count);
+}
+
+}  // namespace cpd

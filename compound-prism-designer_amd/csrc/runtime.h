@@ -12,6 +12,8 @@
 
 namespace cpd {
 
+struct RowSlots;
+
 // Grow-only device allocation.
 struct DeviceBuffer {
     void *ptr = nullptr;
@@ -96,6 +98,32 @@ struct Context {
                         const uint64_t *&rays_host);
     cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
                              const cpd_design **designs_dev, uint32_t *nbin_max);
+    cpd_status stage_rows(const std::vector<uint32_t> &row_design, const std::vector<double> &row_lambda,
+                          const std::vector<uint64_t> &row_out, const std::vector<uint32_t> &row_cap, uint64_t out_n,
+                          uint32_t max_rays, uint32_t nbin_max, double max_err_sq, RowSlots &slots);
+    cpd_status finish_rows(const RowSlots &slots, uint64_t out_n, const double *&rows_host,
+                           const uint64_t *&rays_host);
+    // 3D spectrometers (rows3.hip)
+    DeviceBuffer design3_stage, samples3, samples3_gauss;  // records; beam samples of fiber and Gaussian rays
+    uint64_t samples3_key = ~0ull, samples3_gauss_key = ~0ull;  // seeds of the sample tables
+    uint32_t samples3_n = 0, samples3_gauss_n = 0;              // rays they hold
+    cpd_status ensure_samples3(uint32_t kind, uint64_t seed_fx, uint32_t n, const void **table);
+    cpd_status stage_designs3(const cpd_design3 *designs, const std::vector<uint32_t> &valid,
+                              const cpd_design3 **designs_dev, uint32_t *nbin_max);
+    cpd_status run_rows3(bool chunked, const cpd_design3 *designs_dev, const std::vector<uint32_t> &row_design,
+                         const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
+                         const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays, uint32_t nbin_max,
+                         double max_err_sq, const void *fiber_tab, const void *gauss_tab, const double *&rows_host,
+                         const uint64_t *&rays_host);
+    cpd_status run_fitness3(const cpd_design3 *designs, const cpd_sampling *sampling,
+                            const std::vector<uint32_t> &valid, cpd_fitness *out);
+    cpd_status rows3(const cpd_design3 &g, const double *wavelengths, size_t n_w, uint32_t max_m, uint32_t mode,
+                     double *out);
+    cpd_status run_gpu_fitness3(const cpd_design3 *designs, const std::vector<uint32_t> &valid, const double *seeds,
+                                size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval, cpd_fitness *out,
+                                cpd_status *per_design);
+    cpd_status propagate3(const cpd_design3 &g, const double *wcdf, const double *rcdf, size_t n, uint32_t *bin,
+                          double *t);
     void apply(const cpd_schedule &s);
     cpd_schedule schedule() const;
     cpd_status autotune(const cpd_design *designs, const cpd_sampling *sampling, size_t n,
@@ -103,6 +131,8 @@ struct Context {
 };
 
 cpd_status default_context(Context **out);
+// The context a C entry point runs on: ctx, or the default context when NULL.
+cpd_status resolve_context(cpd_ctx *ctx, Context **out);
 cpd_schedule default_schedule();
 cpd_status upload_beam_table(hipStream_t s, uint64_t seed_fx, uint32_t n, float *d_z32, double *d_z64);
 cpd_status check_schedule(const cpd_schedule &s);

@@ -412,6 +412,9 @@ Context::~Context() {
     design_stage.release();
     seed_z32.release();
     seed_z64.release();
+    design3_stage.release();
+    samples3.release();
+    samples3_gauss.release();
     host_stage.release();
     row_args_pin.release();
     rows_pin.release();
@@ -466,13 +469,15 @@ cpd_status default_context(Context **out) {
 
 using cpd::Context;
 
-static cpd_status resolve(cpd_ctx *ctx, Context **out) {
+cpd_status cpd::resolve_context(cpd_ctx *ctx, Context **out) {
     if (ctx) {
         *out = reinterpret_cast<Context *>(ctx);
         return CPD_OK;
     }
     return cpd::default_context(out);
 }
+
+static cpd_status resolve(cpd_ctx *ctx, Context **out) { return cpd::resolve_context(ctx, out); }
 
 extern "C" {
 

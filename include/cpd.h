@@ -4,7 +4,9 @@
  * The evaluator traces quasi-Monte Carlo rays through a 2D compound prism
  * (P planar faces and one circular-arc exit lens) onto a linear detector
  * array, and reduces the per-bin hit probabilities p(d|lambda) to the mutual
- * information I(Lambda; D), next to the design's size and deviation.
+ * information I(Lambda; D), next to the design's size and deviation.  The
+ * cpd3_* entry points do the same for the 3D variant (toric lenses, fiber
+ * beam).
  *
  * Every entry point is extern "C", takes plain pointers and sizes, returns a
  * cpd_status and never lets a C++ exception cross the boundary.  Buffers are
@@ -417,6 +419,153 @@ cpd_status cpd_server_wait(cpd_server *srv, uint64_t request_id, int64_t timeout
 cpd_status cpd_server_stats_get(cpd_server *srv, cpd_server_stats *out);
 /* Stops the scheduling thread after the step in flight; queued requests are dropped. */
 void cpd_server_destroy(cpd_server *srv);
+
+/* ------------------------------------------------------------------------ */
+/* 3D spectrometer: toric entrance and exit lenses, fiber or Gaussian beam   */
+/* ------------------------------------------------------------------------ */
+
+/* The reference's Dim3 variant (DimensionedSizedCompoundPrism::Dim3,
+ * src/pylib.rs:636-640): CulminatingToricCompoundPrism -- a ToricLens
+ * entrance face, planar interior faces bounded in height and width, a
+ * ToricLens exit face (src/spectrometer/ray.rs:121-127,
+ * src/spectrometer/toric.rs) -- traced in 3D from a FiberBeam
+ * (spectrometer.rs:70-149) or a 3D GaussianBeam (spectrometer.rs:32-46) onto
+ * the same linear detector array.  Geometry lies in the xy plane (z = 0);
+ * rays leave the plane. */
+enum { CPD_BEAM_GAUSSIAN = 0, CPD_BEAM_FIBER = 1 };
+
+/* CompoundPrism(glasses, angles, lengths, ((ip, it), (fp, ft)), height,
+ * width, ar_coated) src/pylib.rs:715-744, DetectorArray as in cpd_params,
+ * FiberBeam(core_radius, numerical_aperture, center_y) src/pylib.rs:396-409
+ * or GaussianBeam(width, y_mean), UniformWavelengthDistribution(bounds). */
+typedef struct cpd_params3 {
+    uint32_t n_glass;
+    uint32_t ar_coated;
+    double glass[CPD_MAX_PRISMS][CPD_GLASS_COEFFS];
+    double angles[CPD_MAX_PRISMS + 1];
+    double lengths[CPD_MAX_PRISMS];
+    double initial_curvature[2];   /* (signed normalised poloidal, normalised toroidal) */
+    double final_curvature[2];
+    double height;
+    double width;
+    uint32_t bin_count;
+    uint32_t flipped;
+    double bin_size;
+    double linear_slope;
+    double linear_intercept;
+    double det_length;
+    double max_incident_angle;
+    double det_angle;
+    double det_position[2];
+    uint32_t beam_kind;            /* CPD_BEAM_* */
+    uint32_t reserved;
+    double beam_width;             /* Gaussian 1/e^2 width, or fiber core radius */
+    double y_mean;                 /* Gaussian y_mean, or fiber center_y */
+    double numerical_aperture;     /* fiber only */
+    double wavelength_lo;
+    double wavelength_hi;
+} cpd_params3;
+
+/* ToricLens<V, 3> (toric.rs:17-24, 240-291): the torus (centre, toroidal
+ * axis, radii) and its SandwichBounds (geometry.rs:94-110).  Vectors are xy;
+ * their z is 0. */
+typedef struct cpd_torus {
+    double center[2];
+    double toroidal_normal[2];
+    double toroidal_radius;
+    double poloidal_radius;
+    double bound_center[2];
+    double bound_normal[2];
+    double bound_height;
+} cpd_torus;
+
+/* A constructed 3D spectrometer (FP64).  lens[0] is the entrance face,
+ * lens[1] the exit face; plane i (1..P-1) the interior face i, bounded by
+ * 0 <= y <= height and |z| <= half_width (PrismBounds<3>,
+ * geometry.rs:69-77). */
+typedef struct cpd_design3 {
+    uint32_t n_glass;
+    uint32_t bin_count;
+    uint32_t ar_coated;
+    uint32_t beam_kind;
+    double glass[CPD_MAX_PRISMS][CPD_GLASS_COEFFS];
+    cpd_torus lens[2];
+    double plane_point[CPD_MAX_PRISMS][2];
+    double plane_normal[CPD_MAX_PRISMS][2];
+    double height;
+    double half_width;
+    double det_position[2];
+    double det_direction[2];
+    double det_normal[2];
+    double det_min_ci;
+    double det_length;
+    double det_slope;
+    double det_intercept;
+    double det_bin_size;
+    double beam_width;             /* Gaussian width, or fiber radius */
+    double y_mean;                 /* Gaussian y_mean, or fiber center_y */
+    double fiber_min_cos;          /* sqrt(1 - NA^2), spectrometer.rs:85-93 */
+    double wavelength_lo;
+    double wavelength_hi;
+} cpd_design3;
+
+/* Host-side (no GPU).  CompoundPrism::new for the Dim3 variant
+ * (src/pylib.rs:715-744, ray.rs:137-183 with ToricLens::from_hyperplane
+ * toric.rs:244-291), LinearDetectorArray::new and FiberBeam::new. */
+cpd_status cpd3_build_design(const cpd_params3 *params, cpd_design3 *out);
+/* detector_array_positioning for a Dim3 prism (spectrometer.rs:322-364 via
+ * position_detector_array, src/pylib.rs:814-838): the median ray starts at
+ * (0, median_y, 0) along x. */
+cpd_status cpd3_position_detector_array(const cpd_params3 *params, double acceptance, double position_out[2],
+                                        uint32_t *flipped_out);
+/* Spectrometer::size_and_deviation, spectrometer.rs:429-437. */
+cpd_status cpd3_size_and_deviation(const cpd_design3 *design, double *size, double *deviation);
+/* The ray leaving the prism for a start ray (0, y, 0) along x: origin xyz,
+ * direction xyz (CompoundPrism.exit_ray, src/pylib.rs:763-768). */
+cpd_status cpd3_exit_ray(const cpd_design3 *design, double y, double wavelength, double out[6]);
+
+/* Synthetic 3D workload (no reference counterpart; the reference's optimiser
+ * draws 3D designs in designer.py:160-265): 3 prisms, 32 bins, a fiber beam
+ * of 50 um core and NA 0.1, the detector positioned for acceptance 1 and
+ * at least 10% of a 64-ray probe at three wavelengths landing in bins;
+ * sampling 64 wavelengths x 4096 rays.  Design `index` is reproducible on
+ * its own; *attempts_out (may be NULL) receives the draws used. */
+cpd_status cpd3_generate_design(uint64_t index, cpd_params3 *params_out, cpd_sampling *sampling_out,
+                                uint32_t *attempts_out);
+/* Designs first_index .. first_index + n - 1 on `threads` host threads (0: all). */
+cpd_status cpd3_generate_designs(uint64_t first_index, size_t n, uint32_t threads, cpd_params3 *params_out,
+                                 cpd_design3 *designs_out, cpd_sampling *sampling_out);
+
+/* fitness() of 3D designs on the GPU (src/fitness.rs:130-147 with the Dim3
+ * spectrometer, as Spectrometer.cpu_fitness calls it, src/pylib.rs:1008-1026):
+ * rows of wavelength samples are traced in FP64 on the device, rays in
+ * order with the SEM <= 5e-3 early exit under CPD_MODE_ADAPTIVE (none under
+ * CPD_MODE_FIXED), and folded on the host as mutual_information folds them.
+ * Rays are the R1 points of Qrng<[f64; 4]> (fiber) or Qrng<[f64; 2]>
+ * (Gaussian) from 0.5 (qrng.rs:167-255).  per_design may be NULL. */
+cpd_status cpd3_fitness_batch(cpd_ctx *ctx, const cpd_design3 *designs, const cpd_sampling *sampling, size_t n,
+                              cpd_fitness *out, cpd_status *per_design);
+/* p(d|lambda) rows of one 3D design: Spectrometer.transmission_probability
+ * (src/pylib.rs:1028-1058) over p_dets_l_wavelength (fitness.rs:48-69). */
+cpd_status cpd3_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design3 *design, const double *wavelengths,
+                                    size_t n_wavelengths, uint32_t max_m, uint32_t mode, double *out);
+/* The reference's GPU estimator for fiber-beam 3D designs (the
+ * ..._fiber_beam_culminating_toric_compound_prism kernels,
+ * src/kernel/kernel.rs:169-170, driven by cuda_fitness,
+ * src/cuda_fitness.rs:347-423): as cpd_gpu_fitness_batch, with rays the
+ * points frac(seed + k alpha_i) of the 4-D sequence.  Traced and folded in
+ * FP64.  Gaussian-beam designs fail with CPD_ERR_INVALID_ARGUMENT, where the
+ * reference raises NotImplementedError (src/pylib.rs:1118-1130). */
+cpd_status cpd3_gpu_fitness_batch(cpd_ctx *ctx, const cpd_design3 *designs, size_t n, const double *seeds,
+                                  size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval,
+                                  cpd_fitness *out, cpd_status *per_design);
+/* One ray per (wavelength cdf, ray cdf) pair through the FP64 device trace:
+ * ray_cdf holds 4 values per ray (fiber: rho, theta, deflection, rotation;
+ * Gaussian: the first 2).  bin index (bin_count on a miss) and
+ * transmittance (0 on a miss).  Replaces propagation_test_kernel
+ * (src/spectrometer/kernel.rs:195-224) for the 3D kernels. */
+cpd_status cpd3_propagate_rays(cpd_ctx *ctx, const cpd_design3 *design, const double *wavelength_cdf,
+                               const double *ray_cdf, size_t n, uint32_t *bin_out, double *t_out);
 
 #ifdef __cplusplus
 }

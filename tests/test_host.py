@@ -22,8 +22,8 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "cpd.h").read_text()
-    names = set(re.findall(r"\b(cpd_[a-z0-9_]+)\s*\(", header))
-    assert len(names) >= 15
+    names = set(re.findall(r"\b(cpd3?_[a-z0-9_]+)\s*\(", header))
+    assert len(names) >= 15 and any(n.startswith("cpd3_") for n in names)
     for name in sorted(names):
         assert hasattr(N.lib, name), name
 
@@ -129,8 +129,10 @@ def test_python_surface_validation():
         cpd.CompoundPrism([g], [0.0, 0.1], [1.0, 2.0], 0.2, 2.0, 2.0, False)
     with pytest.raises(ValueError):
         cpd.UniformWavelengthDistribution((0.8, 0.5))
-    with pytest.raises(NotImplementedError):
-        cpd.CompoundPrism([g], [0.0, 0.1], [1.0], ((0.1, 0.2), (0.1, 0.2)), 2.0, 2.0, False)
+    # a curvature pair selects the 3D toric prism
+    assert cpd.CompoundPrism([g], [0.0, 0.1], [1.0], ((0.1, 0.2), (0.1, 0.2)), 2.0, 2.0, False).dim == 3
+    with pytest.raises(ValueError):
+        cpd.CompoundPrism([g], [0.0, 0.1], [1.0], ((0.1, 0.2),), 2.0, 2.0, False)
 
 
 def test_python_api_builds_known_spectrometer():
