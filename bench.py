@@ -278,6 +278,79 @@ def adaptive_run(cpd, N, np, ctx, args):
             "fitness_sem_batched_designs_per_s": fit}
 
 
+def toric_run(cpd, N, np, ctx, args):
+    """The 3D variant (toric entrance and exit lenses, fiber beam; cpd3_*): a batch of synthetic 3D
+    designs at fixed counts (64 wavelengths x 4096 rays, FP64) in shared launches vs one call per
+    design, the FP64 oracle on the host cores, then fitness() with the SEM early exits at its pyo3
+    defaults and the reference's GPU estimator as designer.py calls it."""
+    n = args.toric
+    t0 = time.time()
+    _, D, S = cpd.generate_designs3(0, n)
+    log(f"toric: generated {n} 3D designs in {time.time() - t0:.1f}s")
+    darr = (N.Design3 * n)(*D)
+    samp = N.Sampling(64, 4096, N.MODE_FIXED, 0)
+    sarr = (N.Sampling * n)(*([samp] * n))
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+
+    def batch(sa=sarr):
+        if N.lib.cpd3_fitness_batch(ctx.handle, darr, sa, n, out, status) != 0:
+            raise RuntimeError(N.last_error())
+
+    batch()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        batch()
+    batched = n * reps / (time.perf_counter() - t)
+    infos = np.array([f.info for f in out])
+    ctx.set_option(N.OPT_PROFILE, 1)
+    batch()
+    st = ctx.last_stats()
+    ctx.set_option(N.OPT_PROFILE, 0)
+    kernel_ms, rays = st["trace_ms"], st["rays"]
+    n_plain = min(32, n)
+    f1 = N.Fitness()
+    per = ctypes.c_int()
+    t = time.perf_counter()
+    for i in range(n_plain):
+        if N.lib.cpd3_fitness_batch(ctx.handle, ctypes.byref(darr[i]), ctypes.byref(samp), 1, ctypes.byref(f1),
+                                    ctypes.byref(per)) != 0:
+            raise RuntimeError(N.last_error())
+    plain = n_plain / (time.perf_counter() - t)
+    log(f"toric: fixed 64 x 4096 batched {batched:.0f} designs/s (row kernels {kernel_ms:.2f} ms, "
+        f"{rays / (kernel_ms * 1e-3) / 1e9:.2f} G rays/s), per design {plain:.0f} designs/s")
+    from oracle import oracle3 as O3
+    n_check = min(4, n)
+    ref = O3.fitness_many(D[:n_check], 64, 4096, adaptive=False)
+    identical = int(np.sum(ref[:, 1] == infos[:n_check]))
+    threads = len(os.sched_getaffinity(0))
+    done, t = 0, time.perf_counter()
+    while time.perf_counter() - t < min(args.cpu_seconds, 8.0):
+        O3.fitness_many([D[(done + i) % n] for i in range(threads)], 64, 4096, adaptive=False, threads=threads)
+        done += threads
+    cpu = done / (time.perf_counter() - t)
+    log(f"toric: FP64 oracle {cpu:.1f} designs/s on {threads} threads; {identical}/{n_check} identical")
+    sa = (N.Sampling * n)(*([N.Sampling(16384, 16384, N.MODE_ADAPTIVE, 0)] * n))
+    t = time.perf_counter()
+    batch(sa)
+    sem = n / (time.perf_counter() - t)
+    seeds = np.array([0.37])
+    t = time.perf_counter()
+    if N.lib.cpd3_gpu_fitness_batch(ctx.handle, darr, n, seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1,
+                                    128, 2, 8192, out, status) != 0:
+        raise RuntimeError(N.last_error())
+    est = n / (time.perf_counter() - t)
+    log(f"toric: fitness() with SEM early exits batched {sem:.0f} designs/s; gpu_fitness batched {est:.0f} designs/s")
+    return {"workload": f"{n} 3D designs x (3 prisms, toric faces, fiber beam NA 0.1, 32 bins, 64 wavelengths x "
+                        "4096 rays), FP64, one call per batch",
+            "batched_designs_per_s": batched, "per_design_designs_per_s": plain,
+            "batched_over_per_design": batched / plain, "row_kernel_ms": kernel_ms,
+            "rays_per_s": rays / (kernel_ms * 1e-3), "identical_to_oracle": f"{identical}/{n_check}",
+            "cpu_oracle_designs_per_s": cpu, "cpu_threads": threads,
+            "fitness_sem_batched_designs_per_s": sem, "gpu_fitness_batched_designs_per_s": est}
+
+
 def stress_run(cpd, N, np, ctx, args):
     """Precision stress: designs of the 6-prism, 256-bin distribution with 1024 wavelengths x 2^20 rays,
     evaluated once under each accumulation policy (CPD_OPT_ACCUMULATE).  The FP64 oracle (fixed count,
@@ -352,6 +425,9 @@ def main() -> int:
     ap.add_argument("--adaptive", type=int, default=1024,
                     help="designs of the adaptive phase: the reference GPU estimator as the optimiser calls it "
                          "(gpu_fitness, max_n 128, max_eval 8192) batched vs per design (0: skip)")
+    ap.add_argument("--toric", type=int, default=256,
+                    help="designs of the 3D phase (toric lenses, fiber beam; 0: skip)")
+    ap.add_argument("--toric-only", action="store_true", help="only the 3D phase (its JSON object)")
     ap.add_argument("--stress", action="store_true",
                     help="only the precision stress: 512 designs (6 prisms, 256 bins, 1024 wavelengths x 2^20 rays) "
                          "under each accumulation policy, MI error against the FP64 oracle on a subset")
@@ -385,6 +461,9 @@ def main() -> int:
     log(f"device {info}")
     if args.stress:
         print(json.dumps({"stress": stress_run(cpd, N, np, ctx, args), "device": info}), flush=True)
+        return 0
+    if args.toric_only:
+        print(json.dumps({"toric3d": toric_run(cpd, N, np, ctx, args), "device": info}), flush=True)
         return 0
     n = args.designs
     t0 = time.time()
@@ -491,6 +570,10 @@ def main() -> int:
     if args.adaptive > 0:
         adaptive = adaptive_run(cpd, N, np, ctx, args)
 
+    toric = None
+    if args.toric > 0:
+        toric = toric_run(cpd, N, np, ctx, args)
+
     serve = None
     if args.serve_rate > 0 or args.serve_sweep:
         t = time.time()
@@ -529,6 +612,7 @@ def main() -> int:
                        "batch_p99": float(np.percentile(step_ms, 99))},
         "uneven": uneven,
         "adaptive": adaptive,
+        "toric3d": toric,
         "served": serve,
         "mi_error_bits": mi_err,
         "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",

@@ -15,8 +15,10 @@
 // Every ray is traced in FP64 in the reference's operation order (trace3.h).
 // The trigonometric part of each ray's beam sample comes from a table built
 // on the host per (beam kind, seed, rays), so rows match the FP64 CPU path
-// bit for bit.  A wavefront owns a row; a row's glass indices are computed
-// once.
+// bit for bit.  Where a ray meets the entrance lens does not depend on the
+// wavelength: one launch computes every (design, ray) entrance hit before the
+// rows, and each row refracts from the table.  A wavefront owns a row; a
+// row's glass indices are computed once.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -50,27 +52,33 @@ struct RowArgs3 {
     const cpd_design3 *designs;
     const BeamSample3 *fiber;  // sample of ray k at k - 1, per beam kind
     const BeamSample3 *gauss;
+    const EntryHit3 *entry;    // [design][k - 1] entrance hits, or null: trace the entrance per row
+    uint32_t entry_rays;       // rays per design in entry
 };
 
-// Rays of a 3D row, all in FP64 (nothing to re-trace).
+// Rays of a 3D row, all in FP64 (nothing to re-trace), from the design's
+// entrance hits when the launch has them.
 struct Tracer3 {
     static constexpr bool kRetrace = false;
     static constexpr size_t kExtraLds = 0;
     const cpd_design3 *g;
     const BeamSample3 *samples;
+    const EntryHit3 *entry;
     double n[CPD_MAX_PRISMS];
     uint32_t nbin;
     bool live = true;
 
     __device__ Tracer3(const RowArgs3 &a, uint32_t r) : g(&a.designs[a.slots.row_design[r]]) {
         samples = g->beam_kind == CPD_BEAM_FIBER ? a.fiber : a.gauss;
+        entry = a.entry ? a.entry + (size_t)a.slots.row_design[r] * a.entry_rays : nullptr;
         glass_indices3(*g, a.slots.row_lambda[r], n);
         nbin = g->bin_count;
     }
     __device__ uint32_t trace(uint32_t k, bool active, double &t, bool &) {
         t = 0.0;
         if (!active) return nbin;
-        return trace3(*g, n, beam_ray3(*g, samples[k - 1]), t);
+        const Ray3 ray = beam_ray3(*g, samples[k - 1]);
+        return entry ? trace3_entered(*g, n, ray, entry[k - 1], t) : trace3(*g, n, ray, t);
     }
     __device__ uint32_t retrace(uint32_t, double &t) {
         t = 0.0;
@@ -94,6 +102,17 @@ __global__ __launch_bounds__(64) void chunk_rows3_kernel(RowArgs3 a) {
     Tracer3 tr(a, r);
     __syncthreads();
     chunk_row(a.slots, r, tr, lds);
+}
+
+// Entrance hits of rays k = 1..n_rays of every design (at [design][k - 1]).
+__global__ __launch_bounds__(256) void entrance3_kernel(const cpd_design3 *designs, uint32_t n_designs,
+                                                        const BeamSample3 *fiber, const BeamSample3 *gauss,
+                                                        uint32_t n_rays, EntryHit3 *out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= (uint64_t)n_designs * n_rays) return;
+    const cpd_design3 &g = designs[i / n_rays];
+    const BeamSample3 *s = g.beam_kind == CPD_BEAM_FIBER ? fiber : gauss;
+    out[i] = entrance_hit3(g, beam_ray3(g, s[i % n_rays]));
 }
 
 // propagation_test_kernel (kernel.rs:195-224) for 3D designs: ray i at
@@ -180,11 +199,32 @@ cpd_status Context::stage_designs3(const cpd_design3 *designs, const std::vector
     return CPD_OK;
 }
 
+// The entrance hits of rays 1..n_rays of the staged designs, or null when the
+// table would pass kEntryBudget (the rows then trace their entrances).
+static constexpr size_t kEntryBudget = size_t(2) << 30;
+
+cpd_status Context::entrance_table3(const cpd_design3 *designs_dev, uint32_t n_designs, const void *fiber_tab,
+                                    const void *gauss_tab, uint32_t n_rays, const void **table) {
+    *table = nullptr;
+    const size_t bytes = (size_t)n_designs * n_rays * sizeof(EntryHit3);
+    if (bytes == 0 || bytes > kEntryBudget) return CPD_OK;
+    cpd_status st;
+    if ((st = entry3.reserve(bytes, false, stream)) != CPD_OK) return st;
+    const uint64_t total = (uint64_t)n_designs * n_rays;
+    hipLaunchKernelGGL(entrance3_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, designs_dev,
+                       n_designs, (const BeamSample3 *)fiber_tab, (const BeamSample3 *)gauss_tab, n_rays,
+                       entry3.as<EntryHit3>());
+    R3_TRY(hipGetLastError());
+    *table = entry3.ptr;
+    return CPD_OK;
+}
+
 cpd_status Context::run_rows3(bool chunked, const cpd_design3 *designs_dev, const std::vector<uint32_t> &row_design,
                               const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
                               const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays,
                               uint32_t nbin_max, double max_err_sq, const void *fiber_tab, const void *gauss_tab,
-                              const double *&rows_host, const uint64_t *&rays_host) {
+                              const void *entry, uint32_t entry_rays, const double *&rows_host,
+                              const uint64_t *&rays_host) {
     const uint32_t n_rows = (uint32_t)row_design.size();
     if (n_rows == 0) return CPD_OK;
     RowArgs3 a{};
@@ -195,6 +235,8 @@ cpd_status Context::run_rows3(bool chunked, const cpd_design3 *designs_dev, cons
     a.designs = designs_dev;
     a.fiber = (const BeamSample3 *)fiber_tab;
     a.gauss = (const BeamSample3 *)gauss_tab;
+    a.entry = (const EntryHit3 *)entry;
+    a.entry_rays = entry_rays;
     const size_t lds = chunked ? chunk_lds_bytes(nbin_max) : (size_t)3 * nbin_max * sizeof(double);
     if (profile) R3_TRY(hipEventRecord(ev[0], stream));
     if (chunked)
@@ -224,6 +266,8 @@ cpd_status Context::run_fitness3(const cpd_design3 *designs, const cpd_sampling 
     const cpd_design3 *ddev;
     uint32_t nbin_max;
     if ((st = stage_designs3(designs, valid, &ddev, &nbin_max)) != CPD_OK) return st;
+    const void *entry = nullptr;
+    if ((st = entrance_table3(ddev, (uint32_t)n, ftab, gtab, max_m, &entry)) != CPD_OK) return st;
     row_kernel_ms = 0.0;
     double fold_ms = 0.0;
     const double e2 = adaptive ? kMaxErr * kMaxErr : 0.0;
@@ -260,7 +304,7 @@ cpd_status Context::run_fitness3(const cpd_design3 *designs, const cpd_sampling 
         }
         if (row_design.empty()) break;
         if ((st = run_rows3(false, ddev, row_design, row_lambda, row_out, row_cap, out_n, max_m, nbin_max, e2, ftab,
-                            gtab, rows, rays)) != CPD_OK)
+                            gtab, entry, max_m, rows, rays)) != CPD_OK)
             return st;
         const auto f0 = std::chrono::steady_clock::now();
         for (size_t r = 0; r < row_design.size();) {
@@ -301,6 +345,8 @@ cpd_status Context::rows3(const cpd_design3 &g, const double *wavelengths, size_
     uint32_t nbin_max;
     std::vector<uint32_t> one{0};
     if ((st = stage_designs3(&g, one, &ddev, &nbin_max)) != CPD_OK) return st;
+    const void *entry = nullptr;
+    if ((st = entrance_table3(ddev, 1, tab, tab, max_m, &entry)) != CPD_OK) return st;
     std::vector<uint32_t> row_design(n_w, 0), row_cap(n_w, max_m);
     std::vector<double> row_lambda(wavelengths, wavelengths + n_w);
     std::vector<uint64_t> row_out(n_w);
@@ -309,7 +355,7 @@ cpd_status Context::rows3(const cpd_design3 &g, const double *wavelengths, size_
     const uint64_t *rays = nullptr;
     const double e2 = mode == CPD_MODE_ADAPTIVE ? kMaxErr * kMaxErr : 0.0;
     if ((st = run_rows3(false, ddev, row_design, row_lambda, row_out, row_cap, n_w * g.bin_count, max_m, nbin_max, e2,
-                        tab, tab, rows, rays)) != CPD_OK)
+                        tab, tab, entry, max_m, rows, rays)) != CPD_OK)
         return st;
     memcpy(out, rows, n_w * g.bin_count * sizeof(double));
     return CPD_OK;
@@ -356,10 +402,11 @@ cpd_status Context::run_gpu_fitness3(const cpd_design3 *designs, const std::vect
             }
         }
         if (row_design.empty()) break;
-        const void *tab = nullptr;
+        const void *tab = nullptr, *entry = nullptr;
         if ((st = ensure_samples3(CPD_BEAM_FIBER, seed_fx, table_n, &tab)) != CPD_OK) return st;
+        if ((st = entrance_table3(ddev, (uint32_t)n, tab, tab, table_n, &entry)) != CPD_OK) return st;
         if ((st = run_rows3(true, ddev, row_design, row_lambda, row_out, std::vector<uint32_t>(), out_n, max_evals,
-                            nbin_max, kMaxErr * kMaxErr, tab, tab, rows, rays)) != CPD_OK)
+                            nbin_max, kMaxErr * kMaxErr, tab, tab, entry, table_n, rows, rays)) != CPD_OK)
             return st;
         const auto f0 = std::chrono::steady_clock::now();
         for (size_t r = 0; r < row_design.size(); ++r) total_rays += rays[r];

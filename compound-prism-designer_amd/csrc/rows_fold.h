@@ -126,6 +126,9 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
     const uint32_t nbin = tr.nbin;
     const uint32_t cap = a.row_cap[r];
     const double max_err_sq = a.max_err_sq;
+    // without an early exit nothing reads m2 (rows give their means): the
+    // fold updates counts and means alone, in the same operations
+    const bool track_m2 = max_err_sq > 0.0;
     double count = 0.0;
     bool stop = false;
     for (uint32_t base = 1; base <= cap && !stop; base += 64) {
@@ -147,12 +150,17 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
             consumed = l + 1;
             const uint32_t b = __builtin_amdgcn_readlane(bin, l);
             W64 w = load_w(lds, a.nbin_max, b);
-            w_skip(w, count);
-            w_next(w, readlane_f64(t, l));
+            if (track_m2) {
+                w_skip(w, count);
+                w_next(w, readlane_f64(t, l));
+            } else {
+                w_skip_mean(w, count);
+                w_next_mean(w, readlane_f64(t, l));
+            }
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) store_w(lds, a.nbin_max, b, w);
             __builtin_amdgcn_wave_barrier();
-            if (w_sem_le(w, max_err_sq)) {
+            if (track_m2 && w_sem_le(w, max_err_sq)) {
                 stop = true;
                 break;
             }

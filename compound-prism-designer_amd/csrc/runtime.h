@@ -105,16 +105,19 @@ struct Context {
                            const uint64_t *&rays_host);
     // 3D spectrometers (rows3.hip)
     DeviceBuffer design3_stage, samples3, samples3_gauss;  // records; beam samples of fiber and Gaussian rays
+    DeviceBuffer entry3;                                    // entrance hits of the staged designs' rays
     uint64_t samples3_key = ~0ull, samples3_gauss_key = ~0ull;  // seeds of the sample tables
     uint32_t samples3_n = 0, samples3_gauss_n = 0;              // rays they hold
     cpd_status ensure_samples3(uint32_t kind, uint64_t seed_fx, uint32_t n, const void **table);
     cpd_status stage_designs3(const cpd_design3 *designs, const std::vector<uint32_t> &valid,
                               const cpd_design3 **designs_dev, uint32_t *nbin_max);
+    cpd_status entrance_table3(const cpd_design3 *designs_dev, uint32_t n_designs, const void *fiber_tab,
+                               const void *gauss_tab, uint32_t n_rays, const void **table);
     cpd_status run_rows3(bool chunked, const cpd_design3 *designs_dev, const std::vector<uint32_t> &row_design,
                          const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
                          const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays, uint32_t nbin_max,
-                         double max_err_sq, const void *fiber_tab, const void *gauss_tab, const double *&rows_host,
-                         const uint64_t *&rays_host);
+                         double max_err_sq, const void *fiber_tab, const void *gauss_tab, const void *entry,
+                         uint32_t entry_rays, const double *&rows_host, const uint64_t *&rays_host);
     cpd_status run_fitness3(const cpd_design3 *designs, const cpd_sampling *sampling,
                             const std::vector<uint32_t> &valid, cpd_fitness *out);
     cpd_status rows3(const cpd_design3 &g, const double *wavelengths, size_t n_w, uint32_t max_m, uint32_t mode,

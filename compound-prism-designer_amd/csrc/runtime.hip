@@ -415,6 +415,7 @@ Context::~Context() {
     design3_stage.release();
     samples3.release();
     samples3_gauss.release();
+    entry3.release();
     host_stage.release();
     row_args_pin.release();
     rows_pin.release();

@@ -213,22 +213,43 @@ CPD_HD int plane3(Ray3 &ray, const double *pt, const double *nm, double height, 
     return refract3(ray, n, n1, n2, ar);
 }
 
-// CompoundPrism::propagate (ray.rs:485-499) given the glasses' indices n[i]
-// at the wavelength (glass_n64 of each glass, as calc_n computes them).  The
-// faces are unrolled so that n stays in registers.
-CPD_HD int prism3(Ray3 &ray, const cpd_design3 &g, const double *n) {
+// The interior faces and the exit lens, for a ray already refracted into
+// glass 0 (the rest of CompoundPrism::propagate, ray.rs:485-499), given the
+// glasses' indices n[i] at the wavelength (glass_n64 of each glass, as calc_n
+// computes them).  The faces are unrolled so that n stays in registers.
+CPD_HD int prism3_inner(Ray3 &ray, const cpd_design3 &g, const double *n) {
     const bool ar = g.ar_coated != 0;
-    int e = lens3(ray, g.lens[0], 1.0, n[0], ar);
-    if (e != CPD_OK) return e;
     double n1 = n[0];
 #pragma unroll
     for (int i = 1; i < CPD_MAX_PRISMS; ++i) {
         if (i >= (int)g.n_glass) break;
-        e = plane3(ray, g.plane_point[i], g.plane_normal[i], g.height, g.half_width, n1, n[i], ar);
+        const int e = plane3(ray, g.plane_point[i], g.plane_normal[i], g.height, g.half_width, n1, n[i], ar);
         if (e != CPD_OK) return e;
         n1 = n[i];
     }
     return lens3(ray, g.lens[1], n1, 1.0, ar);
+}
+
+// CompoundPrism::propagate (ray.rs:485-499).
+CPD_HD int prism3(Ray3 &ray, const cpd_design3 &g, const double *n) {
+    const int e = lens3(ray, g.lens[0], 1.0, n[0], g.ar_coated != 0);
+    if (e != CPD_OK) return e;
+    return prism3_inner(ray, g, n);
+}
+
+// Where a beam ray meets the entrance lens, and the lens normal there: both
+// independent of the wavelength, so computed once per ray and reused by every
+// wavelength row.  nx is NaN when the ray misses the lens.
+struct EntryHit3 {
+    double px, py, pz, nx, ny, nz;
+};
+
+CPD_HD EntryHit3 entrance_hit3(const cpd_design3 &g, const Ray3 &ray) {
+    double d;
+    P3 nrm;
+    if (!torus_hit(g.lens[0], ray.o, ray.v, d, nrm)) return EntryHit3{0.0, 0.0, 0.0, NAN, NAN, NAN};
+    const P3 p = p3_fma(ray.v, d, ray.o);  // the translation lens3 applies
+    return EntryHit3{p.x, p.y, p.z, nrm.x, nrm.y, nrm.z};
 }
 
 // LinearDetectorArray intersection and bin (spectrometer.rs:265-303), the
@@ -323,6 +344,20 @@ CPD_HD Ray3 beam_ray3(const cpd_design3 &g, const BeamSample3 &s) {
 CPD_HD uint32_t trace3(const cpd_design3 &g, const double *n, Ray3 ray, double &t) {
     t = 0.0;
     if (prism3(ray, g, n) != CPD_OK) return g.bin_count;
+    uint32_t bin = g.bin_count;
+    double tt = 0.0;
+    if (detector3(ray, g, bin, tt) != CPD_OK) return g.bin_count;
+    t = tt;
+    return bin;
+}
+
+// The same from the ray's entrance hit (entrance_hit3 of the same ray).
+CPD_HD uint32_t trace3_entered(const cpd_design3 &g, const double *n, Ray3 ray, const EntryHit3 &h, double &t) {
+    t = 0.0;
+    if (isnan(h.nx)) return g.bin_count;
+    ray.o = P3{h.px, h.py, h.pz};
+    if (refract3(ray, P3{h.nx, h.ny, h.nz}, 1.0, n[0], g.ar_coated != 0) != CPD_OK) return g.bin_count;
+    if (prism3_inner(ray, g, n) != CPD_OK) return g.bin_count;
     uint32_t bin = g.bin_count;
     double tt = 0.0;
     if (detector3(ray, g, bin, tt) != CPD_OK) return g.bin_count;

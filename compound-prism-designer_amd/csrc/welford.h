@@ -34,6 +34,21 @@ CPD_HD void w_skip(W64 &w, double new_count) {
     }
 }
 
+// w_skip and w_next without the m2 update (for folds whose m2 is never read):
+// the same count and mean, operation for operation.
+CPD_HD void w_skip_mean(W64 &w, double new_count) {
+#pragma clang fp contract(off)
+    if (w.count < new_count) {
+        w.mean = (w.count * w.mean) / new_count;
+        w.count = new_count;
+    }
+}
+CPD_HD void w_next_mean(W64 &w, double x) {
+#pragma clang fp contract(off)
+    w.count += 1.0;
+    w.mean += (x - w.mean) / w.count;
+}
+
 // welford.rs:55-58: SEM^2 <= e^2 as m2 < e^2 n (n - 1)
 CPD_HD bool w_sem_le(const W64 &w, double error_squared) {
 #pragma clang fp contract(off)
