@@ -147,17 +147,34 @@ __global__ __launch_bounds__(64) void chunk_rows_kernel(RowArgs a) {
 
 }  // namespace
 
+// Reciprocals RN(1/k), k = 0..n - 1 (k = 0 holds 0), computed by IEEE
+// division on the host; grow-only.
+cpd_status Context::ensure_recip(uint32_t n) {
+    if (recip_n >= n) return CPD_OK;
+    cpd_status st;
+    if ((st = recip.reserve((size_t)n * sizeof(double), false, stream)) != CPD_OK) return st;
+    if ((st = host_stage.reserve((size_t)n * sizeof(double))) != CPD_OK) return st;
+    double *h = (double *)host_stage.ptr;
+    h[0] = 0.0;
+    for (uint32_t k = 1; k < n; ++k) h[k] = 1.0 / (double)k;
+    AD_TRY(hipMemcpyAsync(recip.ptr, h, (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream));
+    AD_TRY(hipStreamSynchronize(stream));
+    recip_n = n;
+    return CPD_OK;
+}
+
 // Stages the rows' descriptors (one pinned copy) and reserves their outputs.
 cpd_status Context::stage_rows(const std::vector<uint32_t> &row_design, const std::vector<double> &row_lambda,
                                const std::vector<uint64_t> &row_out, const std::vector<uint32_t> &row_cap,
                                uint64_t out_n, uint32_t max_rays, uint32_t nbin_max, double max_err_sq,
                                RowSlots &a) {
     const uint32_t n_rows = (uint32_t)row_design.size();
+    cpd_status st;
+    if ((st = ensure_recip(max_rays + 40)) != CPD_OK) return st;
     const size_t off_lam = (n_rows * sizeof(uint32_t) + 255) / 256 * 256;
     const size_t off_out = off_lam + (n_rows * sizeof(double) + 255) / 256 * 256;
     const size_t off_cap = off_out + (n_rows * sizeof(uint64_t) + 255) / 256 * 256;
     const size_t bytes = off_cap + n_rows * sizeof(uint32_t);
-    cpd_status st;
     if ((st = row_stage.reserve(bytes, false, stream)) != CPD_OK) return st;
     if ((st = row_buf.reserve(out_n * sizeof(double) + n_rows * sizeof(uint64_t) + 256, false, stream)) != CPD_OK)
         return st;
@@ -179,6 +196,7 @@ cpd_status Context::stage_rows(const std::vector<uint32_t> &row_design, const st
     a.row_cap = (const uint32_t *)(d + off_cap);
     a.out = row_buf.as<double>();
     a.row_rays = (uint64_t *)(row_buf.as<char>() + off_rays);
+    a.recip = recip.as<double>();
     a.max_err_sq = max_err_sq;
     a.n_rows = n_rows;
     a.max_rays = max_rays;

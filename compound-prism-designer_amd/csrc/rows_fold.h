@@ -43,6 +43,7 @@ struct RowSlots {
     const uint32_t *row_cap;     // [n_rows] rays cap of each row (sem rows)
     double *out;                 // p(d|lambda) rows
     uint64_t *row_rays;          // [n_rows] rays folded into the row
+    const double *recip;         // [k] = RN(1/k), k <= max_rays + 32: the folds' integer divisors
     double max_err_sq;           // SEM^2 threshold of the early exits (0: none)
     uint32_t n_rows;
     uint32_t max_rays;           // rays cap (chunked: a multiple of 32)
@@ -118,7 +119,10 @@ __device__ __forceinline__ void finish_row(const RowSlots &a, uint32_t r, const 
     if (threadIdx.x == 0) a.row_rays[r] = (uint64_t)count;
 }
 
-// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).
+// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).  The summary
+// of the bin hit last stays in registers until a hit lands in another bin
+// (rows mostly hit a few adjacent bins, so most hits skip the LDS round
+// trip); the arithmetic is the same either way.
 template <class Tr>
 __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
@@ -131,6 +135,8 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
     const bool track_m2 = max_err_sq > 0.0;
     double count = 0.0;
     bool stop = false;
+    uint32_t held = ~0u;  // bin whose summary is in w
+    W64 w{0.0, 0.0, 0.0};
     for (uint32_t base = 1; base <= cap && !stop; base += 64) {
         const uint32_t k = base + lane;
         double t = 0.0;
@@ -149,23 +155,34 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
             count += (double)(l + 1 - consumed);
             consumed = l + 1;
             const uint32_t b = __builtin_amdgcn_readlane(bin, l);
-            W64 w = load_w(lds, a.nbin_max, b);
-            if (track_m2) {
-                w_skip(w, count);
-                w_next(w, readlane_f64(t, l));
-            } else {
-                w_skip_mean(w, count);
-                w_next_mean(w, readlane_f64(t, l));
+            if (b != held) {
+                if (held != ~0u) {
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) store_w(lds, a.nbin_max, held, w);
+                    __builtin_amdgcn_wave_barrier();
+                }
+                w = load_w(lds, a.nbin_max, b);
+                held = b;
             }
-            __builtin_amdgcn_wave_barrier();
-            if (lane == 0) store_w(lds, a.nbin_max, b, w);
-            __builtin_amdgcn_wave_barrier();
+            // the divisors are the ray index k = count and k + 1
+            const double rk = a.recip[(uint32_t)count], rk1 = a.recip[(uint32_t)count + 1];
+            if (track_m2) {
+                w_skip_r(w, count, rk);
+                w_next_r(w, readlane_f64(t, l), rk1);
+            } else {
+                w_skip_mean_r(w, count, rk);
+                w_next_mean_r(w, readlane_f64(t, l), rk1);
+            }
             if (track_m2 && w_sem_le(w, max_err_sq)) {
                 stop = true;
                 break;
             }
         }
         if (!stop) count += (double)(n_here - consumed);
+    }
+    if (held != ~0u) {
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) store_w(lds, a.nbin_max, held, w);
     }
     finish_row(a, r, lds, nbin, count);
 }
@@ -258,7 +275,8 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
                     const double wmean = readlane_f64(half_sum(tb), 32 * h) / 32.0;
                     const double dev = mine ? tb - wmean : 0.0;
                     const double wm2 = readlane_f64(half_sum(dev * dev), 32 * h);
-                    const W64 w = w_add(load_w(lds, a.nbin_max, bmin), W64{32.0, wmean, wm2});
+                    const W64 held = load_w(lds, a.nbin_max, bmin);
+                    const W64 w = w_add_r(held, W64{32.0, wmean, wm2}, a.recip[(uint32_t)held.count + 32]);
                     __builtin_amdgcn_wave_barrier();
                     if (lane == 0) store_w(lds, a.nbin_max, bmin, w);
                     __builtin_amdgcn_wave_barrier();
@@ -282,7 +300,8 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
                             const double dev = v(j) - wmean;
                             return dev * dev;
                         });
-                        const W64 w = w_add(load_w(lds, a.nbin_max, bin), W64{32.0, wmean, wm2});
+                        const W64 held = load_w(lds, a.nbin_max, bin);
+                        const W64 w = w_add_r(held, W64{32.0, wmean, wm2}, a.recip[(uint32_t)held.count + 32]);
                         store_w(lds, a.nbin_max, bin, w);
                         ok = w_sem_le(w, max_err_sq);
                     }

@@ -98,6 +98,9 @@ struct Context {
                         const uint64_t *&rays_host);
     cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
                              const cpd_design **designs_dev, uint32_t *nbin_max);
+    DeviceBuffer recip;        // [k] = RN(1/k) for the row folds' integer divisors
+    uint32_t recip_n = 0;
+    cpd_status ensure_recip(uint32_t n);
     cpd_status stage_rows(const std::vector<uint32_t> &row_design, const std::vector<double> &row_lambda,
                           const std::vector<uint64_t> &row_out, const std::vector<uint32_t> &row_cap, uint64_t out_n,
                           uint32_t max_rays, uint32_t nbin_max, double max_err_sq, RowSlots &slots);

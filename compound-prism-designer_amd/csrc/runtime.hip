@@ -416,6 +416,7 @@ Context::~Context() {
     samples3.release();
     samples3_gauss.release();
     entry3.release();
+    recip.release();
     host_stage.release();
     row_args_pin.release();
     rows_pin.release();

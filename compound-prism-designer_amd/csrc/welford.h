@@ -34,6 +34,50 @@ CPD_HD void w_skip(W64 &w, double new_count) {
     }
 }
 
+// x / d for an integer d (1 <= d <= 2^24) from rd = RN(1/d): one fused
+// correction of x rd gives the correctly rounded quotient (Markstein's
+// theorem: rd within half an ulp of 1/d and x rd within one ulp of x / d),
+// i.e. exactly the value x / d has, in three dependent operations instead of
+// a division sequence.
+CPD_HD double div_int(double x, double d, double rd) {
+#pragma clang fp contract(off)
+    const double q0 = x * rd;
+    return fma(fma(-q0, d, x), rd, q0);
+}
+
+// w_skip, w_next and w_add with their divisions by integer counts taken from
+// reciprocals: rc = RN(1/new_count) for skip, RN(1/(count + 1)) for next,
+// RN(1/(a.count + b.count)) for add.  Bit-identical to the plain forms.
+CPD_HD void w_skip_r(W64 &w, double new_count, double rc) {
+#pragma clang fp contract(off)
+    if (w.count < new_count) {
+        const double zero_count = new_count - w.count;
+        w.mean = div_int(w.count * w.mean, new_count, rc);
+        w.m2 += div_int(w.mean * w.mean * w.count * zero_count, new_count, rc);
+        w.count = new_count;
+    }
+}
+CPD_HD void w_next_r(W64 &w, double x, double rc) {
+#pragma clang fp contract(off)
+    w.count += 1.0;
+    const double delta = x - w.mean;
+    w.mean += div_int(delta, w.count, rc);
+    const double delta2 = x - w.mean;
+    w.m2 += delta * delta2;
+}
+CPD_HD void w_skip_mean_r(W64 &w, double new_count, double rc) {
+#pragma clang fp contract(off)
+    if (w.count < new_count) {
+        w.mean = div_int(w.count * w.mean, new_count, rc);
+        w.count = new_count;
+    }
+}
+CPD_HD void w_next_mean_r(W64 &w, double x, double rc) {
+#pragma clang fp contract(off)
+    w.count += 1.0;
+    w.mean += div_int(x - w.mean, w.count, rc);
+}
+
 // w_skip and w_next without the m2 update (for folds whose m2 is never read):
 // the same count and mean, operation for operation.
 CPD_HD void w_skip_mean(W64 &w, double new_count) {
@@ -62,6 +106,14 @@ CPD_HD W64 w_add(const W64 &a, const W64 &b) {
     const double delta = b.mean - a.mean;
     const double mean = (a.count * a.mean + b.count * b.mean) / count;
     const double m2 = a.m2 + b.m2 + delta * delta * a.count * b.count / count;
+    return W64{count, mean, m2};
+}
+CPD_HD W64 w_add_r(const W64 &a, const W64 &b, double rc) {
+#pragma clang fp contract(off)
+    const double count = a.count + b.count;
+    const double delta = b.mean - a.mean;
+    const double mean = div_int(a.count * a.mean + b.count * b.mean, count, rc);
+    const double m2 = a.m2 + b.m2 + div_int(delta * delta * a.count * b.count, count, rc);
     return W64{count, mean, m2};
 }
 
