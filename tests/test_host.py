@@ -149,3 +149,23 @@ def test_python_api_builds_known_spectrometer():
     with pytest.raises(cpd.RayTraceError):
         bad = cpd.CompoundPrism(glasses, [0.0, 1.5, -1.5, 1.5], [0.0] * 3, 0.21, 2.5, 2.0, False)
         cpd.position_detector_array(3.2, 0.0, bad, wl, beam, 1.0)
+
+
+def test_integer_division_from_reciprocal_is_exact():
+    # welford.h div_int: q0 = x RN(1/d); q = fma(fma(-q0, d, x), RN(1/d), q0) is the correctly rounded
+    # x / d for integer divisors (Markstein); the row folds rely on it to stay bit-identical
+    libm = ctypes.CDLL(None)
+    libm.fma.argtypes = [ctypes.c_double] * 3
+    libm.fma.restype = ctypes.c_double
+    rng = np.random.default_rng(5)
+    ds = np.concatenate([np.arange(1, 2049), rng.integers(1, 2 ** 24 + 1, 4000),
+                         [2 ** e + o for e in range(1, 24) for o in (-1, 0, 1)]])
+    for d in ds:
+        d = float(d)
+        r = 1.0 / d
+        xs = np.concatenate([rng.random(6), rng.random(3) - rng.random(3), (d - 1) * rng.random(3) * 1e-3,
+                             np.ldexp(rng.random(3) + 0.5, rng.integers(-40, 20, 3))])
+        for x in xs:
+            x = float(x)
+            q0 = x * r
+            assert libm.fma(libm.fma(-q0, d, x), r, q0) == x / d, (x, d)
