@@ -95,7 +95,9 @@ __global__ __launch_bounds__(64) void sem_rows3_kernel(RowArgs3 a) {
     sem_row(a.slots, r, tr, lds);
 }
 
-__global__ __launch_bounds__(64) void chunk_rows3_kernel(RowArgs3 a) {
+// Three wavefronts per SIMD: the compiler fits the FP64 trace and the chunk
+// fold in 166 VGPRs without spilling (182 unconstrained, two wavefronts).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void chunk_rows3_kernel(RowArgs3 a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);

@@ -62,6 +62,12 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     return v;
 }
 
+// Minimum over the 32 lanes of each half-wave.
+__device__ __forceinline__ uint32_t half_min_u32(uint32_t v) {
+    for (int off = 16; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
 __device__ __forceinline__ uint64_t lanemask_below() {
     return (1ull << __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u))) - 1ull;
 }
@@ -190,26 +196,26 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 // Passes of 64 rays (two chunks) traced ahead of the fold.
 constexpr uint32_t kAhead = 4;
 
-// LDS of chunk_row after the 3 nbin_max Welford doubles: a half's
+// LDS of chunk_row after the 3 nbin_max Welford doubles: the pass's
 // (transmittance, bin) pairs, and the traced passes' transmittances, bins and
 // re-trace queue; the tracer's own bytes follow.
 __host__ __device__ constexpr size_t chunk_lds_bytes(uint32_t nbin_max) {
-    return (size_t)3 * nbin_max * sizeof(double) + 32 * 16 + kAhead * 64 * (sizeof(double) + 2 * sizeof(uint32_t));
+    return (size_t)3 * nbin_max * sizeof(double) + 64 * 16 + kAhead * 64 * (sizeof(double) + 2 * sizeof(uint32_t));
 }
 
 // kernel.rs:89-193: a wavefront traces kAhead passes of two chunks (lanes
 // 0-31 and 32-63) before folding them in order; flagged rays of all the
 // passes are queued and re-traced together, and rays traced beyond the chunk
-// where the row stops are not folded.  In the fold, the two lowest bins of a
-// chunk are folded one at a time with half-wave butterflies; when more
-// remain, every lane that hit one of them evaluates the same butterfly sums
-// for its own bin from LDS, so a chunk spread over many bins costs one pass
-// instead of one per bin.
+// where the row stops are not folded.  In the fold, each chunk's (32, mean,
+// m2) per bin comes from half-wave butterflies, both halves at once: the two
+// lowest bins of each half one at a time, then every lane that hit one of the
+// remaining bins evaluates the same butterfly sums for its own bin from LDS.
+// The merges into the row's summaries follow chunk by chunk.
 template <class Tr>
 __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
-    double2 *tpair = (double2 *)(lds + 3 * a.nbin_max);  // bin ~0u: nothing left to fold
-    double *tv = (double *)(tpair + 32);                 // [kAhead * 64] pass p, lane at p * 64 + lane
+    double2 *tpair = (double2 *)(lds + 3 * a.nbin_max);  // [64]; bin ~0u: nothing left to fold
+    double *tv = (double *)(tpair + 64);                 // [kAhead * 64] pass p, lane at p * 64 + lane
     uint32_t *bv = (uint32_t *)(tv + kAhead * 64);
     uint32_t *queue = bv + kAhead * 64;
     const uint32_t lane = threadIdx.x;
@@ -259,57 +265,61 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
             }
         }
         __builtin_amdgcn_wave_barrier();
-        // fold the passes' chunks in order
+        // fold the passes' chunks in order.  A chunk's statistics per bin
+        // depend on its own 32 lanes only, so both halves form theirs at
+        // once; the merges into the row's summaries then run chunk by
+        // chunk, every bin of a chunk at once.
         for (uint32_t p = 0; p < passes && !stop; ++p) {
             const uint32_t i = i0 + 2 * p;
             const double t = tv[p * 64 + lane];
             const uint32_t bin = bv[p * 64 + lane];
-            for (uint32_t h = 0; h < 2 && i + h < iters; ++h) {
-                count += 32.0;
-                const bool mine = half == h;
-                bool pending = mine && bin < nbin;
-                bool finished = __ballot(pending) != 0;
-                for (int serial = 0; serial < 2 && __ballot(pending) != 0; ++serial) {
-                    const uint32_t bmin = wave_min_u32(pending ? bin : 0xffffffffu);
-                    const double tb = (mine && bin == bmin) ? t : 0.0;
-                    const double wmean = readlane_f64(half_sum(tb), 32 * h) / 32.0;
-                    const double dev = mine ? tb - wmean : 0.0;
-                    const double wm2 = readlane_f64(half_sum(dev * dev), 32 * h);
-                    const W64 held = load_w(lds, a.nbin_max, bmin);
-                    const W64 w = w_add_r(held, W64{32.0, wmean, wm2}, a.recip[(uint32_t)held.count + 32]);
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 0) store_w(lds, a.nbin_max, bmin, w);
-                    __builtin_amdgcn_wave_barrier();
-                    finished = finished && w_sem_le(w, max_err_sq);
-                    if (bin == bmin) pending = false;
-                }
-                if (__ballot(pending) != 0) {
-                    // the remaining bins at once; lanes that share a bin
-                    // compute and store the same record
-                    __builtin_amdgcn_wave_barrier();
-                    if (mine) tpair[l] = make_double2(t, __longlong_as_double(pending ? bin : ~0u));
-                    __builtin_amdgcn_wave_barrier();
-                    bool ok = true;
-                    if (pending) {
-                        auto v = [&](int j) {
-                            const double2 pr = tpair[j];
-                            return (uint32_t)__double_as_longlong(pr.y) == bin ? pr.x : 0.0;
-                        };
-                        const double wmean = butterfly_sum(v) / 32.0;
-                        const double wm2 = butterfly_sum([&](int j) {
-                            const double dev = v(j) - wmean;
-                            return dev * dev;
-                        });
-                        const W64 held = load_w(lds, a.nbin_max, bin);
-                        const W64 w = w_add_r(held, W64{32.0, wmean, wm2}, a.recip[(uint32_t)held.count + 32]);
-                        store_w(lds, a.nbin_max, bin, w);
-                        ok = w_sem_le(w, max_err_sq);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    finished = finished && __ballot(pending && !ok) == 0;
+            // (32, cmean, cm2): this lane's bin over its chunk
+            bool pending = i + half < iters && bin < nbin;
+            double cmean = 0.0, cm2 = 0.0;
+            for (int serial = 0; serial < 2 && __ballot(pending) != 0; ++serial) {
+                const uint32_t bmin = half_min_u32(pending ? bin : 0xffffffffu);
+                const double tb = bin == bmin ? t : 0.0;
+                const double wmean = half_sum(tb) / 32.0;
+                const double dev = tb - wmean;
+                const double wm2 = half_sum(dev * dev);
+                if (pending && bin == bmin) {
+                    cmean = wmean;
+                    cm2 = wm2;
                     pending = false;
                 }
-                if (finished) {
+            }
+            if (__ballot(pending) != 0) {
+                // the remaining bins from LDS, each lane for its own bin
+                __builtin_amdgcn_wave_barrier();
+                tpair[lane] = make_double2(t, __longlong_as_double(pending ? bin : ~0u));
+                __builtin_amdgcn_wave_barrier();
+                if (pending) {
+                    const double2 *mine = tpair + 32 * half;
+                    auto v = [&](int j) {
+                        const double2 pr = mine[j];
+                        return (uint32_t)__double_as_longlong(pr.y) == bin ? pr.x : 0.0;
+                    };
+                    cmean = butterfly_sum(v) / 32.0;
+                    cm2 = butterfly_sum([&](int j) {
+                        const double dev = v(j) - cmean;
+                        return dev * dev;
+                    });
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            for (uint32_t h = 0; h < 2 && i + h < iters; ++h) {
+                count += 32.0;
+                const bool merge = half == h && bin < nbin;
+                bool ok = true;
+                if (merge) {
+                    // lanes that share a bin compute and store the same record
+                    const W64 held = load_w(lds, a.nbin_max, bin);
+                    const W64 w = w_add_r(held, W64{32.0, cmean, cm2}, a.recip[(uint32_t)held.count + 32]);
+                    store_w(lds, a.nbin_max, bin, w);
+                    ok = w_sem_le(w, max_err_sq);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (__ballot(merge) != 0 && __ballot(merge && !ok) == 0) {
                     stop = true;
                     break;
                 }

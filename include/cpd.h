@@ -292,7 +292,9 @@ cpd_status cpd_p_dets_l_wavelength(cpd_ctx *ctx, const cpd_design *design, const
  * <= 5e-3 early exit per row; seeds are added until every bin's mean over
  * the rows has an SEM <= 5e-3.  Returns CPD_ERR_NOT_CONVERGED (out holds the
  * estimate after the last seed) where the reference returns None.  Rays are
- * traced in FP64 and folded in FP64 (the reference folds in FP32 with
+ * traced by the FP32 trace with the exact FP64 re-trace of flagged rays (so
+ * bins are the FP64 decisions; CPD_OPT_ROW_TRACE 64 traces every ray in
+ * FP64) and folded in FP64 (the reference traces and folds in FP32 with
  * fast-math).  Replaces cuda_fitness, src/cuda_fitness.rs:347-423, with the
  * kernel src/spectrometer/kernel.rs:89-193, as exposed by
  * Spectrometer.gpu_fitness, src/pylib.rs:1087-1135. */
