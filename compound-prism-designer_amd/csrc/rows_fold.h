@@ -125,10 +125,26 @@ __device__ __forceinline__ void finish_row(const RowSlots &a, uint32_t r, const 
     if (threadIdx.x == 0) a.row_rays[r] = (uint64_t)count;
 }
 
-// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).  The summary
-// of the bin hit last stays in registers until a hit lands in another bin
-// (rows mostly hit a few adjacent bins, so most hits skip the LDS round
-// trip); the arithmetic is the same either way.
+// One hit of the per-ray fold: skip to ray k (the count so far), then the
+// sample; the divisors k and k + 1 come from the reciprocal table.
+__device__ __forceinline__ void fold_hit(W64 &w, double k, double t, const double *recip, bool track_m2) {
+    const double rk = recip[(uint32_t)k], rk1 = recip[(uint32_t)k + 1];
+    if (track_m2) {
+        w_skip_r(w, k, rk);
+        w_next_r(w, t, rk1);
+    } else {
+        w_skip_mean_r(w, k, rk);
+        w_next_mean_r(w, t, rk1);
+    }
+}
+
+// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).  Rays k are
+// folded in order; hits of different bins touch independent summaries, so
+// each pass folds every bin's hits as its own chain at once (the lowest lane
+// of each bin leads it, in ray order).  The early exit stops at the first hit
+// whose bin's SEM passes, which is the earliest such hit over the chains: a
+// pass where a chain passes is folded again one hit at a time from the
+// summaries as they were, and stops there.
 template <class Tr>
 __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
@@ -141,8 +157,6 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
     const bool track_m2 = max_err_sq > 0.0;
     double count = 0.0;
     bool stop = false;
-    uint32_t held = ~0u;  // bin whose summary is in w
-    W64 w{0.0, 0.0, 0.0};
     for (uint32_t base = 1; base <= cap && !stop; base += 64) {
         const uint32_t k = base + lane;
         double t = 0.0;
@@ -151,44 +165,55 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
         if (Tr::kRetrace && wave_any(redo) && redo) bin = tr.retrace(k, t);
         if (!tr.live) t = 0.0;
         const uint32_t n_here = min(64u, cap - base + 1);
-        // visit only the lanes whose ray hit a bin, in ray order; count still
-        // advances by one per ray, as in the reference's loop
-        uint64_t hits = __ballot(lane < n_here && bin < nbin);
-        uint32_t consumed = 0;
-        while (hits) {
-            const uint32_t l = (uint32_t)__builtin_ctzll(hits);
-            hits &= hits - 1;
-            count += (double)(l + 1 - consumed);
-            consumed = l + 1;
-            const uint32_t b = __builtin_amdgcn_readlane(bin, l);
-            if (b != held) {
-                if (held != ~0u) {
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 0) store_w(lds, a.nbin_max, held, w);
-                    __builtin_amdgcn_wave_barrier();
+        const bool hit = lane < n_here && bin < nbin;
+        const uint64_t hits = __ballot(hit);
+        if (hits) {
+            // this lane's bin-mates (leaders only) and whether it leads
+            uint64_t mates = 0;
+            for (uint64_t left = hits; left;) {
+                const uint32_t b = __builtin_amdgcn_readlane(bin, (uint32_t)__builtin_ctzll(left));
+                const uint64_t m = __ballot(hit && bin == b);
+                left &= ~m;
+                if (hit && bin == b && lane == (uint32_t)__builtin_ctzll(m)) mates = m;
+            }
+            const bool lead = mates != 0;
+            W64 w{0.0, 0.0, 0.0};
+            if (lead) w = load_w(lds, a.nbin_max, bin);
+            bool passed = false;
+            for (uint64_t rem = mates; wave_any(rem != 0);) {
+                const uint32_t src = rem ? (uint32_t)__builtin_ctzll(rem) : lane;
+                const double ts = __shfl(t, (int)src);
+                if (rem) {
+                    rem &= rem - 1;
+                    fold_hit(w, (double)(base + src), ts, a.recip, track_m2);
+                    passed = passed || (track_m2 && w_sem_le(w, max_err_sq));
                 }
-                w = load_w(lds, a.nbin_max, b);
-                held = b;
             }
-            // the divisors are the ray index k = count and k + 1
-            const double rk = a.recip[(uint32_t)count], rk1 = a.recip[(uint32_t)count + 1];
-            if (track_m2) {
-                w_skip_r(w, count, rk);
-                w_next_r(w, readlane_f64(t, l), rk1);
+            __builtin_amdgcn_wave_barrier();
+            if (!wave_any(passed)) {
+                if (lead) store_w(lds, a.nbin_max, bin, w);
+                __builtin_amdgcn_wave_barrier();
             } else {
-                w_skip_mean_r(w, count, rk);
-                w_next_mean_r(w, readlane_f64(t, l), rk1);
-            }
-            if (track_m2 && w_sem_le(w, max_err_sq)) {
-                stop = true;
-                break;
+                // again, hit by hit from the summaries in LDS, up to the stop
+                uint64_t left = hits;
+                while (left) {
+                    const uint32_t l = (uint32_t)__builtin_ctzll(left);
+                    left &= left - 1;
+                    const uint32_t b = __builtin_amdgcn_readlane(bin, l);
+                    W64 v = load_w(lds, a.nbin_max, b);
+                    fold_hit(v, (double)(base + l), readlane_f64(t, l), a.recip, track_m2);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) store_w(lds, a.nbin_max, b, v);
+                    __builtin_amdgcn_wave_barrier();
+                    if (w_sem_le(v, max_err_sq)) {
+                        count = (double)(base + l);
+                        stop = true;
+                        break;
+                    }
+                }
             }
         }
-        if (!stop) count += (double)(n_here - consumed);
-    }
-    if (held != ~0u) {
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) store_w(lds, a.nbin_max, held, w);
+        if (!stop) count += (double)n_here;
     }
     finish_row(a, r, lds, nbin, count);
 }
