@@ -64,19 +64,19 @@ struct RowArgs {
     const float *z32;   // the same in FP32
 };
 
-// Per-row constants of the FP32 trace: the planar chain of this wavelength
-// (and its FP64 record, where chain is given).
+// Per-row constants: the planar chain of this wavelength (ok: some ray can
+// leave it), its FP64 record where chain is given (lane 0 writes it), and
+// the FP32 trace's hot record.
 template <bool FP32>
 __device__ __forceinline__ HotRecord row_hot(const cpd_design &g, double lambda, bool &ok,
                                              Chain64 *chain = nullptr) {
     HotRecord hot{};
-    ok = true;
+    LambdaConsts lc;
+    make_lambda_consts(g, lambda, lc, chain);
+    ok = lc.ok != 0;
     if (FP32) {
         TraceDesign td;
         make_trace_design(g, 1, 64, td);
-        LambdaConsts lc;
-        make_lambda_consts(g, lambda, lc, chain);
-        ok = lc.ok != 0;
         hot = make_hot(td, lc);
     }
     return hot;
@@ -85,12 +85,13 @@ __device__ __forceinline__ HotRecord row_hot(const cpd_design &g, double lambda,
 // Rays of a 2D row: FP64 in the reference's operation order, or the FP32
 // trace that flags rays near a decision for the exact FP64 re-trace (the
 // decisions are the FP64 ones; t is rounded to FP32).  k is the ray's
-// 1-based index.  With a chain record in LDS the re-trace starts from the
-// wavelength's planar faces.
+// 1-based index.  The FP64 traces start from the wavelength's planar-face
+// record in LDS (trace64_chain: the same arithmetic as trace64 with the
+// ray-independent directions and transmittances computed once per row).
 template <bool FP32>
 struct Tracer2 {
     static constexpr bool kRetrace = FP32;
-    static constexpr size_t kExtraLds = FP32 ? sizeof(Chain64) : 0;
+    static constexpr size_t kExtraLds = sizeof(Chain64);
     const cpd_design *g;
     const double *z64;
     const float *z32;
@@ -110,7 +111,7 @@ struct Tracer2 {
     }
     __device__ uint32_t trace(uint32_t k, bool active, double &t, bool &redo) {
         t = 0.0;
-        if (!FP32) return active ? trace64(*g, lambda, beam_y64(*g, z64[k - 1]), t) : nbin;
+        if (!FP32) return active ? trace64_chain(*g, *chain, beam_y64(*g, z64[k - 1]), t) : nbin;
         const float z = active ? z32[k - 1] : 0.f;
         const Trace32Result tr = trace32<false>(hot, hot.y_mean - hot.beam_w * z);
         redo = active && tr.guard;
@@ -118,10 +119,7 @@ struct Tracer2 {
         t = tr.t;
         return tr.bin;
     }
-    __device__ uint32_t retrace(uint32_t k, double &t) {
-        const double y = beam_y64(*g, z64[k - 1]);
-        return chain ? trace64_chain(*g, *chain, y, t) : trace64(*g, lambda, y, t);
-    }
+    __device__ uint32_t retrace(uint32_t k, double &t) { return trace64_chain(*g, *chain, beam_y64(*g, z64[k - 1]), t); }
 };
 
 template <bool FP32>
@@ -129,7 +127,7 @@ __global__ __launch_bounds__(64) void sem_rows_kernel(RowArgs a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
-    Tracer2<FP32> tr(a, r, nullptr);
+    Tracer2<FP32> tr(a, r, (Chain64 *)(lds + 3 * a.slots.nbin_max));
     __syncthreads();
     sem_row(a.slots, r, tr, lds);
 }
@@ -139,8 +137,7 @@ __global__ __launch_bounds__(64) void chunk_rows_kernel(RowArgs a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
-    Chain64 *chain = FP32 ? (Chain64 *)((char *)lds + chunk_lds_bytes(a.slots.nbin_max)) : nullptr;
-    Tracer2<FP32> tr(a, r, chain);
+    Tracer2<FP32> tr(a, r, (Chain64 *)((char *)lds + chunk_lds_bytes(a.slots.nbin_max)));
     __syncthreads();
     chunk_row(a.slots, r, tr, lds);
 }
@@ -242,8 +239,8 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
     a.designs = designs_dev;
     a.z64 = z64_dev;
     a.z32 = z32_dev;
-    const size_t lds = chunked ? chunk_lds_bytes(nbin_max) + (fp32 ? Tracer2<true>::kExtraLds : 0)
-                               : (size_t)3 * nbin_max * sizeof(double);
+    const size_t lds = (chunked ? chunk_lds_bytes(nbin_max) : (size_t)3 * nbin_max * sizeof(double)) +
+                       Tracer2<true>::kExtraLds;
     if (profile) AD_TRY(hipEventRecord(ev[0], stream));
     if (chunked && fp32)
         hipLaunchKernelGGL(chunk_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
