@@ -21,7 +21,9 @@
 // order, from beam quantiles built on the host, or (CPD_OPT_ROW_TRACE 32, the
 // default of the GPU estimator) with the FP32 trace and its FP64 re-trace of
 // flagged rays.  The fold is FP64 in the reference's operation order, so with
-// the FP64 trace a row is the FP64 oracle's row bit for bit.  A wavefront owns one row; the sequential fold runs on wave-uniform
+// the FP64 trace a row is the FP64 oracle's row bit for bit.  A block owns
+// one row (four wavefronts tracing for the SEM rows, one for the GPU
+// estimator's); the sequential fold runs in one wavefront on wave-uniform
 // values (each lane computes the same numbers) over summaries kept in LDS.
 // The rows of one design are independent, so the host launches blocks of
 // rows speculatively and scans them in order, discarding rows past the stop.
@@ -123,11 +125,11 @@ struct Tracer2 {
 };
 
 template <bool FP32>
-__global__ __launch_bounds__(64) void sem_rows_kernel(RowArgs a) {
+__global__ __launch_bounds__(64 * kSemWaves) void sem_rows_kernel(RowArgs a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
-    Tracer2<FP32> tr(a, r, (Chain64 *)(lds + 3 * a.slots.nbin_max));
+    Tracer2<FP32> tr(a, r, (Chain64 *)((char *)lds + sem_lds_bytes(a.slots.nbin_max)));
     __syncthreads();
     sem_row(a.slots, r, tr, lds);
 }
@@ -239,7 +241,7 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
     a.designs = designs_dev;
     a.z64 = z64_dev;
     a.z32 = z32_dev;
-    const size_t lds = (chunked ? chunk_lds_bytes(nbin_max) : (size_t)3 * nbin_max * sizeof(double)) +
+    const size_t lds = (chunked ? chunk_lds_bytes(nbin_max) : sem_lds_bytes(nbin_max)) +
                        Tracer2<true>::kExtraLds;
     if (profile) AD_TRY(hipEventRecord(ev[0], stream));
     if (chunked && fp32)
@@ -247,9 +249,9 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
     else if (chunked)
         hipLaunchKernelGGL(chunk_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
     else if (fp32)
-        hipLaunchKernelGGL(sem_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
+        hipLaunchKernelGGL(sem_rows_kernel<true>, dim3(n_rows), dim3(sem_threads(n_rows, cu_count)), lds, stream, a);
     else
-        hipLaunchKernelGGL(sem_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
+        hipLaunchKernelGGL(sem_rows_kernel<false>, dim3(n_rows), dim3(sem_threads(n_rows, cu_count)), lds, stream, a);
     return finish_rows(a.slots, out_n, rows_host, rays_host);
 }
 
@@ -331,7 +333,9 @@ cpd_status Context::run_adaptive(const cpd_design *designs, const cpd_sampling *
                 if (fold[q].fold(rows + row_out[x]) || fold[q].rows >= s.n_wavelengths) fold[q].done = true;
             }
             next_j[q] += e - r;
-            block[q] = std::min<uint64_t>(block[q] * 4, 1024);
+            // a row's time is mostly its serial fold, and a launch rarely fills the
+            // GPU: fewer, larger rounds beat the rows traced past the stop
+            block[q] = std::min<uint64_t>(block[q] * 16, 1024);
             r = e;
         }
         fold_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();

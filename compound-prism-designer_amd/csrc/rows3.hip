@@ -86,7 +86,7 @@ struct Tracer3 {
     }
 };
 
-__global__ __launch_bounds__(64) void sem_rows3_kernel(RowArgs3 a) {
+__global__ __launch_bounds__(64 * kSemWaves) void sem_rows3_kernel(RowArgs3 a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
@@ -239,12 +239,12 @@ cpd_status Context::run_rows3(bool chunked, const cpd_design3 *designs_dev, cons
     a.gauss = (const BeamSample3 *)gauss_tab;
     a.entry = (const EntryHit3 *)entry;
     a.entry_rays = entry_rays;
-    const size_t lds = chunked ? chunk_lds_bytes(nbin_max) : (size_t)3 * nbin_max * sizeof(double);
+    const size_t lds = chunked ? chunk_lds_bytes(nbin_max) : sem_lds_bytes(nbin_max);
     if (profile) R3_TRY(hipEventRecord(ev[0], stream));
     if (chunked)
         hipLaunchKernelGGL(chunk_rows3_kernel, dim3(n_rows), dim3(64), lds, stream, a);
     else
-        hipLaunchKernelGGL(sem_rows3_kernel, dim3(n_rows), dim3(64), lds, stream, a);
+        hipLaunchKernelGGL(sem_rows3_kernel, dim3(n_rows), dim3(sem_threads(n_rows, cu_count)), lds, stream, a);
     return finish_rows(a.slots, out_n, rows_host, rays_host);
 }
 
@@ -320,6 +320,8 @@ cpd_status Context::run_fitness3(const cpd_design3 *designs, const cpd_sampling 
                 if (fold[q].fold(rows + row_out[x]) || fold[q].rows >= s.n_wavelengths) fold[q].done = true;
             }
             next_j[q] += e - r;
+            // 3D rows are trace-bound: rows past the stop cost throughput, so the
+            // blocks grow more slowly than the 2D rows'
             if (adaptive) block[q] = std::min<uint64_t>(block[q] * 4, 1024);
             r = e;
         }

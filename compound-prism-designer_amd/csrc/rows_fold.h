@@ -2,14 +2,14 @@
 // ray tracer, shared by the 2D rows (adaptive.hip) and the 3D rows
 // (rows3.hip).
 //
-// A wavefront owns one row (a design at one wavelength) and keeps the row's
+// A block owns one row (a design at one wavelength) and keeps the row's
 // per-bin Welford summaries (count, mean, m2) in LDS.
-//  * sem_row: p_dets_l_wavelength with the early exit of
-//    src/fitness.rs:5-34 -- rays k = 1, 2, ... traced 64 at a time and folded
-//    one at a time, in order, into the summary of the bin each hit, padded
+//  * sem_row (kSemWaves wavefronts): p_dets_l_wavelength with the early exit
+//    of src/fitness.rs:5-34 -- rays k = 1, 2, ... traced 64 per wavefront and
+//    folded one at a time, in order, into the summary of the bin each hit, padded
 //    with zeros (skip); the row stops as soon as the bin just updated has a
 //    standard error of the mean below 5e-3.
-//  * chunk_row: the reference's GPU estimator, src/spectrometer/kernel.rs:89-193
+//  * chunk_row (one wavefront): the reference's GPU estimator, src/spectrometer/kernel.rs:89-193
 //    -- 32-ray chunks folded in order, each chunk's (32, mean, m2) over its
 //    32 lanes into every bin it hit, lowest bin first, with the SEM test after
 //    each chunk.
@@ -23,7 +23,8 @@
 //   uint32_t retrace(k, t)         the exact re-trace of a flagged ray
 //   static constexpr bool kRetrace;
 //   static constexpr size_t kExtraLds;   LDS bytes the tracer keeps after
-//                                  chunk_lds_bytes (a chain record, say)
+//                                  sem_lds_bytes or chunk_lds_bytes (a chain
+//                                  record, say)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -110,14 +111,14 @@ __device__ __forceinline__ void store_w(double *lds, uint32_t nbin_max, uint32_t
 // Clears the row's summaries (before the tracer is built; the caller
 // synchronises after).
 __device__ __forceinline__ void clear_row(double *lds, uint32_t nbin_max, uint32_t nbin) {
-    for (uint32_t b = threadIdx.x; b < nbin; b += 64) store_w(lds, nbin_max, b, W64{0.0, 0.0, 0.0});
+    for (uint32_t b = threadIdx.x; b < nbin; b += blockDim.x) store_w(lds, nbin_max, b, W64{0.0, 0.0, 0.0});
 }
 
 // Pads every summary to the rays folded and writes the row's means.
 __device__ __forceinline__ void finish_row(const RowSlots &a, uint32_t r, const double *lds, uint32_t nbin,
                                            double count) {
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < nbin; b += 64) {
+    for (uint32_t b = threadIdx.x; b < nbin; b += blockDim.x) {
         W64 w = load_w(lds, a.nbin_max, b);
         w_skip(w, count);
         a.out[a.row_out[r] + b] = w.mean;
@@ -126,9 +127,8 @@ __device__ __forceinline__ void finish_row(const RowSlots &a, uint32_t r, const 
 }
 
 // One hit of the per-ray fold: skip to ray k (the count so far), then the
-// sample; the divisors k and k + 1 come from the reciprocal table.
-__device__ __forceinline__ void fold_hit(W64 &w, double k, double t, const double *recip, bool track_m2) {
-    const double rk = recip[(uint32_t)k], rk1 = recip[(uint32_t)k + 1];
+// sample; rk and rk1 are the reciprocals of k and k + 1.
+__device__ __forceinline__ void fold_hit_r(W64 &w, double k, double t, double rk, double rk1, bool track_m2) {
     if (track_m2) {
         w_skip_r(w, k, rk);
         w_next_r(w, t, rk1);
@@ -137,18 +137,47 @@ __device__ __forceinline__ void fold_hit(W64 &w, double k, double t, const doubl
         w_next_mean_r(w, t, rk1);
     }
 }
+// The same with the reciprocals from the table.
+__device__ __forceinline__ void fold_hit(W64 &w, double k, double t, const double *recip, bool track_m2) {
+    fold_hit_r(w, k, t, recip[(uint32_t)k], recip[(uint32_t)k + 1], track_m2);
+}
 
-// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).  Rays k are
-// folded in order; hits of different bins touch independent summaries, so
-// each pass folds every bin's hits as its own chain at once (the lowest lane
-// of each bin leads it, in ray order).  The early exit stops at the first hit
-// whose bin's SEM passes, which is the earliest such hit over the chains: a
-// pass where a chain passes is folded again one hit at a time from the
-// summaries as they were, and stops there.
+// Most wavefronts of a sem_row block (blockDim.x / 64 of them): each traces
+// one pass of 64 rays, and the first folds the passes in ray order.
+constexpr uint32_t kSemWaves = 4;
+
+// Threads of a sem_row launch of n_rows rows: one wavefront per row when the
+// rows alone give every SIMD four wavefronts (the other wavefronts of a block
+// would wait through its fold), kSemWaves when fewer rows leave the GPU short
+// of wavefronts.
+inline uint32_t sem_threads(size_t n_rows, int cu_count) {
+    return n_rows >= (size_t)16 * (size_t)(cu_count > 1 ? cu_count : 1) ? 64u : 64u * kSemWaves;
+}
+
+// LDS of sem_row after the 3 nbin_max Welford doubles: the traced passes'
+// transmittances and bins, and the fold's stop flag and ray count; the
+// tracer's own bytes follow.
+__host__ __device__ constexpr size_t sem_lds_bytes(uint32_t nbin_max) {
+    return (size_t)3 * nbin_max * sizeof(double) + kSemWaves * 64 * (sizeof(double) + sizeof(uint32_t)) +
+           2 * sizeof(double);
+}
+
+// fitness.rs:5-34, specialised to p_dets_l_wavelength (48-69).  The block's
+// wavefronts trace as many passes at once; the first wavefront folds them.
+// Rays k are folded in order; hits of different bins touch independent
+// summaries, so each pass folds every bin's hits as its own chain at once
+// (the lowest lane of each bin leads it, in ray order).  The
+// early exit stops at the first hit whose bin's SEM passes, which is the
+// earliest such hit over the chains: a pass where a chain passes is folded
+// again one hit at a time from the summaries as they were, and stops there.
+// Passes traced beyond the stop are not folded.
 template <class Tr>
 __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
-    const uint32_t lane = threadIdx.x;
+    double *ring_t = lds + 3 * a.nbin_max;  // [kSemWaves * 64]: pass p, lane at p * 64 + lane
+    uint32_t *ring_b = (uint32_t *)(ring_t + kSemWaves * 64);
+    double *ctl = (double *)(ring_b + kSemWaves * 64);  // stop flag, rays folded
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
     const uint32_t nbin = tr.nbin;
     const uint32_t cap = a.row_cap[r];
     const double max_err_sq = a.max_err_sq;
@@ -157,63 +186,89 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
     const bool track_m2 = max_err_sq > 0.0;
     double count = 0.0;
     bool stop = false;
-    for (uint32_t base = 1; base <= cap && !stop; base += 64) {
-        const uint32_t k = base + lane;
-        double t = 0.0;
-        bool redo = false;
-        uint32_t bin = tr.live ? tr.trace(k, k <= cap, t, redo) : nbin;
-        if (Tr::kRetrace && wave_any(redo) && redo) bin = tr.retrace(k, t);
-        if (!tr.live) t = 0.0;
-        const uint32_t n_here = min(64u, cap - base + 1);
-        const bool hit = lane < n_here && bin < nbin;
-        const uint64_t hits = __ballot(hit);
-        if (hits) {
-            // this lane's bin-mates (leaders only) and whether it leads
-            uint64_t mates = 0;
-            for (uint64_t left = hits; left;) {
-                const uint32_t b = __builtin_amdgcn_readlane(bin, (uint32_t)__builtin_ctzll(left));
-                const uint64_t m = __ballot(hit && bin == b);
-                left &= ~m;
-                if (hit && bin == b && lane == (uint32_t)__builtin_ctzll(m)) mates = m;
-            }
-            const bool lead = mates != 0;
-            W64 w{0.0, 0.0, 0.0};
-            if (lead) w = load_w(lds, a.nbin_max, bin);
-            bool passed = false;
-            for (uint64_t rem = mates; wave_any(rem != 0);) {
-                const uint32_t src = rem ? (uint32_t)__builtin_ctzll(rem) : lane;
-                const double ts = __shfl(t, (int)src);
-                if (rem) {
-                    rem &= rem - 1;
-                    fold_hit(w, (double)(base + src), ts, a.recip, track_m2);
-                    passed = passed || (track_m2 && w_sem_le(w, max_err_sq));
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (!wave_any(passed)) {
-                if (lead) store_w(lds, a.nbin_max, bin, w);
-                __builtin_amdgcn_wave_barrier();
-            } else {
-                // again, hit by hit from the summaries in LDS, up to the stop
-                uint64_t left = hits;
-                while (left) {
-                    const uint32_t l = (uint32_t)__builtin_ctzll(left);
-                    left &= left - 1;
-                    const uint32_t b = __builtin_amdgcn_readlane(bin, l);
-                    W64 v = load_w(lds, a.nbin_max, b);
-                    fold_hit(v, (double)(base + l), readlane_f64(t, l), a.recip, track_m2);
+    for (uint32_t group = 1; group <= cap && !stop; group += 64 * n_waves) {
+        // this wavefront's pass (one wavefront: kept in registers)
+        double t_own = 0.0;
+        uint32_t bin_own = nbin;
+        if (group + 64 * wave <= cap) {
+            const uint32_t k = group + 64 * wave + lane;
+            bool redo = false;
+            bin_own = tr.live ? tr.trace(k, k <= cap, t_own, redo) : nbin;
+            if (Tr::kRetrace && wave_any(redo) && redo) bin_own = tr.retrace(k, t_own);
+            if (!tr.live) t_own = 0.0;
+        }
+        if (n_waves > 1) {
+            ring_t[wave * 64 + lane] = t_own;
+            ring_b[wave * 64 + lane] = bin_own;
+            __syncthreads();
+        }
+        if (wave == 0) {
+            for (uint32_t p = 0; p < n_waves && !stop; ++p) {
+                const uint32_t base = group + 64 * p;
+                if (base > cap) break;
+                const double t = n_waves > 1 ? ring_t[p * 64 + lane] : t_own;
+                const uint32_t bin = n_waves > 1 ? ring_b[p * 64 + lane] : bin_own;
+                const uint32_t n_here = min(64u, cap - base + 1);
+                const bool hit = lane < n_here && bin < nbin;
+                const uint64_t hits = __ballot(hit);
+                if (hits) {
+                    // this lane's bin-mates (leaders only) and whether it leads
+                    uint64_t mates = 0;
+                    for (uint64_t left = hits; left;) {
+                        const uint32_t b = __builtin_amdgcn_readlane(bin, (uint32_t)__builtin_ctzll(left));
+                        const uint64_t m = __ballot(hit && bin == b);
+                        left &= ~m;
+                        if (hit && bin == b && lane == (uint32_t)__builtin_ctzll(m)) mates = m;
+                    }
+                    const bool lead = mates != 0;
+                    W64 w{0.0, 0.0, 0.0};
+                    if (lead) w = load_w(lds, a.nbin_max, bin);
+                    bool passed = false;
+                    for (uint64_t rem = mates; wave_any(rem != 0);) {
+                        const uint32_t src = rem ? (uint32_t)__builtin_ctzll(rem) : lane;
+                        const double ts = __shfl(t, (int)src);
+                        if (rem) {
+                            rem &= rem - 1;
+                            fold_hit(w, (double)(base + src), ts, a.recip, track_m2);
+                            passed = passed || (track_m2 && w_sem_le(w, max_err_sq));
+                        }
+                    }
                     __builtin_amdgcn_wave_barrier();
-                    if (lane == 0) store_w(lds, a.nbin_max, b, v);
-                    __builtin_amdgcn_wave_barrier();
-                    if (w_sem_le(v, max_err_sq)) {
-                        count = (double)(base + l);
-                        stop = true;
-                        break;
+                    if (!wave_any(passed)) {
+                        if (lead) store_w(lds, a.nbin_max, bin, w);
+                        __builtin_amdgcn_wave_barrier();
+                    } else {
+                        // again, hit by hit from the summaries in LDS, up to the stop
+                        uint64_t left = hits;
+                        while (left) {
+                            const uint32_t l = (uint32_t)__builtin_ctzll(left);
+                            left &= left - 1;
+                            const uint32_t b = __builtin_amdgcn_readlane(bin, l);
+                            W64 v = load_w(lds, a.nbin_max, b);
+                            fold_hit(v, (double)(base + l), readlane_f64(t, l), a.recip, track_m2);
+                            __builtin_amdgcn_wave_barrier();
+                            if (lane == 0) store_w(lds, a.nbin_max, b, v);
+                            __builtin_amdgcn_wave_barrier();
+                            if (w_sem_le(v, max_err_sq)) {
+                                count = (double)(base + l);
+                                stop = true;
+                                break;
+                            }
+                        }
                     }
                 }
+                if (!stop) count += (double)n_here;
+            }
+            if (n_waves > 1 && lane == 0) {
+                ctl[0] = stop ? 1.0 : 0.0;
+                ctl[1] = count;
             }
         }
-        if (!stop) count += (double)n_here;
+        if (n_waves > 1) {
+            __syncthreads();
+            stop = ctl[0] != 0.0;
+            count = ctl[1];
+        }
     }
     finish_row(a, r, lds, nbin, count);
 }
