@@ -124,14 +124,14 @@ struct Tracer2 {
     __device__ uint32_t retrace(uint32_t k, double &t) { return trace64_chain(*g, *chain, beam_y64(*g, z64[k - 1]), t); }
 };
 
-template <bool FP32>
-__global__ __launch_bounds__(64 * kSemWaves) void sem_rows_kernel(RowArgs a) {
+template <bool FP32, uint32_t W>
+__global__ __launch_bounds__(64 * W) void sem_rows_kernel(RowArgs a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
     Tracer2<FP32> tr(a, r, (Chain64 *)((char *)lds + sem_lds_bytes(a.slots.nbin_max)));
     __syncthreads();
-    sem_row(a.slots, r, tr, lds);
+    sem_row<W>(a.slots, r, tr, lds);
 }
 
 template <bool FP32>
@@ -248,10 +248,13 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
         hipLaunchKernelGGL(chunk_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
     else if (chunked)
         hipLaunchKernelGGL(chunk_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
-    else if (fp32)
-        hipLaunchKernelGGL(sem_rows_kernel<true>, dim3(n_rows), dim3(sem_threads(n_rows, cu_count)), lds, stream, a);
-    else
-        hipLaunchKernelGGL(sem_rows_kernel<false>, dim3(n_rows), dim3(sem_threads(n_rows, cu_count)), lds, stream, a);
+    else {
+        const uint32_t waves = sem_waves(n_rows, cu_count);
+        void (*kernel)(RowArgs) = waves == 1 ? (fp32 ? sem_rows_kernel<true, 1> : sem_rows_kernel<false, 1>)
+                                             : (fp32 ? sem_rows_kernel<true, kSemWaves>
+                                                     : sem_rows_kernel<false, kSemWaves>);
+        hipLaunchKernelGGL(kernel, dim3(n_rows), dim3(64 * waves), lds, stream, a);
+    }
     return finish_rows(a.slots, out_n, rows_host, rays_host);
 }
 

@@ -86,13 +86,14 @@ struct Tracer3 {
     }
 };
 
-__global__ __launch_bounds__(64 * kSemWaves) void sem_rows3_kernel(RowArgs3 a) {
+template <uint32_t W>
+__global__ __launch_bounds__(64 * W) void sem_rows3_kernel(RowArgs3 a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
     Tracer3 tr(a, r);
     __syncthreads();
-    sem_row(a.slots, r, tr, lds);
+    sem_row<W>(a.slots, r, tr, lds);
 }
 
 // Three wavefronts per SIMD: the compiler fits the FP64 trace and the chunk
@@ -243,8 +244,10 @@ cpd_status Context::run_rows3(bool chunked, const cpd_design3 *designs_dev, cons
     if (profile) R3_TRY(hipEventRecord(ev[0], stream));
     if (chunked)
         hipLaunchKernelGGL(chunk_rows3_kernel, dim3(n_rows), dim3(64), lds, stream, a);
+    else if (sem_waves(n_rows, cu_count) == 1)
+        hipLaunchKernelGGL(sem_rows3_kernel<1>, dim3(n_rows), dim3(64), lds, stream, a);
     else
-        hipLaunchKernelGGL(sem_rows3_kernel, dim3(n_rows), dim3(sem_threads(n_rows, cu_count)), lds, stream, a);
+        hipLaunchKernelGGL(sem_rows3_kernel<kSemWaves>, dim3(n_rows), dim3(64 * kSemWaves), lds, stream, a);
     return finish_rows(a.slots, out_n, rows_host, rays_host);
 }
 

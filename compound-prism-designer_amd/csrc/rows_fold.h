@@ -142,16 +142,16 @@ __device__ __forceinline__ void fold_hit(W64 &w, double k, double t, const doubl
     fold_hit_r(w, k, t, recip[(uint32_t)k], recip[(uint32_t)k + 1], track_m2);
 }
 
-// Most wavefronts of a sem_row block (blockDim.x / 64 of them): each traces
+// Most wavefronts of a sem_row block (its template argument W): each traces
 // one pass of 64 rays, and the first folds the passes in ray order.
 constexpr uint32_t kSemWaves = 4;
 
-// Threads of a sem_row launch of n_rows rows: one wavefront per row when the
-// rows alone give every SIMD four wavefronts (the other wavefronts of a block
+// Wavefronts per row of a sem_row launch of n_rows rows: one when the rows
+// alone give every SIMD four wavefronts (the other wavefronts of a block
 // would wait through its fold), kSemWaves when fewer rows leave the GPU short
 // of wavefronts.
-inline uint32_t sem_threads(size_t n_rows, int cu_count) {
-    return n_rows >= (size_t)16 * (size_t)(cu_count > 1 ? cu_count : 1) ? 64u : 64u * kSemWaves;
+inline uint32_t sem_waves(size_t n_rows, int cu_count) {
+    return n_rows >= (size_t)16 * (size_t)(cu_count > 1 ? cu_count : 1) ? 1u : kSemWaves;
 }
 
 // LDS of sem_row after the 3 nbin_max Welford doubles: the traced passes'
@@ -170,14 +170,16 @@ __host__ __device__ constexpr size_t sem_lds_bytes(uint32_t nbin_max) {
 // early exit stops at the first hit whose bin's SEM passes, which is the
 // earliest such hit over the chains: a pass where a chain passes is folded
 // again one hit at a time from the summaries as they were, and stops there.
-// Passes traced beyond the stop are not folded.
-template <class Tr>
+// Passes traced beyond the stop are not folded.  W = 1 keeps the pass in
+// registers, with no LDS ring and no barrier.
+template <uint32_t W, class Tr>
 __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
     double *ring_t = lds + 3 * a.nbin_max;  // [kSemWaves * 64]: pass p, lane at p * 64 + lane
     uint32_t *ring_b = (uint32_t *)(ring_t + kSemWaves * 64);
     double *ctl = (double *)(ring_b + kSemWaves * 64);  // stop flag, rays folded
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    constexpr uint32_t n_waves = W;
+    const uint32_t lane = threadIdx.x & 63u, wave = W > 1 ? threadIdx.x >> 6 : 0u;
     const uint32_t nbin = tr.nbin;
     const uint32_t cap = a.row_cap[r];
     const double max_err_sq = a.max_err_sq;
