@@ -93,6 +93,21 @@ def test_fitness_batch_matches_oracle(mode):
     assert np.all(res[:, 1] <= np.log2([d.bin_count for d in D]) + 1e-12)
 
 
+@pytest.mark.parametrize("mode", [N.MODE_ADAPTIVE, N.MODE_FIXED])
+def test_large_launches_match_oracle(mode):
+    # launches of at least 16 rows per CU run one wavefront per row (the
+    # smaller ones above run four)
+    cus = cpd.Context(0).info()["compute_units"]
+    n, nw = (1024, 8) if mode == N.MODE_ADAPTIVE else (512, 16)
+    assert (4 if mode == N.MODE_ADAPTIVE else nw) * n >= 16 * cus
+    _, D = designs(n, first=50)
+    res, st = cpd.fitness_batch3(D, N.Sampling(nw, 256, mode, 0))
+    assert (st == 0).all()
+    ref = O3.fitness_many(D, nw, 256, adaptive=mode == N.MODE_ADAPTIVE)
+    bad = np.flatnonzero((res != ref).any(axis=1))
+    assert bad.size == 0, (bad[:5], res[bad[:5]], ref[bad[:5]])
+
+
 def test_batch_results_do_not_depend_on_companions():
     _, D = designs(6)
     s = N.Sampling(12, 1024, N.MODE_ADAPTIVE, 0)

@@ -46,6 +46,19 @@ def test_adaptive_batch_matches_oracle():
         assert tuple(got[i]) == ref, (i, got[i], ref)
 
 
+def test_large_adaptive_launch_matches_oracle():
+    # a first round of at least 16 rows per CU runs one wavefront per row (the
+    # smaller launches above run four): the same rows, bit for bit
+    ctx = cpd.Context(0)
+    _, D, _, _ = cpd.generate_designs(0, 300, 1280)
+    assert 4 * len(D) >= 16 * ctx.info()["compute_units"]
+    got, st = cpd.fitness_batch(D, N.Sampling(64, 512, N.MODE_ADAPTIVE, 0), ctx=ctx)
+    assert (st == 0).all()
+    ref, _ = O.fitness_many([to_oracle(d) for d in D], 64, 512, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+    bad = np.flatnonzero((got != ref).any(axis=1))
+    assert bad.size == 0, (bad[:5], got[bad[:5]], ref[bad[:5]])
+
+
 def test_adaptive_and_fixed_share_a_batch():
     _, D, _, _ = cpd.generate_designs(0, 100, 8)
     samp = [N.Sampling(32, 2048, N.MODE_ADAPTIVE if i % 2 else N.MODE_FIXED, 0) for i in range(8)]
