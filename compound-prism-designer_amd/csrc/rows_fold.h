@@ -175,6 +175,7 @@ __host__ __device__ constexpr size_t sem_lds_bytes(uint32_t nbin_max) {
 template <uint32_t W, class Tr>
 __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
+    static_assert(W >= 1 && W <= kSemWaves, "the LDS ring holds kSemWaves passes");
     double *ring_t = lds + 3 * a.nbin_max;  // [kSemWaves * 64]: pass p, lane at p * 64 + lane
     uint32_t *ring_b = (uint32_t *)(ring_t + kSemWaves * 64);
     double *ctl = (double *)(ring_b + kSemWaves * 64);  // stop flag, rays folded
