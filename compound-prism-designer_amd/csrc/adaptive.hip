@@ -134,14 +134,14 @@ __global__ __launch_bounds__(64 * W) void sem_rows_kernel(RowArgs a) {
     sem_row<W>(a.slots, r, tr, lds);
 }
 
-template <bool FP32>
-__global__ __launch_bounds__(64) void chunk_rows_kernel(RowArgs a) {
+template <bool FP32, uint32_t W>
+__global__ __launch_bounds__(64 * W) void chunk_rows_kernel(RowArgs a) {
     extern __shared__ double lds[];
     const uint32_t r = blockIdx.x;
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
-    Tracer2<FP32> tr(a, r, (Chain64 *)((char *)lds + chunk_lds_bytes(a.slots.nbin_max)));
+    Tracer2<FP32> tr(a, r, (Chain64 *)((char *)lds + chunk_lds_bytes(a.slots.nbin_max, W)));
     __syncthreads();
-    chunk_row(a.slots, r, tr, lds);
+    chunk_row<W>(a.slots, r, tr, lds);
 }
 
 }  // namespace
@@ -241,20 +241,18 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
     a.designs = designs_dev;
     a.z64 = z64_dev;
     a.z32 = z32_dev;
-    const size_t lds = (chunked ? chunk_lds_bytes(nbin_max) : sem_lds_bytes(nbin_max)) +
+    const uint32_t waves = sem_waves(n_rows, cu_count);
+    const size_t lds = (chunked ? chunk_lds_bytes(nbin_max, waves) : sem_lds_bytes(nbin_max)) +
                        Tracer2<true>::kExtraLds;
     if (profile) AD_TRY(hipEventRecord(ev[0], stream));
-    if (chunked && fp32)
-        hipLaunchKernelGGL(chunk_rows_kernel<true>, dim3(n_rows), dim3(64), lds, stream, a);
-    else if (chunked)
-        hipLaunchKernelGGL(chunk_rows_kernel<false>, dim3(n_rows), dim3(64), lds, stream, a);
-    else {
-        const uint32_t waves = sem_waves(n_rows, cu_count);
-        void (*kernel)(RowArgs) = waves == 1 ? (fp32 ? sem_rows_kernel<true, 1> : sem_rows_kernel<false, 1>)
-                                             : (fp32 ? sem_rows_kernel<true, kSemWaves>
-                                                     : sem_rows_kernel<false, kSemWaves>);
-        hipLaunchKernelGGL(kernel, dim3(n_rows), dim3(64 * waves), lds, stream, a);
-    }
+    void (*kernel)(RowArgs);
+    if (chunked)
+        kernel = waves == 1 ? (fp32 ? chunk_rows_kernel<true, 1> : chunk_rows_kernel<false, 1>)
+                            : (fp32 ? chunk_rows_kernel<true, kSemWaves> : chunk_rows_kernel<false, kSemWaves>);
+    else
+        kernel = waves == 1 ? (fp32 ? sem_rows_kernel<true, 1> : sem_rows_kernel<false, 1>)
+                            : (fp32 ? sem_rows_kernel<true, kSemWaves> : sem_rows_kernel<false, kSemWaves>);
+    hipLaunchKernelGGL(kernel, dim3(n_rows), dim3(64 * waves), lds, stream, a);
     return finish_rows(a.slots, out_n, rows_host, rays_host);
 }
 

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ch
     clear_row(lds, a.slots.nbin_max, a.designs[a.slots.row_design[r]].bin_count);
     Tracer3 tr(a, r);
     __syncthreads();
-    chunk_row(a.slots, r, tr, lds);
+    chunk_row<1>(a.slots, r, tr, lds);
 }
 
 // Entrance hits of rays k = 1..n_rays of every design (at [design][k - 1]).
@@ -240,7 +240,7 @@ cpd_status Context::run_rows3(bool chunked, const cpd_design3 *designs_dev, cons
     a.gauss = (const BeamSample3 *)gauss_tab;
     a.entry = (const EntryHit3 *)entry;
     a.entry_rays = entry_rays;
-    const size_t lds = chunked ? chunk_lds_bytes(nbin_max) : sem_lds_bytes(nbin_max);
+    const size_t lds = chunked ? chunk_lds_bytes(nbin_max, 1) : sem_lds_bytes(nbin_max);
     if (profile) R3_TRY(hipEventRecord(ev[0], stream));
     if (chunked)
         hipLaunchKernelGGL(chunk_rows3_kernel, dim3(n_rows), dim3(64), lds, stream, a);

@@ -279,41 +279,49 @@ __device__ void sem_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 // Passes of 64 rays (two chunks) traced ahead of the fold.
 constexpr uint32_t kAhead = 4;
 
-// LDS of chunk_row after the 3 nbin_max Welford doubles: the pass's
-// (transmittance, bin) pairs, and the traced passes' transmittances, bins and
-// re-trace queue; the tracer's own bytes follow.
-__host__ __device__ constexpr size_t chunk_lds_bytes(uint32_t nbin_max) {
-    return (size_t)3 * nbin_max * sizeof(double) + 64 * 16 + kAhead * 64 * (sizeof(double) + 2 * sizeof(uint32_t));
+// LDS of chunk_row with W wavefronts after the 3 nbin_max Welford doubles:
+// the pass's (transmittance, bin) pairs, the traced passes' transmittances,
+// bins and re-trace queues, and the fold's stop flag and chunk count; the
+// tracer's own bytes follow.
+__host__ __device__ constexpr size_t chunk_lds_bytes(uint32_t nbin_max, uint32_t W) {
+    return (size_t)3 * nbin_max * sizeof(double) + 64 * 16 +
+           (size_t)W * kAhead * 64 * (sizeof(double) + 2 * sizeof(uint32_t)) + 2 * sizeof(double);
 }
 
-// kernel.rs:89-193: a wavefront traces kAhead passes of two chunks (lanes
-// 0-31 and 32-63) before folding them in order; flagged rays of all the
-// passes are queued and re-traced together, and rays traced beyond the chunk
-// where the row stops are not folded.  In the fold, each chunk's (32, mean,
+// kernel.rs:89-193: each of the block's W wavefronts traces kAhead passes
+// of two chunks (lanes 0-31 and 32-63), and the first folds all of them in
+// order; a wavefront's flagged rays are queued and re-traced together, and
+// rays traced beyond the chunk where the row stops are not folded.  In the fold, each chunk's (32, mean,
 // m2) per bin comes from half-wave butterflies, both halves at once: the two
 // lowest bins of each half one at a time, then every lane that hit one of the
 // remaining bins evaluates the same butterfly sums for its own bin from LDS.
 // The merges into the row's summaries follow chunk by chunk.
-template <class Tr>
+template <uint32_t W, class Tr>
 __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
 #pragma clang fp contract(off)
+    static_assert(W >= 1 && W <= kSemWaves, "chunk_lds_bytes sizes the passes of W wavefronts");
     double2 *tpair = (double2 *)(lds + 3 * a.nbin_max);  // [64]; bin ~0u: nothing left to fold
-    double *tv = (double *)(tpair + 64);                 // [kAhead * 64] pass p, lane at p * 64 + lane
-    uint32_t *bv = (uint32_t *)(tv + kAhead * 64);
-    uint32_t *queue = bv + kAhead * 64;
-    const uint32_t lane = threadIdx.x;
+    double *tv = (double *)(tpair + 64);                 // [W * kAhead * 64] pass p, lane at p * 64 + lane
+    uint32_t *bv = (uint32_t *)(tv + W * kAhead * 64);
+    uint32_t *queue = bv + W * kAhead * 64;              // wavefront w's at w * kAhead * 64
+    double *ctl = (double *)(queue + W * kAhead * 64);   // stop flag, chunks folded
+    const uint32_t lane = threadIdx.x & 63u, wave = W > 1 ? threadIdx.x >> 6 : 0u;
     const uint32_t nbin = tr.nbin;
     const double max_err_sq = a.max_err_sq;
     const uint32_t iters = a.max_rays / 32;
     const uint32_t half = lane >> 5, l = lane & 31;
     double count = 0.0;
     bool stop = false;
-    for (uint32_t i0 = 0; i0 < iters && !stop; i0 += 2 * kAhead) {
-        const uint32_t passes = min(kAhead, (iters - i0 + 1) / 2);
-        // trace: ray k = l + 32 (c + 1) of chunk c = i0 + 2 p + half
+    for (uint32_t i0 = 0; i0 < iters && !stop; i0 += 2 * kAhead * W) {
+        // this wavefront's passes: pass p of the group is chunks i0 + 2 p and
+        // i0 + 2 p + 1, and the wavefront traces p = wave kAhead ... + kAhead - 1
+        const uint32_t w0 = i0 + 2 * kAhead * wave;
+        const uint32_t passes = w0 < iters ? min(kAhead, (iters - w0 + 1) / 2) : 0u;
+        uint32_t *wq = queue + wave * kAhead * 64;
+        // trace: ray k = l + 32 (c + 1) of chunk c = w0 + 2 p + half
         uint32_t queued = 0;
         for (uint32_t p = 0; p < passes; ++p) {
-            const uint32_t c = i0 + 2 * p + half;
+            const uint32_t c = w0 + 2 * p + half;
             const bool active = tr.live && c < iters;
             const uint32_t k = l + 32 * (c + 1);
             double t = 0.0;
@@ -323,12 +331,12 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
                 bin = redo ? bin : nbin;
                 t = 0.0;
             }
-            const uint32_t slot = p * 64 + lane;
+            const uint32_t slot = (wave * kAhead + p) * 64 + lane;
             tv[slot] = t;
             bv[slot] = bin;
             if (Tr::kRetrace) {
                 const uint64_t mask = __builtin_amdgcn_ballot_w64(redo);
-                if (redo) queue[queued + __popcll(mask & lanemask_below())] = slot;
+                if (redo) wq[queued + __popcll(mask & lanemask_below())] = slot;
                 queued += __popcll(mask);
             }
         }
@@ -337,7 +345,7 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
             for (uint32_t q0 = 0; q0 < queued; q0 += 64) {
                 __builtin_amdgcn_wave_barrier();
                 if (q0 + lane < queued) {
-                    const uint32_t slot = queue[q0 + lane];
+                    const uint32_t slot = wq[q0 + lane];
                     const uint32_t c = i0 + 2 * (slot >> 6) + ((slot >> 5) & 1);
                     const uint32_t k = (slot & 31) + 32 * (c + 1);
                     double t;
@@ -347,12 +355,16 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        // fold the passes' chunks in order.  A chunk's statistics per bin
-        // depend on its own 32 lanes only, so both halves form theirs at
-        // once; the merges into the row's summaries then run chunk by
-        // chunk, every bin of a chunk at once.
-        for (uint32_t p = 0; p < passes && !stop; ++p) {
+        if (W > 1)
+            __syncthreads();
+        else
+            __builtin_amdgcn_wave_barrier();
+        // fold the passes' chunks in order (the first wavefront).  A chunk's
+        // statistics per bin depend on its own 32 lanes only, so both halves
+        // form theirs at once; the merges into the row's summaries then run
+        // chunk by chunk, every bin of a chunk at once.
+        const uint32_t group_passes = wave == 0 ? min(kAhead * W, (iters - i0 + 1) / 2) : 0u;
+        for (uint32_t p = 0; p < group_passes && !stop; ++p) {
             const uint32_t i = i0 + 2 * p;
             const double t = tv[p * 64 + lane];
             const uint32_t bin = bv[p * 64 + lane];
@@ -408,7 +420,17 @@ __device__ void chunk_row(const RowSlots &a, uint32_t r, Tr &tr, double *lds) {
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
+        if (W > 1) {
+            if (wave == 0 && lane == 0) {
+                ctl[0] = stop ? 1.0 : 0.0;
+                ctl[1] = count;
+            }
+            __syncthreads();
+            stop = ctl[0] != 0.0;
+            count = ctl[1];
+        } else {
+            __builtin_amdgcn_wave_barrier();
+        }
     }
     finish_row(a, r, lds, nbin, count);
 }

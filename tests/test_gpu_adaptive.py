@@ -119,6 +119,25 @@ def test_gpu_fitness_not_converged_is_none_like_the_reference():
 
 
 @pytest.mark.parametrize("trace", [64, 32])
+def test_gpu_fitness_large_batch_matches_single_designs(trace):
+    # a batch of at least 16 rows per CU runs one wavefront per row, a single
+    # design (128 rows) four: the same results bit for bit, and the FP64 rows
+    # the oracle's
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ROW_TRACE, trace)
+    _, D, _, _ = cpd.generate_designs(0, 400, 48)
+    assert 128 * len(D) >= 16 * ctx.info()["compute_units"]
+    seeds = np.array([0.37])
+    got, st = cpd.gpu_fitness_batch(D, seeds, max_n=128, nwarp=2, max_eval=2048, ctx=ctx)
+    for i in range(0, len(D), 5):
+        one, st1 = cpd.gpu_fitness_batch([D[i]], seeds, max_n=128, nwarp=2, max_eval=2048, ctx=ctx)
+        assert st1[0] == st[i] and np.array_equal(one[0], got[i]), (i, one[0], got[i])
+        if trace == 64:
+            ref, converged, _ = O.gpu_fitness(to_oracle(D[i]), seeds, 128, 2, 2048)
+            assert (st[i] == 0) == converged and tuple(got[i]) == ref, (i, got[i], ref)
+
+
+@pytest.mark.parametrize("trace", [64, 32])
 def test_gpu_fitness_batch_matches_single_designs(trace):
     _, D, _, _ = cpd.generate_designs(0, 200, 12)
     D = list(D) + [kat_design()]
