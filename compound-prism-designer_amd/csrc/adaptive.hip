@@ -22,8 +22,8 @@
 // default of the GPU estimator) with the FP32 trace and its FP64 re-trace of
 // flagged rays.  The fold is FP64 in the reference's operation order, so with
 // the FP64 trace a row is the FP64 oracle's row bit for bit.  A block owns
-// one row (four wavefronts tracing for the SEM rows, one for the GPU
-// estimator's); the sequential fold runs in one wavefront on wave-uniform
+// one row (four wavefronts tracing when a launch has fewer than 16 rows per
+// CU, else one); the sequential fold runs in one wavefront on wave-uniform
 // values (each lane computes the same numbers) over summaries kept in LDS.
 // The rows of one design are independent, so the host launches blocks of
 // rows speculatively and scans them in order, discarding rows past the stop.

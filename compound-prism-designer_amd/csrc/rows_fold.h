@@ -4,12 +4,13 @@
 //
 // A block owns one row (a design at one wavelength) and keeps the row's
 // per-bin Welford summaries (count, mean, m2) in LDS.
-//  * sem_row (kSemWaves wavefronts): p_dets_l_wavelength with the early exit
+//  * sem_row (W wavefronts tracing, the first folding): p_dets_l_wavelength with the early exit
 //    of src/fitness.rs:5-34 -- rays k = 1, 2, ... traced 64 per wavefront and
 //    folded one at a time, in order, into the summary of the bin each hit, padded
 //    with zeros (skip); the row stops as soon as the bin just updated has a
 //    standard error of the mean below 5e-3.
-//  * chunk_row (one wavefront): the reference's GPU estimator, src/spectrometer/kernel.rs:89-193
+//  * chunk_row (W wavefronts tracing, the first folding): the reference's GPU
+//    estimator, src/spectrometer/kernel.rs:89-193
 //    -- 32-ray chunks folded in order, each chunk's (32, mean, m2) over its
 //    32 lanes into every bin it hit, lowest bin first, with the SEM test after
 //    each chunk.
