@@ -7,6 +7,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -28,15 +29,22 @@ def build(verbose: bool = False) -> Path:
     objdir = HERE / "_build"
     objdir.mkdir(exist_ok=True)
     inc = ["-I", str(INCLUDE), "-I", str(CSRC)]
-    objs = []
+    cmds, objs = [], []
     for src in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp")):
         obj = objdir / (src.name + ".o")
         flags = HIP_FLAGS if src.suffix == ".hip" else HOST_FLAGS
-        cmd = [hipcc, *flags, *inc, "-c", str(src), "-o", str(obj)]
+        cmds.append([hipcc, *flags, *inc, "-c", str(src), "-o", str(obj)])
+        objs.append(str(obj))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lpthread"]
     if verbose:

@@ -52,6 +52,7 @@ __all__ = [
     "Server",
     "generate_design",
     "generate_designs",
+    "hit_fraction",
     "fitness_batch3",
     "gpu_fitness_batch3",
     "generate_designs3",
@@ -813,8 +814,9 @@ class Server:
 
 
 def generate_design(config: int, index: int):
-    """Synthetic design `index` of benchmark configuration `config` (0, 1, 2 or 4):
-    returns (Params, Design, Sampling, attempts)."""
+    """Synthetic design `index` of workload configuration `config` (0, 1, 2, 4, or 5..7 with the
+    detector and coating settings of the reference's shipped designer configurations; see
+    cpd_generate_design): returns (Params, Design, Sampling, attempts)."""
     p, s, att = Params(), Sampling(), ctypes.c_uint32()
     _check(N.lib.cpd_generate_design(config, index, ctypes.byref(p), ctypes.byref(s), ctypes.byref(att)))
     d = Design()
@@ -830,6 +832,13 @@ def generate_designs(config: int, first: int, n: int, threads: int = 0):
     _check(N.lib.cpd_generate_designs(config, first, n, threads, P, D, S, att))
     return list(P), list(D), list(S), np.array(att[:], np.uint32)
 
+
+def hit_fraction(design: Design, n_lambda: int = 4, n_rays: int = 256) -> float:
+    """Fraction of rays k = 1..n_rays at wavelength samples j = 1..n_lambda that land in a bin
+    (FP64 host probe; a workload statistic, see cpd_hit_fraction)."""
+    out = ctypes.c_double()
+    _check(N.lib.cpd_hit_fraction(ctypes.byref(design), int(n_lambda), int(n_rays), ctypes.byref(out)))
+    return out.value
 
 
 def generate_designs3(first: int, n: int, threads: int = 0):

@@ -251,6 +251,7 @@ def _load() -> ctypes.CDLL:
         "cpd_fit_sellmeier1": ([P(d), P(d), d, d, u32, P(d), P(d)], st),
         "cpd_generate_design": ([u32, u64, P(Params), P(Sampling), P(u32)], st),
         "cpd_generate_designs": ([u32, u64, sz, u32, P(Params), P(Design), P(Sampling), P(u32)], st),
+        "cpd_hit_fraction": ([P(Design), u32, u32, P(d)], st),
         "cpd_ctx_create": ([i32, P(vp)], st),
         "cpd_ctx_destroy": ([vp], None),
         "cpd_ctx_info": ([vp, ctypes.c_char_p, sz, P(i32), P(i32)], st),

@@ -102,6 +102,25 @@ struct SplitMix64 {
 
 const double kPi = 3.14159265358979323846;
 
+// Transmission probe of the generated workloads: the fraction of rays k =
+// 1..n_rays of the beam at wavelength samples j = 1..n_lambda (the R1
+// sequences of fitness.rs:19,94) that land in a bin, traced in FP64.
+constexpr uint32_t kProbeLambdas = 4, kProbeRays = 256;
+constexpr double kMinProbeHits = 0.1;
+
+double probe_hit_fraction(const cpd_design &g, uint32_t n_lambda, uint32_t n_rays) {
+    uint64_t hits = 0;
+    for (uint32_t j = 1; j <= n_lambda; ++j) {
+        const double w = cpd::lambda_of(g, cpd::qmc_u64(j));
+        for (uint32_t k = 1; k <= n_rays; ++k) {
+            double t;
+            const double y = cpd::beam_y64(g, cpd::norminv64(cpd::qmc_u64(k)));
+            hits += cpd::trace64(g, w, y, t) < g.bin_count;
+        }
+    }
+    return (double)hits / ((double)n_lambda * (double)n_rays);
+}
+
 }  // namespace
 
 namespace cpd {
@@ -284,13 +303,14 @@ cpd_status cpd_fit_sellmeier1(const double b[3], const double c[3], double lo, d
 
 // Draw order per design: sampling counts once; then, per attempt, face angles,
 // height, separation lengths, glasses (B1..B3, C1..C3 each), lens curvature,
-// beam mean height, detector angle.  Attempts whose construction or detector
-// positioning fails are redrawn.
+// beam mean height, detector angle, and for config 7 the detector's bin size,
+// intercept and anti-reflection coating.  Attempts whose construction or
+// detector positioning fails are redrawn; for configs 5..7 so are attempts
+// whose transmission probe lands fewer than kMinProbeHits of its rays in a bin.
 cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *params_out, cpd_sampling *sampling_out,
                                uint32_t *attempts_out) {
     if (!params_out || !sampling_out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL output");
-    if (config != 0 && config != 1 && config != 2 && config != 4)
-        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "config must be 0, 1, 2 or 4");
+    if (config == 3 || config > 7) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "config must be 0, 1, 2, 4, 5, 6 or 7");
     SplitMix64 rng{((0xC0FFEE00ull + (config == 1 ? 0 : config)) << 32) ^ (index * 0xD1B54A32D192ED03ull)};
     rng.next();
     uint32_t P = 3, nbin = 32, n_lambda = 64, n_rays = 16384;
@@ -305,6 +325,10 @@ cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *para
         nbin = 256;
         n_lambda = 1024;
         n_rays = 1u << 20;
+    } else if (config == 5) {
+        P = 4;  // spring.toml nglass; design_config.toml max-count
+    } else if (config == 6) {
+        nbin = 64;  // bch.toml
     }
     cpd_params p;
     memset(&p, 0, sizeof p);
@@ -319,10 +343,41 @@ cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *para
     p.beam_width = 0.2;
     p.wavelength_lo = 0.5;
     p.wavelength_hi = 0.82;
+    // the designer's variable ranges (designer.py:122-130, 216-227) in mm
+    double max_height = 0.0;
+    if (config == 5) {
+        // spring.toml:4-19 and design_config.toml:17-41: 32 bins of 0.8 at
+        // slope 1 from intercept 0.1 over 32 mm, anti-reflection coated
+        max_height = 25.0;
+        p.width = 7.0;
+        p.ar_coated = 1;
+        p.bin_size = 0.8;
+        p.linear_slope = 1.0;
+        p.linear_intercept = 0.1;
+        p.det_length = 32.0;
+        p.beam_width = 3.2;
+    } else if (config == 6) {
+        // bch.toml:3-15: 64 bins of 0.42 over a 26.6 mm array (the last bin
+        // overhangs its end), wavelengths 0.48..1.0 um, anti-reflection coated
+        max_height = 20.0;
+        p.width = 6.6;
+        p.ar_coated = 1;
+        p.bin_size = 0.42;
+        p.linear_slope = 0.42;
+        p.det_length = 26.6;
+        p.beam_width = 2.0;
+        p.wavelength_lo = 0.48;
+        p.wavelength_hi = 1.0;
+    }
     for (uint32_t attempt = 1; attempt <= 1000000; ++attempt) {
         for (uint32_t i = 0; i <= P; ++i) p.angles[i] = rng.uniform(-kPi / 2, kPi / 2);
-        p.height = rng.uniform(0.5, 2.0);
-        for (uint32_t i = 0; i < P; ++i) p.lengths[i] = rng.uniform(0.5, 2.0);
+        if (max_height > 0.0) {
+            p.height = rng.uniform(1e-4 * max_height, max_height);
+            for (uint32_t i = 0; i < P; ++i) p.lengths[i] = atanh(rng.uniform()) * max_height / 4.0;
+        } else {
+            p.height = rng.uniform(0.5, 2.0);
+            for (uint32_t i = 0; i < P; ++i) p.lengths[i] = rng.uniform(0.5, 2.0);
+        }
         bool glass_ok = true;
         for (uint32_t i = 0; i < P; ++i) {
             double B[3], C[3];
@@ -331,9 +386,17 @@ cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *para
             if (cpd_fit_sellmeier1(B, C, p.wavelength_lo, p.wavelength_hi, 100, p.glass[i], nullptr) != CPD_OK)
                 glass_ok = false;
         }
-        p.curvature = 1.0 - rng.uniform();
+        p.curvature = max_height > 0.0 ? rng.uniform(1e-5, 1.0) : 1.0 - rng.uniform();
         p.y_mean = rng.uniform() * p.height;
         p.det_angle = rng.uniform(-kPi, kPi);
+        if (config == 7) {
+            // gaps between bins, an intercept of up to three bin pitches (so
+            // positions before it fall in bin 0 by the saturating cast when
+            // their remainder is below the bin size), a coin for the coating
+            p.bin_size = p.linear_slope * rng.uniform(0.5, 1.0);
+            p.linear_intercept = p.linear_slope * rng.uniform(0.0, 3.0);
+            p.ar_coated = rng.uniform() < 0.5 ? 1u : 0u;
+        }
         if (!glass_ok) continue;
         double pos[2];
         uint32_t flipped = 0;
@@ -343,6 +406,7 @@ cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *para
         p.flipped = flipped;
         cpd_design d;
         if (cpd_build_design(&p, &d) != CPD_OK) continue;
+        if (config >= 5 && probe_hit_fraction(d, kProbeLambdas, kProbeRays) < kMinProbeHits) continue;
         *params_out = p;
         sampling_out->n_wavelengths = n_lambda;
         sampling_out->n_rays = n_rays;
@@ -353,6 +417,15 @@ cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *para
         return CPD_OK;
     }
     return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "no valid design after 1e6 draws");
+}
+
+cpd_status cpd_hit_fraction(const cpd_design *design, uint32_t n_lambda, uint32_t n_rays, double *out) {
+    if (!design || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (design->n_glass < 1 || design->n_glass > CPD_MAX_PRISMS || design->bin_count < 1 || n_lambda < 1 ||
+        n_rays < 1)
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "need 1..6 glasses, bins, wavelengths and rays");
+    *out = probe_hit_fraction(*design, n_lambda, n_rays);
+    return CPD_OK;
 }
 
 cpd_status cpd_generate_designs(uint32_t config, uint64_t first_index, size_t n, uint32_t threads,

@@ -747,8 +747,11 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         const float rem = fmaf(-qb, d.slope, v);
         const float edge = fminf(fminf(fabsf(rem), fabsf(d.slope - rem)), fabsf(rem - d.bin_size));
         guard |= alive & (edge * den <= plim);
-        alive &= (qb >= 0.f) & (qb < d.nbin_f) & (rem < d.bin_size);
-        res.bin = alive ? (uint32_t)qb : d.nbin;
+        // the quotient is cast to u32 saturating (bin_index, spectrometer.rs:
+        // 291-303): a position before the intercept has a negative quotient
+        // and counts in bin 0 when its remainder is below the bin size
+        alive &= (qb < d.nbin_f) & (rem < d.bin_size);
+        res.bin = alive ? (uint32_t)fmaxf(qb, 0.f) : d.nbin;
         res.t = alive ? (ts + tp) * 0.5f : 0.f;
         if (COUNT) res.ops += alive ? kOpsHit : 0u;
     }

@@ -193,12 +193,30 @@ cpd_status cpd_fit_sellmeier1(const double b[3], const double c[3], double lo, d
  * counterpart; the reference's optimiser draws designs in designer.py:203-265).
  * config 0/1: 3 prisms, 32 bins, 64 wavelengths x 16384 rays;
  * config 2:   1..6 prisms, 16..128 bins, 32..256 wavelengths x 4096..65536 rays;
- * config 4:   6 prisms, 256 bins, 1024 wavelengths x 2^20 rays.
+ * config 4:   6 prisms, 256 bins, 1024 wavelengths x 2^20 rays;
+ * the detector and coating settings of the reference's shipped designer
+ * configurations, geometry drawn from designer.py:122-130's ranges (mm):
+ * config 5:   spring.toml / design_config.toml: 4 prisms, anti-reflection
+ *             coated, 32 bins of 0.8 at slope 1 from intercept 0.1 over 32,
+ *             beam width 3.2, max height 25;
+ * config 6:   bch.toml: 3 prisms, coated, 64 bins of 0.42 over 26.6 (the last
+ *             bin overhangs the array), wavelengths 0.48..1.0, beam width 2;
+ * config 7:   config 0 with gaps (bin size 0.5..1 of the slope), an intercept
+ *             of 0..3 slopes and a random coating;
+ * configs 5..7 sample 64 wavelengths x 16384 rays and keep only designs that
+ * land at least 10% of a probe's rays in a bin (cpd_hit_fraction over 4
+ * wavelengths x 256 rays).
  * Design `index` of a configuration is reproducible on its own.  Draws that
- * fail construction or positioning are redrawn; *attempts_out (may be NULL)
- * receives the number of draws used. */
+ * fail construction, positioning or the probe are redrawn; *attempts_out (may
+ * be NULL) receives the number of draws used. */
 cpd_status cpd_generate_design(uint32_t config, uint64_t index, cpd_params *params_out,
                                cpd_sampling *sampling_out, uint32_t *attempts_out);
+
+/* Fraction of the rays that land in a bin, over rays k = 1..n_rays of the
+ * beam at wavelength samples j = 1..n_lambda (the sequences fitness() uses,
+ * src/fitness.rs:19,94), traced in FP64 on the host.  A workload statistic
+ * (no reference counterpart). */
+cpd_status cpd_hit_fraction(const cpd_design *design, uint32_t n_lambda, uint32_t n_rays, double *out);
 
 /* Designs first_index .. first_index + n - 1 of a configuration, generated
  * on `threads` host threads (0: all); arrays of n.  attempts_out may be NULL. */

@@ -847,6 +847,9 @@ int ora_size_and_deviation(const cpd_design *d, double *size, double *deviation)
 }
 
 double ora_norminv_f64(double u) { return norminv<double>(u); }
+void ora_norminv_many(size_t n, const double *u, double *out) {
+    for (size_t i = 0; i < n; ++i) out[i] = norminv<double>(u[i]);
+}
 float ora_norminv_f32(float u) { return norminv<float>(u); }
 
 // u_k of the R1 sequence: iterative (the reference) or fixed point (exact).

@@ -138,6 +138,7 @@ def lib() -> ctypes.CDLL:
     L.ora_size_and_deviation.argtypes = [P(Design), P(d), P(d)]
     L.ora_norminv_f64.argtypes = [d]
     L.ora_norminv_f64.restype = d
+    L.ora_norminv_many.argtypes = [sz, P(d), P(d)]
     L.ora_norminv_f32.argtypes = [f32]
     L.ora_norminv_f32.restype = f32
     L.ora_qmc_f64.argtypes = [u64, i32, P(d)]
@@ -213,6 +214,14 @@ def size_and_deviation(design: Design):
 
 def norminv(u: float, precision: int = 64) -> float:
     return lib().ora_norminv_f64(u) if precision == 64 else lib().ora_norminv_f32(u)
+
+
+def norminv_many(u) -> np.ndarray:
+    """norminv (erf.rs:62-73) in FP64 of every element of u."""
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.empty_like(u)
+    lib().ora_norminv_many(u.size, _ptr(u, ctypes.c_double), _ptr(out, ctypes.c_double))
+    return out
 
 
 def qmc(n: int, kind: int = QRNG_ITERATIVE, precision: int = 64) -> np.ndarray:

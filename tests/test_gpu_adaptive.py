@@ -20,6 +20,7 @@ from oracle import oracle as O
 import kat
 
 from test_gpu_fitness import INFO_TOL, kat_design, to_oracle
+from workloads import transmitting
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +38,8 @@ def test_known_design_cpu_fitness_matches_oracle_and_known_answer():
 
 
 def test_adaptive_batch_matches_oracle():
-    _, D, _, _ = cpd.generate_designs(0, 0, 24)
+    # designs that land >= 10% of a probe's rays in a bin
+    _, D, _, _ = transmitting(0, 24)
     samp = N.Sampling(256, 4096, N.MODE_ADAPTIVE, 0)
     got, st = cpd.fitness_batch(D, samp)
     assert (st == 0).all()
@@ -50,13 +52,19 @@ def test_large_adaptive_launch_matches_oracle():
     # a first round of at least 16 rows per CU runs one wavefront per row (the
     # smaller launches above run four): the same rows, bit for bit
     ctx = cpd.Context(0)
-    _, D, _, _ = cpd.generate_designs(0, 300, 1280)
+    _, D, _, _ = transmitting(0, 1280, first=300)
     assert 4 * len(D) >= 16 * ctx.info()["compute_units"]
     got, st = cpd.fitness_batch(D, N.Sampling(64, 512, N.MODE_ADAPTIVE, 0), ctx=ctx)
     assert (st == 0).all()
     ref, _ = O.fitness_many([to_oracle(d) for d in D], 64, 512, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+    print(f"1280 transmitting designs: {int(np.sum(ref[:, 1] == 0))} with info 0, "
+          f"{int(np.sum(ref[:, 1] > 0.1))} above 0.1 bits", flush=True)
     bad = np.flatnonzero((got != ref).any(axis=1))
     assert bad.size == 0, (bad[:5], got[bad[:5]], ref[bad[:5]])
+    # the same designs one at a time run four wavefronts per row
+    for i in range(0, len(D), 160):
+        one, _ = cpd.fitness_batch([D[i]], N.Sampling(64, 512, N.MODE_ADAPTIVE, 0), ctx=ctx)
+        assert np.array_equal(one[0], got[i]), i
 
 
 def test_adaptive_and_fixed_share_a_batch():
@@ -125,11 +133,11 @@ def test_gpu_fitness_large_batch_matches_single_designs(trace):
     # the oracle's
     ctx = cpd.Context(0)
     ctx.set_option(N.OPT_ROW_TRACE, trace)
-    _, D, _, _ = cpd.generate_designs(0, 400, 48)
+    _, D, _, _ = transmitting(0, 48, first=400)
     assert 128 * len(D) >= 16 * ctx.info()["compute_units"]
     seeds = np.array([0.37])
     got, st = cpd.gpu_fitness_batch(D, seeds, max_n=128, nwarp=2, max_eval=2048, ctx=ctx)
-    for i in range(0, len(D), 5):
+    for i in range(len(D)):
         one, st1 = cpd.gpu_fitness_batch([D[i]], seeds, max_n=128, nwarp=2, max_eval=2048, ctx=ctx)
         assert st1[0] == st[i] and np.array_equal(one[0], got[i]), (i, one[0], got[i])
         if trace == 64:
@@ -139,7 +147,7 @@ def test_gpu_fitness_large_batch_matches_single_designs(trace):
 
 @pytest.mark.parametrize("trace", [64, 32])
 def test_gpu_fitness_batch_matches_single_designs(trace):
-    _, D, _, _ = cpd.generate_designs(0, 200, 12)
+    _, D, _, _ = transmitting(0, 12, first=200)
     D = list(D) + [kat_design()]
     seeds = np.array([0.1, 0.7])
     ctx = cpd.Context(0)
@@ -159,7 +167,7 @@ def test_gpu_fitness_batch_matches_single_designs(trace):
 
 
 def test_fp32_rows_fitness_close_to_oracle():
-    _, D, _, _ = cpd.generate_designs(0, 0, 12)
+    _, D, _, _ = transmitting(0, 12)
     ctx = cpd.Context(0)
     ctx.set_option(N.OPT_ROW_TRACE, 32)
     got, st = cpd.fitness_batch(D, N.Sampling(256, 4096, N.MODE_ADAPTIVE, 0), ctx=ctx)
@@ -167,3 +175,55 @@ def test_fp32_rows_fitness_close_to_oracle():
     for i, d in enumerate(D):
         ref, _ = O.fitness(to_oracle(d), 256, 4096, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
         assert abs(got[i][1] - ref[1]) <= INFO_TOL, (i, got[i], ref)
+
+
+def wide_design(params, nbin):
+    """A generated design re-binned: nbin bins of 3.2 / nbin on the same array."""
+    p = N.Params()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(params), ctypes.sizeof(p))
+    p.bin_count = nbin
+    p.bin_size = p.linear_slope = 3.2 / nbin
+    d = N.Design()
+    assert N.lib.cpd_build_design(ctypes.byref(p), ctypes.byref(d)) == 0
+    return d
+
+
+@pytest.mark.parametrize("trace", [64, 32])
+def test_gpu_estimator_mixed_bin_counts(trace):
+    # one launch over designs of 16..128 bins and one of 1024 bins (the largest
+    # LDS footprint of the chunk fold), 32 designs x 128 rows = at least 16 rows
+    # per CU, so one wavefront per row; each design alone runs four.  FP64 rows
+    # are the oracle's bit for bit.
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ROW_TRACE, trace)
+    P, D, _, _ = transmitting(2, 31, first=500)
+    D = list(D) + [wide_design(transmitting(0, 1, first=50)[0][0], 1024)]
+    assert len({d.bin_count for d in D}) >= 4 and 128 * len(D) >= 16 * ctx.info()["compute_units"]
+    seeds = np.array([0.61])
+    got, st = cpd.gpu_fitness_batch(D, seeds, max_n=128, nwarp=2, max_eval=4096, ctx=ctx)
+    for i, d in enumerate(D):
+        one, st1 = cpd.gpu_fitness_batch([d], seeds, max_n=128, nwarp=2, max_eval=4096, ctx=ctx)
+        assert st1[0] == st[i] and np.array_equal(one[0], got[i]), (i, d.bin_count, one[0], got[i])
+        if trace == 64:
+            ref, converged, _ = O.gpu_fitness(to_oracle(d), seeds, 128, 2, 4096)
+            assert (st[i] == 0) == converged and tuple(got[i]) == ref, (i, d.bin_count, got[i], ref)
+
+
+def test_adaptive_fitness_mixed_bin_counts():
+    # fitness() with the SEM exits over a launch that mixes 16..128-bin designs
+    # with 512- and 1024-bin ones: large enough for one wavefront per row in its first
+    # round; every design against the oracle, and a few alone (four wavefronts)
+    ctx = cpd.Context(0)
+    P, D, _, _ = transmitting(0, 1000, first=2000)
+    P2, D2, _, _ = transmitting(2, 64, first=2000)
+    D = list(D) + list(D2) + [wide_design(P[0], 1024), wide_design(P[1], 512)]
+    assert len({d.bin_count for d in D}) >= 6
+    samp = N.Sampling(64, 2048, N.MODE_ADAPTIVE, 0)
+    got, st = cpd.fitness_batch(D, samp, ctx=ctx)
+    assert (st == 0).all()
+    ref, _ = O.fitness_many([to_oracle(d) for d in D], 64, 2048, 64, adaptive=True, qrng=O.QRNG_FIXED_POINT)
+    bad = np.flatnonzero((got != ref).any(axis=1))
+    assert bad.size == 0, (bad[:5], got[bad[:5]], ref[bad[:5]])
+    for i in (0, 1, len(D) - 2, len(D) - 1):
+        one, _ = cpd.fitness_batch([D[i]], samp, ctx=ctx)
+        assert np.array_equal(one[0], got[i]), (i, D[i].bin_count)

@@ -14,6 +14,7 @@ import compound_prism_designer as cpd
 from compound_prism_designer import _native as N
 from oracle import oracle as O
 import kat
+from workloads import transmitting
 
 pytestmark = pytest.mark.gpu
 
@@ -75,15 +76,18 @@ def test_kat_known_answer_on_gpu():
 # LDS columns into the u64 row 16 times
 @pytest.mark.parametrize("config,n,nl,nr", [(0, 24, 16, 4096), (2, 24, 8, 2048), (4, 6, 4, 65536)])
 def test_generated_batch_matches_oracle(config, n, nl, nr):
-    designs = [cpd.generate_design(config, i)[1] for i in range(n)]
+    # designs that land >= 10% of a probe's rays in a bin
+    _, designs, _, hit = transmitting(config, n)
     samp = [N.Sampling(nl, nr, N.MODE_FIXED, 0) for _ in range(n)]
     res, status = cpd.fitness_batch(designs, samp)
     assert (status == 0).all()
     ref, _ = O.fitness_many([to_oracle(d) for d in designs], nl, nr, precision=64, qrng=O.QRNG_FIXED_POINT)
     err = np.abs(res[:, 1] - ref[:, 1])
-    print(f"config {config}: max |dI| {err.max():.3e}, infos {np.round(ref[:, 1], 4)}")
+    print(f"config {config}: max |dI| {err.max():.3e}, hit fractions {np.round(hit, 2)}, "
+          f"infos {np.round(ref[:, 1], 4)}, {int(np.sum(ref[:, 1] == 0))} with info 0")
     for i in range(n):
         check_close(res[i], ref[i], f"design {i}")
+    assert np.sum(ref[:, 1] > 0.05) >= n // 4
 
 
 def test_batch_result_independent_of_companions():
@@ -152,7 +156,7 @@ def test_bin_probabilities_of_generated_designs_match_oracle(config):
     # in FP64, so every ray must land in the oracle's bin: one ray in another bin
     # moves two cells by t / 16384 (about 3e-5 for t = 0.5), far above the 2e-6
     # that FP32 transmittances leave.
-    _, D, _, _ = cpd.generate_designs(config, 0, 12)
+    _, D, _, _ = transmitting(config, 12)
     checked = 0
     dp = ctypes.POINTER(ctypes.c_double)
     for d in D:
@@ -166,7 +170,7 @@ def test_bin_probabilities_of_generated_designs_match_oracle(config):
         assert st == 0, N.last_error()
         np.testing.assert_allclose(out, ref, atol=2e-6, err_msg=f"config {config}")
         checked += 1
-    assert checked >= 2
+    assert checked >= 10
 
 
 @pytest.mark.parametrize("acc", [N.ACC_EXACT, N.ACC_F32, N.ACC_F32C, N.ACC_F64])
@@ -182,3 +186,30 @@ def test_accumulation_policies_match_oracle(acc):
     for i, d in enumerate(D):
         ref, _ = O.fitness(to_oracle(d), 4, 65536, 64, adaptive=False, qrng=O.QRNG_FIXED_POINT)
         check_close(got[i], np.array(ref), f"policy {acc} design {i}")
+
+
+@pytest.fixture(scope="module")
+def config4_rows():
+    # two transmitting designs of configuration 4 (6 prisms, 256 bins) at the
+    # configuration's own row length, 2^20 rays, over 4 wavelengths
+    _, D, _, _ = transmitting(4, 2)
+    ref, _ = O.fitness_many([to_oracle(d) for d in D], 4, 1 << 20, 64, qrng=O.QRNG_FIXED_POINT)
+    return D, ref
+
+
+@pytest.mark.parametrize("acc,tile", [(N.ACC_EXACT, 0), (N.ACC_EXACT, 1 << 20), (N.ACC_F32, 0), (N.ACC_F32C, 0),
+                                      (N.ACC_F64, 0)])
+def test_config4_rows_of_a_million_rays(config4_rows, acc, tile):
+    # tile 2^20 makes a whole wavelength one work item (the largest the trace
+    # takes, folding its u32 columns 265 times); FP32C always takes one item per
+    # wavelength.  Every policy is within INFO_TOL of the FP64 oracle.
+    D, ref = config4_rows
+    ctx = cpd.Context(0)
+    ctx.set_option(N.OPT_ACCUMULATE, acc)
+    if tile:
+        ctx.set_option(N.OPT_TILE_RAYS, tile)
+    got, st = cpd.fitness_batch(D, N.Sampling(4, 1 << 20, N.MODE_FIXED, 0), ctx=ctx)
+    assert (st == 0).all()
+    print(f"policy {acc} tile {tile}: dI {got[:, 1] - ref[:, 1]}, info {ref[:, 1]}", flush=True)
+    for i in range(len(D)):
+        check_close(got[i], ref[i], f"policy {acc} design {i}")
