@@ -29,6 +29,11 @@ ERR_DEVICE = 33
 ERR_OUT_OF_MEMORY = 34
 ERR_CAPACITY = 35
 
+NOALLOC_MAX_DESIGNS = 4096
+NOALLOC_MAX_RAYS = 65536
+NOALLOC_MAX_ROWS = 1 << 18
+NOALLOC_MAX_CELLS = 1 << 21
+
 MODE_FIXED = 0
 MODE_ADAPTIVE = 1
 
@@ -265,6 +270,7 @@ def _load() -> ctypes.CDLL:
         "cpd_server_destroy": ([vp], None),
         "cpd_fitness_gpu": ([vp, P(Design), P(Sampling), P(Fitness)], st),
         "cpd_fitness_batch": ([vp, P(Design), P(Sampling), sz, P(Fitness), P(st)], st),
+        "cpd_fitness_batch_noalloc": ([vp, P(Design), P(Sampling), sz, P(Fitness), P(st)], st),
         "cpd_p_dets_l_wavelength": ([vp, P(Design), P(d), sz, u32, u32, P(d)], st),
         "cpd_propagate_rays": ([vp, P(Design), P(ctypes.c_float), P(ctypes.c_float), sz, P(u32),
                                 P(ctypes.c_float)], st),

@@ -295,6 +295,30 @@ cpd_status cpd_fitness_gpu(cpd_ctx *ctx, const cpd_design *design, const cpd_sam
 cpd_status cpd_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling,
                              size_t n, cpd_fitness *out, cpd_status *per_design);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, code that must not call hipMalloc, harnesses
+ * that forbid it): cpd_fitness_batch for fixed-count sampling, with every
+ * device buffer it uses (beam tables, staged records, per-bin accumulators,
+ * last-ray bins, results, work counter) inside one static device array of
+ * CPD_NOALLOC_SCRATCH_BYTES (22 MiB) that each GPU holds from library load
+ * until the process ends; the call allocates nothing (no hipMalloc, hipFree,
+ * pinned or heap buffers).  Results are cpd_fitness_batch's bit for bit.
+ * Limits: n <= CPD_NOALLOC_MAX_DESIGNS; per design CPD_MODE_FIXED and
+ * n_rays <= CPD_NOALLOC_MAX_RAYS (otherwise that design's status is
+ * CPD_ERR_INVALID_ARGUMENT); over the valid designs sum n_wavelengths <=
+ * CPD_NOALLOC_MAX_ROWS and sum n_wavelengths * bin_count <=
+ * CPD_NOALLOC_MAX_CELLS (otherwise the call fails with
+ * CPD_ERR_INVALID_ARGUMENT).  The first call on a device fills the array's beam
+ * tables (a synchronous upload).  Calls in one process are serialised; each
+ * process has its own copy of the array. */
+#define CPD_NOALLOC_MAX_DESIGNS 4096
+#define CPD_NOALLOC_MAX_RAYS 65536
+#define CPD_NOALLOC_MAX_ROWS (1u << 18)
+#define CPD_NOALLOC_MAX_CELLS (1u << 21)
+#define CPD_NOALLOC_SCRATCH_BYTES 23134720
+cpd_status cpd_fitness_batch_noalloc(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling,
+                                     size_t n, cpd_fitness *out, cpd_status *per_design);
+
 /* Per-bin transmission probabilities p(d|l) for the given wavelengths with
  * max_m rays each; out is row-major [n_wavelengths][bin_count].  With
  * CPD_MODE_ADAPTIVE a row stops at the reference's SEM early exit, as
