@@ -11,14 +11,24 @@ Printed as ONE JSON line:
   value            designs/s of the batched path (steps x 1024 / timed seconds)
   plain            designs/s of the plain one-design-per-launch path (the
                    reference's cuda_fitness structure) and the batched/plain ratio
-  mi_error_bits    max |info_GPU - info_FP64 oracle| over a checked subset
-  roofline         trace kernel: algorithmic operations per launch (stages
-                   reached, planar faces hoisted per wavelength) / mean kernel
-                   time (HIP events on the context's stream) against MI355X's
-                   157.3 TFLOP/s FP32 peak; also the same rays priced with the
-                   reference formulas (85 + 42 P operations per ray)
+  mi_error_bits    max |info_GPU - info_FP64 oracle| over the first designs of
+                   the batch that land >= 10% of a probe's rays in a bin
+  workload_stats   generator draws per design, the batch's hit fractions
+  transmitting     the same measurement on 1024 designs that all land >= 10%
+                   of a probe's rays in a bin (what an optimiser's population
+                   looks like after a few generations)
+  roofline         trace kernel: the algorithmic operations of the launch
+                   counted with the reference formulas (85 + 42 P per ray and
+                   12 P + 3 nbin per wavelength, only per-design and
+                   per-wavelength invariants hoisted) / mean kernel time (HIP
+                   events on the context's stream) against MI355X's
+                   157.3 TFLOP/s FP32 vector peak; also the count of the stages
+                   the rays reach with the planar faces hoisted
   cpu_baseline     the FP64 oracle (C++ restatement of the reference's
-                   fitness.rs) on the host cores, one design per thread
+                   fitness.rs) on the host cores, one design per thread; also
+                   FP32, and both on one core
+  stress           the precision-stress configuration under the accumulation
+                   policies, MI error against the FP64 oracle on a subset
 """
 from __future__ import annotations
 
@@ -37,8 +47,35 @@ METRIC = "designs/sec (batched vs plain GPU) at ≤1e-4 bit MI error; p50/p99 re
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the batched trace kernel on this workload, from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy's readings
-# (profiles/trace_kernel_batch1024_p3.md)
 TRACE_TRAFFIC_BYTES = 15.48e6
+TRACE_TRAFFIC_SOURCE = "profiles/trace_kernel_batch1024_p3.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+
+
+def reference_ops(designs, samplings):
+    """Algorithmic operations of fixed-count evaluations, counted with the
+    reference formulas (add/mul/div/sqrt/log = 1, fma = 2): per ray 85 + 42 P
+    (beam 2, each planar face 42, lens 58, detector 24, hit 1) and per
+    wavelength 12 P + 3 nbin (Horner n(lambda), ratios, entropy terms), with
+    only per-design and per-wavelength invariants hoisted."""
+    return sum(s.n_wavelengths * (s.n_rays * (85 + 42 * d.n_glass) + 12 * d.n_glass + 3 * d.bin_count)
+               for d, s in zip(designs, samplings))
+
+
+def hit_fractions(cpd, designs):
+    """Probe hit fraction of every design (cpd_hit_fraction, 4 wavelengths x 256 rays, host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
+        return list(ex.map(cpd.hit_fraction, designs))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(msg: str) -> None:
@@ -59,8 +96,12 @@ def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed, step_items=
         arrivals.append(t)
     sizes = rng.integers(100, 2001, len(arrivals))
     picks = [rng.integers(0, len(pool_d), n) for n in sizes]
-    reqs = [((N.Design * len(ix))(*[pool_d[i] for i in ix]), (N.Sampling * len(ix))(*[pool_s[i] for i in ix]))
-            for ix in picks]
+    pd = (N.Design * len(pool_d))(*pool_d)
+    ps = (N.Sampling * len(pool_s))(*pool_s)
+    rec_d = np.frombuffer(pd, dtype=np.uint8).reshape(len(pool_d), ctypes.sizeof(N.Design))
+    rec_s = np.frombuffer(ps, dtype=np.uint8).reshape(len(pool_s), ctypes.sizeof(N.Sampling))
+    reqs = [((N.Design * len(ix)).from_buffer_copy(rec_d[ix].tobytes()),
+             (N.Sampling * len(ix)).from_buffer_copy(rec_s[ix].tobytes())) for ix in picks]
     srv = cpd.Server(ctx, step_items=step_items)
     ids = []
     t0 = time.perf_counter()
@@ -402,6 +443,140 @@ def stress_run(cpd, N, np, ctx, args):
             "oracle_seconds": cpu_s, "oracle_threads": min(threads, len(pick)), "variants": variants}
 
 
+def transmitting_run(cpd, N, np, ctx, args):
+    """The headline measurement on designs that transmit: the first designs of the 3-prism
+    distribution (from index 0) whose probe lands >= 10% of the rays in a bin, as an optimiser's
+    population does after a few generations.  Batched designs/s, the trace kernel's time (HIP
+    events) and rate, the plain path, and the MI error against the FP64 oracle."""
+    n = args.transmitting
+    D, H, first = [], [], 0
+    while len(D) < n:
+        _, d, _, _ = cpd.generate_designs(1, first, 2048)
+        h = hit_fractions(cpd, d)
+        for dd, hh in zip(d, h):
+            if hh >= 0.1 and len(D) < n:
+                D.append(dd)
+                H.append(hh)
+        first += 2048
+    samp = N.Sampling(64, 16384, N.MODE_FIXED, 0)
+    darr = (N.Design * n)(*D)
+    sarr = (N.Sampling * n)(*([samp] * n))
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+
+    def batch():
+        if N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status) != 0:
+            raise RuntimeError(N.last_error())
+
+    ctx.set_option(N.OPT_COUNT_OPS, 1)
+    batch()
+    stage_ops = ctx.last_stats()["ops"]
+    ctx.set_option(N.OPT_COUNT_OPS, 0)
+    for _ in range(2):
+        batch()
+    reps = 10
+    t = time.perf_counter()
+    for _ in range(reps):
+        batch()
+    elapsed = time.perf_counter() - t
+    value = n * reps / elapsed
+    ctx.set_option(N.OPT_PROFILE, 1)
+    kms = []
+    for _ in range(5):
+        batch()
+        kms.append(ctx.last_stats()["trace_ms"])
+    ctx.set_option(N.OPT_PROFILE, 0)
+    kernel_ms = float(np.mean(kms))
+    infos = np.array([f.info for f in out])
+    ref_ops = reference_ops(D, [samp] * n)
+    achieved = ref_ops / (kernel_ms * 1e-3) / 1e12
+    f1 = N.Fitness()
+    n_plain = min(64, n)
+    t = time.perf_counter()
+    for i in range(n_plain):
+        if N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(darr[i]), ctypes.byref(samp), ctypes.byref(f1)) != 0:
+            raise RuntimeError(N.last_error())
+    plain = n_plain / (time.perf_counter() - t)
+    from oracle import oracle as O
+    n_check = min(args.check, n)
+    ref, _ = O.fitness_many(D[:n_check], 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT)
+    err = np.abs(ref[:, 1] - infos[:n_check])
+    log(f"transmitting: {value:.0f} designs/s batched, trace kernel {kernel_ms:.3f} ms ({achieved:.1f} TFLOP/s in "
+        f"reference-formula operations), plain {plain:.0f} designs/s; max |dI| {err.max():.2e} over {n_check}")
+    return {"workload": f"{n} designs x (3 prisms, 32 bins, 64 wavelengths x 16384 rays) whose probe hit "
+                        f"fraction is >= 0.1 (the first such of the 3-prism distribution, {first} draws scanned)",
+            "designs_per_s": value, "ms_per_step": elapsed / reps * 1e3, "kernel_ms": kernel_ms,
+            "achieved_tflops": achieved, "frac": achieved / PEAK_FP32_TFLOPS,
+            "stage_reached_tflops": stage_ops / (kernel_ms * 1e-3) / 1e12,
+            "hit_fraction_median": float(np.median(H)),
+            "plain_designs_per_s": plain, "batched_over_plain": value / plain,
+            "mi_error_bits": float(err.max()), "oracle_info_bits": [float(x) for x in ref[:, 1]]}
+
+
+def stress_short_run(cpd, N, np, ctx, args):
+    """A bounded form of the precision stress in the default run: designs of the 6-prism,
+    256-bin distribution at their full sampling (1024 wavelengths x 2^20 rays) timed under each
+    accumulation policy; MI error per policy on the two designs with the most information,
+    evaluated by the GPU and the FP64 oracle at 64 wavelengths x 2^20 rays (2^26 rays per design).
+    The designs are the first of the distribution whose probe hit fraction is >= 0.1."""
+    n = args.stress_short
+    # the first designs whose probe lands >= 10% of the rays in a bin
+    _, D0, S0, _ = cpd.generate_designs(4, 0, 8 * n)
+    h = hit_fractions(cpd, D0)
+    order = [i for i in range(len(D0)) if h[i] >= 0.1] + [i for i in range(len(D0)) if h[i] < 0.1]
+    D = [D0[i] for i in order[:n]]
+    S = [S0[i] for i in order[:n]]
+    darr = (N.Design * n)(*D)
+    sarr = (N.Sampling * n)(*S)
+    out = (N.Fitness * n)()
+    status = (ctypes.c_int * n)()
+    names = {N.ACC_EXACT: "exact fixed-point + FP64 compensated entropy", N.ACC_F32: "FP32 naive",
+             N.ACC_F32C: "FP32 compensated", N.ACC_F64: "FP64"}
+    timing, infos = {}, None
+    # untimed first run: builds the context's beam tables for 2^20 rays
+    if N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status) != 0:
+        raise RuntimeError(N.last_error())
+    ctx.set_option(N.OPT_PROFILE, 1)
+    for acc in names:
+        ctx.set_option(N.OPT_ACCUMULATE, acc)
+        t = time.perf_counter()
+        if N.lib.cpd_fitness_batch(ctx.handle, darr, sarr, n, out, status) != 0:
+            raise RuntimeError(N.last_error())
+        wall = time.perf_counter() - t
+        timing[acc] = (wall, ctx.last_stats()["trace_ms"])
+        if acc == N.ACC_EXACT:
+            infos = np.array([f.info for f in out])
+    pick = [int(i) for i in np.argsort(-infos)[:2]]
+    check_s = N.Sampling(64, 1 << 20, N.MODE_FIXED, 0)
+    sub = (N.Design * 2)(*[D[i] for i in pick])
+    ssub = (N.Sampling * 2)(check_s, check_s)
+    out2 = (N.Fitness * 2)()
+    st2 = (ctypes.c_int * 2)()
+    got = {}
+    for acc in names:
+        ctx.set_option(N.OPT_ACCUMULATE, acc)
+        if N.lib.cpd_fitness_batch(ctx.handle, sub, ssub, 2, out2, st2) != 0:
+            raise RuntimeError(N.last_error())
+        got[acc] = np.array([f.info for f in out2])
+    ctx.set_option(N.OPT_ACCUMULATE, N.ACC_EXACT)
+    ctx.set_option(N.OPT_PROFILE, 0)
+    from oracle import oracle as O
+    t = time.perf_counter()
+    ref, _ = O.fitness_many([D[i] for i in pick], 64, 1 << 20, precision=64, qrng=O.QRNG_FIXED_POINT, threads=2)
+    oracle_s = time.perf_counter() - t
+    variants = []
+    for acc, name in names.items():
+        err = np.abs(got[acc] - ref[:, 1])
+        variants.append({"policy": name, "designs_per_s": n / timing[acc][0], "seconds": timing[acc][0],
+                         "trace_ms": timing[acc][1], "mi_error_bits_max": float(err.max()),
+                         "mi_error_bits_mean": float(err.mean())})
+        log(f"stress: {name}: {n / timing[acc][0]:.1f} designs/s, max |dI| {err.max():.2e} bits")
+    return {"workload": f"{n} designs: 6 prisms, 256 bins, 1024 wavelengths x 2^20 rays, FP32 trace",
+            "check": f"designs {pick} (largest information) at 64 wavelengths x 2^20 rays against the FP64 "
+                     f"oracle ({oracle_s:.1f} s on 2 threads)",
+            "oracle_info_bits": [float(x) for x in ref[:, 1]], "variants": variants}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,10 +587,13 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="designs checked against the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="req/s of the short served phase (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
-    ap.add_argument("--serve-sweep", action="store_true",
-                    help="served runs at 20, 50, 100 and 200 req/s (the served configuration), 2 s of arrivals each")
+    ap.add_argument("--serve-rates", default="20,50,100,200",
+                    help="offered req/s of the served phase, comma-separated (the served configuration; '' skips)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="seconds of Poisson arrivals per rate")
+    ap.add_argument("--transmitting", type=int, default=1024,
+                    help="designs of the transmitting line (probe hit fraction >= 0.1; 0: skip)")
+    ap.add_argument("--stress-short", type=int, default=16,
+                    help="designs of the bounded precision-stress phase of the default run (0: skip)")
     ap.add_argument("--uneven", type=int, default=4096,
                     help="designs of the uneven batch (1-6 prisms, 16-128 bins, 32-256 wavelengths x 4096-65536 "
                          "rays), autotuned and replayed as a captured generation (0: skip)")
@@ -467,10 +645,19 @@ def main() -> int:
         return 0
     n = args.designs
     t0 = time.time()
-    gen = [cpd.generate_design(1, i) for i in range(n)]
-    designs = [g[1] for g in gen]
-    attempts = sum(g[3] for g in gen)
-    log(f"generated {n} designs in {time.time() - t0:.1f}s ({attempts} draws)")
+    _, designs, _, draws = cpd.generate_designs(1, 0, n)
+    attempts = int(draws.sum())
+    hits = np.array(hit_fractions(cpd, designs))
+    log(f"generated {n} designs in {time.time() - t0:.1f}s ({attempts} draws); hit fraction median "
+        f"{np.median(hits):.3f}, {np.mean(hits >= 0.1) * 100:.1f}% of designs >= 0.1")
+    workload_stats = {
+        "generator_draws": attempts, "draws_per_design": attempts / n,
+        "rejected_draws_fraction": 1.0 - n / attempts,
+        "hit_fraction_probe": "cpd_hit_fraction: 4 wavelengths x 256 rays per design, FP64",
+        "hit_fraction_percentiles": {str(q): float(np.percentile(hits, q)) for q in (10, 25, 50, 75, 90, 99)},
+        "designs_hit_fraction_zero": int(np.sum(hits == 0.0)),
+        "designs_hit_fraction_ge_0_1": int(np.sum(hits >= 0.1)),
+    }
     samp = N.Sampling(64, 16384, N.MODE_FIXED, 0)
     darr = (N.Design * n)(*designs)
     sarr = (N.Sampling * n)(*([samp] * n))
@@ -513,12 +700,11 @@ def main() -> int:
         fin_ms.append(s["finalize_ms"])
     ctx.set_option(N.OPT_PROFILE, 0)
     trace_mean = float(np.mean(trace_ms))
-    achieved = ops_per_launch / (trace_mean * 1e-3) / 1e12
-    # the same rays priced with the reference formulas (every stage, 85 + 42 P
-    # per ray, only per-design and per-wavelength invariants hoisted)
-    reference_ops = n * 64 * 16384 * (85 + 42 * 3)
-    log(f"trace kernel {trace_mean:.3f} ms, finalize {np.mean(fin_ms):.3f} ms, {achieved:.2f} TFLOP/s "
-        f"({reference_ops / (trace_mean * 1e-3) / 1e12:.2f} in reference-formula operations)")
+    stage_tflops = ops_per_launch / (trace_mean * 1e-3) / 1e12
+    ref_ops = reference_ops(designs, [samp] * n)
+    achieved = ref_ops / (trace_mean * 1e-3) / 1e12
+    log(f"trace kernel {trace_mean:.3f} ms, finalize {np.mean(fin_ms):.3f} ms, {achieved:.2f} TFLOP/s in "
+        f"reference-formula operations ({stage_tflops:.2f} in operations of the stages reached)")
 
     # plain path: one design per launch, synchronised and copied back
     f1 = N.Fitness()
@@ -538,29 +724,57 @@ def main() -> int:
     plain_equal = bool(np.array_equal(np.array(plain_info), infos[:n_plain]))
     log(f"plain: {plain_value:.0f} designs/s; identical to batched: {plain_equal}")
 
-    # accuracy against the FP64 oracle on a subset
+    # accuracy against the FP64 oracle: the first designs of the batch that
+    # land >= 10% of the probe's rays in a bin
     mi_err = None
+    mi_check = None
     n_check = min(args.check, n)
     if n_check:
         from oracle import oracle as O
-        ref, _ = O.fitness_many(designs[:n_check], 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT)
-        mi_err = float(np.max(np.abs(ref[:, 1] - infos[:n_check])))
-        log(f"max |dI| vs FP64 oracle over {n_check} designs: {mi_err:.3e} bits")
+        pick = [int(i) for i in np.flatnonzero(hits >= 0.1)[:n_check]]
+        ref, _ = O.fitness_many([designs[i] for i in pick], 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT)
+        err = np.abs(ref[:, 1] - infos[pick])
+        mi_err = float(err.max())
+        mi_check = {"designs": pick, "selection": "first designs of the batch with probe hit fraction >= 0.1",
+                    "hit_fractions": [float(hits[i]) for i in pick],
+                    "oracle_info_bits": [float(x) for x in ref[:, 1]],
+                    "designs_with_info_zero": int(np.sum(ref[:, 1] == 0.0)),
+                    "max_abs_dinfo_bits": mi_err, "mean_abs_dinfo_bits": float(err.mean()),
+                    "size_max_rel_err": float(np.max(np.abs(ref[:, 0] - np.array([out[i].size for i in pick]))
+                                                     / ref[:, 0])),
+                    "deviation_max_rel_err": float(np.max(np.abs(ref[:, 2] - np.array([out[i].deviation
+                                                                                        for i in pick]))
+                                                          / np.maximum(ref[:, 2], 1e-300)))}
+        log(f"max |dI| vs FP64 oracle over {len(pick)} transmitting designs: {mi_err:.3e} bits "
+            f"(oracle info {np.round(ref[:, 1], 3)})")
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         threads = len(os.sched_getaffinity(0))
-        done, t = 0, time.perf_counter()
-        while time.perf_counter() - t < args.cpu_seconds:
-            chunk = [designs[(done + i) % n] for i in range(threads)]
-            O.fitness_many(chunk, 64, 16384, precision=64, qrng=O.QRNG_FIXED_POINT, threads=threads)
-            done += len(chunk)
-        cpu_s = time.perf_counter() - t
+
+        def cpu_rate(precision, nthreads, seconds):
+            done, t = 0, time.perf_counter()
+            while time.perf_counter() - t < seconds:
+                chunk = [designs[(done + i) % n] for i in range(nthreads)]
+                O.fitness_many(chunk, 64, 16384, precision=precision, qrng=O.QRNG_FIXED_POINT, threads=nthreads)
+                done += len(chunk)
+            return done, time.perf_counter() - t
+
+        done, cpu_s = cpu_rate(64, threads, args.cpu_seconds)
         cpu = {"value": done / cpu_s, "unit": "designs/s", "cores": threads, "kind": "port",
                "sample": f"{done} design evaluations cycling over the {n} designs, 64 wavelengths x 16384 rays each, FP64, "
-                         f"fixed-count, one design per thread, {cpu_s:.1f} s"}
-        log(f"cpu baseline: {cpu['value']:.2f} designs/s on {threads} threads")
+                         f"fixed-count, one design per thread, {cpu_s:.1f} s",
+               "cpu_model": cpu_model()}
+        log(f"cpu baseline: {cpu['value']:.2f} designs/s on {threads} threads ({cpu['cpu_model']})")
+        short = max(2.0, args.cpu_seconds / 3)
+        variants = {}
+        for name, prec, nt in (("fp32_all_cores", 32, threads), ("fp64_one_core", 64, 1), ("fp32_one_core", 32, 1)):
+            d_, s_ = cpu_rate(prec, nt, short)
+            variants[name] = {"value": d_ / s_, "unit": "designs/s", "cores": nt,
+                              "sample": f"{d_} design evaluations, FP{prec}, {s_:.1f} s"}
+            log(f"cpu baseline {name}: {d_ / s_:.2f} designs/s")
+        cpu["variants"] = variants
 
     uneven = None
     if args.uneven > 0:
@@ -574,13 +788,16 @@ def main() -> int:
     if args.toric > 0:
         toric = toric_run(cpd, N, np, ctx, args)
 
+    transmitting = transmitting_run(cpd, N, np, ctx, args) if args.transmitting > 0 else None
+    stress = stress_short_run(cpd, N, np, ctx, args) if args.stress_short > 0 else None
+
     serve = None
-    if args.serve_rate > 0 or args.serve_sweep:
+    rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
+    if rates:
         t = time.time()
         _, pool_d, pool_s, _ = cpd.generate_designs(2, 0, 2048)
         log(f"serving pool: 2048 uneven designs in {time.time() - t:.1f}s")
-        rates = [20.0, 50.0, 100.0, 200.0] if args.serve_sweep else [args.serve_rate]
-        window = 2.0 if args.serve_sweep else args.serve_seconds
+        window = args.serve_seconds
         serve = []
         for q, r in enumerate(rates):
             res = serve_run(cpd, N, np, ctx, pool_d, pool_s, r, window, 0x5E12 + q)
@@ -615,12 +832,19 @@ def main() -> int:
         "toric3d": toric,
         "served": serve,
         "mi_error_bits": mi_err,
-        "roofline": {"bound": "mfma", "pipe": "fp32 vector ALU (no MFMA on this path)",
+        "mi_check": mi_check,
+        "workload_stats": workload_stats,
+        "transmitting": transmitting,
+        "stress": stress,
+        "roofline": {"bound": "valu", "pipe": "FP32 vector ALU (no MFMA on this path: no dense contraction)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRACE_TRAFFIC_BYTES,
-                     "ops_per_launch": ops_per_launch, "kernel_ms": trace_mean,
-                     "reference_formula_ops_per_launch": reference_ops,
-                     "reference_formula_tflops": reference_ops / (trace_mean * 1e-3) / 1e12},
+                     "traffic_source": TRACE_TRAFFIC_SOURCE,
+                     "ops_per_launch": ref_ops,
+                     "ops_count": "reference formulas: 85 + 42 P per ray, 12 P + 3 nbin per wavelength",
+                     "kernel_ms": trace_mean,
+                     "stage_reached_ops_per_launch": ops_per_launch,
+                     "stage_reached_tflops": stage_tflops},
         "cpu_baseline": cpu,
         "device": info,
     }
