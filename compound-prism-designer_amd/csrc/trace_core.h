@@ -532,6 +532,15 @@ CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // Whether any lane of the wavefront holds v (one lane on the host).
 CPD_HD bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
+#elif defined(CPD_EMULATE_DEVICE_ROUNDING)
+// A host model of the device's approximate reciprocal and square root (each
+// within 1 ulp of the exact value): the correctly rounded result moved by the
+// ulp a harness draws (tests/guard).
+float cpd_emu_rcp(float x);
+float cpd_emu_sqrt(float x);
+inline float fast_rcp(float x) { return cpd_emu_rcp(x); }
+inline float fast_sqrt(float x) { return cpd_emu_sqrt(x); }
+inline bool wave_any(bool v) { return v; }
 #else
 CPD_HD float fast_rcp(float x) { return 1.0f / x; }
 CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
@@ -678,7 +687,9 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     const float disc = fmaf(b, b, -c);
     guard |= alive & (fabsf(disc) <= d.disc_lim);
     alive &= disc > 0.f;
-    const float sd = fast_sqrt(fmaxf(disc, 0.f));
+    // a negative discriminant (a lane already lost) gives NaN, which only
+    // reaches values of lost lanes: every later decision is masked by alive
+    const float sd = fast_sqrt(disc);
     const float big = -(b + copysignf(sd, b));
     const float small = c * fast_rcp(big);
     const bool bpos = !signbit(b);
@@ -703,7 +714,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     const float ci = ok1 ? -sdr : sdr;
     // 1 - r^2 (1 - ci^2) with ci^2 = disc / R^2
     const float cr_sq = fmaf(d.lr2r, disc, d.lomr2);
-    const float cr = fast_sqrt(fmaxf(cr_sq, 0.f));
+    const float cr = fast_sqrt(cr_sq);  // NaN only for lanes lost to reflection
     const float rci = d.lr * ci;
     const float k = rci - cr;
     guard |= alive & (fabsf(cr_sq) <= kGuardCos);
@@ -721,7 +732,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         alive &= cd > d.min_ci;
         // most wavefronts lose every ray here (the acceptance cosine): skip
         // the rest of the stage when no lane has a ray left or near a decision
-        if (!wave_any(alive | guard)) return res;
+        if (!wave_any(alive) && !wave_any(guard)) return res;
         const float qx = ok1 ? q1x : q2x, qy = ok1 ? q1y : q2y;
         const float path = d.path + fabsf(tq);
         const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;

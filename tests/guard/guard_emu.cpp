@@ -1,0 +1,139 @@
+// guard_emu.cpp -- soundness check of the FP32 trace's error margins, on the
+// host (test infrastructure; built by tests/test_guard_soundness.py).
+//
+// The device traces rays in FP32 (trace32 in csrc/trace_core.h) and re-traces
+// in FP64 (trace64_chain, the reference's operation order) every ray whose
+// FP32 outcome lies within an error margin of a hit/miss or bin decision.  The
+// margins are sound when no ray that trace32 does NOT flag lands in a
+// different bin than the FP64 trace.  This harness compiles the same
+// trace32 source for the host, with the device's approximate reciprocal and
+// square root modelled as the correctly rounded value moved by -1, 0 or +1 ulp
+// (drawn per call), and compares every ray of every (design, wavelength) of a
+// batch with the FP64 trace: the same beam quantiles z_k = norminv(u_k), the
+// same per-wavelength constants (make_lambda_consts, make_hot), y computed in
+// FP32 as the kernel does.  It counts unflagged bin mismatches (must be 0),
+// flagged rays, hits, and the largest transmittance error of unflagged hits.
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "cpd.h"
+#include "trace_core.h"
+
+namespace {
+thread_local uint64_t g_state = 0x9E3779B97F4A7C15ull;
+thread_local int g_mode = 0;  // 0: exact; 1: random -1/0/+1 ulp per call
+
+inline int draw() {
+    // SplitMix64 step; three outcomes
+    uint64_t z = (g_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (int)(z % 3) - 1;
+}
+
+inline float perturb(float v) {
+    if (g_mode == 0 || !isfinite(v) || v == 0.f) return v;
+    const int d = draw();
+    if (d > 0) return nextafterf(v, INFINITY);
+    if (d < 0) return nextafterf(v, -INFINITY);
+    return v;
+}
+}  // namespace
+
+namespace cpd {
+float cpd_emu_rcp(float x) { return perturb(1.0f / x); }
+float cpd_emu_sqrt(float x) { return perturb(sqrtf(x)); }
+}  // namespace cpd
+
+extern "C" {
+
+// stats[0] rays, [1] flagged, [2] hits (FP64), [3] unflagged bin mismatches,
+// [4] unflagged hits with |t32 - t64| > 1e-5 t64, [5] flagged rays whose FP32
+// bin differs from the FP64 one (what the margins catch); terr[0] and terr[1]
+// the largest absolute and relative transmittance error of unflagged hits.
+// mode 0: exact rcp/sqrt, 1: +-1 ulp.
+int guard_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t n_rays, int mode, int threads,
+                uint64_t *stats, double *terr) {
+    std::vector<double> z64(n_rays);
+    std::vector<float> z32(n_rays);
+    for (uint32_t k = 0; k < n_rays; ++k) {
+        z64[k] = cpd::norminv64(cpd::qmc_u64((uint64_t)k + 1));
+        z32[k] = (float)z64[k];
+    }
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> tot[6];
+    for (auto &t : tot) t = 0;
+    std::vector<double> worst(threads > 0 ? threads : 1, 0.0), worst_rel(worst);
+    auto work = [&](int tid) {
+        g_mode = mode;
+        g_state = 0x9E3779B97F4A7C15ull * (uint64_t)(tid + 1);
+        uint64_t loc[6] = {0, 0, 0, 0, 0, 0};
+        double w = 0.0, wr = 0.0;
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const cpd_design &g = designs[i];
+            cpd::TraceDesign td;
+            cpd::make_trace_design(g, n_lambda, n_rays, td);
+            for (uint32_t j = 1; j <= n_lambda; ++j) {
+                const double lambda = cpd::lambda_of(g, cpd::qmc_u64(j));
+                cpd::LambdaConsts lc;
+                cpd::Chain64 chain;
+                cpd::make_lambda_consts(g, lambda, lc, &chain);
+                const cpd::HotRecord hot = cpd::make_hot(td, lc);
+                for (uint32_t k = 0; k < n_rays; ++k) {
+                    double t64 = 0.0;
+                    const uint32_t b64 = cpd::trace64(g, lambda, cpd::beam_y64(g, z64[k]), t64);
+                    loc[0]++;
+                    loc[2] += b64 < g.bin_count;
+                    if (!lc.ok) {
+                        // the kernel traces nothing for this wavelength: every ray is lost
+                        if (b64 < g.bin_count) loc[3]++;
+                        continue;
+                    }
+                    const float y = hot.y_mean - hot.beam_w * z32[k];
+                    const cpd::Trace32Result r = cpd::trace32<false>(hot, y);
+                    const uint32_t b32 = r.bin < g.bin_count ? r.bin : g.bin_count;
+                    if (r.guard) {
+                        loc[1]++;
+                        loc[5] += b32 != b64;
+                        continue;
+                    }
+                    if (b32 != b64) {
+                        loc[3]++;
+                        continue;
+                    }
+                    if (b64 < g.bin_count) {
+                        const double e = fabs((double)r.t - t64);
+                        if (e > w) w = e;
+                        if (e / t64 > wr) wr = e / t64;
+                        if (e > 1e-5 * t64) loc[4]++;
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < 6; ++q) tot[q] += loc[q];
+        worst[tid] = w;
+        worst_rel[tid] = wr;
+    };
+    const int T = threads > 0 ? threads : 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    for (int q = 0; q < 6; ++q) stats[q] = tot[q];
+    terr[0] = terr[1] = 0.0;
+    for (size_t t = 0; t < worst.size(); ++t) {
+        terr[0] = worst[t] > terr[0] ? worst[t] : terr[0];
+        terr[1] = worst_rel[t] > terr[1] ? worst_rel[t] : terr[1];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+"""Soundness of the FP32 trace's error margins, checked on the host.
+
+The device traces rays in FP32 and re-traces in FP64 only the rays whose FP32
+outcome lies within an error margin of a decision (compound-prism-designer_amd/
+csrc/trace_core.h).  If a margin were too small, a ray could land in another
+bin than the reference's FP64 trace without being re-traced.  tests/guard/
+guard_emu.cpp compiles the same trace32 source for the host, with the
+device's approximate reciprocal and square root modelled as the correctly
+rounded value moved by a random -1, 0 or +1 ulp per call, and compares every
+ray with the FP64 trace (trace64, the reference's operation order).
+
+Per generator (the benchmark distributions 0, 2 and 4, and the reference's
+shipped detector and coating settings 5, 6 and 7), designs that land >= 10% of
+a probe's rays in a bin, >= 1e7 rays: no unflagged bin mismatch, and the
+transmittance of every unflagged hit within the reference's own per-ray
+tolerance of its GPU trace, 5e-3 relative (src/cuda_fitness.rs:605-607).  The
+FP32 transmittance is least accurate near total internal reflection at the
+lens, where cr = sqrt(1 - r^2 (1 - ci^2)) cancels (up to ~2e-3 relative);
+a hit moves its bin's probability by t / M, so such rays change information
+by far less than the 1e-4 bit tolerance.  Two builds: FP32
+contraction as the device compiles it (fast, honouring the FP64 code's
+contract(off) pragmas) and none at all."""
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from compound_prism_designer import _native as N
+from workloads import transmitting
+
+HERE = Path(__file__).resolve().parent / "guard"
+ROOT = HERE.parent.parent
+CLANG = "/opt/rocm/llvm/bin/clang++"
+T_RTOL = 5e-3
+
+_libs = {}
+
+
+def guard_lib(contract: str):
+    if contract in _libs:
+        return _libs[contract]
+    out = HERE / "_build" / f"guard_emu_{contract}.so"
+    out.parent.mkdir(exist_ok=True)
+    src = HERE / "guard_emu.cpp"
+    hdr = ROOT / "compound-prism-designer_amd" / "csrc" / "trace_core.h"
+    if not out.exists() or out.stat().st_mtime < max(src.stat().st_mtime, hdr.stat().st_mtime):
+        cmd = [CLANG, "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", f"-ffp-contract={contract}",
+               "-DCPD_EMULATE_DEVICE_ROUNDING", "-I", str(ROOT / "include"),
+               "-I", str(ROOT / "compound-prism-designer_amd" / "csrc"), str(src), "-o", str(out) + ".tmp"]
+        subprocess.run(cmd, check=True)
+        os.replace(str(out) + ".tmp", out)
+    L = ctypes.CDLL(str(out))
+    P = ctypes.POINTER
+    L.guard_check.argtypes = [P(N.Design), ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                              ctypes.c_int, P(ctypes.c_uint64), ctypes.c_double * 2]
+    _libs[contract] = L
+    return L
+
+
+def check(designs, n_lambda, n_rays, mode, contract):
+    L = guard_lib(contract)
+    n = len(designs)
+    arr = (N.Design * n)(*designs)
+    stats = (ctypes.c_uint64 * 6)()
+    terr = (ctypes.c_double * 2)()
+    L.guard_check(arr, n, n_lambda, n_rays, mode, min(16, os.cpu_count() or 1), stats, terr)
+    return dict(zip(["rays", "flagged", "hits", "unflagged_mismatch", "t_off", "caught"], stats[:])), tuple(terr)
+
+
+@pytest.mark.parametrize("config", [0, 2, 4, 5, 6, 7])
+def test_no_unflagged_bin_mismatch_with_device_rounding(config):
+    _, D, _, _ = transmitting(config, 40, first=1000)
+    st, (t_abs, t_rel) = check(D, 16, 16384, 1, "fast-honor-pragmas")
+    print(f"config {config}: {st}, flagged {st['flagged'] / st['rays']:.4%}, max |t32 - t64| {t_abs:.2e} "
+          f"({t_rel:.2e} relative)")
+    assert st["rays"] >= 10_000_000 and st["hits"] >= 1_000_000
+    assert st["unflagged_mismatch"] == 0, st
+    assert t_rel <= T_RTOL, t_rel
+
+
+@pytest.mark.parametrize("config", [0, 5, 7])
+def test_no_unflagged_bin_mismatch_without_contraction(config):
+    _, D, _, _ = transmitting(config, 10, first=3000)
+    st, (t_abs, t_rel) = check(D, 16, 16384, 1, "off")
+    assert st["unflagged_mismatch"] == 0 and t_rel <= T_RTOL, (st, t_rel)
+
+
+def test_the_margins_catch_real_fp32_errors():
+    # rays whose FP32 bin differs from the FP64 one exist, and every one of them
+    # is flagged: without the margins the comparison above would fail
+    _, D, _, _ = transmitting(7, 40, first=1000)
+    st, _ = check(D, 16, 16384, 1, "fast-honor-pragmas")
+    print(st)
+    assert st["caught"] > 0 and st["unflagged_mismatch"] == 0
