@@ -14,6 +14,9 @@ namespace cpd {
 // integers and independent of the order in which rays and tiles are added.
 static constexpr double kQuantScale = 4194304.0;
 static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 re-trace
+// Positions of a tile whose bands are tested at once before their rays (one
+// 64-bit skip mask per 4096 positions in LDS).
+static constexpr uint32_t kMaskChunk = 65536;
 
 // One launch's work on the device.  Design records live in a pool
 // (designs/traces, indexed by slot); the launch covers n_entries entries,

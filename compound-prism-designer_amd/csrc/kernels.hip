@@ -160,6 +160,9 @@ struct TileCtx {
     uint32_t full_end;   // positions 1..full_end are banded, the rest in ray order
     uint32_t last_pos;   // position of ray k = n_rays
     const Chain64 *chain;  // the wavelength's planar faces in FP64 (LDS)
+    const LensBand64 *band;  // the wavelength's lens and acceptance constants in FP64 (LDS)
+    uint64_t *masks;         // skip masks of the current chunk's bands (LDS)
+    const LambdaConsts *lc_lds;  // lc, as lane 0 stored it (LDS)
 };
 
 // FP64 re-trace of `count` (<= 64) queued rays starting at ring slot `head`.
@@ -212,71 +215,115 @@ __device__ void fold_columns(void *hist_v, void *row_v, uint32_t nbin, uint32_t 
     __syncthreads();
 }
 
+// Skip masks of the bands of positions [chunk, chunk_end) (at most
+// kMaskChunk): band i (positions chunk + 64 i .. + 63) is skipped when all its
+// positions are banded and band_may_hit proves that the FP64 trace loses every
+// one of its rays.  Word g of `masks` holds bands 64 g .. 64 g + 63, one per
+// lane; every lane of the wavefront evaluates its own band of each word.
+__device__ void band_masks(const TileCtx &c, uint32_t chunk, uint32_t chunk_end, uint64_t *masks) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t words = (chunk_end - chunk + 4095u) >> 12;
+    for (uint32_t w = 0; w < words; ++w) {
+        const uint32_t b0 = chunk + (w << 12) + 64u * lane;  // this lane's band: positions b0 .. b0 + 63
+        bool sk = false;
+        if (b0 < chunk_end && b0 + 63u <= c.full_end) {
+            const double ya = beam_y64(*c.g, c.zs64[b0 - 1]);
+            const double yb = beam_y64(*c.g, c.zs64[b0 + 62]);
+            sk = !band_may_hit(*c.band, ya, yb);
+        }
+        const uint64_t m = __builtin_amdgcn_ballot_w64(sk);
+        if (lane == 0) masks[w] = m;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+}
+
 template <bool COUNT, int A>
 __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     using Cell = typename Acc<A>::Cell;
     const uint32_t lane = threadIdx.x;
     const uint32_t col = lane & (c.cols - 1);
-    // register copies of the per-item records: the loop stores through
-    // last_bin, so loads through c.td would be repeated every iteration
-    HotRecord hot = to_sgprs(make_hot(*c.td, c.lc));
-    // operands paired with another uniform operand, the miss bin, and the
-    // detector stage's constants (which would otherwise be spilled)
-    hot.y_mean = in_vgpr(hot.y_mean);
-    hot.e0x = in_vgpr(hot.e0x);
-    hot.e0y = in_vgpr(hot.e0y);
-    hot.lb0 = in_vgpr(hot.lb0);
-    hot.lc1 = in_vgpr(hot.lc1);
-    hot.lomr2 = in_vgpr(hot.lomr2);
-    hot.nd0 = in_vgpr(hot.nd0);
-    hot.nbin = in_vgpr(hot.nbin);
-    hot.ts = in_vgpr(hot.ts);
-    hot.tp = in_vgpr(hot.tp);
-    hot.mpx = in_vgpr(hot.mpx);
-    hot.mpy = in_vgpr(hot.mpy);
-    hot.ddx = in_vgpr(hot.ddx);
-    hot.ddy = in_vgpr(hot.ddy);
-    hot.half_len = in_vgpr(hot.half_len);
-    hot.slope = in_vgpr(hot.slope);
-    hot.inv_slope = in_vgpr(hot.inv_slope);
-    hot.intercept = in_vgpr(hot.intercept);
-    hot.bin_size = in_vgpr(hot.bin_size);
-    hot.nbin_f = in_vgpr(hot.nbin_f);
     Cell *__restrict__ hist = (Cell *)c.hist;
     const uint32_t k_end = c.k_end, nbin = c.nbin, cols = c.cols, full_end = c.full_end, last_pos = c.last_pos;
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
     const uint32_t fold = Acc<A>::fold_every(cols);
     uint32_t since_fold = 0;
-    for (uint32_t base = c.k_begin; base < k_end; base += 64) {
-        // positions base .. base + 63 are all banded or all in ray order
-        const float *__restrict__ zp = base + 63 <= full_end ? c.zs32 : c.z32;
-        const uint32_t k = base + lane;  // position in the tables
-        const bool active = k < k_end;
-        // a 32-bit byte offset from a scalar base: one load, no 64-bit address
-        // arithmetic per lane (the tables hold at most 2^30 rays)
-        const float z = active ? *(const float *)((const char *)zp + ((k - 1u) << 2)) : 0.f;
-        const float y = hot.y_mean - hot.beam_w * z;
-        const Trace32Result r = trace32<COUNT>(hot, y);
-        if (COUNT && active) ops += r.ops;
-        const bool flagged = active && r.guard;
-        if (active && !r.guard) {
-            if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
-            if (k == last_pos) *c.last_bin = r.bin;
-        }
-        const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
-        if (mask) {
-            if (flagged) c.queue[(tail + __popcll(mask & lanemask_lt())) & (kQueueCap - 1)] = k;
-            tail += __popcll(mask);
-            if (tail - head >= 64) {
-                __builtin_amdgcn_wave_barrier();
-                drain_queue<A>(c, head, 64);
-                head += 64;
+    // The tile runs in chunks of kMaskChunk positions.  Before a chunk's rays,
+    // the wavefront marks the bands of 64 banded positions whose every ray the
+    // FP64 trace provably loses at the lens, its acceptance test or the array
+    // ends (band_masks); the ray loop skips them.  Rays in ray order (past the
+    // last whole block) are never skipped, and op counting traces every band.
+    for (uint32_t chunk = c.k_begin; chunk < k_end; chunk += kMaskChunk) {
+        const uint32_t chunk_end = min(chunk + kMaskChunk, k_end);
+        if (!COUNT) band_masks(c, chunk, chunk_end, c.masks);
+        // register copies of the per-item records (made after the band tests,
+        // whose FP64 registers they would otherwise share): the loop stores
+        // through last_bin, so loads through c.td would be repeated every
+        // iteration
+        HotRecord hot = to_sgprs(make_hot(*c.td, *c.lc_lds));
+        // operands paired with another uniform operand, the miss bin, and the
+        // detector stage's constants (which would otherwise be spilled)
+        hot.y_mean = in_vgpr(hot.y_mean);
+        hot.e0x = in_vgpr(hot.e0x);
+        hot.e0y = in_vgpr(hot.e0y);
+        hot.lb0 = in_vgpr(hot.lb0);
+        hot.lc1 = in_vgpr(hot.lc1);
+        hot.lomr2 = in_vgpr(hot.lomr2);
+        hot.nd0 = in_vgpr(hot.nd0);
+        hot.nbin = in_vgpr(hot.nbin);
+        hot.ts = in_vgpr(hot.ts);
+        hot.tp = in_vgpr(hot.tp);
+        hot.mpx = in_vgpr(hot.mpx);
+        hot.mpy = in_vgpr(hot.mpy);
+        hot.ddx = in_vgpr(hot.ddx);
+        hot.ddy = in_vgpr(hot.ddy);
+        hot.half_len = in_vgpr(hot.half_len);
+        hot.slope = in_vgpr(hot.slope);
+        hot.inv_slope = in_vgpr(hot.inv_slope);
+        hot.intercept = in_vgpr(hot.intercept);
+        hot.bin_size = in_vgpr(hot.bin_size);
+        hot.nbin_f = in_vgpr(hot.nbin_f);
+        uint64_t skip = 0;
+        for (uint32_t base = chunk; base < chunk_end; base += 64) {
+            const uint32_t it = (base - chunk) >> 6;
+            if (!COUNT) {
+                if ((it & 63u) == 0) skip = c.masks[it >> 6];
+                if ((skip >> (it & 63u)) & 1u) {
+                    // every ray of the band misses: only the last ray's bin is recorded
+                    if (last_pos >= base && last_pos < base + 64u && lane == 0) *c.last_bin = nbin;
+                    continue;
+                }
             }
-        }
-        since_fold += 64;
-        if (since_fold >= fold) {
-            fold_columns<A>(c.hist, c.row, nbin, cols);
-            since_fold = 0;
+            // positions base .. base + 63 are all banded or all in ray order
+            const float *__restrict__ zp = base + 63 <= full_end ? c.zs32 : c.z32;
+            const uint32_t k = base + lane;  // position in the tables
+            const bool active = k < k_end;
+            // a 32-bit byte offset from a scalar base: one load, no 64-bit address
+            // arithmetic per lane (the tables hold at most 2^30 rays)
+            const float z = active ? *(const float *)((const char *)zp + ((k - 1u) << 2)) : 0.f;
+            const float y = hot.y_mean - hot.beam_w * z;
+            const Trace32Result r = trace32<COUNT>(hot, y);
+            if (COUNT && active) ops += r.ops;
+            const bool flagged = active && r.guard;
+            if (active && !r.guard) {
+                if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
+                if (k == last_pos) *c.last_bin = r.bin;
+            }
+            const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
+            if (mask) {
+                if (flagged) c.queue[(tail + __popcll(mask & lanemask_lt())) & (kQueueCap - 1)] = k;
+                tail += __popcll(mask);
+                if (tail - head >= 64) {
+                    __builtin_amdgcn_wave_barrier();
+                    drain_queue<A>(c, head, 64);
+                    head += 64;
+                }
+            }
+            since_fold += 64;
+            if (since_fold >= fold) {
+                fold_columns<A>(c.hist, c.row, nbin, cols);
+                since_fold = 0;
+            }
         }
     }
     if (tail != head) {
@@ -349,6 +396,9 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
     uint32_t *queue = lds + hist_end;
     uint32_t *row32 = lds + ((hist_end + kQueueCap + 1) & ~1u);
     Chain64 *chain = (Chain64 *)((char *)lds + trace_chain_offset(a.hist_words, a.row_words, A));
+    LensBand64 *band = (LensBand64 *)(chain + 1);
+    uint64_t *masks = (uint64_t *)(band + 1);
+    LambdaConsts *lc_lds = (LambdaConsts *)(masks + (kMaskChunk >> 12));
     const uint32_t lane = threadIdx.x;
     for (uint32_t i = lane; i < a.hist_words; i += 64) hist[i] = 0;
     for (uint32_t i = lane; i < a.row_words * (Acc<A>::kRowBytes / 4); i += 64) row32[i] = 0;
@@ -383,9 +433,13 @@ __global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
         c.td = td;
         const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
         // every lane computes the same constants; lane 0 stores the chain record
-        make_lambda_consts(*g, lambda, c.lc, lane == 0 ? chain : nullptr);
-        __syncthreads();  // the queue drains of this item read the record
+        make_lambda_consts(*g, lambda, c.lc, lane == 0 ? chain : nullptr, lane == 0 ? band : nullptr);
+        if (lane == 0) *lc_lds = c.lc;
+        __syncthreads();  // the queue drains and the band tests of this item read the records
         c.chain = chain;
+        c.band = band;
+        c.masks = masks;
+        c.lc_lds = lc_lds;
         c.k_begin = tile * tile_rays + 1;
         c.k_end = min(c.k_begin + tile_rays, M + 1);
         c.n_rays = M;
@@ -635,7 +689,8 @@ hipError_t launch_trace_c(hipStream_t s, const BatchArgs &a, uint32_t grid, int 
 }  // namespace
 
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc) {
-    return (size_t)trace_chain_offset(hist_words, row_words, acc) + sizeof(Chain64);
+    return (size_t)trace_chain_offset(hist_words, row_words, acc) + sizeof(Chain64) + sizeof(LensBand64) +
+           (kMaskChunk >> 12) * sizeof(uint64_t) + sizeof(LambdaConsts);
 }
 
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, int acc, hipEvent_t start,

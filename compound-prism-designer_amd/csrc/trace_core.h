@@ -328,6 +328,21 @@ struct Chain64 {
     double ldx, ldy, ts, tp, n_last;
 };
 
+// The lens and detector-acceptance constants of one wavelength in FP64, for
+// the band test below: the same quantities as LambdaConsts (relative to the
+// lens chord midpoint), before rounding.
+struct LensBand64 {
+    double lb0, lb1, lc0, lc1, lc2;  // b(y) = lb0 + lb1 y; c(y) = lc0 + lc1 y + lc2 y^2
+    double e0x, e0y, e1x, e1y;       // last planar hit point minus the chord midpoint: e0 + e1 y
+    double dx, dy;                   // direction into the lens
+    double lr2r, lomr2, r;           // cr^2 = lomr2 + lr2r disc; lens ratio r
+    double radius, inv_r, max_dist_sq;
+    double nd0, nd1, ddn, rddn, min_ci;
+    double cmx, cmy;                 // lens centre minus the chord midpoint
+    double mpx, mpy;                 // chord midpoint minus the array's start
+    double dnx, dny, ddx, ddy, length;  // array normal, direction, length
+};
+
 // Build the FP32 record from the FP64 design (host or device).
 // tile_cap (a multiple of 64, 0 for none) bounds the rays of one work item.
 CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n_rays, TraceDesign &t,
@@ -403,7 +418,8 @@ CPD_HD double refraction_gain(double dx, double dy, double nx, double ny, double
 // order.  Face i's hit point is o_i(y) = A_i + B_i y and its distance along
 // the shared direction d is (p.n - o_i.n) / (d.n), affine in y as well; the
 // conditions d >= 0 and 0 <= hit y <= h become bounds on y.
-CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts &lc, Chain64 *chain = nullptr) {
+CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts &lc, Chain64 *chain = nullptr,
+                               LensBand64 *band = nullptr) {
 #pragma clang fp contract(off)
     const int P = (int)g.n_glass;
     const bool ar = g.ar_coated != 0;
@@ -494,6 +510,156 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.nd1 = (float)(-(bx * dnx + by * dny));
     lc.ddn = (float)(ray.dx * dnx + ray.dy * dny);
     lc.rddn = (float)(r * (ray.dx * dnx + ray.dy * dny));
+    if (band) {
+        band->lb0 = ray.dx * (e0x - cmx) + ray.dy * (e0y - cmy);
+        band->lb1 = ray.dx * bx + ray.dy * by;
+        band->lc0 = e0x * e0x + e0y * e0y - 2.0 * (e0x * cmx + e0y * cmy) + kappa;
+        band->lc1 = 2.0 * (e0x * bx + e0y * by) - 2.0 * (bx * cmx + by * cmy);
+        band->lc2 = bx * bx + by * by;
+        band->e0x = e0x;
+        band->e0y = e0y;
+        band->e1x = bx;
+        band->e1y = by;
+        band->dx = ray.dx;
+        band->dy = ray.dy;
+        band->r = r;
+        band->lr2r = r * r / (g.arc_radius * g.arc_radius);
+        band->lomr2 = 1.0 - r * r;
+        band->radius = g.arc_radius;
+        band->inv_r = 1.0 / g.arc_radius;
+        band->max_dist_sq = g.arc_max_dist_sq;
+        band->nd0 = (cmx - e0x) * dnx + (cmy - e0y) * dny;
+        band->nd1 = -(bx * dnx + by * dny);
+        band->ddn = ray.dx * dnx + ray.dy * dny;
+        band->rddn = r * band->ddn;
+        band->min_ci = g.det_min_ci;
+        band->cmx = cmx;
+        band->cmy = cmy;
+        band->mpx = g.arc_midpt[0] - g.det_position[0];
+        band->mpy = g.arc_midpt[1] - g.det_position[1];
+        band->dnx = dnx;
+        band->dny = dny;
+        band->ddx = g.det_direction[0];
+        band->ddy = g.det_direction[1];
+        band->length = g.det_length;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Band test: can any ray with beam height in [ya, yb] reach the detector?
+// ---------------------------------------------------------------------------
+// Interval arithmetic in FP64 over the lens intersection, the refraction and
+// the detector's acceptance cosine as functions of the beam height (the
+// planar faces are affine in it).  It returns false only when, for every
+// height of the band, every root of the lens the trace could take (the near
+// one, and the far one unless the near one is valid for the whole band)
+// either does not exist, lies behind the ray or outside the arc's bounds,
+// refracts by total internal reflection, leaves at an acceptance cosine at
+// most the detector's minimum, or meets the array's line off its ends.  Each
+// test keeps a margin of 1e-9 of the
+// magnitudes it was formed from (the discriminant's the FP64 trace's own
+// cancellation of (R + |o - m|)^2 terms), far above the rounding of either
+// computation, so the FP64 trace in the reference's operation order misses
+// with every ray of such a band, whatever the FP32 trace would have decided,
+// and the band can be skipped.
+struct Iv {
+    double lo, hi;
+};
+CPD_HD Iv iv_add(Iv a, Iv b) { return Iv{a.lo + b.lo, a.hi + b.hi}; }
+CPD_HD Iv iv_sub(Iv a, Iv b) { return Iv{a.lo - b.hi, a.hi - b.lo}; }
+CPD_HD Iv iv_scale(Iv a, double k) { return k >= 0.0 ? Iv{a.lo * k, a.hi * k} : Iv{a.hi * k, a.lo * k}; }
+CPD_HD Iv iv_affine(double c0, double c1, Iv y) {  // c0 + c1 y
+    const double a = c0 + c1 * y.lo, b = c0 + c1 * y.hi;
+    return Iv{fmin(a, b), fmax(a, b)};
+}
+CPD_HD Iv iv_mul(Iv a, Iv b) {
+    const double p1 = a.lo * b.lo, p2 = a.lo * b.hi, p3 = a.hi * b.lo, p4 = a.hi * b.hi;
+    return Iv{fmin(fmin(p1, p2), fmin(p3, p4)), fmax(fmax(p1, p2), fmax(p3, p4))};
+}
+CPD_HD Iv iv_sqr(Iv a) {
+    if (a.lo >= 0.0) return Iv{a.lo * a.lo, a.hi * a.hi};
+    if (a.hi <= 0.0) return Iv{a.hi * a.hi, a.lo * a.lo};
+    return Iv{0.0, fmax(a.lo * a.lo, a.hi * a.hi)};
+}
+CPD_HD Iv iv_sqrt0(Iv a) {  // sqrt over the non-negative part
+    return Iv{sqrt(fmax(a.lo, 0.0)), sqrt(fmax(a.hi, 0.0))};
+}
+CPD_HD double iv_mag(Iv a) { return fmax(fabs(a.lo), fabs(a.hi)); }
+
+// Whether a ray of the band can pass the acceptance test through lens root
+// `far` (0: -b - sqrt(disc), 1: -b + sqrt(disc)); *valid is set when the root
+// is certainly in front of the ray and inside the arc for every height of the
+// band (so the trace never looks at the other root).
+CPD_HD bool band_root_may_hit(const LensBand64 &L, Iv y, Iv b, Iv sd, Iv disc, bool far, bool *valid) {
+    *valid = false;
+    // root distance d = -b -/+ sqrt(disc)
+    const Iv nb{-b.hi, -b.lo};
+    const Iv d = far ? iv_add(nb, sd) : iv_sub(nb, sd);
+    const double dm = 1e-9 * (iv_mag(d) + iv_mag(b) + iv_mag(sd));
+    if (d.hi < -dm) return false;  // behind the ray
+    // bounds: |e + d dir|^2 <= max_dist_sq, e = e0 + e1 y
+    const Iv qx = iv_add(iv_affine(L.e0x, L.e1x, y), iv_scale(d, L.dx));
+    const Iv qy = iv_add(iv_affine(L.e0y, L.e1y, y), iv_scale(d, L.dy));
+    const Iv qx2 = iv_sqr(qx), qy2 = iv_sqr(qy);
+    const Iv s = iv_sub(iv_add(qx2, qy2), Iv{L.max_dist_sq, L.max_dist_sq});
+    const double sm = 1e-9 * (L.max_dist_sq + qx2.hi + qy2.hi);
+    if (s.lo > sm) return false;  // outside the arc
+    *valid = d.lo > dm && s.hi < -sm;
+    // ci = -/+ sqrt(disc) / R, cr = sqrt(1 - r^2 (1 - ci^2)), k = r ci - cr
+    const Iv cr2 = iv_add(Iv{L.lomr2, L.lomr2}, iv_scale(disc, L.lr2r));
+    if (cr2.hi < -1e-9) return false;  // total internal reflection
+    const Iv cr = iv_sqrt0(cr2);
+    const Iv sdr = iv_scale(sd, L.inv_r);
+    const Iv ci = far ? sdr : Iv{-sdr.hi, -sdr.lo};
+    const Iv k = iv_sub(iv_scale(ci, L.r), cr);
+    // acceptance cosine cd = -(r dir.dn + k n.dn), n.dn = (nd0 + nd1 y - d ddn) / R
+    const Iv ndn = iv_scale(iv_sub(iv_affine(L.nd0, L.nd1, y), iv_scale(d, L.ddn)), L.inv_r);
+    const Iv kn = iv_mul(k, ndn);
+    const Iv cd{-(L.rddn + kn.hi), -(L.rddn + kn.lo)};
+    const double cm = 1e-9 * (1.0 + fabs(L.rddn) + iv_mag(k) * (1.0 + iv_mag(ndn)));
+    if (cd.hi <= L.min_ci - cm) return false;  // below the acceptance cosine
+    // array ends (spectrometer.rs:291-295): rays accepted (cd > min_ci) meet the
+    // array's line at w + v dist, w = q - P0, v = r dir + k n, n = (cm - q) / R,
+    // dist = w.dn / cd; a position off [0, length] misses
+    const double cd_lo = fmax(cd.lo, L.min_ci - cm);
+    if (!(cd_lo > 1e-3)) return true;
+    const Iv nx = iv_scale(iv_sub(Iv{L.cmx, L.cmx}, qx), L.inv_r);
+    const Iv ny = iv_scale(iv_sub(Iv{L.cmy, L.cmy}, qy), L.inv_r);
+    const Iv vx = iv_add(Iv{L.r * L.dx, L.r * L.dx}, iv_mul(k, nx));
+    const Iv vy = iv_add(Iv{L.r * L.dy, L.r * L.dy}, iv_mul(k, ny));
+    const Iv wx = iv_add(Iv{L.mpx, L.mpx}, qx), wy = iv_add(Iv{L.mpy, L.mpy}, qy);
+    const Iv wn = iv_add(iv_scale(wx, L.dnx), iv_scale(wy, L.dny));
+    const double r1 = wn.lo / cd_lo, r2 = wn.lo / cd.hi, r3 = wn.hi / cd_lo, r4 = wn.hi / cd.hi;
+    const Iv dist{fmin(fmin(r1, r2), fmin(r3, r4)), fmax(fmax(r1, r2), fmax(r3, r4))};
+    const Iv px = iv_add(wx, iv_mul(vx, dist)), py = iv_add(wy, iv_mul(vy, dist));
+    const Iv pos = iv_add(iv_scale(px, L.ddx), iv_scale(py, L.ddy));
+    const double pm = 1e-9 * (iv_mag(px) + iv_mag(py) + L.length);
+    return !(pos.hi < -pm || pos.lo > L.length + pm);
+}
+
+CPD_HD bool band_may_hit(const LensBand64 &L, double ya, double yb) {
+    const Iv y{fmin(ya, yb), fmax(ya, yb)};
+    const Iv b = iv_affine(L.lb0, L.lb1, y);
+    // c(y) = lc0 + y (lc1 + lc2 y)
+    const Iv c = iv_add(Iv{L.lc0, L.lc0}, iv_mul(y, iv_affine(L.lc1, L.lc2, y)));
+    Iv disc = iv_sub(iv_sqr(b), c);
+    // The FP64 trace forms disc = (d.(o - c))^2 - |o - c|^2 + R^2 from terms
+    // of size (R + |o - m|)^2, so its own disc is uncertain by a few ulp of
+    // that; every decision downstream (roots, bounds, refraction, acceptance)
+    // sees the widened interval.
+    const Iv ex = iv_affine(L.e0x, L.e1x, y), ey = iv_affine(L.e0y, L.e1y, y);
+    const double em = iv_mag(ex) + iv_mag(ey);
+    const double tol = 1e-14 * (L.radius + em) * (L.radius + em) + 1e-9 * (iv_mag(b) * iv_mag(b) + iv_mag(c));
+    if (disc.hi <= -tol) return false;
+    disc = Iv{disc.lo - tol, disc.hi + tol};
+    const Iv sd = iv_sqrt0(disc);
+    // the trace takes the near root wherever it is valid (nearest valid root
+    // first, geometry.rs:203-209); where it is valid for the whole band the far
+    // root is never looked at
+    bool near_valid, far_valid;
+    if (band_root_may_hit(L, y, b, sd, disc, false, &near_valid)) return true;
+    if (near_valid) return false;
+    return band_root_may_hit(L, y, b, sd, disc, true, &far_valid);
 }
 
 // trace64 of one ray with the planar faces' directions and transmittances

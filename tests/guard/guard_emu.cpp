@@ -137,3 +137,80 @@ int guard_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t
 }
 
 }  // extern "C"
+
+#include <algorithm>
+
+extern "C" {
+
+// The band test (band_may_hit, trace_core.h) over the banded beam order the
+// device traces in (within each block of 4096 rays sorted by height, 64 rays
+// to a band): stats[0] bands, [1] bands a wavefront would take past the
+// planar-chain test, [2] of those the band test skips, [3] skipped bands
+// holding an FP64 hit (must be 0), [4] bands holding an FP64 hit.
+int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t n_rays, int threads,
+               uint64_t *stats) {
+    const uint32_t full = n_rays / cpd::kBandRays * cpd::kBandRays;
+    std::vector<double> z64(full);
+    std::vector<uint32_t> order(full);
+    for (uint32_t k = 0; k < full; ++k) {
+        z64[k] = cpd::norminv64(cpd::qmc_u64((uint64_t)k + 1));
+        order[k] = k;
+    }
+    for (uint32_t b = 0; b < full; b += cpd::kBandRays)
+        std::sort(order.begin() + b, order.begin() + b + cpd::kBandRays, [&](uint32_t x, uint32_t y) {
+            return z64[x] < z64[y] || (z64[x] == z64[y] && x < y);
+        });
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> tot[5];
+    for (auto &t : tot) t = 0;
+    auto work = [&]() {
+        uint64_t loc[5] = {0, 0, 0, 0, 0};
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const cpd_design &g = designs[i];
+            cpd::TraceDesign td;
+            cpd::make_trace_design(g, n_lambda, n_rays, td);
+            for (uint32_t j = 1; j <= n_lambda; ++j) {
+                const double lambda = cpd::lambda_of(g, cpd::qmc_u64(j));
+                cpd::LambdaConsts lc;
+                cpd::Chain64 chain;
+                cpd::LensBand64 band;
+                cpd::make_lambda_consts(g, lambda, lc, &chain, &band);
+                const cpd::HotRecord hot = cpd::make_hot(td, lc);
+                for (uint32_t p0 = 0; p0 < full; p0 += 64) {
+                    loc[0]++;
+                    bool chain_pass = false, hit = false;
+                    for (uint32_t q = 0; q < 64; ++q) {
+                        const uint32_t k = order[p0 + q];
+                        const float y = hot.y_mean - hot.beam_w * (float)z64[k];
+                        const bool alive = (y >= hot.ylo) & (y <= hot.yhi);
+                        const bool guard = fminf(fabsf(y - hot.ylo), fabsf(y - hot.yhi)) <= hot.ymarg;
+                        chain_pass |= alive | guard;
+                        double t;
+                        hit |= lc.ok && cpd::trace64(g, lambda, cpd::beam_y64(g, z64[k]), t) < g.bin_count;
+                    }
+                    loc[4] += hit;
+                    if (!chain_pass || !lc.ok) continue;
+                    loc[1]++;
+                    const double ya = cpd::beam_y64(g, z64[order[p0]]);
+                    const double yb = cpd::beam_y64(g, z64[order[p0 + 63]]);
+                    if (!cpd::band_may_hit(band, ya, yb)) {
+                        loc[2]++;
+                        loc[3] += hit;
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < 5; ++q) tot[q] += loc[q];
+    };
+    const int T = threads > 0 ? threads : 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    for (int q = 0; q < 5; ++q) stats[q] = tot[q];
+    return 0;
+}
+
+}  // extern "C"

@@ -95,3 +95,28 @@ def test_the_margins_catch_real_fp32_errors():
     st, _ = check(D, 16, 16384, 1, "fast-honor-pragmas")
     print(st)
     assert st["caught"] > 0 and st["unflagged_mismatch"] == 0
+
+
+def band_stats(designs, n_lambda, n_rays):
+    L = guard_lib("fast-honor-pragmas")
+    L.band_check.argtypes = [ctypes.POINTER(N.Design), ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                             ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    n = len(designs)
+    arr = (N.Design * n)(*designs)
+    st = (ctypes.c_uint64 * 5)()
+    L.band_check(arr, n, n_lambda, n_rays, min(16, os.cpu_count() or 1), st)
+    return dict(zip(["bands", "past_chain", "skipped", "skipped_with_hit", "with_hit"], st[:]))
+
+
+@pytest.mark.parametrize("config,transmit", [(1, False), (2, False), (4, False), (0, True), (5, True), (6, True),
+                                             (7, True)])
+def test_skipped_bands_hold_no_hit(config, transmit):
+    # the trace kernel skips a band of 64 banded rays when band_may_hit proves
+    # (interval arithmetic on the lens and the acceptance cosine) that the FP64
+    # trace loses every ray of it; no skipped band may hold an FP64 hit
+    import compound_prism_designer as cpd
+    D = transmitting(config, 48)[1] if transmit else cpd.generate_designs(config, 0, 96)[1]
+    st = band_stats(D, 16, 16384)
+    print(f"config {config}: {st}, skipped {st['skipped'] / st['past_chain']:.3f} of the bands past the chain test")
+    assert st["skipped_with_hit"] == 0, st
+    assert st["skipped"] > 0
