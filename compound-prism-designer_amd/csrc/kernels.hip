@@ -115,16 +115,19 @@ __device__ __forceinline__ float plog2pf(float p) { return p > 0.f ? p * log2f(p
 //    FP32 compensated entropies.
 //  * f64: FP64 LDS columns and FP64 global atomics, FP64 compensated entropies.
 template <int A> struct Acc;
+// kTScale: the scale trace32's transmittance arrives in (make_hot's tscale).
 template <> struct Acc<CPD_ACC_EXACT> {
     using Cell = uint32_t;
     static constexpr uint32_t kRowBytes = 8;
-    static __device__ Cell of32(float t) { return quant32(t); }
+    static constexpr float kTScale = 0.5f * (float)kQuantScale;  // t arrives as t 2^22: round it
+    static __device__ Cell of32(float t) { return __float2uint_rn(t); }
     static __device__ Cell of64(double t) { return quant64(t); }
     static __device__ uint32_t fold_every(uint32_t cols) { return fold_rays(cols); }
 };
 template <> struct Acc<CPD_ACC_F32> {
     using Cell = float;
     static constexpr uint32_t kRowBytes = 4;
+    static constexpr float kTScale = 0.5f;
     static __device__ Cell of32(float t) { return t; }
     static __device__ Cell of64(double t) { return (float)t; }
     static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
@@ -132,6 +135,7 @@ template <> struct Acc<CPD_ACC_F32> {
 template <> struct Acc<CPD_ACC_F32C> {
     using Cell = float;
     static constexpr uint32_t kRowBytes = 8;
+    static constexpr float kTScale = 0.5f;
     static __device__ Cell of32(float t) { return t; }
     static __device__ Cell of64(double t) { return (float)t; }
     static __device__ uint32_t fold_every(uint32_t) { return 1024; }
@@ -139,6 +143,7 @@ template <> struct Acc<CPD_ACC_F32C> {
 template <> struct Acc<CPD_ACC_F64> {
     using Cell = double;
     static constexpr uint32_t kRowBytes = 8;
+    static constexpr float kTScale = 0.5f;
     static __device__ Cell of32(float t) { return (double)t; }
     static __device__ Cell of64(double t) { return t; }
     static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
@@ -260,7 +265,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         // whose FP64 registers they would otherwise share): the loop stores
         // through last_bin, so loads through c.td would be repeated every
         // iteration
-        HotRecord hot = to_sgprs(make_hot(*c.td, *c.lc_lds));
+        HotRecord hot = to_sgprs(make_hot(*c.td, *c.lc_lds, Acc<A>::kTScale));
         // operands paired with another uniform operand, the miss bin, and the
         // detector stage's constants (which would otherwise be spilled)
         hot.y_mean = in_vgpr(hot.y_mean);
@@ -388,7 +393,7 @@ __device__ void flush_row(void *hist_v, void *row_v, uint32_t nbin, uint32_t col
 }
 
 template <bool COUNT, int A>
-__global__ __launch_bounds__(64) void trace_kernel(BatchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel(BatchArgs a) {
     using Cell = typename Acc<A>::Cell;
     extern __shared__ __align__(8) uint32_t lds[];
     Cell *hist = (Cell *)lds;

@@ -312,7 +312,7 @@ struct LambdaConsts {
     float path;                // length scale of the FP32 error budget up to the lens
     float lr, lomr2;           // lens ratio r = n_{P-1}, 1 - r^2
     float lr2r;                // r^2 / R^2 (R the lens radius)
-    float nd0, nd1, ddn, rddn; // detector normal dn: (cm - e).dn = nd0 + nd1 y, dir.dn, r dir.dn
+    float nd0, nd1, ddn, rddn; // detector normal dn: (cm - e).dn / R = nd0 + nd1 y, dir.dn / R, r dir.dn
     int ok;                    // 0 when every ray is lost in the planar faces (TIR, parallel face)
     double lambda;
 };
@@ -506,9 +506,9 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.lr2r = (float)(r * r / (g.arc_radius * g.arc_radius));
     lc.lomr2 = (float)(1.0 - r * r);
     const double dnx = g.det_normal[0], dny = g.det_normal[1];
-    lc.nd0 = (float)((cmx - e0x) * dnx + (cmy - e0y) * dny);
-    lc.nd1 = (float)(-(bx * dnx + by * dny));
-    lc.ddn = (float)(ray.dx * dnx + ray.dy * dny);
+    lc.nd0 = (float)(((cmx - e0x) * dnx + (cmy - e0y) * dny) / g.arc_radius);
+    lc.nd1 = (float)(-(bx * dnx + by * dny) / g.arc_radius);
+    lc.ddn = (float)((ray.dx * dnx + ray.dy * dny) / g.arc_radius);
     lc.rddn = (float)(r * (ray.dx * dnx + ray.dy * dny));
     if (band) {
         band->lb0 = ray.dx * (e0x - cmx) + ray.dy * (e0y - cmy);
@@ -694,6 +694,8 @@ CPD_HD uint32_t trace64_chain(const cpd_design &g, const Chain64 &ch, double y, 
 // FP32 trace with error margins
 // ---------------------------------------------------------------------------
 #if defined(__HIP_DEVICE_COMPILE__)
+// max of two non-NaN values in one instruction (median of a, b and +inf)
+CPD_HD float max2f(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, __builtin_inff()); }
 CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // Whether any lane of the wavefront holds v (one lane on the host).
@@ -704,10 +706,12 @@ CPD_HD bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
 // ulp a harness draws (tests/guard).
 float cpd_emu_rcp(float x);
 float cpd_emu_sqrt(float x);
+inline float max2f(float a, float b) { return fmaxf(a, b); }
 inline float fast_rcp(float x) { return cpd_emu_rcp(x); }
 inline float fast_sqrt(float x) { return cpd_emu_sqrt(x); }
 inline bool wave_any(bool v) { return v; }
 #else
+CPD_HD float max2f(float a, float b) { return fmaxf(a, b); }
 CPD_HD float fast_rcp(float x) { return 1.0f / x; }
 CPD_HD float fast_sqrt(float x) { return sqrtf(x); }
 CPD_HD bool wave_any(bool v) { return v; }
@@ -753,13 +757,17 @@ struct alignas(4) HotRecord {
     float cd_lim;    // margin of the acceptance cosine, times cos(theta_t), for every ray
     float e0x, e0y, e1x, e1y, lb0, lb1, lc0, lc1, lc2;
     float dx, dy, rdx, rdy, ts, tp, amp, path;
+    float num0k;     // kGuardRel amp
     float cmx, cmy, inv_r, max_dist_sq, bound_scale;
     float lr2r, lomr2, lr, nd0, nd1, ddn, rddn;
     float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
     float slope, inv_slope, intercept, bin_size, nbin_f;
 };
 
-CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
+// tscale multiplies the planar faces' transmittances, so that trace32's t =
+// tscale (Ts + Tp) (0.5: the mean the reference takes; a power of two, so
+// any other scale gives the same bits scaled).
+CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc, float tscale = 0.5f) {
     HotRecord h;
     h.nbin = d.nbin;
     h.ar_coated = d.ar_coated;
@@ -794,8 +802,9 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc) {
     h.dy = lc.dy;
     h.rdx = lc.rdx;
     h.rdy = lc.rdy;
-    h.ts = lc.ts;
-    h.tp = lc.tp;
+    h.ts = lc.ts * tscale;
+    h.tp = lc.tp * tscale;
+    h.num0k = kGuardRel * lc.amp;
     h.amp = lc.amp;
     h.path = lc.path;
     h.cmx = d.cmx;
@@ -891,8 +900,9 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     if (COUNT) res.ops += alive ? kOpsDetector : 0u;
     {
         // acceptance cosine -v.dn of v = r dir + k n: -(r dir.dn + k n.dn),
-        // n.dn = ((cm - e).dn - tq dir.dn) / R and (cm - e).dn = nd0 + nd1 y
-        const float ndn = fmaf(-tq, d.ddn, fmaf(d.nd1, y, d.nd0)) * d.inv_r;
+        // n.dn = ((cm - e).dn - tq dir.dn) / R = nd0 + nd1 y - tq ddn (the
+        // constants carry the 1 / R)
+        const float ndn = fmaf(-tq, d.ddn, fmaf(d.nd1, y, d.nd0));
         const float cd = -fmaf(k, ndn, d.rddn);
         guard |= alive & (fabsf(cd - d.min_ci) * den <= d.cd_lim);
         alive &= cd > d.min_ci;
@@ -914,8 +924,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         const float wx = d.mpx + qx, wy = d.mpy + qy;
         const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(cd);
         const float pos = fmaf(fmaf(vx, dist, wx), d.ddx, fmaf(vy, dist, wy) * d.ddy);
-        const float num = num0 * fmaxf(rci, cr);
-        const float plim = kGuardRel * (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * num;
+        const float plim = (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * (d.num0k * max2f(rci, cr));
         const float apos = pos - d.half_len;
         guard |= alive & (fabsf(fabsf(apos) - d.half_len) * den <= plim);
         alive &= fabsf(apos) <= d.half_len;
@@ -929,7 +938,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         // and counts in bin 0 when its remainder is below the bin size
         alive &= (qb < d.nbin_f) & (rem < d.bin_size);
         res.bin = alive ? (uint32_t)fmaxf(qb, 0.f) : d.nbin;
-        res.t = alive ? (ts + tp) * 0.5f : 0.f;
+        res.t = alive ? ts + tp : 0.f;  // ts, tp carry make_hot's tscale
         if (COUNT) res.ops += alive ? kOpsHit : 0u;
     }
     res.guard = guard;
