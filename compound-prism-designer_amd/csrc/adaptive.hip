@@ -146,6 +146,83 @@ __global__ __launch_bounds__(64 * W) void chunk_rows_kernel(RowArgs a) {
 
 }  // namespace
 
+// The GPU estimator's outer fold (cuda_fitness.rs:366-380) on the device:
+// one wavefront per design of the launch, whose rows are contiguous and in id
+// order.  Each row updates every bin's Welford summary of p(d) (lanes own
+// bins; the host's operations in the host's order, so bit for bit the host's
+// values) and the summary of the row's entropy sum H = sum_d plog2p(p(d|l))
+// (each row's sum in bin order, the summary in row order).  The design is done when every bin's mean has an SEM
+// below 5e-3; info = mean H - sum_d plog2p(mean p(d)) is refreshed after
+// every launch.  log2 is the device's, so info agrees with a host fold to
+// ~1e-15 while the summaries and the stop decisions are the host's exactly.
+struct OuterArgs {
+    const double *rows;          // p(d|l) of the launch's rows at row_out offsets
+    const uint64_t *row_out;
+    const uint64_t *row_rays;
+    const uint32_t *row_design;  // design index q of each row
+    const cpd_design *designs;   // indexed by q
+    W64 *pw;                     // per-bin summaries, design q's at pw_off[q]
+    const uint64_t *pw_off;
+    W64 *hw;                     // [q] summary of the rows' entropy sums
+    uint32_t *done;              // [q]
+    double *info;                // [q]
+    unsigned long long *rays;    // rays folded, all designs
+    uint32_t rows_per;
+    double e2;
+};
+
+__device__ __forceinline__ double dev_plog2p(double p) { return p > 0.0 ? p * log2(p) : 0.0; }
+
+__global__ __launch_bounds__(64) void outer_fold_kernel(OuterArgs a) {
+    __shared__ double hrow[64];
+    extern __shared__ double terms[];  // [nbin]
+    const uint32_t lane = threadIdx.x;
+    const uint32_t r0 = blockIdx.x * a.rows_per, r_end = r0 + a.rows_per;
+    const uint32_t q = a.row_design[r0];
+    const uint32_t nbin = a.designs[q].bin_count;
+    W64 *pw = a.pw + a.pw_off[q];
+    W64 h = a.hw[q];
+    unsigned long long rays = 0;
+    // 64 rows at a time: each lane sums one row's entropy terms in bin order,
+    // then every lane folds its bins through the rows in order, and lane 0
+    // folds the rows' sums in order
+    for (uint32_t c = r0; c < r_end; c += 64) {
+        const uint32_t m = min(64u, r_end - c);
+        if (lane < m) {
+            const double *row = a.rows + a.row_out[c + lane];
+            double hr = 0.0;
+            for (uint32_t b = 0; b < nbin; ++b) hr += dev_plog2p(row[b]);
+            hrow[lane] = hr;
+            rays += a.row_rays[c + lane];
+        }
+        for (uint32_t b = lane; b < nbin; b += 64) {
+            W64 w = pw[b];
+            for (uint32_t i = 0; i < m; ++i) w_next(w, a.rows[a.row_out[c + i] + b]);
+            pw[b] = w;
+        }
+        __syncthreads();
+        if (lane == 0)
+            for (uint32_t i = 0; i < m; ++i) w_next(h, hrow[i]);
+        __syncthreads();
+    }
+    for (int off = 32; off > 0; off >>= 1) rays += __shfl_xor(rays, off);
+    bool ok = true;
+    for (uint32_t b = lane; b < nbin; b += 64) {
+        ok = ok && w_sem_le(pw[b], a.e2);
+        terms[b] = dev_plog2p(pw[b].mean);
+    }
+    const bool conv = __builtin_amdgcn_ballot_w64(!ok) == 0;
+    __syncthreads();
+    if (lane == 0) {
+        double h_det = 0.0;
+        for (uint32_t b = 0; b < nbin; ++b) h_det = h_det + terms[b];
+        a.hw[q] = h;
+        a.done[q] = conv ? 1u : 0u;
+        a.info[q] = h.mean - h_det;
+        atomicAdd(a.rays, rays);
+    }
+}
+
 // Reciprocals RN(1/k), k = 0..n - 1 (k = 0 holds 0), computed by IEEE
 // division on the host; grow-only.
 cpd_status Context::ensure_recip(uint32_t n) {
@@ -230,7 +307,8 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
                              const std::vector<uint32_t> &row_design, const std::vector<double> &row_lambda,
                              const std::vector<uint64_t> &row_out, const std::vector<uint32_t> &row_cap,
                              uint64_t out_n, uint32_t max_rays, uint32_t nbin_max, const double *z64_dev,
-                             const float *z32_dev, const double *&rows_host, const uint64_t *&rays_host) {
+                             const float *z32_dev, const double *&rows_host, const uint64_t *&rays_host,
+                             RowSlots *dev_slots) {
     const uint32_t n_rows = (uint32_t)row_design.size();
     if (n_rows == 0) return CPD_OK;
     RowArgs a{};
@@ -253,6 +331,13 @@ cpd_status Context::run_rows(bool chunked, bool fp32, const cpd_design *designs_
         kernel = waves == 1 ? (fp32 ? sem_rows_kernel<true, 1> : sem_rows_kernel<false, 1>)
                             : (fp32 ? sem_rows_kernel<true, kSemWaves> : sem_rows_kernel<false, kSemWaves>);
     hipLaunchKernelGGL(kernel, dim3(n_rows), dim3(64 * waves), lds, stream, a);
+    if (dev_slots) {
+        // the rows stay on the device for a fold there
+        AD_TRY(hipGetLastError());
+        if (profile) AD_TRY(hipEventRecord(ev[1], stream));
+        *dev_slots = a.slots;
+        return CPD_OK;
+    }
     return finish_rows(a.slots, out_n, rows_host, rays_host);
 }
 
@@ -395,15 +480,41 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
     uint32_t nbin_max;
     if ((st = stage_designs(designs, valid, &ddev, &nbin_max)) != CPD_OK) return st;
     row_kernel_ms = 0.0;
-    double fold_ms = 0.0;
-    std::vector<OuterFold> fold(n);
-    for (size_t q = 0; q < n; ++q) fold[q].reset(designs[valid[q]].bin_count);
+    // the outer fold's state on the device: per-bin summaries, the entropy
+    // summary, done flags, information, the ray count and the offsets
+    std::vector<uint64_t> pw_off(n);
+    uint64_t n_pw = 0;
+    for (size_t q = 0; q < n; ++q) {
+        pw_off[q] = n_pw;
+        n_pw += designs[valid[q]].bin_count;
+    }
+    const size_t o_hw = n_pw * sizeof(W64), o_done = o_hw + n * sizeof(W64);
+    const size_t o_info = (o_done + n * sizeof(uint32_t) + 7) / 8 * 8, o_rays = o_info + n * sizeof(double);
+    const size_t o_off = o_rays + 8, fold_bytes = o_off + n * sizeof(uint64_t);
+    if ((st = fold_buf.reserve(fold_bytes, false, stream)) != CPD_OK) return st;
+    if ((st = host_out.reserve(std::max(n * sizeof(uint64_t), o_off - o_done))) != CPD_OK) return st;
+    char *fb = fold_buf.as<char>();
+    // zero summaries, flags and count (a fill kernel, not a memset)
+    AD_TRY(launch_zero_u32(stream, (unsigned int *)fb, (uint32_t)(o_off / 4)));
+    memcpy(host_out.ptr, pw_off.data(), n * sizeof(uint64_t));
+    AD_TRY(hipMemcpyAsync(fb + o_off, host_out.ptr, n * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    OuterArgs oa{};
+    oa.designs = ddev;
+    oa.pw = (W64 *)fb;
+    oa.pw_off = (const uint64_t *)(fb + o_off);
+    oa.hw = (W64 *)(fb + o_hw);
+    oa.done = (uint32_t *)(fb + o_done);
+    oa.info = (double *)(fb + o_info);
+    oa.rays = (unsigned long long *)(fb + o_rays);
+    oa.rows_per = n_rows_per;
+    oa.e2 = kMaxErr * kMaxErr;
+    std::vector<uint32_t> done(n, 0u);
+    bool launched = false;
     std::vector<uint32_t> row_design;
     std::vector<double> row_lambda;
     std::vector<uint64_t> row_out;
     const double *rows = nullptr;
     const uint64_t *rays = nullptr;
-    uint64_t total_rays = 0;
     for (size_t si = 0; si < n_seeds; ++si) {
         const uint64_t seed_fx = seed_to_fx(seeds[si]);
         row_design.clear();
@@ -411,7 +522,7 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
         row_out.clear();
         uint64_t out_n = 0;
         for (size_t q = 0; q < n; ++q) {
-            if (fold[q].done) continue;
+            if (done[q]) continue;
             const cpd_design &g = designs[valid[q]];
             for (uint32_t id = 0; id < n_rows_per; ++id) {
                 row_design.push_back((uint32_t)q);
@@ -424,33 +535,45 @@ cpd_status Context::run_gpu_fitness(const cpd_design *designs, const std::vector
         if ((st = upload_beam_table(stream, seed_fx, table_n, seed_z32.as<float>(), seed_z64.as<double>())) !=
             CPD_OK)
             return st;
+        RowSlots slots;
         if ((st = run_rows(true, chunk_fp32(), ddev, row_design, row_lambda, row_out, std::vector<uint32_t>(), out_n,
-                           max_evals, nbin_max, seed_z64.as<double>(), seed_z32.as<float>(), rows, rays)) != CPD_OK)
+                           max_evals, nbin_max, seed_z64.as<double>(), seed_z32.as<float>(), rows, rays, &slots)) !=
+            CPD_OK)
             return st;
-        const auto f0 = std::chrono::steady_clock::now();
-        for (size_t r = 0; r < row_design.size(); ++r) total_rays += rays[r];
-        // each design's rows are contiguous and in id order; designs fold in parallel
-        std::vector<size_t> starts;
-        for (size_t r = 0; r < row_design.size(); r += n_rows_per) starts.push_back(r);
-        parallel_for(starts.size(), [&](size_t i) {
-            const size_t r0 = starts[i];
-            OuterFold &f = fold[row_design[r0]];
-            for (size_t r = r0; r < r0 + n_rows_per; ++r) f.fold(rows + row_out[r]);
-            if (f.rows && f.converged()) f.done = true;
-        });
-        fold_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - f0).count();
+        oa.rows = slots.out;
+        oa.row_out = slots.row_out;
+        oa.row_rays = slots.row_rays;
+        oa.row_design = slots.row_design;
+        const uint32_t blocks = (uint32_t)(row_design.size() / n_rows_per);
+        hipLaunchKernelGGL(outer_fold_kernel, dim3(blocks), dim3(64), nbin_max * sizeof(double), stream, oa);
+        AD_TRY(hipGetLastError());
+        // done flags, information and the ray count in one copy
+        AD_TRY(hipMemcpyAsync(host_out.ptr, fb + o_done, o_off - o_done, hipMemcpyDeviceToHost, stream));
+        AD_TRY(hipStreamSynchronize(stream));
+        if (profile) {
+            float ms = 0.f;
+            AD_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            row_kernel_ms += ms;
+        }
+        memcpy(done.data(), host_out.ptr, n * sizeof(uint32_t));
+        launched = true;
+    }
+    std::vector<double> info(n, 0.0);  // no seed: the empty fold's 0 - 0
+    uint64_t total_rays = 0;
+    if (launched) {
+        memcpy(info.data(), (const char *)host_out.ptr + (o_info - o_done), n * sizeof(double));
+        memcpy(&total_rays, (const char *)host_out.ptr + (o_rays - o_done), 8);
     }
     for (size_t q = 0; q < n; ++q) {
         const cpd_design &g = designs[valid[q]];
         double size, dev;
         size_and_deviation64(g, size, dev);
-        out[valid[q]] = cpd_fitness{size, fold[q].info(), dev};
-        if (per_design) per_design[valid[q]] = fold[q].done ? CPD_OK : CPD_ERR_NOT_CONVERGED;
+        out[valid[q]] = cpd_fitness{size, info[q], dev};
+        if (per_design) per_design[valid[q]] = done[q] ? CPD_OK : CPD_ERR_NOT_CONVERGED;
     }
     last = cpd_run_stats{};
     last.rays = total_rays;
     last.trace_ms = row_kernel_ms;  // with CPD_OPT_PROFILE
-    last.finalize_ms = fold_ms;     // the host fold over rows
     return CPD_OK;
 }
 

@@ -95,7 +95,8 @@ struct Context {
                         const std::vector<double> &row_lambda, const std::vector<uint64_t> &row_out,
                         const std::vector<uint32_t> &row_cap, uint64_t out_n, uint32_t max_rays, uint32_t nbin_max,
                         const double *z64_dev, const float *z32_dev, const double *&rows_host,
-                        const uint64_t *&rays_host);
+                        const uint64_t *&rays_host, RowSlots *dev_slots = nullptr);
+    DeviceBuffer fold_buf;     // the GPU estimator's outer fold state
     cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
                              const cpd_design **designs_dev, uint32_t *nbin_max);
     DeviceBuffer recip;        // [k] = RN(1/k) for the row folds' integer divisors

@@ -408,6 +408,7 @@ Context::~Context() {
     row_buf.release();
     prop_buf.release();
     build_buf.release();
+    fold_buf.release();
     row_stage.release();
     design_stage.release();
     seed_z32.release();
