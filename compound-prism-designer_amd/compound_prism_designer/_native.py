@@ -254,6 +254,7 @@ def _load() -> ctypes.CDLL:
         "cpd_exit_ray": ([P(Design), d, d, P(d)], st),
         "cpd_ray_trace": ([P(Design), d, d, P(d), P(u32)], st),
         "cpd_fit_sellmeier1": ([P(d), P(d), d, d, u32, P(d), P(d)], st),
+        "cpd_fit_sellmeier1_batch": ([vp, P(d), P(d), sz, d, d, u32, P(d), P(d), P(st)], st),
         "cpd_generate_design": ([u32, u64, P(Params), P(Sampling), P(u32)], st),
         "cpd_generate_designs": ([u32, u64, sz, u32, P(Params), P(Design), P(Sampling), P(u32)], st),
         "cpd_hit_fraction": ([P(Design), u32, u32, P(d)], st),

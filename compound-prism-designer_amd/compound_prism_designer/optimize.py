@@ -39,6 +39,31 @@ def sellmeier_glass(name: str, b: Sequence[float], c: Sequence[float], wavelengt
     return Glass(name, list(out))
 
 
+def fit_sellmeier_catalog(ctx: Context, b, c, wavelength_range=(0.5, 0.82), npts: int = 100,
+                          names: Optional[Sequence[str]] = None):
+    """Fits n Sellmeier1 glasses (b, c: n x 3) on the GPU in one call, one glass
+    per thread (cpd_fit_sellmeier1_batch; glasscat.py:114-127 per glass).
+    Returns (glasses, chisq); a glass whose fit fails (a pole in range) raises
+    ValueError naming it."""
+    b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 3)
+    c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3)
+    if b.shape != c.shape:
+        raise ValueError("b and c must have the same number of glasses")
+    n = b.shape[0]
+    names = list(names) if names is not None else [f"G{i:03d}" for i in range(n)]
+    coeffs = np.empty((n, 6))
+    chisq = np.empty(n)
+    status = (ctypes.c_int * max(n, 1))()
+    dp = ctypes.POINTER(ctypes.c_double)
+    _check(N.lib.cpd_fit_sellmeier1_batch(ctx.handle, b.ctypes.data_as(dp), c.ctypes.data_as(dp), n,
+                                          float(wavelength_range[0]), float(wavelength_range[1]), npts,
+                                          coeffs.ctypes.data_as(dp), chisq.ctypes.data_as(dp), status))
+    bad = [names[i] for i in range(n) if status[i] != N.OK]
+    if bad:
+        raise ValueError(f"Sellmeier fit failed (pole in range) for {bad[:8]}")
+    return [Glass(names[i], list(coeffs[i])) for i in range(n)], chisq
+
+
 def synthetic_catalog(n: int, seed: int = 0, wavelength_range=(0.5, 0.82)) -> List[Glass]:
     """n Sellmeier1 glasses with B ~ U[0.3, 1.5], C ~ U[0.001, 0.2] um^2 (the
     benchmark generator's ranges), standing in for a glass catalogue file."""

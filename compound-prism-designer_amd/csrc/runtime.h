@@ -97,6 +97,7 @@ struct Context {
                         const double *z64_dev, const float *z32_dev, const double *&rows_host,
                         const uint64_t *&rays_host, RowSlots *dev_slots = nullptr);
     DeviceBuffer fold_buf;     // the GPU estimator's outer fold state
+    DeviceBuffer fit_buf;      // glass fits (cpd_fit_sellmeier1_batch)
     cpd_status stage_designs(const cpd_design *designs, const std::vector<uint32_t> &valid,
                              const cpd_design **designs_dev, uint32_t *nbin_max);
     DeviceBuffer recip;        // [k] = RN(1/k) for the row folds' integer divisors

@@ -409,6 +409,7 @@ Context::~Context() {
     prop_buf.release();
     build_buf.release();
     fold_buf.release();
+    fit_buf.release();
     row_stage.release();
     design_stage.release();
     seed_z32.release();

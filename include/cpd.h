@@ -189,6 +189,18 @@ cpd_status cpd_fit_sellmeier1(const double b[3], const double c[3], double lo, d
                               uint32_t npts, double coeffs_out[CPD_GLASS_COEFFS],
                               double *chisq_out);
 
+/* The same fit for n glasses on the GPU, one glass per thread (a catalogue
+ * fitted in one call, as designer.py's glass_catalog is before an
+ * optimisation): b and c hold n triples, coeffs_out n records of 6 and
+ * chisq_out (may be NULL) n residual sums of squares; per_glass[i] is CPD_OK
+ * or CPD_ERR_INVALID_ARGUMENT (a pole in range or dependent columns, NaN
+ * results).  Least squares by Givens rotations row by row (no m x 6 matrix
+ * per thread): the same optimum as cpd_fit_sellmeier1's Householder
+ * factorisation, to rounding.  AbstractGlass.horner_polynomial_fit /
+ * into_glass, compound_prism_designer/glasscat.py:114-127. */
+cpd_status cpd_fit_sellmeier1_batch(cpd_ctx *ctx, const double *b, const double *c, size_t n, double lo, double hi,
+                                    uint32_t npts, double *coeffs_out, double *chisq_out, cpd_status *per_glass);
+
 /* Synthetic workloads of the benchmark configurations (no reference
  * counterpart; the reference's optimiser draws designs in designer.py:203-265).
  * config 0/1: 3 prisms, 32 bins, 64 wavelengths x 16384 rays;

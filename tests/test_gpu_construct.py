@@ -114,3 +114,75 @@ def test_plan_generation_from_parameters():
         np.testing.assert_array_equal(res2[ok], ref[ok])
     finally:
         plan.close()
+
+
+def oracle_record(p: N.Params, acceptance: float) -> N.Design:
+    # the oracle's construction (CompoundPrism::new, detector_array_positioning,
+    # LinearDetectorArray::new restated in FP64): the detector positioned by the
+    # oracle first when acceptance > 0
+    from oracle import oracle as O
+
+    q = O._as(p, O.Params)
+    if acceptance > 0:
+        st, pos, fl = O.position_detector_array(q, acceptance)
+        assert st == 0
+        q.det_position[0], q.det_position[1], q.flipped = pos[0], pos[1], int(fl)
+    d = N.Design()
+    ctypes.memmove(ctypes.byref(d), ctypes.byref(O.build_design(q)), ctypes.sizeof(d))
+    return d
+
+
+@pytest.mark.parametrize("config", [0, 2, 4, 5, 6, 7])
+def test_device_records_match_oracle(config):
+    P, _, _, _ = cpd.generate_designs(config, 0, 128)
+    dev, st = cpd.build_designs(P, 1.0)
+    assert (st == 0).all()
+    for p, a in zip(P, dev):
+        b = oracle_record(p, 1.0)
+        assert ints(a) == ints(b)
+        np.testing.assert_allclose(doubles(a), doubles(b), rtol=REL, atol=1e-300)
+
+
+def test_device_sellmeier_fits_match_golden_and_host():
+    # glasscat.py:114-127: fitted indices within 1e-6 of scipy's curve_fit at the
+    # golden check wavelengths, and a residual no larger than scipy's; the
+    # device's Givens least squares and the host's Householder one reach the
+    # same optimum (indices within 1e-10 of each other)
+    import json
+    from pathlib import Path
+
+    from compound_prism_designer.optimize import fit_sellmeier_catalog
+
+    cases = json.loads((Path(__file__).resolve().parent / "golden" / "sellmeier.json").read_text())
+    ctx = cpd.Context(0)
+    try:
+        for lo, hi in sorted({(cs["lo"], cs["hi"]) for cs in cases}):
+            sel = [cs for cs in cases if (cs["lo"], cs["hi"]) == (lo, hi)]
+            glasses, chisq = fit_sellmeier_catalog(ctx, [cs["b"] for cs in sel], [cs["c"] for cs in sel], (lo, hi))
+            for cs, g, x2 in zip(sel, glasses, chisq):
+                n_dev = np.array([g(w) for w in cs["check_w"]])
+                np.testing.assert_allclose(n_dev, cs["check_n_fit"], rtol=0, atol=1e-6)
+                assert x2 <= cs["chisq"] * (1 + 1e-6) + 1e-18
+                b = (ctypes.c_double * 3)(*cs["b"])
+                c = (ctypes.c_double * 3)(*cs["c"])
+                host = (ctypes.c_double * 6)()
+                assert N.lib.cpd_fit_sellmeier1(b, c, lo, hi, 100, host, None) == 0
+                h = cpd.Glass("host", list(host))
+                np.testing.assert_allclose(n_dev, [h(w) for w in cs["check_w"]], rtol=0, atol=1e-10)
+        # a catalogue of 4096 glasses in one call; a pole inside the range fails by name
+        rng = np.random.default_rng(3)
+        b, c = rng.uniform(0.3, 1.5, (4096, 3)), rng.uniform(0.001, 0.2, (4096, 3))
+        glasses, chisq = fit_sellmeier_catalog(ctx, b, c)
+        assert len(glasses) == 4096 and np.isfinite(chisq).all()
+        for i in (0, 1000, 4095):
+            bb, cc = (ctypes.c_double * 3)(*b[i]), (ctypes.c_double * 3)(*c[i])
+            host = (ctypes.c_double * 6)()
+            assert N.lib.cpd_fit_sellmeier1(bb, cc, 0.5, 0.82, 100, host, None) == 0
+            ws = np.linspace(0.5, 0.82, 33)
+            np.testing.assert_allclose([glasses[i](w) for w in ws], [cpd.Glass("h", list(host))(w) for w in ws],
+                                       rtol=0, atol=1e-10)
+        b[7, 1], c[7, 1] = 1.0, 0.6 ** 2
+        with pytest.raises(ValueError, match="G007"):
+            fit_sellmeier_catalog(ctx, b[:16], c[:16])
+    finally:
+        ctx.close()

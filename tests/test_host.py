@@ -5,6 +5,7 @@ deterministic; the Python surface validates like the reference's pyo3 classes.""
 import ctypes
 import json
 import math
+import os
 import re
 from pathlib import Path
 
@@ -97,6 +98,59 @@ def test_sellmeier_fit_matches_curve_fit():
         np.testing.assert_allclose(n_ours, case["check_n_fit"], rtol=0, atol=1e-6)
         # no worse a fit than scipy's
         assert chisq.value <= case["chisq"] * (1 + 1e-6) + 1e-18
+
+
+def givens_lib():
+    # csrc/glassfit.h (the per-thread fit cpd_fit_sellmeier1_batch runs on the
+    # device) built for the host with the device's FP64 contraction rules
+    import subprocess
+
+    root = Path(__file__).resolve().parent.parent
+    src = root / "tests" / "guard" / "givens_host.cpp"
+    out = root / "tests" / "guard" / "_build" / "givens_host.so"
+    out.parent.mkdir(exist_ok=True)
+    deps = [src, *(root / "compound-prism-designer_amd" / "csrc").glob("*.h")]
+    if not out.exists() or out.stat().st_mtime < max(d.stat().st_mtime for d in deps):
+        subprocess.run(["/opt/rocm/llvm/bin/clang++", "-O2", "-std=c++17", "-fPIC", "-shared",
+                        "-ffp-contract=fast-honor-pragmas", "-I", str(root / "include"),
+                        "-I", str(root / "compound-prism-designer_amd" / "csrc"), str(src), "-o",
+                        str(out) + ".tmp"], check=True)
+        os.replace(str(out) + ".tmp", out)
+    L = ctypes.CDLL(str(out))
+    d = ctypes.POINTER(ctypes.c_double)
+    L.givens_fit.argtypes = [d, d, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, d, d]
+    return L
+
+
+def test_givens_sellmeier_fit_matches_curve_fit_and_householder():
+    # the device fit's least squares (Givens rotations, one row at a time)
+    # against scipy's curve_fit (golden) and the host's Householder fit: the
+    # optimum is unique, so fitted indices agree to rounding
+    L = givens_lib()
+    for case in json.loads((GOLDEN / "sellmeier.json").read_text()):
+        b = (ctypes.c_double * 3)(*case["b"])
+        c = (ctypes.c_double * 3)(*case["c"])
+        g6, h6 = (ctypes.c_double * 6)(), (ctypes.c_double * 6)()
+        gx, hx = ctypes.c_double(), ctypes.c_double()
+        assert L.givens_fit(b, c, case["lo"], case["hi"], 100, g6, ctypes.byref(gx)) == 0
+        assert N.lib.cpd_fit_sellmeier1(b, c, case["lo"], case["hi"], 100, h6, ctypes.byref(hx)) == 0
+        g, h = cpd.Glass("givens", list(g6)), cpd.Glass("householder", list(h6))
+        n_g = np.array([g(w) for w in case["check_w"]])
+        np.testing.assert_allclose(n_g, case["check_n_fit"], rtol=0, atol=1e-6)
+        np.testing.assert_allclose(n_g, [h(w) for w in case["check_w"]], rtol=0, atol=1e-10)
+        assert gx.value <= case["chisq"] * (1 + 1e-6) + 1e-18
+        np.testing.assert_allclose(gx.value, hx.value, rtol=1e-6)
+    # an exact degree-5 polynomial is not a Sellmeier curve, but a pole inside
+    # the range is refused and a one-term glass with tiny C is fitted closely
+    b = (ctypes.c_double * 3)(1.0, 0.0, 0.0)
+    c = (ctypes.c_double * 3)(0.6 ** 2, 0.01, 0.01)
+    out = (ctypes.c_double * 6)()
+    assert L.givens_fit(b, c, 0.5, 0.82, 100, out, None) == -1
+    c = (ctypes.c_double * 3)(1e-6, 0.01, 0.01)
+    assert L.givens_fit(b, c, 0.5, 0.82, 100, out, None) == 0
+    ws = np.linspace(0.5, 0.82, 17)
+    exact = np.sqrt(1 + ws ** 2 / (ws ** 2 - 1e-6))
+    np.testing.assert_allclose([cpd.Glass("g", list(out))(w) for w in ws], exact, rtol=0, atol=1e-9)
 
 
 def test_generator_is_deterministic_and_in_range():
