@@ -689,7 +689,8 @@ class Plan:
     def run_params(self, params: Sequence["N.Params"], acceptance: float = 1.0):
         """One generation from parameter vectors, built into designs on the GPU
         (cpd_plan_run_params): acceptance > 0 positions each detector as
-        position_detector_array does, 0 takes params' position as given.
+        position_detector_array does, 0 takes params' position as given and
+        N.POSITION_FROM_EXIT reads it as (distance, angle) from the exit face.
         Returns (fitness float64 [n, 3], status int32 [n])."""
         if len(params) != self.n:
             raise ValueError(f"plan holds {self.n} designs, got {len(params)}")
@@ -722,7 +723,9 @@ class Plan:
 def build_designs(params: Sequence["N.Params"], acceptance: float = 1.0, *, ctx: Optional[Context] = None):
     """Design records built on the GPU from parameter vectors
     (cpd_build_designs): acceptance > 0 positions each detector as
-    position_detector_array does, 0 takes params' position as given.  Returns
+    position_detector_array does, 0 takes params' position as given and
+    N.POSITION_FROM_EXIT reads it as (distance, angle) from the exit face
+    (designer.py:236-243).  Returns
     (list of Design, status int32 [n]); a design is meaningful where its status
     is 0."""
     n = len(params)

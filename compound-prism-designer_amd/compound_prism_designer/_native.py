@@ -37,6 +37,10 @@ NOALLOC_MAX_CELLS = 1 << 21
 MODE_FIXED = 0
 MODE_ADAPTIVE = 1
 
+# acceptance of cpd_build_designs / cpd_plan_run_params: det_position is
+# (distance, angle) from the exit face (designer.py:236-243)
+POSITION_FROM_EXIT = -1.0
+
 BEAM_GAUSSIAN = 0
 BEAM_FIBER = 1
 

@@ -4,8 +4,16 @@ The reference optimises with pymoo, one individual per `_evaluate` call
 (designer.py:115-300: CompoundPrismSpectrometerProblem builds a Spectrometer
 from each parameter vector and calls gpu_fitness on it).  Here a whole
 population is evaluated per call: its parameter vectors become `cpd_params`
-records, and one captured generation (cpd_plan_run_params) builds the designs
-on the GPU, positions their detectors and evaluates them.
+records, the designs are built on the GPU (detectors positioned, or placed
+from the exit face) and evaluated by one of three estimators:
+
+- "plan": the fixed-count fitness in one captured generation
+  (cpd_plan_run_params: construction and evaluation replay as HIP graphs);
+- "server": the same fitness through a continuous-batching server
+  (cpd_server_*), so several problems or callers share the GPU and their
+  generations overlap;
+- "gpu_fitness": the designer's own estimator, designer.py:276
+  (gpu_fitness with max_n 128 and max_eval 8192, cpd_gpu_fitness_batch).
 
 pymoo is not part of this build, so `evolve` is a small elitist
 multi-objective loop of its own (non-dominated sorting with crowding
@@ -23,7 +31,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from . import Context, Glass, Plan, _check, build_designs
+from . import Context, Glass, Plan, Server, _check, build_designs, gpu_fitness_batch
 from . import _native as N
 
 
@@ -85,12 +93,22 @@ def _params_dtype():
 
 @dataclass
 class SpectrometerProblem:
-    """The designer's problem (designer.py:115-201), Gaussian beam, detector
-    positioned automatically.  Variables are normalised to [0, 1]: glass
-    indices (nglass), face angles (nglass + 1, mapped to [-pi/2, pi/2]),
-    tanh of the separation lengths (nglass; length = artanh(t) max_height / 4,
-    designer.py:217), curvature ([1e-5, 1]), height ([1e-4, 1] max_height),
-    normalised beam height ([0, 1]) and detector angle ([-pi, pi])."""
+    """The designer's problem (designer.py:115-201), Gaussian beam.  Variables
+    are normalised to [0, 1]: glass indices (nglass), face angles (nglass + 1,
+    mapped to [-pi/2, pi/2]), tanh of the separation lengths (nglass; length =
+    artanh(t) max_height / 4, designer.py:217), curvature ([1e-5, 1]), height
+    ([1e-4, 1] max_height), normalised beam height ([0, 1]) and detector angle
+    ([-pi, pi]).  With acceptance None (the designer's
+    auto_position_detector_acceptance unset) two more variables place the
+    detector from the exit face: distance ([0, 40] max_height) and angle
+    ([-pi/2, pi/2]) (designer.py:131-132, 236-243); otherwise the detector is
+    positioned for that acceptance.
+
+    estimator: "plan", "server" or "gpu_fitness" (module docstring).  The
+    designer draws one random seed per individual for gpu_fitness; here one
+    seed per generation serves all its individuals, since a seed fixes the
+    beam table that every design of the launch reads.  A design that does not
+    converge gets the designer's RayTraceError objectives."""
 
     glasses: Sequence[Glass]
     nglass: int
@@ -105,17 +123,40 @@ class SpectrometerProblem:
     linear_intercept: float = 0.1
     detector_length: float = 32.0
     max_incident_angle: float = math.radians(45.0)
-    acceptance: float = 1.0
+    acceptance: Optional[float] = 1.0
     n_wavelengths: int = 64
     n_rays: int = 16384
     optimize_size: bool = True
     optimize_deviation: bool = True
     ctx: Optional[Context] = None
+    estimator: str = "plan"
+    gpu_fitness_max_n: int = 128
+    gpu_fitness_max_eval: int = 16_384 // 2
+    seed: int = 0
+    server: Optional[Server] = None
     _plan: Optional[Plan] = field(default=None, init=False, repr=False)
+    _own_server: bool = field(default=False, init=False, repr=False)
+    _rng: Optional[np.random.Generator] = field(default=None, init=False, repr=False)
+
+    def __post_init__(self):
+        if self.estimator not in ("plan", "server", "gpu_fitness"):
+            raise ValueError(f"unknown estimator {self.estimator!r}")
+        if self.acceptance is not None and not 0.0 < self.acceptance <= 1.0:
+            raise ValueError("acceptance must be in (0, 1] or None")
+        self._rng = np.random.default_rng(self.seed)
+
+    @property
+    def explicit_position(self) -> bool:
+        return self.acceptance is None
+
+    @property
+    def build_acceptance(self) -> float:
+        """The construction's acceptance argument (cpd_build_designs)."""
+        return N.POSITION_FROM_EXIT if self.acceptance is None else float(self.acceptance)
 
     @property
     def n_var(self) -> int:
-        return 3 * self.nglass + 1 + 4
+        return 3 * self.nglass + 1 + 4 + (2 if self.explicit_position else 0)
 
     @property
     def n_obj(self) -> int:
@@ -149,6 +190,10 @@ class SpectrometerProblem:
         a["max_incident_angle"] = self.max_incident_angle
         a["beam_width"] = self.beam_width
         a["wavelength_lo"], a["wavelength_hi"] = self.wavelength_range
+        if self.explicit_position:
+            a["det_position"][:, 0] = self.max_height * 40 * X[:, 3 * P + 5]
+            a["det_position"][:, 1] = -math.pi / 2 + math.pi * X[:, 3 * P + 6]
+            a["flipped"] = 0
         return arr
 
     def _plan_for(self, params: List["N.Params"]) -> Plan:
@@ -161,12 +206,13 @@ class SpectrometerProblem:
         # individual is a valid stand-in record for all of them; random
         # parameter vectors mostly fail (total internal reflection), so draw
         # more when the population has none
-        designs, st = build_designs(params, self.acceptance, ctx=self.ctx)
+        designs, st = build_designs(params, self.build_acceptance, ctx=self.ctx)
         rng = np.random.default_rng(0)
         for _ in range(64):
             if (st == 0).any():
                 break
-            designs, st = build_designs(self.params(rng.random((256, self.n_var))), self.acceptance, ctx=self.ctx)
+            designs, st = build_designs(self.params(rng.random((256, self.n_var))), self.build_acceptance,
+                                        ctx=self.ctx)
         ok = np.flatnonzero(st == 0)
         if ok.size == 0:
             raise RuntimeError("no buildable design found for the plan's stand-in record")
@@ -177,12 +223,52 @@ class SpectrometerProblem:
                           schedule=N.Schedule(0, 16, N.ORDER_GIVEN, 0))
         return self._plan
 
+    def _built(self, params):
+        """Designs built on the GPU; (designs, status, indices of the built ones)."""
+        designs, st = build_designs(params, self.build_acceptance, ctx=self.ctx)
+        return designs, st, np.flatnonzero(st == 0)
+
+    def _server_fitness(self, params):
+        designs, st, ok = self._built(params)
+        fit = np.full((len(params), 3), np.nan)
+        if ok.size:
+            if self.server is None:
+                self.server = Server(self.ctx)
+                self._own_server = True
+            rid = self.server.submit([designs[i] for i in ok], (self.n_wavelengths, self.n_rays))
+            res, rst, _ = self.server.wait(rid)
+            fit[ok] = res
+            st = st.copy()
+            st[ok] = rst
+        return fit, st
+
+    def _gpu_fitness(self, params):
+        designs, st, ok = self._built(params)
+        fit = np.full((len(params), 3), np.nan)
+        if ok.size:
+            seeds = self._rng.random(1)
+            res, rst = gpu_fitness_batch([designs[i] for i in ok], seeds, max_n=self.gpu_fitness_max_n,
+                                         max_eval=self.gpu_fitness_max_eval, ctx=self.ctx)
+            fit[ok] = res
+            st = st.copy()
+            st[ok] = rst
+        return fit, st
+
+    def fitness(self, X: np.ndarray):
+        """Raw fitness [pop, 3] (size, info, deviation) and per-design status of
+        a population, by the problem's estimator."""
+        params = self.params(X)
+        if self.estimator == "server":
+            return self._server_fitness(params)
+        if self.estimator == "gpu_fitness":
+            return self._gpu_fitness(params)
+        return self._plan_for(params).run_params(params, self.build_acceptance)
+
     def evaluate(self, X: np.ndarray):
         """Objectives F [pop, n_obj] (size, log2(nbin) - info, deviation, as
         designer.py:267-300 orders them), constraint G [pop] (0 feasible),
         the raw fitness [pop, 3] and the per-design status."""
-        params = self.params(X)
-        fit, st = self._plan_for(params).run_params(params, self.acceptance)
+        fit, st = self.fitness(X)
         max_size = self.max_height * 40 if self.optimize_size else np.inf
         max_info = math.log2(self.bin_count)
         size, info, dev = fit[:, 0], fit[:, 1], fit[:, 2]
@@ -203,6 +289,10 @@ class SpectrometerProblem:
         if self._plan is not None:
             self._plan.close()
             self._plan = None
+        if self._own_server and self.server is not None:
+            self.server.close()
+            self.server = None
+            self._own_server = False
 
 
 def nondominated_ranks(F: np.ndarray) -> np.ndarray:
@@ -274,7 +364,8 @@ def evolve(problem: SpectrometerProblem, pop_size: int = 64, generations: int = 
            crossover_eta: float = 4.0, mutation_eta: float = 2.0, mutation_prob: Optional[float] = None,
            X0: Optional[np.ndarray] = None):
     """Elitist (mu + mu) multi-objective loop; every generation of pop_size
-    offspring is one cpd_plan_run_params call.  The distribution indices and
+    offspring is one evaluate call (one captured generation, one server
+    request or one batched gpu_fitness call, by the problem's estimator).  The distribution indices and
     the per-variable mutation probability default to design_config.toml's
     optimiser section (4, 2 and 0.05).  Returns the final population (X, F, G)
     and a per-generation history."""

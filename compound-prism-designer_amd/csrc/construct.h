@@ -208,9 +208,26 @@ CPD_HD cpd_status position_detector(const cpd_params &p, double acceptance, doub
     return CPD_OK;
 }
 
-// The whole design from its parameters: checks, then, when acceptance > 0,
-// the detector positioned for that acceptance (the parameters' position and
-// flip are used as given otherwise), then the records.
+// The detector position given relative to the exit face (designer.py:236-243):
+// det_position holds (distance, angle); the array sits that distance from the
+// final face's midpoint (the arc's midpoint, ray.rs:231-240) along the face
+// normal (cos a, sin a) of the last face angle, rotated by the angle.  The
+// rotation is numpy's R @ v and the sums are unfused, as in the designer.
+CPD_HD void position_from_exit(const cpd_params &p, const cpd_design &prism, double pos[2]) {
+#pragma clang fp contract(off)
+    const double a = p.angles[p.n_glass];
+    const double ex = cos(a), ey = sin(a);
+    const double c = cos(p.det_position[1]), s = sin(p.det_position[1]);
+    const double dx = c * ex + -s * ey, dy = s * ex + c * ey;
+    const double dv = p.det_position[0];
+    pos[0] = prism.arc_midpt[0] + dx * dv;
+    pos[1] = prism.arc_midpt[1] + dy * dv;
+}
+
+// The whole design from its parameters: checks, then the detector position
+// (acceptance > 0: positioned for that acceptance; CPD_POSITION_FROM_EXIT:
+// from the exit face, not flipped; 0: the parameters' position and flip as
+// given), then the records.
 CPD_HD cpd_status construct_design(const cpd_params &p, double acceptance, cpd_design &d) {
     if (check_design_params(p) != PARAM_OK) return CPD_ERR_INVALID_ARGUMENT;
     double pos[2] = {p.det_position[0], p.det_position[1]};
@@ -221,6 +238,10 @@ CPD_HD cpd_status construct_design(const cpd_params &p, double acceptance, cpd_d
     }
     d = cpd_design{};
     build_prism(p, d);
+    if (acceptance < 0.0) {
+        position_from_exit(p, d, pos);
+        flipped = 0;
+    }
     build_detector(p, pos, flipped, d);
     return CPD_OK;
 }

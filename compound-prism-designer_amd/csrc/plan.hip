@@ -347,8 +347,8 @@ cpd_status cpd_plan_run(cpd_plan *plan, const cpd_design *designs, cpd_fitness *
 cpd_status cpd_plan_run_params(cpd_plan *plan, const cpd_params *params, double acceptance, cpd_fitness *out,
                                cpd_status *per_design) {
     if (!plan || !params || !out) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (!(acceptance >= 0.0 && acceptance <= 1.0))
-        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "acceptance must be in [0, 1]");
+    if (!(acceptance >= 0.0 && acceptance <= 1.0) && acceptance != CPD_POSITION_FROM_EXIT)
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "acceptance must be in [0, 1] or CPD_POSITION_FROM_EXIT");
     return reinterpret_cast<cpd::Plan *>(plan)->run_params(params, acceptance, out, per_design);
 }
 

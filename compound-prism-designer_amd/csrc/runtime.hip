@@ -700,8 +700,8 @@ cpd_status cpd_build_designs(cpd_ctx *ctx, const cpd_params *params, size_t n, d
     if (n == 0) return CPD_OK;
     if (!params || !designs_out || !per_design) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n > (1u << 24)) return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "more than 2^24 designs");
-    if (!(acceptance >= 0.0 && acceptance <= 1.0))
-        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "acceptance must be in [0, 1]");
+    if (!(acceptance >= 0.0 && acceptance <= 1.0) && acceptance != CPD_POSITION_FROM_EXIT)
+        return cpd::fail(CPD_ERR_INVALID_ARGUMENT, "acceptance must be in [0, 1] or CPD_POSITION_FROM_EXIT");
     Context *c;
     cpd_status st = resolve(ctx, &c);
     if (st != CPD_OK) return st;

@@ -360,15 +360,25 @@ cpd_status cpd_gpu_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, size_t
                                  size_t n_seeds, uint32_t max_n, uint32_t nwarp, uint32_t max_eval,
                                  cpd_fitness *out, cpd_status *per_design);
 
+/* Acceptance value of cpd_build_designs and cpd_plan_run_params that places
+ * the detector relative to the exit face, as the designer does without
+ * automatic positioning (designer.py:236-243): params' det_position holds
+ * (distance, angle), and the array's position is the final face's midpoint
+ * (the arc midpoint, src/spectrometer/ray.rs:231-240) plus distance times the
+ * face normal (cos, sin of the last face angle) rotated by angle; not
+ * flipped. */
+#define CPD_POSITION_FROM_EXIT (-1.0)
+
 /* n designs built on the GPU from their parameters: cpd_build_design, and
  * first, when acceptance > 0, cpd_position_detector_array with that
  * acceptance (params' det_position and flipped are then not read; with
- * acceptance 0 they are used as given, designer.py:236-243's explicit
- * position).  The same FP64 construction code as the host entry points; only
+ * acceptance 0 they are used as given; with CPD_POSITION_FROM_EXIT
+ * det_position is read as (distance, angle) from the exit face).  The same FP64 construction code as the host entry points; only
  * the device's cos and sin may differ from the host's in the last bit.
  * per_design[i] receives design i's status (the host entry points' codes,
  * without their messages); designs_out[i] is meaningful where it is CPD_OK.
- * acceptance in [0, 1].  designer.py:203-265 per individual. */
+ * acceptance in [0, 1] or CPD_POSITION_FROM_EXIT.  designer.py:203-265 per
+ * individual. */
 cpd_status cpd_build_designs(cpd_ctx *ctx, const cpd_params *params, size_t n, double acceptance,
                              cpd_design *designs_out, cpd_status *per_design);
 

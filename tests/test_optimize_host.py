@@ -47,3 +47,23 @@ def test_sellmeier_catalog_is_smooth_and_dispersive():
     for g in cat:
         n = np.array([g(x) for x in w])
         assert (n > 1.0).all() and (np.diff(n) < 0).all()  # normal dispersion over the range
+
+
+def test_explicit_position_variables():
+    # designer.py:131-132, 236-243: without automatic positioning two more
+    # variables give the detector's distance ([0, 40 max_height]) and angle
+    # ([-pi/2, pi/2]) from the exit face
+    import pytest
+    from compound_prism_designer import _native as N
+
+    cat = O.synthetic_catalog(5, seed=3)
+    pb = O.SpectrometerProblem(cat, 2, max_height=20.0, acceptance=None)
+    assert pb.n_var == 3 * 2 + 5 + 2 and pb.build_acceptance == N.POSITION_FROM_EXIT
+    x = np.full((1, pb.n_var), 0.5)
+    x[0, -2:] = [0.25, 1.0]
+    p = pb.params(x)[0]
+    assert p.det_position[0] == 0.25 * 800.0 and p.det_position[1] == math.pi / 2 and p.flipped == 0
+    with pytest.raises(ValueError):
+        O.SpectrometerProblem(cat, 2, acceptance=0.0)
+    with pytest.raises(ValueError):
+        O.SpectrometerProblem(cat, 2, estimator="pymoo")
