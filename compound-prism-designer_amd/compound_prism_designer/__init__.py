@@ -752,12 +752,16 @@ class Server:
 
     submit() queues a batch of designs and returns a request id; poll()/wait()
     return (fitness [n, 3], status [n], latency_ms) once the request is done.
-    Requests that arrive while others run join the next scheduling step."""
+    Requests that arrive while others run join the next scheduling step, or,
+    with resident=True, the device-resident loop, which admits, traces and
+    finalizes them on the GPU without a launch per request (designs of at most
+    max_bins bins and max_rays rays per wavelength)."""
 
     def __init__(self, ctx: Optional[Context] = None, *, memory_budget: int = 0, step_items: int = 0,
-                 max_designs: int = 0):
+                 max_designs: int = 0, resident: bool = False, max_bins: int = 0, max_rays: int = 0):
         h = ctypes.c_void_p()
-        opts = N.ServerOpts(memory_budget, step_items, max_designs)
+        opts = N.ServerOpts(memory_budget, step_items, max_designs,
+                            N.SERVER_RESIDENT if resident else N.SERVER_STEPS, max_bins, max_rays, 0)
         _check(N.lib.cpd_server_create(_ctx_handle(ctx), ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
         self._sizes = {}

@@ -221,8 +221,14 @@ class Schedule(ctypes.Structure):
                 "order": "cost" if self.order == ORDER_COST else "given"}
 
 
+SERVER_STEPS = 0
+SERVER_RESIDENT = 1
+
+
 class ServerOpts(ctypes.Structure):
-    _fields_ = [("memory_budget", ctypes.c_uint64), ("step_items", ctypes.c_uint32), ("max_designs", ctypes.c_uint32)]
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("step_items", ctypes.c_uint32), ("max_designs", ctypes.c_uint32),
+                ("mode", ctypes.c_uint32), ("max_bins", ctypes.c_uint32), ("max_rays", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):

@@ -70,6 +70,74 @@ struct FinalizeArgs {
     uint32_t n_designs;
 };
 
+// ---------------------------------------------------------------------------
+// Device-resident request loop (the server's resident mode).
+//
+// The host stages requests in host memory mapped into the GPU's address
+// space (coherent, never cached stale by the GPU) and publishes them as
+// chunks of at most 64 designs; a persistent kernel whose wavefronts loop
+// until told to stop, or until they find no work for idle_limit polls,
+// ingests chunks into the device pools, claims their work items, traces them
+// and finalizes each design when its last item is done, writing the result
+// and the request's tag back to host memory.  Only plain loads and stores
+// touch host memory; the counters the wavefronts share live in device memory.
+
+struct ResidentHost {                  // control block, host memory
+    unsigned long long published;     // chunks the host has written (host stores, release)
+    unsigned int stop;                // launches with epoch <= stop exit at their next poll
+    unsigned int error;               // nonzero: a wavefront met an inconsistent chunk (device stores)
+};
+struct ResidentChunk {                 // descriptor of chunk c at ring position c % chunk_cap, host memory
+    unsigned long long seq1;          // c + 1 (0: never written)
+    uint32_t slot0, n;                // pool slots slot0 .. slot0 + n - 1, n <= 64
+    uint32_t tag, pad;                // the request's tag, written per slot when a design is done
+};
+struct alignas(16) StagedDesign {      // one design as the host stages it (host memory, indexed by slot)
+    cpd_design design;
+    TraceDesign trace;
+    uint64_t acc_begin, lb_begin;     // its accumulator and last-bin offsets
+    uint32_t items, pad[3];           // work items (wavelengths x tiles)
+};
+struct ResidentResult {                // per slot, host memory
+    cpd_fitness fit;
+    uint32_t tag, pad;                // written after fit: the request whose design finished here
+};
+
+// device counters: chunks taken for ingest, the head chunk claims come from,
+// the newest `published` and stop epoch the polling wavefronts have read
+enum { kResIngest = 0, kResHead = 1, kResPublished = 2, kResStop = 3, kResCtrlWords = 4 };
+
+struct ResidentArgs {
+    ResidentHost *host;
+    const ResidentChunk *chunks;      // [chunk_cap]
+    const StagedDesign *staged;       // [max_designs]
+    ResidentResult *results;          // [max_designs]
+    uint32_t chunk_cap, max_designs;
+    // device memory
+    cpd_design *designs;
+    TraceDesign *traces;
+    uint64_t *acc_begin, *lb_begin;
+    unsigned long long *acc;
+    uint32_t *last_bin;
+    unsigned int *items_total, *items_done;  // per slot
+    uint32_t *slot_tag;                      // per slot
+    unsigned long long *ingested;            // per ring position: c + 1 once chunk c's records are in the pools
+    unsigned long long *claim_word;          // per ring position: (c mod 2^24, items, claimed)
+    uint32_t *chunk_slot0, *chunk_n;         // per ring position
+    uint32_t *prefix;                        // per ring position: 65 inclusive item prefix sums over its designs
+    unsigned long long *ctrl;                // kRes* counters
+    const float *z32, *zs32;
+    const double *z64, *zs64;
+    const uint32_t *inv;
+    uint32_t max_rays;                       // beam table length
+    uint32_t hist_words, row_words;
+    uint32_t idle_limit;                     // polls without work before a wavefront exits
+    uint32_t epoch;                          // this launch's number (from 1)
+};
+hipError_t launch_resident(hipStream_t s, const ResidentArgs &a, uint32_t grid);
+hipError_t resident_occupancy(size_t lds, int *per_cu);
+size_t resident_lds_bytes(uint32_t hist_words, uint32_t row_words);
+
 // LDS of one trace wavefront: columns | flagged-ray queue | row; sizes of the
 // accumulation policy `acc` (CPD_ACC_*).
 CPD_HD uint32_t trace_hist_end_words(uint32_t hist_words, uint32_t cell_bytes) {

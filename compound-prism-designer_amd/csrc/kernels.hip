@@ -392,22 +392,94 @@ __device__ void flush_row(void *hist_v, void *row_v, uint32_t nbin, uint32_t col
     __syncthreads();
 }
 
+// The LDS regions of one trace wavefront (trace_lds_bytes).
+struct TraceLds {
+    void *hist;
+    uint32_t *queue, *row32;
+    Chain64 *chain;
+    LensBand64 *band;
+    uint64_t *masks;
+    LambdaConsts *lc_lds;
+};
+template <int A>
+__device__ __forceinline__ TraceLds trace_lds(uint32_t *lds, uint32_t hist_words, uint32_t row_words) {
+    using Cell = typename Acc<A>::Cell;
+    TraceLds L;
+    const uint32_t hist_end = trace_hist_end_words(hist_words, sizeof(Cell));
+    L.hist = lds;
+    L.queue = lds + hist_end;
+    L.row32 = lds + ((hist_end + kQueueCap + 1) & ~1u);
+    L.chain = (Chain64 *)((char *)lds + trace_chain_offset(hist_words, row_words, A));
+    L.band = (LensBand64 *)(L.chain + 1);
+    L.masks = (uint64_t *)(L.band + 1);
+    L.lc_lds = (LambdaConsts *)(L.masks + (kMaskChunk >> 12));
+    return L;
+}
+template <int A>
+__device__ __forceinline__ void trace_lds_clear(uint32_t *lds, uint32_t hist_words, uint32_t row_words) {
+    using Cell = typename Acc<A>::Cell;
+    const TraceLds L = trace_lds<A>(lds, hist_words, row_words);
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < hist_words; i += 64) ((Cell *)L.hist)[i] = 0;
+    for (uint32_t i = lane; i < row_words * (Acc<A>::kRowBytes / 4); i += 64) L.row32[i] = 0;
+    __syncthreads();
+}
+
+// One work item: wavelength sample j (at `lambda`) and ray tile `tile` of the
+// design (g, td); its row is added to acc[acc_j + b] and the bin of the
+// design's last ray, when the tile holds it, stored at *last_bin_j.  Args
+// supplies the beam tables (z32, zs32, z64, zs64, inv) and the LDS shape
+// (hist_words, row_words); the LDS regions are derived here from the kernel's
+// arguments, which keeps them out of the registers live across items.
+template <bool COUNT, int A, class Args>
+__device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd_design *g, const TraceDesign *td,
+                                         uint32_t j, uint32_t tile, double lambda, unsigned long long *acc,
+                                         uint64_t acc_j, uint32_t *last_bin_j, unsigned long long &ops) {
+    const TraceLds L = trace_lds<A>(lds, a.hist_words, a.row_words);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t M = td->n_rays, nbin = td->nbin;
+    const uint32_t tile_rays = td->tile_rays, cols = td->hist_cols;
+    TileCtx c;
+    c.g = g;
+    c.td = td;
+    // every lane computes the same constants; lane 0 stores the chain record
+    make_lambda_consts(*g, lambda, c.lc, lane == 0 ? L.chain : nullptr, lane == 0 ? L.band : nullptr);
+    if (lane == 0) *L.lc_lds = c.lc;
+    __syncthreads();  // the queue drains and the band tests of this item read the records
+    c.chain = L.chain;
+    c.band = L.band;
+    c.masks = L.masks;
+    c.lc_lds = L.lc_lds;
+    c.k_begin = tile * tile_rays + 1;
+    c.k_end = min(c.k_begin + tile_rays, M + 1);
+    c.n_rays = M;
+    c.nbin = nbin;
+    c.cols = cols;
+    c.hist = L.hist;
+    c.row = L.row32;
+    c.queue = L.queue;
+    c.last_bin = last_bin_j;
+    c.z32 = a.z32;
+    c.z64 = a.z64;
+    c.zs32 = a.zs32;
+    c.zs64 = a.zs64;
+    c.full_end = M / kBandRays * kBandRays;
+    c.last_pos = M <= c.full_end ? a.inv[M - 1] : M;
+    if (!c.lc.ok || td->skip) {
+        // every ray of this wavelength is lost in the planar faces, or the
+        // design is a stand-in for one that failed construction
+        if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
+    } else {
+        tile_loop<COUNT, A>(c, ops);
+        flush_row<A>(L.hist, L.row32, nbin, cols, acc, acc_j);
+    }
+}
+
 template <bool COUNT, int A>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel(BatchArgs a) {
-    using Cell = typename Acc<A>::Cell;
     extern __shared__ __align__(8) uint32_t lds[];
-    Cell *hist = (Cell *)lds;
-    const uint32_t hist_end = trace_hist_end_words(a.hist_words, sizeof(Cell));
-    uint32_t *queue = lds + hist_end;
-    uint32_t *row32 = lds + ((hist_end + kQueueCap + 1) & ~1u);
-    Chain64 *chain = (Chain64 *)((char *)lds + trace_chain_offset(a.hist_words, a.row_words, A));
-    LensBand64 *band = (LensBand64 *)(chain + 1);
-    uint64_t *masks = (uint64_t *)(band + 1);
-    LambdaConsts *lc_lds = (LambdaConsts *)(masks + (kMaskChunk >> 12));
+    trace_lds_clear<A>(lds, a.hist_words, a.row_words);
     const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < a.hist_words; i += 64) hist[i] = 0;
-    for (uint32_t i = lane; i < a.row_words * (Acc<A>::kRowBytes / 4); i += 64) row32[i] = 0;
-    __syncthreads();
     unsigned long long ops = 0;
     uint64_t item = blockIdx.x;
     // items claimed by the wavefront's last counter fetch: [next, end)
@@ -428,46 +500,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void tr
         const uint32_t d = a.slot ? a.slot[e] : e;
         const cpd_design *g = a.designs + d;
         const TraceDesign *td = a.traces + d;
-        const uint32_t M = td->n_rays, nbin = td->nbin;
-        const uint32_t n_tiles = td->n_tiles, tile_rays = td->tile_rays, cols = td->hist_cols;
+        const uint32_t n_tiles = td->n_tiles;
         const uint64_t local = item - a.item_begin[e] + (a.item_offset ? a.item_offset[e] : 0);
         const uint32_t j = (uint32_t)(local / n_tiles), tile = (uint32_t)(local % n_tiles);
-
-        TileCtx c;
-        c.g = g;
-        c.td = td;
         const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
-        // every lane computes the same constants; lane 0 stores the chain record
-        make_lambda_consts(*g, lambda, c.lc, lane == 0 ? chain : nullptr, lane == 0 ? band : nullptr);
-        if (lane == 0) *lc_lds = c.lc;
-        __syncthreads();  // the queue drains and the band tests of this item read the records
-        c.chain = chain;
-        c.band = band;
-        c.masks = masks;
-        c.lc_lds = lc_lds;
-        c.k_begin = tile * tile_rays + 1;
-        c.k_end = min(c.k_begin + tile_rays, M + 1);
-        c.n_rays = M;
-        c.nbin = nbin;
-        c.cols = cols;
-        c.hist = hist;
-        c.row = row32;
-        c.queue = queue;
-        c.last_bin = a.last_bin + a.lb_begin[d] + j;
-        c.z32 = a.z32;
-        c.z64 = a.z64;
-        c.zs32 = a.zs32;
-        c.zs64 = a.zs64;
-        c.full_end = M / kBandRays * kBandRays;
-        c.last_pos = M <= c.full_end ? a.inv[M - 1] : M;
-        if (!c.lc.ok || td->skip) {
-            // every ray of this wavelength is lost in the planar faces, or the
-            // design is a stand-in for one that failed construction
-            if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
-        } else {
-            tile_loop<COUNT, A>(c, ops);
-            flush_row<A>(hist, row32, nbin, cols, a.acc, a.acc_begin[d] + (uint64_t)j * nbin);
-        }
+        run_item<COUNT, A>(a, lds, g, td, j, tile, lambda, a.acc, a.acc_begin[d] + (uint64_t)j * td->nbin,
+                           a.last_bin + a.lb_begin[d] + j, ops);
         if (!a.work_counter) break;
     }
     if (COUNT) {
@@ -484,17 +522,14 @@ __device__ double cell_p(const unsigned long long *acc, uint64_t idx, double div
     return (double)(*(const float *)(acc + idx) / (float)div);
 }
 
+// One wavefront reduces a design's sums (N wavelengths x nbin cells at acc,
+// the last ray's bins at lb) to its information, writes p(d|l) rows when
+// `rows` is set, and leaves the cells zero and the last bins at nbin for the
+// next use.  Every lane returns the information.
 template <int A>
-__global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
-    const uint32_t e = blockIdx.x;
-    const uint32_t d = a.slot ? a.slot[e] : e;
+__device__ __forceinline__ double finalize_design(unsigned long long *acc, uint32_t *lb, double *rows, uint32_t N,
+                                                  uint32_t M, uint32_t nbin) {
     const uint32_t lane = threadIdx.x;
-    const cpd_design &g = a.designs[d];
-    const TraceDesign &td = a.traces[d];
-    const uint32_t N = td.n_lambda, M = td.n_rays, nbin = td.nbin;
-    unsigned long long *acc = a.acc + a.acc_begin[d];
-    uint32_t *lb = a.last_bin + a.lb_begin[d];
-    double *rows = a.rows ? a.rows + a.row_begin[d] : nullptr;
     double info;
     if (A == CPD_ACC_EXACT || A == CPD_ACC_F64) {
         CSum h_cond, h_det;
@@ -583,6 +618,19 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
     }
     __syncthreads();
     for (uint32_t j = lane; j < N; j += 64) lb[j] = nbin;
+    return info;
+}
+
+template <int A>
+__global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
+    const uint32_t e = blockIdx.x;
+    const uint32_t d = a.slot ? a.slot[e] : e;
+    const uint32_t lane = threadIdx.x;
+    const cpd_design &g = a.designs[d];
+    const TraceDesign &td = a.traces[d];
+    const double info = finalize_design<A>(a.acc + a.acc_begin[d], a.last_bin + a.lb_begin[d],
+                                           a.rows ? a.rows + a.row_begin[d] : nullptr, td.n_lambda, td.n_rays,
+                                           td.nbin);
     if (lane == 0 && a.out) {
         double size, deviation;
         size_and_deviation64(g, size, deviation);
@@ -591,6 +639,230 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
         f.info = info;
         f.deviation = deviation;
         a.out[a.out_stride_slot ? d : e] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident request loop (ResidentArgs in kernels.h).
+//
+// Chunk c (at most 64 designs) sits at ring position c % chunk_cap.  Its
+// claim word packs (c mod 2^24, items of the chunk, items claimed) as bits
+// 63..40, 39..20, 19..0; ingest stores it last, with release, so a wavefront
+// whose atomic add returns chunk c's word owns item `claimed` of chunk c and
+// sees c's tables, even if it meant to claim from an older chunk at that
+// position.  A chunk holds fewer than 2^18 items, and the count never nears
+// 2^20: past the total only wavefronts that find the chunk exhausted add to
+// it, once each, before the head moves on.  Items per design are counted down
+// in device memory; the wavefront that finishes a design's last item
+// finalizes it.  Host memory is read only after a system-scope acquire and
+// written only with plain stores followed by a system-scope release.
+constexpr uint32_t kResCountBits = 20;
+constexpr unsigned long long kResCountMask = (1ull << kResCountBits) - 1;
+
+__device__ __forceinline__ unsigned long long bcast64(unsigned long long v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long agent_load(unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Copies chunk c's staged records (host memory) into the pools, builds its
+// item table and publishes its claim word.  Returns false on an inconsistent
+// descriptor (the chunk is then published empty and the host told).
+__device__ __attribute__((noinline)) bool resident_ingest(const ResidentArgs &a, unsigned long long c) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t pos = (uint32_t)(c % a.chunk_cap);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's descriptor and records
+    const ResidentChunk *ch = a.chunks + pos;
+    const unsigned long long seq1 = ch->seq1;
+    uint32_t slot0 = ch->slot0, n = ch->n;
+    const uint32_t tag = ch->tag;
+    const bool ok = seq1 == c + 1 && n >= 1 && n <= 64 && slot0 < a.max_designs && n <= a.max_designs - slot0;
+    if (!ok) n = 0;
+    constexpr uint32_t WD = sizeof(cpd_design) / 4, WT = sizeof(TraceDesign) / 4;
+    for (uint32_t w = lane; w < n * WD; w += 64) {
+        const uint32_t q = w / WD, o = w - q * WD;
+        ((uint32_t *)(a.designs + slot0 + q))[o] = ((const uint32_t *)&a.staged[slot0 + q].design)[o];
+    }
+    for (uint32_t w = lane; w < n * WT; w += 64) {
+        const uint32_t q = w / WT, o = w - q * WT;
+        ((uint32_t *)(a.traces + slot0 + q))[o] = ((const uint32_t *)&a.staged[slot0 + q].trace)[o];
+    }
+    uint32_t items = 0;
+    if (lane < n) {
+        const StagedDesign &sd = a.staged[slot0 + lane];
+        items = sd.items;
+        a.acc_begin[slot0 + lane] = sd.acc_begin;
+        a.lb_begin[slot0 + lane] = sd.lb_begin;
+        a.items_total[slot0 + lane] = items;
+        a.items_done[slot0 + lane] = 0;
+        a.slot_tag[slot0 + lane] = tag;
+    }
+    // inclusive prefix sums of the items over the chunk's designs
+    uint32_t x = items;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(x, off);
+        if (lane >= off) x += v;
+    }
+    uint32_t *pre = a.prefix + (size_t)pos * 65;
+    if (lane == 0) pre[0] = 0;
+    pre[lane + 1] = x;
+    const uint32_t total = __shfl(x, 63);
+    a.chunk_slot0[pos] = slot0;
+    a.chunk_n[pos] = n;
+    const bool fits = total < (1u << 18);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) {
+        const unsigned long long word = ((c & 0xffffffull) << 40) | ((unsigned long long)(fits ? total : 0) << kResCountBits);
+        __hip_atomic_store(a.claim_word + pos, word, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.ingested + pos, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!ok || !fits) a.host->error = 1;
+    }
+    return ok && fits;
+}
+
+// The finished design in `slot`: information, size and deviation to host
+// memory, then the request's tag.
+__device__ __attribute__((noinline)) void resident_finalize(const ResidentArgs &a, uint32_t slot) {
+    const uint32_t lane = threadIdx.x;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every item's sums of this design
+    const TraceDesign &td = a.traces[slot];
+    const double info = finalize_design<CPD_ACC_EXACT>(a.acc + a.acc_begin[slot], a.last_bin + a.lb_begin[slot],
+                                                       nullptr, td.n_lambda, td.n_rays, td.nbin);
+    if (lane == 0) {
+        double size, deviation;
+        size_and_deviation64(a.designs[slot], size, deviation);
+        ResidentResult *r = a.results + slot;
+        r->fit.size = size;
+        r->fit.info = info;
+        r->fit.deviation = deviation;
+    }
+    // the result and the cleared sums reach memory before the tag
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (lane == 0) __hip_atomic_store(&a.results[slot].tag, a.slot_tag[slot], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Wavefronts 0..kResPollers-1 poll host memory (new chunks, the stop epoch)
+// once per loop and copy what they read into device counters; the others read
+// only those counters, so the host link carries a handful of polls.
+constexpr uint32_t kResPollers = 4;
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void resident_kernel(ResidentArgs a) {
+    extern __shared__ __align__(8) uint32_t lds[];
+    trace_lds_clear<CPD_ACC_EXACT>(lds, a.hist_words, a.row_words);
+    const uint32_t lane = threadIdx.x;
+    const bool poller = blockIdx.x < kResPollers;
+    unsigned long long ops = 0;
+    uint32_t idle = 0;
+    for (;;) {
+        if (poller && lane == 0) {
+            const unsigned int stop = __hip_atomic_load(&a.host->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            const unsigned long long hp =
+                __hip_atomic_load(&a.host->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_max(a.ctrl + kResPublished, hp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(a.ctrl + kResStop, (unsigned long long)stop, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        unsigned long long pub = 0, stop = 0;
+        if (lane == 0) {
+            pub = agent_load(a.ctrl + kResPublished);
+            stop = __hip_atomic_load(a.ctrl + kResStop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        pub = bcast64(pub);
+        if (bcast64(stop) >= a.epoch) break;
+        // 1. ingest the oldest chunk nobody has taken
+        unsigned long long took = 0;
+        if (lane == 0) {
+            unsigned long long ing = agent_load(a.ctrl + kResIngest);
+            if (ing < pub &&
+                __hip_atomic_compare_exchange_strong(a.ctrl + kResIngest, &ing, ing + 1, __ATOMIC_ACQ_REL,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                took = ing + 1;
+        }
+        took = bcast64(took);
+        if (took) {
+            resident_ingest(a, took - 1);
+            idle = 0;
+            continue;
+        }
+        // 2. claim an item of the head chunk
+        // state: 0 nothing, 1 retry now, 2 item (chunk position pos, item x)
+        unsigned long long st = 0, pos_x = 0;
+        if (lane == 0) {
+            unsigned long long h = agent_load(a.ctrl + kResHead);
+            if (h < pub) {
+                const uint32_t pos = (uint32_t)(h % a.chunk_cap);
+                const unsigned long long ing = agent_load(a.ingested + pos);
+                if (ing > h + 1) {
+                    // the position holds a later chunk: chunk h is complete
+                    __hip_atomic_compare_exchange_strong(a.ctrl + kResHead, &h, h + 1, __ATOMIC_ACQ_REL,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    st = 1;
+                } else if (ing == h + 1) {
+                    const unsigned long long w = __hip_atomic_fetch_add(a.claim_word + pos, 1ull, __ATOMIC_ACQ_REL,
+                                                                        __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long x = w & kResCountMask, total = (w >> kResCountBits) & kResCountMask;
+                    if (x < total) {
+                        st = 2;
+                        pos_x = ((unsigned long long)pos << 32) | x;
+                    } else {
+                        if ((w >> 40) == (h & 0xffffffull))
+                            __hip_atomic_compare_exchange_strong(a.ctrl + kResHead, &h, h + 1, __ATOMIC_ACQ_REL,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        st = 1;
+                    }
+                }
+                // ing < h + 1: chunk h is being ingested
+            }
+        }
+        st = bcast64(st);
+        if (st == 1) {
+            idle = 0;
+            continue;
+        }
+        if (st == 0) {
+            // nothing to claim: back off (at most ~30 us between polls), and
+            // exit after idle_limit polls even if the host never asks
+            if (++idle > a.idle_limit) break;
+            for (uint32_t k = 0, n = idle < 8 ? idle : 8; k < n; ++k) __builtin_amdgcn_s_sleep(127);
+            continue;
+        }
+        idle = 0;
+        pos_x = bcast64(pos_x);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the chunk's tables and records
+        const uint32_t pos = (uint32_t)(pos_x >> 32), x = (uint32_t)pos_x;
+        const uint32_t *pre = a.prefix + (size_t)pos * 65;
+        const uint32_t n = a.chunk_n[pos];
+        // the design holding item x: the number of designs whose items end at or before x
+        const uint64_t before = __builtin_amdgcn_ballot_w64(lane < n && pre[lane + 1] <= x);
+        const uint32_t q = (uint32_t)__builtin_popcountll(before);
+        const uint32_t slot = a.chunk_slot0[pos] + q;
+        if (q >= n || slot >= a.max_designs) {
+            if (lane == 0) a.host->error = 3;
+            continue;
+        }
+        const cpd_design *g = a.designs + slot;
+        const TraceDesign *td = a.traces + slot;
+        const uint32_t local = x - pre[q];
+        const uint32_t j = local / td->n_tiles, tile = local - j * td->n_tiles;
+        if (j < td->n_lambda) {
+            run_item<false, CPD_ACC_EXACT>(a, lds, g, td, j, tile, lambda_of(*g, qmc_u64(j + 1)), a.acc,
+                                           a.acc_begin[slot] + (uint64_t)j * td->nbin,
+                                           a.last_bin + a.lb_begin[slot] + j, ops);
+        } else if (lane == 0) {
+            a.host->error = 2;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this item's sums and last bin
+        unsigned long long last = 0;
+        if (lane == 0) {
+            const unsigned int done =
+                __hip_atomic_fetch_add(a.items_done + slot, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+            last = done == a.items_total[slot];
+        }
+        if (bcast64(last)) resident_finalize(a, slot);
     }
 }
 
@@ -713,6 +985,20 @@ hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a, int acc) {
         case CPD_ACC_F64: hipLaunchKernelGGL(finalize_kernel<CPD_ACC_F64>, dim3(a.n_designs), dim3(64), 0, s, a); break;
         default: hipLaunchKernelGGL(finalize_kernel<CPD_ACC_EXACT>, dim3(a.n_designs), dim3(64), 0, s, a); break;
     }
+    return hipGetLastError();
+}
+
+size_t resident_lds_bytes(uint32_t hist_words, uint32_t row_words) {
+    return trace_lds_bytes(hist_words, row_words, CPD_ACC_EXACT);
+}
+
+hipError_t resident_occupancy(size_t lds, int *per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, resident_kernel, 64, lds);
+}
+
+hipError_t launch_resident(hipStream_t s, const ResidentArgs &a, uint32_t grid) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(resident_kernel, dim3(grid), dim3(64), resident_lds_bytes(a.hist_words, a.row_words), s, a);
     return hipGetLastError();
 }
 

@@ -1,15 +1,22 @@
 // server.hip -- continuous batching of fitness requests (cpd_server_* in cpd.h).
 //
-// A scheduling thread owns a HIP stream and three device pools: design
-// records (one slot per design), u64 accumulators and last-ray bins.  Requests
-// are admitted first come, first served while their pool regions fit.  Each
-// step gives every admitted request an equal share of the step's work items
-// (leftovers go to the oldest), launches one persistent trace over all the
-// shares, finalizes the requests whose last item ran, copies their results
-// back and frees their regions.  A request arriving during a step is admitted
-// before the next one, so its work runs alongside the requests already in
-// flight.  Accumulation is exact integer arithmetic, so a design's result does
-// not depend on which requests share its steps.
+// A host thread owns a HIP stream and three device pools: design records (one
+// slot per design), u64 accumulators and last-ray bins.  Requests are
+// admitted first come, first served while their pool regions fit.
+// Accumulation is exact integer arithmetic, so a design's result does not
+// depend on which requests share the GPU with it.
+//
+// Steps mode: each step gives every admitted request an equal share of the
+// step's work items (leftovers go to the oldest), launches one persistent
+// trace over all the shares, finalizes the requests whose last item ran,
+// copies their results back and frees their regions.  A request arriving
+// during a step is admitted before the next one, so its work runs alongside
+// the requests already in flight.
+//
+// Resident mode: the thread stages admitted requests in mapped host memory,
+// publishes them in chunks, keeps the resident loop (resident_kernel) running
+// while any request is open, and collects results as the loop writes their
+// tags; it launches nothing per request.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -100,6 +107,9 @@ struct Request {
     uint64_t cursor_item = 0;         // next item of that design
     uint64_t slot0 = 0, acc0 = 0, lb0 = 0, acc_n = 0, lb_n = 0;
     uint32_t max_rays = 0;
+    uint32_t tag = 0;                 // resident mode: written per slot when a design is done
+    uint64_t chunk0 = 0, n_chunks = 0;  // resident mode: its chunk numbers
+    size_t checked = 0;               // resident mode: designs known done
     bool finished = false;
     cpd_status error = CPD_OK;
     Clock::time_point t_submit, t_done;
@@ -135,9 +145,33 @@ struct Server {
     cpd_server_stats stats{};
     std::thread worker;
 
+    // resident mode
+    bool resident = false;
+    uint32_t max_bins = 0, max_rays_cap = 0, chunk_cap = 0, grid = 0, hist_words = 0;
+    uint64_t next_chunk = 0;
+    std::vector<uint8_t> chunk_busy;  // ring positions held by open requests
+    DeviceBuffer d_items_total, d_items_done, d_slot_tag, d_ingested, d_claim, d_chunk_slot0, d_chunk_n, d_prefix,
+        d_ctrl;
+    void *h_mapped = nullptr;         // control block | chunks | staged designs | results (mapped host memory)
+    ResidentHost *h_ctrl = nullptr;
+    ResidentChunk *h_chunks = nullptr;
+    StagedDesign *h_staged = nullptr;
+    ResidentResult *h_res = nullptr;
+    ResidentArgs rargs{};
+    hipEvent_t ev_loop = nullptr;
+    bool loop_running = false;
+    Clock::time_point loop_t0;
+
     cpd_status init(Context *ctx, const cpd_server_opts *o);
+    cpd_status init_resident(const cpd_server_opts *o);
     void run();
     cpd_status step();
+    cpd_status run_resident();
+    bool admit_resident(Request &r);
+    void publish(Request &r);
+    cpd_status ensure_loop();
+    bool collect_done(Request &r);
+    static void chunk_split(const Request &r, std::vector<std::pair<uint32_t, uint32_t>> *out);
     bool admit(Request &r);
     void release(Request &r);
     cpd_status upload(const std::vector<std::shared_ptr<Request>> &fresh);
@@ -177,7 +211,256 @@ cpd_status Server::init(Context *ctx, const cpd_server_opts *o) {
     slots.reset(max_designs);
     accs.reset(acc_cap);
     lbs.reset(lb_cap);
+    if (o && o->mode == CPD_SERVER_RESIDENT) {
+        if ((st = init_resident(o)) != CPD_OK) return st;
+    } else if (o && o->mode != CPD_SERVER_STEPS) {
+        return fail(CPD_ERR_INVALID_ARGUMENT, "unknown server mode %u", o->mode);
+    }
     worker = std::thread([this] { run(); });
+    return CPD_OK;
+}
+
+// Resident mode: ring of chunk positions, per-slot counters, mapped host
+// memory, the beam tables for max_rays rays and the loop's launch shape.
+cpd_status Server::init_resident(const cpd_server_opts *o) {
+    resident = true;
+    max_bins = o->max_bins ? o->max_bins : 256;
+    max_rays_cap = o->max_rays ? o->max_rays : 65536;
+    if (max_bins > CPD_MAX_BINS || max_rays_cap > kMaxTileRays)
+        return fail(CPD_ERR_INVALID_ARGUMENT, "max_bins <= %d and max_rays <= %u", CPD_MAX_BINS, kMaxTileRays);
+    chunk_cap = (uint32_t)std::max<uint64_t>(64, max_designs);
+    chunk_busy.assign(chunk_cap, 0);
+    hist_words = 1;
+    for (uint32_t b = 1; b <= max_bins; ++b) hist_words = std::max(hist_words, hist_cols(b) * b);
+    cpd_status st;
+    const size_t D = max_designs, C = chunk_cap;
+    if ((st = d_items_total.reserve(D * 4, false, stream)) != CPD_OK) return st;
+    if ((st = d_items_done.reserve(D * 4, false, stream)) != CPD_OK) return st;
+    if ((st = d_slot_tag.reserve(D * 4, false, stream)) != CPD_OK) return st;
+    if ((st = d_ingested.reserve(C * 8, true, stream)) != CPD_OK) return st;
+    if ((st = d_claim.reserve(C * 8, true, stream)) != CPD_OK) return st;
+    if ((st = d_chunk_slot0.reserve(C * 4, false, stream)) != CPD_OK) return st;
+    if ((st = d_chunk_n.reserve(C * 4, false, stream)) != CPD_OK) return st;
+    if ((st = d_prefix.reserve(C * 65 * 4, false, stream)) != CPD_OK) return st;
+    if ((st = d_ctrl.reserve(kResCtrlWords * 8, true, stream)) != CPD_OK) return st;
+    if ((st = beam.ensure(max_rays_cap, stream)) != CPD_OK) return st;
+    const size_t off_chunks = 256, off_staged = off_chunks + (C * sizeof(ResidentChunk) + 255) / 256 * 256;
+    const size_t off_res = off_staged + (D * sizeof(StagedDesign) + 255) / 256 * 256;
+    const size_t mapped = off_res + D * sizeof(ResidentResult);
+    SRV_TRY(hipHostMalloc(&h_mapped, mapped, hipHostMallocMapped | hipHostMallocCoherent));
+    memset(h_mapped, 0, mapped);
+    h_ctrl = (ResidentHost *)h_mapped;
+    h_chunks = (ResidentChunk *)((char *)h_mapped + off_chunks);
+    h_staged = (StagedDesign *)((char *)h_mapped + off_staged);
+    h_res = (ResidentResult *)((char *)h_mapped + off_res);
+    void *dev = nullptr;
+    SRV_TRY(hipHostGetDevicePointer(&dev, h_mapped, 0));
+    ResidentArgs &a = rargs;
+    a.host = (ResidentHost *)dev;
+    a.chunks = (const ResidentChunk *)((char *)dev + off_chunks);
+    a.staged = (const StagedDesign *)((char *)dev + off_staged);
+    a.results = (ResidentResult *)((char *)dev + off_res);
+    a.chunk_cap = chunk_cap;
+    a.max_designs = (uint32_t)max_designs;
+    a.designs = d_designs.as<cpd_design>();
+    a.traces = d_traces.as<TraceDesign>();
+    a.acc_begin = d_acc_begin.as<uint64_t>();
+    a.lb_begin = d_lb_begin.as<uint64_t>();
+    a.acc = d_acc.as<unsigned long long>();
+    a.last_bin = d_last_bin.as<uint32_t>();
+    a.items_total = d_items_total.as<unsigned int>();
+    a.items_done = d_items_done.as<unsigned int>();
+    a.slot_tag = d_slot_tag.as<uint32_t>();
+    a.ingested = d_ingested.as<unsigned long long>();
+    a.claim_word = d_claim.as<unsigned long long>();
+    a.chunk_slot0 = d_chunk_slot0.as<uint32_t>();
+    a.chunk_n = d_chunk_n.as<uint32_t>();
+    a.prefix = d_prefix.as<uint32_t>();
+    a.ctrl = d_ctrl.as<unsigned long long>();
+    a.z32 = beam.z32.as<float>();
+    a.zs32 = beam.zs32.as<float>();
+    a.z64 = beam.z64.as<double>();
+    a.zs64 = beam.zs64.as<double>();
+    a.inv = beam.inv.as<uint32_t>();
+    a.max_rays = beam.n;
+    a.hist_words = hist_words;
+    a.row_words = max_bins;
+    // a safety net: the host retires the loop when no request is open; the
+    // wavefronts also exit by themselves after about a second without work
+    a.idle_limit = 1u << 15;
+    a.epoch = 0;
+    const size_t lds = resident_lds_bytes(a.hist_words, a.row_words);
+    int per_cu = 0;
+    SRV_TRY(resident_occupancy(lds, &per_cu));
+    grid = (uint32_t)cu_count * (uint32_t)std::max(1, std::min<int>(per_cu, (int)waves_per_cu));
+    SRV_TRY(hipEventCreateWithFlags(&ev_loop, hipEventDisableTiming));
+    SRV_TRY(hipStreamSynchronize(stream));
+    return CPD_OK;
+}
+
+// Chunks of a request: consecutive designs, at most 64 and fewer than 2^18
+// work items each (the claim word's fields); (first design, count) pairs.
+void Server::chunk_split(const Request &r, std::vector<std::pair<uint32_t, uint32_t>> *out) {
+    out->clear();
+    uint32_t first = 0, n = 0;
+    uint64_t items = 0;
+    for (uint32_t q = 0; q < (uint32_t)r.designs.size(); ++q) {
+        if (n == 64 || (n && items + r.items[q] >= (1u << 18))) {
+            out->emplace_back(first, n);
+            first = q;
+            n = 0;
+            items = 0;
+        }
+        ++n;
+        items += r.items[q];
+    }
+    if (n) out->emplace_back(first, n);
+}
+
+bool Server::admit_resident(Request &r) {
+    std::vector<std::pair<uint32_t, uint32_t>> ch;
+    chunk_split(r, &ch);
+    if (ch.size() > chunk_cap) return false;
+    for (size_t i = 0; i < ch.size(); ++i)
+        if (chunk_busy[(next_chunk + i) % chunk_cap]) return false;
+    if (!admit(r)) return false;
+    r.chunk0 = next_chunk;
+    r.n_chunks = ch.size();
+    for (size_t i = 0; i < ch.size(); ++i) chunk_busy[(next_chunk + i) % chunk_cap] = 1;
+    next_chunk += ch.size();
+    return true;
+}
+
+// Stage an admitted request's designs and chunk descriptors in mapped host
+// memory, then publish its chunks.
+void Server::publish(Request &r) {
+    const size_t n = r.designs.size();
+    uint64_t acc = r.acc0, lb = r.lb0;
+    for (size_t q = 0; q < n; ++q) {
+        StagedDesign &sd = h_staged[r.slot0 + q];
+        sd.design = r.designs[q];
+        sd.trace = r.traces[q];
+        sd.acc_begin = acc;
+        sd.lb_begin = lb;
+        sd.items = (uint32_t)r.items[q];
+        acc += (uint64_t)r.sampling[q].n_wavelengths * r.designs[q].bin_count;
+        lb += r.sampling[q].n_wavelengths;
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> ch;
+    chunk_split(r, &ch);
+    for (size_t i = 0; i < ch.size(); ++i) {
+        const uint64_t c = r.chunk0 + i;
+        ResidentChunk &d = h_chunks[c % chunk_cap];
+        d.slot0 = (uint32_t)(r.slot0 + ch[i].first);
+        d.n = ch[i].second;
+        d.tag = r.tag;
+        d.pad = 0;
+        d.seq1 = c + 1;
+    }
+    __atomic_store_n(&h_ctrl->published, (unsigned long long)(r.chunk0 + r.n_chunks), __ATOMIC_RELEASE);
+}
+
+// Launch the loop unless it is running; true work for it exists whenever a
+// request is open, and the loop exits by itself once it finds none.
+cpd_status Server::ensure_loop() {
+    if (loop_running) {
+        const hipError_t e = hipEventQuery(ev_loop);
+        if (e == hipErrorNotReady) return CPD_OK;
+        SRV_TRY(e);
+        loop_running = false;
+        std::lock_guard<std::mutex> lk(mu);
+        stats.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - loop_t0).count();
+    }
+    rargs.epoch++;
+    SRV_TRY(launch_resident(stream, rargs, grid));
+    SRV_TRY(hipEventRecord(ev_loop, stream));
+    loop_running = true;
+    loop_t0 = Clock::now();
+    std::lock_guard<std::mutex> lk(mu);
+    stats.steps++;
+    return CPD_OK;
+}
+
+// Whether every design of r has its tag in the results (designs finish
+// roughly in order: the scan resumes where it stopped).
+bool Server::collect_done(Request &r) {
+    const size_t n = r.designs.size();
+    while (r.checked < n &&
+           __atomic_load_n(&h_res[r.slot0 + r.checked].tag, __ATOMIC_ACQUIRE) == r.tag)
+        ++r.checked;
+    return r.checked == n;
+}
+
+cpd_status Server::run_resident() {
+    SRV_TRY(hipSetDevice(device));
+    std::vector<std::shared_ptr<Request>> fresh, act;
+    for (;;) {
+        fresh.clear();
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            if (active.empty() && queue.empty() && !stop) {
+                // idle: the loop exits by itself; wake for work or to note its exit
+                cv_work.wait_for(lk, std::chrono::milliseconds(loop_running ? 2 : 50),
+                                 [&] { return stop || !queue.empty(); });
+            }
+            if (stop) break;
+            while (!queue.empty()) {
+                std::shared_ptr<Request> r = queue.front();
+                if (!admit_resident(*r)) break;
+                queue.pop_front();
+                active.push_back(r);
+                fresh.push_back(r);
+            }
+            act = active;
+            stats.queued = (uint32_t)queue.size();
+            stats.admitted = (uint32_t)active.size();
+        }
+        for (auto &r : fresh) publish(*r);
+        if (!act.empty()) {
+            cpd_status st = ensure_loop();
+            if (st != CPD_OK) return st;
+        } else if (loop_running) {
+            // no open request: retire this launch (its wavefronts exit at their next poll)
+            __atomic_store_n(&h_ctrl->stop, rargs.epoch, __ATOMIC_RELEASE);
+            if (hipEventQuery(ev_loop) != hipErrorNotReady) {
+                loop_running = false;
+                std::lock_guard<std::mutex> lk(mu);
+                stats.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - loop_t0).count();
+            }
+        }
+        const unsigned int err = __atomic_load_n(&h_ctrl->error, __ATOMIC_ACQUIRE);
+        if (err) return fail(CPD_ERR_DEVICE, "resident loop found an inconsistent chunk (code %u)", err);
+        bool any_done = false;
+        const Clock::time_point now = Clock::now();
+        for (auto &rp : act) {
+            Request &r = *rp;
+            if (!collect_done(r)) continue;
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t q = 0; q < r.designs.size(); ++q) r.out[r.index[q]] = h_res[r.slot0 + q].fit;
+            r.finished = true;
+            r.t_done = now;
+            release(r);
+            for (uint64_t c = r.chunk0; c < r.chunk0 + r.n_chunks; ++c) chunk_busy[c % chunk_cap] = 0;
+            stats.completed++;
+            stats.designs += r.designs.size();
+            stats.items += r.total_items;
+            any_done = true;
+        }
+        if (any_done) {
+            std::lock_guard<std::mutex> lk(mu);
+            std::vector<std::shared_ptr<Request>> keep;
+            for (const auto &r : active)
+                if (!r->finished) keep.push_back(r);
+            active.swap(keep);
+            cv_done.notify_all();
+        } else if (!act.empty()) {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+    }
+    // stop: the loop's wavefronts exit at their next poll
+    __atomic_store_n(&h_ctrl->stop, 0xffffffffu, __ATOMIC_RELEASE);
+    SRV_TRY(hipStreamSynchronize(stream));
+    loop_running = false;
     return CPD_OK;
 }
 
@@ -404,6 +687,28 @@ cpd_status Server::step() {
 }
 
 void Server::run() {
+    if (resident) {
+        const cpd_status st = run_resident();
+        if (st != CPD_OK) {
+            if (h_ctrl) __atomic_store_n(&h_ctrl->stop, 0xffffffffu, __ATOMIC_RELEASE);
+            (void)hipStreamSynchronize(stream);
+            loop_running = false;
+            std::lock_guard<std::mutex> lk(mu);
+            fatal = st;
+            for (auto &kv : all) {
+                Request &r = *kv.second;
+                if (!r.finished) {
+                    r.finished = true;
+                    r.error = st;
+                    r.t_done = Clock::now();
+                }
+            }
+            queue.clear();
+            active.clear();
+            cv_done.notify_all();
+        }
+        return;
+    }
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -480,6 +785,19 @@ cpd_status cpd_server_submit(cpd_server *srv, const cpd_design *designs, const c
         r->lb_n += sampling[i].n_wavelengths;
         r->max_rays = std::max(r->max_rays, sampling[i].n_rays);
     }
+    if (s->resident) {
+        for (size_t q = 0; q < r->designs.size(); ++q)
+            if (r->designs[q].bin_count > s->max_bins || r->sampling[q].n_rays > s->max_rays_cap ||
+                r->items[q] >= (1u << 18)) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->stats.rejected++;
+                return cpd::fail(CPD_ERR_CAPACITY,
+                                 "resident server: a design has %u bins, %u rays per wavelength, %llu work items "
+                                 "(limits %u, %u, 2^18)",
+                                 r->designs[q].bin_count, r->sampling[q].n_rays, (unsigned long long)r->items[q],
+                                 s->max_bins, s->max_rays_cap);
+            }
+    }
     if (r->designs.size() > s->max_designs || r->acc_n > s->acc_cap || r->lb_n > s->lb_cap) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stats.rejected++;
@@ -490,6 +808,7 @@ cpd_status cpd_server_submit(cpd_server *srv, const cpd_design *designs, const c
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->fatal != CPD_OK) return cpd::fail(s->fatal, "server stopped after a device error");
     r->id = s->next_id++;
+    r->tag = (uint32_t)((r->id - 1) % 0xffffffffull) + 1;  // never 0, the results' initial tag
     *request_id = r->id;
     s->all[r->id] = r;
     s->stats.submitted++;
@@ -563,8 +882,12 @@ void cpd_server_destroy(cpd_server *srv) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (cpd::DeviceBuffer *b : {&s->d_designs, &s->d_traces, &s->d_acc_begin, &s->d_lb_begin, &s->d_acc,
-                                 &s->d_last_bin, &s->d_results, &s->d_step, &s->d_counter})
+                                 &s->d_last_bin, &s->d_results, &s->d_step, &s->d_counter, &s->d_items_total,
+                                 &s->d_items_done, &s->d_slot_tag, &s->d_ingested, &s->d_claim, &s->d_chunk_slot0,
+                                 &s->d_chunk_n, &s->d_prefix, &s->d_ctrl})
         b->release();
+    if (s->h_mapped) (void)hipHostFree(s->h_mapped);
+    if (s->ev_loop) (void)hipEventDestroy(s->ev_loop);
     s->beam.release();
     s->h_stage.release();
     s->h_results.release();

@@ -449,22 +449,44 @@ void cpd_plan_destroy(cpd_plan *plan);
 
 /* No reference counterpart: the reference evaluates one individual per call
  * (designer.py:267-300).  A server owns a HIP stream and device pools on its
- * context's GPU and a scheduling thread.  Submitted requests (batches of
- * designs) are admitted first come, first served while their records and
- * accumulators fit the memory budget.  Every scheduling step is one
- * persistent launch over a share of the remaining work of each admitted
- * request, so requests that arrive while others run join the next step and
- * their work overlaps on the device.  A request completes when its last work
- * item has run; its results are the same as from cpd_fitness_batch. */
+ * context's GPU and a host thread.  Submitted requests (batches of designs)
+ * are admitted first come, first served while their records and accumulators
+ * fit the memory budget.  A request completes when its last work item has
+ * run; its results are the same as from cpd_fitness_batch, bit for bit, in
+ * either mode.
+ *
+ * CPD_SERVER_STEPS: every scheduling step is one persistent launch over a
+ * share of the remaining work of each admitted request, so requests that
+ * arrive while others run join the next step and their work overlaps on the
+ * device.
+ *
+ * CPD_SERVER_RESIDENT: the host stages each admitted request in mapped host
+ * memory and publishes it in chunks of up to 64 designs; a persistent kernel
+ * stays on the GPU while there is work: its wavefronts copy published chunks
+ * into the device pools, claim work items from the oldest chunk first, and
+ * the wavefront that finishes a design's last item reduces it and writes the
+ * result back to host memory, with no launch or copy per request.  The loop
+ * exits after a few milliseconds without work and is launched again when work
+ * arrives.  Designs must have at most max_bins bins and max_rays rays per
+ * wavelength (fixed-count sampling); submit fails with CPD_ERR_CAPACITY
+ * otherwise. */
+enum {
+    CPD_SERVER_STEPS = 0,      /* a host thread launches one persistent trace per scheduling step */
+    CPD_SERVER_RESIDENT = 1    /* a device-resident loop admits, traces and finalizes requests */
+};
 typedef struct cpd_server_opts {
     uint64_t memory_budget;    /* device bytes for admitted requests (0: 4 GiB) */
-    uint32_t step_items;       /* work items per step, a cap (0: 64 per resident wavefront) */
+    uint32_t step_items;       /* work items per step, a cap (0: 64 per resident wavefront; steps mode) */
     uint32_t max_designs;      /* design slots (0: 65536) */
+    uint32_t mode;             /* CPD_SERVER_* */
+    uint32_t max_bins;         /* resident mode: largest bin count of a design (0: 256) */
+    uint32_t max_rays;         /* resident mode: largest rays per wavelength (0: 65536) */
+    uint32_t reserved;
 } cpd_server_opts;
 
 typedef struct cpd_server_stats {
     uint64_t submitted, completed, rejected;
-    uint64_t steps, items, designs;
+    uint64_t steps, items, designs;   /* steps: scheduling steps, or launches of the resident loop */
     uint32_t queued, admitted;
     double busy_ms;            /* time with work on the device */
 } cpd_server_stats;

@@ -127,11 +127,12 @@ def test_short_rows_claimed_several_at_a_time():
 
 
 def test_propagation_matches_oracle_per_ray():
-    # src/cuda_fitness.rs:533-607: 1024 seeded (wavelength cdf, ray cdf) pairs on the known design
+    # src/cuda_fitness.rs:533-607: the reference harness's 1024 (wavelength cdf,
+    # ray cdf) pairs (Xoshiro256** seed_from_u64(0), Standard f32) on the known design
+    from xoshiro import harness_pairs
+
     d = kat_design()
-    rng = np.random.default_rng(0)
-    wcdf = rng.random(1024, dtype=np.float32)
-    rcdf = rng.random(1024, dtype=np.float32)
+    wcdf, rcdf = harness_pairs(1024)
     bins = np.empty(1024, np.uint32)
     ts = np.empty(1024, np.float32)
     fp = ctypes.POINTER(ctypes.c_float)

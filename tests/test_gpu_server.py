@@ -108,3 +108,72 @@ def test_two_contexts_in_parallel_threads():
     for t in threads:
         t.join()
     assert np.array_equal(np.vstack([out["a"], out["b"]]), ref)
+
+
+# ---------------------------------------------------------------------------
+# The device-resident loop (resident=True): the same results, bit for bit.
+
+def test_resident_server_matches_standalone_batches():
+    reqs = small_requests([40, 7, 120, 1, 60], seed=11)
+    ref = [cpd.fitness_batch(d, s)[0] for d, s in reqs]
+    with cpd.Server(resident=True) as srv:
+        ids = [srv.submit(*reqs[0]), srv.submit(*reqs[1])]
+        time.sleep(0.002)  # the first requests are on the GPU when the rest arrive
+        ids += [srv.submit(*r) for r in reqs[2:]]
+        got = [srv.wait(i, 60000) for i in ids]
+        stats = srv.stats()
+    for (res, status, lat), want in zip(got, ref):
+        assert (status == 0).all()
+        assert np.array_equal(res, want)
+        assert lat > 0
+    assert stats["completed"] == len(reqs) and stats["designs"] == 228 and stats["steps"] >= 1
+
+
+def test_resident_server_relaunches_after_retiring():
+    # with no open request the loop retires; a later request launches it again
+    reqs = small_requests([33, 5, 70], seed=12)
+    with cpd.Server(resident=True) as srv:
+        for r in reqs:
+            res, status, _ = srv.wait(srv.submit(*r), 60000)
+            assert (status == 0).all() and np.array_equal(res, cpd.fitness_batch(*r)[0])
+            time.sleep(0.05)
+        assert srv.stats()["steps"] >= 2
+
+
+def test_resident_server_concurrent_submitters_and_slot_reuse():
+    # 64 design slots: requests queue for slots, slots and chunk positions are
+    # reused, and four threads submit while the loop runs
+    import threading
+    reqs = small_requests(list(np.random.default_rng(13).integers(1, 49, 40)), seed=13)
+    ref = [cpd.fitness_batch(d, s)[0] for d, s in reqs]
+    got = [None] * len(reqs)
+    with cpd.Server(resident=True, max_designs=64) as srv:
+        def worker(w):
+            for i in range(w, len(reqs), 4):
+                got[i] = srv.wait(srv.submit(*reqs[i]), 60000)
+        threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        stats = srv.stats()
+    for (res, status, _), want in zip(got, ref):
+        assert (status == 0).all()
+        assert np.array_equal(res, want)
+    assert stats["completed"] == len(reqs)
+
+
+def test_resident_server_limits():
+    (designs, samp), = small_requests([6], seed=14)
+    with cpd.Server(resident=True, max_bins=32, max_rays=4096) as srv:
+        big_bins = [d for d in designs if d.bin_count > 32][:1]
+        if big_bins:
+            with pytest.raises(cpd.CudaError):
+                srv.submit(big_bins, [N.Sampling(4, 1024, N.MODE_FIXED, 0)])
+        with pytest.raises(cpd.CudaError):
+            srv.submit(designs[:1], [N.Sampling(4, 8192, N.MODE_FIXED, 0)])
+        small = [d for d in designs if d.bin_count <= 32]
+        if small:
+            s = [N.Sampling(4, 4096, N.MODE_FIXED, 0)] * len(small)
+            res, status, _ = srv.wait(srv.submit(small, s), 60000)
+            assert np.array_equal(res, cpd.fitness_batch(small, s)[0])

@@ -153,6 +153,29 @@ def test_givens_sellmeier_fit_matches_curve_fit_and_householder():
     np.testing.assert_allclose([cpd.Glass("g", list(out))(w) for w in ws], exact, rtol=0, atol=1e-9)
 
 
+def test_harness_generator_matches_published_outputs():
+    # xoshiro256** from state (1, 2, 3, 4): the outputs of the authors' C
+    # reference (xoshiro.di.unimi.it/xoshiro256starstar.c); SplitMix64 from
+    # state 0: its well-known first output; the harness draws in [0, 1)
+    from xoshiro import Xoshiro256StarStar, harness_pairs, splitmix64
+
+    r = Xoshiro256StarStar([1, 2, 3, 4])
+    assert [r.next_u64() for _ in range(10)] == [
+        11520, 0, 1509978240, 1215971899390074240, 1216172134540287360, 607988272756665600,
+        16172922978634559625, 8476171486693032832, 10595114339597558777, 2904607092377533576]
+    assert splitmix64(0)[1] == 0xE220A8397B1DCDAF
+    s = Xoshiro256StarStar.seed_from_u64(0).s
+    st, words = 0, []
+    for _ in range(4):
+        st, z = splitmix64(st)
+        words.append(z)
+    assert s == words
+    w, q = harness_pairs(1024)
+    assert w.dtype == np.float32 and ((w >= 0) & (w < 1)).all() and ((q >= 0) & (q < 1)).all()
+    assert np.all(np.round(w * (1 << 24)) == w * (1 << 24))  # 24-bit mantissas
+    assert abs(w.mean() - 0.5) < 0.05 and abs(q.mean() - 0.5) < 0.05
+
+
 def test_generator_is_deterministic_and_in_range():
     for cfg in (0, 2, 4):
         for i in (0, 7):
