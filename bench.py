@@ -82,11 +82,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed, step_items=0):
+def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed, step_items=0, resident=False):
     """Poisson arrivals at `rate` req/s for `duration` s; each request holds
     U{100..2000} designs drawn from the pool.  Returns latency percentiles and
     the served throughput (designs completed / time from first arrival to last
-    completion)."""
+    completion).  resident: the device-resident loop, else host-launched
+    scheduling steps."""
     rng = np.random.default_rng(seed)
     arrivals, t = [], 0.0
     while True:
@@ -102,7 +103,7 @@ def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed, step_items=
     rec_s = np.frombuffer(ps, dtype=np.uint8).reshape(len(pool_s), ctypes.sizeof(N.Sampling))
     reqs = [((N.Design * len(ix)).from_buffer_copy(rec_d[ix].tobytes()),
              (N.Sampling * len(ix)).from_buffer_copy(rec_s[ix].tobytes())) for ix in picks]
-    srv = cpd.Server(ctx, step_items=step_items)
+    srv = cpd.Server(ctx, step_items=step_items, resident=resident, max_bins=128, max_rays=65536)
     ids = []
     t0 = time.perf_counter()
     for at, (darr, sarr), n in zip(arrivals, reqs, sizes):
@@ -123,10 +124,12 @@ def serve_run(cpd, N, np, ctx, pool_d, pool_s, rate, duration, seed, step_items=
     stats = srv.stats()
     srv.close()
     designs = int(sizes.sum())
-    return {"rate_req_s": rate, "arrival_window_s": duration, "requests": len(ids), "designs": designs,
+    return {"mode": "resident" if resident else "steps", "rate_req_s": rate, "arrival_window_s": duration,
+            "requests": len(ids), "designs": designs,
             "p50_ms": float(np.percentile(lat, 50)) if lat else None,
             "p99_ms": float(np.percentile(lat, 99)) if lat else None,
-            "designs_per_s": designs / (t_end - t0), "steps": stats["steps"]}
+            "designs_per_s": designs / (t_end - t0), "launches": stats["steps"],
+            "busy_ms": stats["busy_ms"]}
 
 
 def uneven_run(cpd, N, np, ctx, args):
@@ -593,6 +596,8 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="20,50,100,200",
                     help="offered req/s of the served phase, comma-separated (the served configuration; '' skips)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="seconds of Poisson arrivals per rate")
+    ap.add_argument("--serve-modes", default="resident,steps",
+                    help="server modes of the served phase: resident (device-resident loop), steps (host-launched)")
     ap.add_argument("--transmitting", type=int, default=1024,
                     help="designs of the transmitting line (probe hit fraction >= 0.1; 0: skip)")
     ap.add_argument("--stress-short", type=int, default=16,
@@ -628,8 +633,8 @@ def main() -> int:
         print(json.dumps({"copy_bytes": src.numel() * 4, "copies": 3}), flush=True)
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (designs shard across GPUs "
-              "with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (designs would shard across GPUs "
+              "with no exchange). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -802,11 +807,14 @@ def main() -> int:
         log(f"serving pool: 2048 uneven designs in {time.time() - t:.1f}s")
         window = args.serve_seconds
         serve = []
-        for q, r in enumerate(rates):
-            res = serve_run(cpd, N, np, ctx, pool_d, pool_s, r, window, 0x5E12 + q)
-            log(f"served {r:g} req/s: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
-                f"{res['designs_per_s']:.0f} designs/s over {res['requests']} requests")
-            serve.append(res)
+        for mode in [m.strip() for m in args.serve_modes.split(",") if m.strip()]:
+            for q, r in enumerate(rates):
+                # the same arrivals and requests for both modes
+                res = serve_run(cpd, N, np, ctx, pool_d, pool_s, r, window, 0x5E12 + q, resident=mode == "resident")
+                log(f"served ({mode}) {r:g} req/s: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
+                    f"{res['designs_per_s']:.0f} designs/s over {res['requests']} requests, "
+                    f"{res['launches']} launches")
+                serve.append(res)
 
     lat_sorted = sorted(lat)
     line = {

@@ -104,8 +104,10 @@ struct ResidentResult {                // per slot, host memory
 };
 
 // device counters: chunks taken for ingest, the head chunk claims come from,
-// the newest `published` and stop epoch the polling wavefronts have read
+// the newest `published` and stop epoch the polling wavefronts have read;
+// counter k at ctrl[k * kResLine] (a 128-byte line each)
 enum { kResIngest = 0, kResHead = 1, kResPublished = 2, kResStop = 3, kResCtrlWords = 4 };
+static constexpr uint32_t kResLine = 16;
 
 struct ResidentArgs {
     ResidentHost *host;
@@ -166,6 +168,10 @@ CPD_HD uint32_t trace_chain_offset(uint32_t hist_words, uint32_t row_words, int 
     const uint32_t row = acc == CPD_ACC_F32 ? 4 : 8;
     const uint32_t end = trace_hist_end_words(hist_words, cell);
     return (((end + kQueueCap + 1) & ~1u) * 4 + row_words * row + 7) & ~7u;
+}
+CPD_HD inline size_t trace_lds_bytes_d(uint32_t hist_words, uint32_t row_words, int acc) {
+    return (size_t)trace_chain_offset(hist_words, row_words, acc) + sizeof(Chain64) + sizeof(LensBand64) +
+           (kMaskChunk >> 12) * sizeof(uint64_t) + sizeof(LambdaConsts);
 }
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc);
 

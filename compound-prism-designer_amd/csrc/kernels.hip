@@ -149,6 +149,9 @@ template <> struct Acc<CPD_ACC_F64> {
     static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
 };
 
+// The bin of a design's last ray for one wavelength.
+__device__ __forceinline__ void set_last_bin(uint32_t *p, uint32_t b) { *p = b; }
+
 struct TileCtx {
     const cpd_design *g;
     const TraceDesign *td;
@@ -181,7 +184,7 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
         double t;
         const uint32_t b = trace64_chain(*c.g, *c.chain, y, t);
         if (b < c.nbin) atomicAdd((Cell *)c.hist + b * c.cols + (lane & (c.cols - 1)), Acc<A>::of64(t));
-        if (i == c.last_pos) *c.last_bin = b;
+        if (i == c.last_pos) set_last_bin(c.last_bin, b);
     }
 }
 
@@ -295,7 +298,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
                 if ((it & 63u) == 0) skip = c.masks[it >> 6];
                 if ((skip >> (it & 63u)) & 1u) {
                     // every ray of the band misses: only the last ray's bin is recorded
-                    if (last_pos >= base && last_pos < base + 64u && lane == 0) *c.last_bin = nbin;
+                    if (last_pos >= base && last_pos < base + 64u && lane == 0) set_last_bin(c.last_bin, nbin);
                     continue;
                 }
             }
@@ -312,7 +315,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
             const bool flagged = active && r.guard;
             if (active && !r.guard) {
                 if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
-                if (k == last_pos) *c.last_bin = r.bin;
+                if (k == last_pos) set_last_bin(c.last_bin, r.bin);
             }
             const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
             if (mask) {
@@ -468,7 +471,7 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     if (!c.lc.ok || td->skip) {
         // every ray of this wavelength is lost in the planar faces, or the
         // design is a stand-in for one that failed construction
-        if (c.k_end == M + 1 && lane == 0) *c.last_bin = nbin;
+        if (c.k_end == M + 1 && lane == 0) set_last_bin(c.last_bin, nbin);
     } else {
         tile_loop<COUNT, A>(c, ops);
         flush_row<A>(L.hist, L.row32, nbin, cols, acc, acc_j);
@@ -656,6 +659,16 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
 // in device memory; the wavefront that finishes a design's last item
 // finalizes it.  Host memory is read only after a system-scope acquire and
 // written only with plain stores followed by a system-scope release.
+//
+// Ordering: the counters are read and updated with relaxed atomics (an
+// acquire at agent scope invalidates the wavefront's XCD L2, which costs the
+// trace its cached beam tables; with acquiring loads on every claim the loop
+// ran at half the throughput).  Data another wavefront wrote is read after one
+// acquire fence, paired with that writer's release: after a claim (ingest
+// released the chunk's tables and records before its claim word), in
+// finalize (every worker released its sums before counting its item), and
+// in ingest (the host released its records before `published`, which a
+// poller passes on with release once it has acquired it).
 constexpr uint32_t kResCountBits = 20;
 constexpr unsigned long long kResCountMask = (1ull << kResCountBits) - 1;
 
@@ -664,8 +677,10 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long v) {
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
     return ((unsigned long long)hi << 32) | lo;
 }
+// Relaxed: an acquire at agent scope invalidates the L2 of the wavefront's
+// XCD, so the loop fences only where it needs data another wavefront wrote.
 __device__ __forceinline__ unsigned long long agent_load(unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Copies chunk c's staged records (host memory) into the pools, builds its
@@ -761,24 +776,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void re
         if (poller && lane == 0) {
             const unsigned int stop = __hip_atomic_load(&a.host->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             const unsigned long long hp =
-                __hip_atomic_load(&a.host->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_fetch_max(a.ctrl + kResPublished, hp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_max(a.ctrl + kResStop, (unsigned long long)stop, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_load(&a.host->published, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (hp > agent_load(a.ctrl + kResPublished * kResLine)) {
+                // new chunks: acquire the host's release, then pass it on
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+                __hip_atomic_fetch_max(a.ctrl + kResPublished * kResLine, hp, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (stop > __hip_atomic_load(a.ctrl + kResStop * kResLine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                __hip_atomic_fetch_max(a.ctrl + kResStop * kResLine, (unsigned long long)stop, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
         }
         unsigned long long pub = 0, stop = 0;
         if (lane == 0) {
-            pub = agent_load(a.ctrl + kResPublished);
-            stop = __hip_atomic_load(a.ctrl + kResStop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pub = agent_load(a.ctrl + kResPublished * kResLine);
+            stop = __hip_atomic_load(a.ctrl + kResStop * kResLine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         pub = bcast64(pub);
         if (bcast64(stop) >= a.epoch) break;
         // 1. ingest the oldest chunk nobody has taken
         unsigned long long took = 0;
         if (lane == 0) {
-            unsigned long long ing = agent_load(a.ctrl + kResIngest);
+            unsigned long long ing = agent_load(a.ctrl + kResIngest * kResLine);
             if (ing < pub &&
-                __hip_atomic_compare_exchange_strong(a.ctrl + kResIngest, &ing, ing + 1, __ATOMIC_ACQ_REL,
+                __hip_atomic_compare_exchange_strong(a.ctrl + kResIngest * kResLine, &ing, ing + 1, __ATOMIC_RELAXED,
                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 took = ing + 1;
         }
@@ -792,17 +813,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void re
         // state: 0 nothing, 1 retry now, 2 item (chunk position pos, item x)
         unsigned long long st = 0, pos_x = 0;
         if (lane == 0) {
-            unsigned long long h = agent_load(a.ctrl + kResHead);
+            unsigned long long h = agent_load(a.ctrl + kResHead * kResLine);
             if (h < pub) {
                 const uint32_t pos = (uint32_t)(h % a.chunk_cap);
                 const unsigned long long ing = agent_load(a.ingested + pos);
                 if (ing > h + 1) {
                     // the position holds a later chunk: chunk h is complete
-                    __hip_atomic_compare_exchange_strong(a.ctrl + kResHead, &h, h + 1, __ATOMIC_ACQ_REL,
+                    __hip_atomic_compare_exchange_strong(a.ctrl + kResHead * kResLine, &h, h + 1, __ATOMIC_RELAXED,
                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     st = 1;
                 } else if (ing == h + 1) {
-                    const unsigned long long w = __hip_atomic_fetch_add(a.claim_word + pos, 1ull, __ATOMIC_ACQ_REL,
+                    const unsigned long long w = __hip_atomic_fetch_add(a.claim_word + pos, 1ull, __ATOMIC_RELAXED,
                                                                         __HIP_MEMORY_SCOPE_AGENT);
                     const unsigned long long x = w & kResCountMask, total = (w >> kResCountBits) & kResCountMask;
                     if (x < total) {
@@ -810,7 +831,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void re
                         pos_x = ((unsigned long long)pos << 32) | x;
                     } else {
                         if ((w >> 40) == (h & 0xffffffull))
-                            __hip_atomic_compare_exchange_strong(a.ctrl + kResHead, &h, h + 1, __ATOMIC_ACQ_REL,
+                            __hip_atomic_compare_exchange_strong(a.ctrl + kResHead * kResLine, &h, h + 1, __ATOMIC_RELAXED,
                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         st = 1;
                     }
@@ -859,7 +880,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void re
         unsigned long long last = 0;
         if (lane == 0) {
             const unsigned int done =
-                __hip_atomic_fetch_add(a.items_done + slot, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+                __hip_atomic_fetch_add(a.items_done + slot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
             last = done == a.items_total[slot];
         }
         if (bcast64(last)) resident_finalize(a, slot);
@@ -966,8 +987,7 @@ hipError_t launch_trace_c(hipStream_t s, const BatchArgs &a, uint32_t grid, int 
 }  // namespace
 
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc) {
-    return (size_t)trace_chain_offset(hist_words, row_words, acc) + sizeof(Chain64) + sizeof(LensBand64) +
-           (kMaskChunk >> 12) * sizeof(uint64_t) + sizeof(LambdaConsts);
+    return trace_lds_bytes_d(hist_words, row_words, acc);
 }
 
 hipError_t launch_trace(hipStream_t s, const BatchArgs &a, uint32_t grid, bool count_ops, int acc, hipEvent_t start,

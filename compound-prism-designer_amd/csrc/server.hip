@@ -242,7 +242,7 @@ cpd_status Server::init_resident(const cpd_server_opts *o) {
     if ((st = d_chunk_slot0.reserve(C * 4, false, stream)) != CPD_OK) return st;
     if ((st = d_chunk_n.reserve(C * 4, false, stream)) != CPD_OK) return st;
     if ((st = d_prefix.reserve(C * 65 * 4, false, stream)) != CPD_OK) return st;
-    if ((st = d_ctrl.reserve(kResCtrlWords * 8, true, stream)) != CPD_OK) return st;
+    if ((st = d_ctrl.reserve(kResCtrlWords * kResLine * 8, true, stream)) != CPD_OK) return st;
     if ((st = beam.ensure(max_rays_cap, stream)) != CPD_OK) return st;
     const size_t off_chunks = 256, off_staged = off_chunks + (C * sizeof(ResidentChunk) + 255) / 256 * 256;
     const size_t off_res = off_staged + (D * sizeof(StagedDesign) + 255) / 256 * 256;

@@ -177,3 +177,34 @@ def test_resident_server_limits():
             s = [N.Sampling(4, 4096, N.MODE_FIXED, 0)] * len(small)
             res, status, _ = srv.wait(srv.submit(small, s), 60000)
             assert np.array_equal(res, cpd.fitness_batch(small, s)[0])
+
+
+def test_resident_server_ring_laps():
+    # 32 design slots and 64 chunk positions: 240 small requests from three
+    # threads wrap the slot pool and the chunk ring many times over; every
+    # result equals the standalone batch of its designs
+    import threading
+    sizes = list(np.random.default_rng(15).integers(1, 21, 240))
+    reqs = small_requests(sizes, seed=15)
+    reqs = [(d, [N.Sampling(s.n_wavelengths, min(s.n_rays, 2048), N.MODE_FIXED, 0) for s in ss]) for d, ss in reqs]
+    got = [None] * len(reqs)
+    with cpd.Server(resident=True, max_designs=32) as srv:
+        def worker(w):
+            for i in range(w, len(reqs), 3):
+                got[i] = srv.wait(srv.submit(*reqs[i]), 60000)
+        threads = [threading.Thread(target=worker, args=(w,)) for w in range(3)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        stats = srv.stats()
+    flat_d = [d for r in reqs for d in r[0]]
+    flat_s = [s for r in reqs for s in r[1]]
+    ref, rst = cpd.fitness_batch(flat_d, flat_s)
+    assert (rst == 0).all()
+    k = 0
+    for (res, status, _), (d, _s) in zip(got, reqs):
+        assert (status == 0).all()
+        assert np.array_equal(res, ref[k:k + len(d)])
+        k += len(d)
+    assert stats["completed"] == len(reqs)
