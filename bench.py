@@ -47,8 +47,9 @@ METRIC = "designs/sec (batched vs plain GPU) at ≤1e-4 bit MI error; p50/p99 re
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the batched trace kernel on this workload, from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy's readings
-TRACE_TRAFFIC_BYTES = 15.48e6
-TRACE_TRAFFIC_SOURCE = "profiles/trace_kernel_batch1024_p3.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+TRACE_TRAFFIC_BYTES = 23.78e6
+TRACE_TRAFFIC_SOURCE = ("profiles/trace_kernel_batch1024_p3.md (rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x "
+                        "1.000, factors from a 256 MiB copy)")
 
 
 def reference_ops(designs, samplings):
@@ -614,6 +615,8 @@ def main() -> int:
     ap.add_argument("--toric", type=int, default=256,
                     help="designs of the 3D phase (toric lenses, fiber beam; 0: skip)")
     ap.add_argument("--toric-only", action="store_true", help="only the 3D phase (its JSON object)")
+    ap.add_argument("--transmitting-only", action="store_true",
+                    help="only the transmitting line (its JSON object; for profiling its launches alone)")
     ap.add_argument("--stress", action="store_true",
                     help="only the precision stress: 512 designs (6 prisms, 256 bins, 1024 wavelengths x 2^20 rays) "
                          "under each accumulation policy, MI error against the FP64 oracle on a subset")
@@ -650,6 +653,9 @@ def main() -> int:
         return 0
     if args.toric_only:
         print(json.dumps({"toric3d": toric_run(cpd, N, np, ctx, args), "device": info}), flush=True)
+        return 0
+    if args.transmitting_only:
+        print(json.dumps({"transmitting": transmitting_run(cpd, N, np, ctx, args), "device": info}), flush=True)
         return 0
     n = args.designs
     t0 = time.time()
