@@ -26,14 +26,27 @@ path (src/fitness.rs and src/spectrometer/*, 2D variant).  What pins each part
                                       closed-form hit points and bins
   construction (geometry.rs)          closed-form trapezoid geometry, arc sagitta
   detector fit (geometry.rs:495-542)  both end rays land on the array ends
-  early-exit stop points              parity unpinned (only the KAT pins the
-                                      resulting info to 1%)
+  anti-reflection branch (ray.rs:      T = 0.99 per coated face at normal
+  336-340)                            incidence; the strict ci > 0.5 threshold
+                                      on either side of 60 degrees
+  gap bins, intercept (spectrometer.  a flat geometry where a ray in a gap
+  rs:291-303)                         misses, the intercept shifts the bins and
+                                      a position before it saturates to bin 0
+  early-exit stop points              closed forms: one bin with constant t
+  (fitness.rs:20-28,108-114,          stops at k + 1 > t / e (the phantom zero
+  welford.rs:34-58)                   and skip's post-update mean, which a
+                                      textbook skip would move); two split bins
+                                      in exact decimals; two equal or all-zero
+                                      rows stop the outer loop
   GPU estimator (kernel.rs:89-193,    a row equals the direct mean of t over the
   cuda_fitness.rs:347-423)            rays it folded (brute force per ray); the
+                                      32-lane chunk (count 32, mean, m2) merge
+                                      and its stop on chunks of known bins; the
                                       KAT's info within the bench's 1e-2
-                                      (benches/benchmark.rs:147-157); its stop
-                                      points are parity unpinned (the reference
-                                      folds in FP32 with fast-math)
+                                      (benches/benchmark.rs:147-157).  Unpinned:
+                                      the reference's own FP32 fast-math bits
+                                      (FastFloat is not vendored); this
+                                      restatement folds in FP64
 """
 from __future__ import annotations
 
