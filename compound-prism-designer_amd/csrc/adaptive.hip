@@ -118,6 +118,7 @@ struct Tracer2 {
         const Trace32Result tr = trace32<false>(hot, hot.y_mean - hot.beam_w * z);
         redo = active && tr.guard;
         if (!active) return nbin;
+        if (!tr.hit) return nbin;
         t = tr.t;
         return tr.bin;
     }

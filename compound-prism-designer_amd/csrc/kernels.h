@@ -17,6 +17,10 @@ static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 r
 // Positions of a tile whose bands are tested at once before their rays (one
 // 64-bit skip mask per 4096 positions in LDS).
 static constexpr uint32_t kMaskChunk = 65536;
+// Readable entries past the last ray of the FP32 beam tables (z32, zs32) the
+// trace loop needs: it loads 64 consecutive entries from any position up to
+// the last ray.
+static constexpr uint32_t kBeamPad = 64;
 
 // One launch's work on the device.  Design records live in a pool
 // (designs/traces, indexed by slot); the launch covers n_entries entries,

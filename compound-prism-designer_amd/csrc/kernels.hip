@@ -115,19 +115,22 @@ __device__ __forceinline__ float plog2pf(float p) { return p > 0.f ? p * log2f(p
 //    FP32 compensated entropies.
 //  * f64: FP64 LDS columns and FP64 global atomics, FP64 compensated entropies.
 template <int A> struct Acc;
-// kTScale: the scale trace32's transmittance arrives in (make_hot's tscale).
+// kTScale, kTOff: the scale and offset trace32's transmittance arrives with
+// (make_hot's tscale and toff).
 template <> struct Acc<CPD_ACC_EXACT> {
     using Cell = uint32_t;
     static constexpr uint32_t kRowBytes = 8;
-    static constexpr float kTScale = 0.5f * (float)kQuantScale;  // t arrives as t 2^22: round it
-    static __device__ Cell of32(float t) { return __float2uint_rn(t); }
+    // t arrives as t 2^22 + 1/2 (the fused multiply-add that forms it adds
+    // the half): truncation rounds it to the nearest integer
+    static constexpr float kTScale = 0.5f * (float)kQuantScale, kTOff = 0.5f;
+    static __device__ Cell of32(float t) { return (Cell)t; }
     static __device__ Cell of64(double t) { return quant64(t); }
     static __device__ uint32_t fold_every(uint32_t cols) { return fold_rays(cols); }
 };
 template <> struct Acc<CPD_ACC_F32> {
     using Cell = float;
     static constexpr uint32_t kRowBytes = 4;
-    static constexpr float kTScale = 0.5f;
+    static constexpr float kTScale = 0.5f, kTOff = 0.f;
     static __device__ Cell of32(float t) { return t; }
     static __device__ Cell of64(double t) { return (float)t; }
     static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
@@ -135,7 +138,7 @@ template <> struct Acc<CPD_ACC_F32> {
 template <> struct Acc<CPD_ACC_F32C> {
     using Cell = float;
     static constexpr uint32_t kRowBytes = 8;
-    static constexpr float kTScale = 0.5f;
+    static constexpr float kTScale = 0.5f, kTOff = 0.f;
     static __device__ Cell of32(float t) { return t; }
     static __device__ Cell of64(double t) { return (float)t; }
     static __device__ uint32_t fold_every(uint32_t) { return 1024; }
@@ -143,7 +146,7 @@ template <> struct Acc<CPD_ACC_F32C> {
 template <> struct Acc<CPD_ACC_F64> {
     using Cell = double;
     static constexpr uint32_t kRowBytes = 8;
-    static constexpr float kTScale = 0.5f;
+    static constexpr float kTScale = 0.5f, kTOff = 0.f;
     static __device__ Cell of32(float t) { return (double)t; }
     static __device__ Cell of64(double t) { return t; }
     static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
@@ -252,7 +255,12 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     const uint32_t lane = threadIdx.x;
     const uint32_t col = lane & (c.cols - 1);
     Cell *__restrict__ hist = (Cell *)c.hist;
-    const uint32_t k_end = c.k_end, nbin = c.nbin, cols = c.cols, full_end = c.full_end, last_pos = c.last_pos;
+    const uint32_t k_end = c.k_end, nbin = c.nbin, cols = c.cols, full_end = c.full_end;
+    const uint32_t last_pos = (uint32_t)__builtin_amdgcn_readfirstlane(c.last_pos);
+    // the lane's column of bin b is (b << cell_shift) bytes past hist_col
+    // (cols is a power of two)
+    const uint32_t cell_shift = 31u - __builtin_clz(cols) + (sizeof(Cell) == 8 ? 3u : 2u);
+    char *__restrict__ hist_col = (char *)(hist + col);
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
     const uint32_t fold = Acc<A>::fold_every(cols);
     uint32_t since_fold = 0;
@@ -268,55 +276,58 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         // whose FP64 registers they would otherwise share): the loop stores
         // through last_bin, so loads through c.td would be repeated every
         // iteration
-        HotRecord hot = to_sgprs(make_hot(*c.td, *c.lc_lds, Acc<A>::kTScale));
-        // operands paired with another uniform operand, the miss bin, and the
-        // detector stage's constants (which would otherwise be spilled)
+        HotRecord hot = to_sgprs(make_hot(*c.td, *c.lc_lds, Acc<A>::kTScale, Acc<A>::kTOff));
+        // operands paired with another uniform operand, and the detector
+        // stage's constants (which would otherwise be spilled)
         hot.y_mean = in_vgpr(hot.y_mean);
-        hot.e0x = in_vgpr(hot.e0x);
-        hot.e0y = in_vgpr(hot.e0y);
+        hot.g0 = in_vgpr(hot.g0);
         hot.lb0 = in_vgpr(hot.lb0);
         hot.lc1 = in_vgpr(hot.lc1);
         hot.lomr2 = in_vgpr(hot.lomr2);
         hot.nd0 = in_vgpr(hot.nd0);
+        hot.wdn0 = in_vgpr(hot.wdn0);
+        hot.wdd0 = in_vgpr(hot.wdd0);
+        hot.ndd0 = in_vgpr(hot.ndd0);
+        hot.dlen0 = in_vgpr(hot.dlen0);
+        hot.tcoat = in_vgpr(hot.tcoat);
         hot.nbin = in_vgpr(hot.nbin);
-        hot.ts = in_vgpr(hot.ts);
-        hot.tp = in_vgpr(hot.tp);
-        hot.mpx = in_vgpr(hot.mpx);
-        hot.mpy = in_vgpr(hot.mpy);
-        hot.ddx = in_vgpr(hot.ddx);
-        hot.ddy = in_vgpr(hot.ddy);
         hot.half_len = in_vgpr(hot.half_len);
         hot.slope = in_vgpr(hot.slope);
         hot.inv_slope = in_vgpr(hot.inv_slope);
-        hot.intercept = in_vgpr(hot.intercept);
         hot.bin_size = in_vgpr(hot.bin_size);
-        hot.nbin_f = in_vgpr(hot.nbin_f);
         uint64_t skip = 0;
         for (uint32_t base = chunk; base < chunk_end; base += 64) {
             const uint32_t it = (base - chunk) >> 6;
             if (!COUNT) {
-                if ((it & 63u) == 0) skip = c.masks[it >> 6];
+                // the chunk's masks are wave-uniform: scalar registers
+                if ((it & 63u) == 0) {
+                    const uint64_t m = c.masks[it >> 6];
+                    // (readfirstlane returns int: each half is taken as u32,
+                    // so the low half is not sign-extended into the high one)
+                    skip = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
+                }
                 if ((skip >> (it & 63u)) & 1u) {
                     // every ray of the band misses: only the last ray's bin is recorded
                     if (last_pos >= base && last_pos < base + 64u && lane == 0) set_last_bin(c.last_bin, nbin);
                     continue;
                 }
             }
-            // positions base .. base + 63 are all banded or all in ray order
-            const float *__restrict__ zp = base + 63 <= full_end ? c.zs32 : c.z32;
-            const uint32_t k = base + lane;  // position in the tables
-            const bool active = k < k_end;
-            // a 32-bit byte offset from a scalar base: one load, no 64-bit address
-            // arithmetic per lane (the tables hold at most 2^30 rays)
-            const float z = active ? *(const float *)((const char *)zp + ((k - 1u) << 2)) : 0.f;
+            // positions base .. base + 63 are all banded or all in ray order;
+            // the tables extend 63 entries past their last ray, so every lane
+            // loads from them (lanes at or past k_end only in the tile's last
+            // iteration, whose results they discard)
+            const float *__restrict__ zp = (base + 63 <= full_end ? c.zs32 : c.z32) + (base - 1u);
+            const bool active = lane < k_end - base;  // position base + lane
+            const float z = zp[lane];
             const float y = hot.y_mean - hot.beam_w * z;
             const Trace32Result r = trace32<COUNT>(hot, y);
             if (COUNT && active) ops += r.ops;
             const bool flagged = active && r.guard;
-            if (active && !r.guard) {
-                if (r.bin < nbin) atomicAdd(&hist[r.bin * cols + col], Acc<A>::of32(r.t));
-                if (k == last_pos) set_last_bin(c.last_bin, r.bin);
-            }
+            if (active && !r.guard && r.hit) atomicAdd((Cell *)(hist_col + (r.bin << cell_shift)), Acc<A>::of32(r.t));
+            if (last_pos - base < 64u && lane == last_pos - base && active && !r.guard)
+                set_last_bin(c.last_bin, r.hit ? r.bin : nbin);
+            const uint32_t k = base + lane;
             const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
             if (mask) {
                 if (flagged) c.queue[(tail + __popcll(mask & lanemask_lt())) & (kQueueCap - 1)] = k;
@@ -493,7 +504,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void tr
             if (next == end) {
                 uint32_t v = 0;
                 if (lane == 0) v = atomicAdd(a.work_counter, per_fetch);
-                next = (uint64_t)__builtin_amdgcn_readfirstlane(v);
+                next = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(v);
                 end = next + per_fetch;
             }
             item = next++;  // claims ascend, so the first item past n_items ends the loop
@@ -900,10 +911,11 @@ __global__ void propagate_kernel(const cpd_design *g, const TraceDesign *td, con
     const double lambda = lambda_of(*g, (double)wcdf[i]);
     make_lambda_consts(*g, lambda, lc);
     const double z = norminv64((double)rcdf[i]);
-    Trace32Result r{td->nbin, 0.f, false, 0};
+    Trace32Result r{td->nbin, 0.f, false, false, 0};
     if (lc.ok) {
         const float y = td->y_mean - td->beam_w * (float)z;
         r = trace32<false>(make_hot(*td, lc), y);
+        if (!r.hit) r.bin = td->nbin;
     }
     if (!lc.ok || r.guard) {
         double t64;

@@ -38,6 +38,8 @@ constexpr size_t kStAcc = up(kStItems + (kMaxDesigns + 1) * sizeof(uint64_t));
 constexpr size_t kStLb = up(kStAcc + kMaxDesigns * sizeof(uint64_t));
 constexpr size_t kStageBytes = up(kStLb + kMaxDesigns * sizeof(uint64_t));
 // the scratch array: beam tables | stage | accumulators | last-ray bins | results | counter
+// (the trace loop's reads past the last ray of z32 and zs32, kBeamPad entries
+// at most, land in z64 and zs64, which follow them: values it discards)
 constexpr size_t kOffZ32 = 0;
 constexpr size_t kOffZ64 = up(kOffZ32 + kMaxRays * sizeof(float));
 constexpr size_t kOffZs32 = up(kOffZ64 + kMaxRays * sizeof(double));
@@ -50,6 +52,9 @@ constexpr size_t kOffResults = up(kOffLastBin + kMaxRows * sizeof(uint32_t));
 constexpr size_t kOffCounter = up(kOffResults + kMaxDesigns * sizeof(cpd_fitness));
 constexpr size_t kScratchBytes = up(kOffCounter + 256);
 static_assert(kScratchBytes == CPD_NOALLOC_SCRATCH_BYTES, "cpd.h states the scratch size");
+static_assert(kOffZ32 + (kMaxRays + kBeamPad) * sizeof(float) <= kOffZs32 &&
+                  kOffZs32 + (kMaxRays + kBeamPad) * sizeof(float) <= kOffInv,
+              "the FP32 beam tables' padding reads stay inside the beam tables");
 
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
 

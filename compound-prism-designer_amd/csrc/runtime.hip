@@ -83,13 +83,15 @@ cpd_status BeamTables::ensure(uint32_t want, hipStream_t s) {
     if (want <= n) return CPD_OK;
     const uint32_t m = std::max<uint32_t>(want, 65536);
     cpd_status st;
-    if ((st = z32.reserve((size_t)m * sizeof(float), false, s)) != CPD_OK) return st;
+    // the FP32 tables carry kBeamPad zeros past ray m: the trace loop loads a
+    // whole wavefront's entries from any position up to the last ray
+    if ((st = z32.reserve((size_t)(m + kBeamPad) * sizeof(float), false, s)) != CPD_OK) return st;
     if ((st = z64.reserve((size_t)m * sizeof(double), false, s)) != CPD_OK) return st;
-    if ((st = zs32.reserve((size_t)m * sizeof(float), false, s)) != CPD_OK) return st;
+    if ((st = zs32.reserve((size_t)(m + kBeamPad) * sizeof(float), false, s)) != CPD_OK) return st;
     if ((st = zs64.reserve((size_t)m * sizeof(double), false, s)) != CPD_OK) return st;
     if ((st = inv.reserve((size_t)m * sizeof(uint32_t), false, s)) != CPD_OK) return st;
     std::vector<double> z(m), zs(m);
-    std::vector<float> zf(m), zsf(m);
+    std::vector<float> zf(m + kBeamPad, 0.f), zsf(m + kBeamPad, 0.f);
     std::vector<uint32_t> order(m), pos(m);
     for (uint32_t i = 0; i < m; ++i) {
         z[i] = norminv64(qmc_seeded(kHalfFx, (uint64_t)i + 1));
@@ -105,9 +107,9 @@ cpd_status BeamTables::ensure(uint32_t want, hipStream_t s) {
         pos[order[i]] = i + 1;
     }
     HIP_TRY(hipMemcpyAsync(z64.ptr, z.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(z32.ptr, zf.data(), (size_t)m * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(z32.ptr, zf.data(), (size_t)(m + kBeamPad) * sizeof(float), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(zs64.ptr, zs.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(zs32.ptr, zsf.data(), (size_t)m * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(zs32.ptr, zsf.data(), (size_t)(m + kBeamPad) * sizeof(float), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(inv.ptr, pos.data(), (size_t)m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     n = m;

@@ -307,12 +307,23 @@ struct LambdaConsts {
     float lb0, lb1;            // lens: b(y) = dir.(e - cm) = lb0 + lb1 y
     float lc0, lc1, lc2;       // lens: |e|^2 - 2 e.cm + |cm|^2 - R^2 = lc0 + lc1 y + lc2 y^2
     float dx, dy, rdx, rdy;    // direction into the lens, and times the lens ratio r
+    // arc bounds: on the lens circle |q|^2 - max_dist_sq = 2 q.cm + R^2 -
+    // |cm|^2 - max_dist_sq, affine in the hit point q = e + s dir:
+    // lg0 + lg1 y + lcmd2 s
+    float lg0, lg1, lcmd2;
     float ts, tp;              // Fresnel transmittances through the planar faces
     float amp;                 // angular amplification through the planar faces, >= 1
     float path;                // length scale of the FP32 error budget up to the lens
     float lr, lomr2;           // lens ratio r = n_{P-1}, 1 - r^2
     float lr2r;                // r^2 / R^2 (R the lens radius)
     float nd0, nd1, ddn, rddn; // detector normal dn: (cm - e).dn / R = nd0 + nd1 y, dir.dn / R, r dir.dn
+    // The array, for the lens hit q = e + tq dir and w = q + m - P0 (P0 the
+    // array's start, dd its direction): w.dn = wdn0 + wdn1 y + wdn2 tq;
+    // w.dd - intercept = wdd0 + wdd1 y + wdd2 tq; the lens normal's
+    // n.dd = ndd0 + ndd1 y + ndd2 tq; r dir.dd
+    float wdn0, wdn1, wdn2, wdd0, wdd1, wdd2, ndd0, ndd1, ndd2, rdd;
+    float hc;                  // half the array's length minus the intercept
+    float dlen0, dlen1;        // the detector stage's length scale is dlen0 + dlen1 |tq| + |dist|
     int ok;                    // 0 when every ray is lost in the planar faces (TIR, parallel face)
     double lambda;
 };
@@ -361,7 +372,9 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.inv_r = (float)(1.0 / g.arc_radius);
     t.max_dist_sq = (float)g.arc_max_dist_sq;
     t.lens_scale = (float)(g.arc_radius + sqrt(g.arc_max_dist_sq));
-    t.bound_scale = (float)(4.0 * (sqrt(g.arc_max_dist_sq) + 1.0));
+    t.bound_scale = (float)(2.0 * (sqrt((g.arc_center[0] - g.arc_midpt[0]) * (g.arc_center[0] - g.arc_midpt[0]) +
+                                        (g.arc_center[1] - g.arc_midpt[1]) * (g.arc_center[1] - g.arc_midpt[1])) +
+                                   1.0));
     t.nbin_f = (float)g.bin_count;
     t.skip = 0;
     t.mpx = (float)(g.arc_midpt[0] - g.det_position[0]);
@@ -493,6 +506,10 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.lc0 = (float)(e0x * e0x + e0y * e0y - 2.0 * (e0x * cmx + e0y * cmy) + kappa);
     lc.lc1 = (float)(2.0 * (e0x * bx + e0y * by) - 2.0 * (bx * cmx + by * cmy));
     lc.lc2 = (float)(bx * bx + by * by);
+    lc.lg0 = (float)(2.0 * (e0x * cmx + e0y * cmy) + (g.arc_radius * g.arc_radius - (cmx * cmx + cmy * cmy)) -
+                     g.arc_max_dist_sq);
+    lc.lg1 = (float)(2.0 * (bx * cmx + by * cmy));
+    lc.lcmd2 = (float)(2.0 * (ray.dx * cmx + ray.dy * cmy));
     const double r = n[P - 1];
     lc.dx = (float)ray.dx;
     lc.dy = (float)ray.dy;
@@ -510,6 +527,33 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
     lc.nd1 = (float)(-(bx * dnx + by * dny) / g.arc_radius);
     lc.ddn = (float)((ray.dx * dnx + ray.dy * dny) / g.arc_radius);
     lc.rddn = (float)(r * (ray.dx * dnx + ray.dy * dny));
+    {
+        const double ddx = g.det_direction[0], ddy = g.det_direction[1];
+        const double wx0 = g.arc_midpt[0] - g.det_position[0] + e0x, wy0 = g.arc_midpt[1] - g.det_position[1] + e0y;
+        const double dir_dd = ray.dx * ddx + ray.dy * ddy, e1_dd = bx * ddx + by * ddy;
+        lc.wdn0 = (float)(wx0 * dnx + wy0 * dny);
+        lc.wdn1 = (float)(bx * dnx + by * dny);
+        lc.wdn2 = (float)(ray.dx * dnx + ray.dy * dny);
+        lc.wdd0 = (float)(wx0 * ddx + wy0 * ddy - g.det_intercept);
+        lc.wdd1 = (float)e1_dd;
+        lc.wdd2 = (float)dir_dd;
+        lc.ndd0 = (float)(((cmx - e0x) * ddx + (cmy - e0y) * ddy) / g.arc_radius);
+        lc.ndd1 = (float)(-e1_dd / g.arc_radius);
+        lc.ndd2 = (float)(-dir_dd / g.arc_radius);
+        lc.rdd = (float)(r * dir_dd);
+        lc.hc = (float)(g.det_length * 0.5 - g.det_intercept);
+        // |w|_1 <= |w(y)|_1 + |dir|_1 |tq| with w(y) = w(0) + e1 y, whose
+        // 1-norm is convex in y: its larger value at the FP32 ends of the
+        // accepted heights bounds it for every ray that can reach the array.
+        // With the path to the lens (which holds |e1|_1 h >= |w(0) - w(y)|_1)
+        // and the intercept it bounds the magnitudes the detector stage's
+        // FP32 values are formed from.
+        const double pl = path + fabs(e0x) + fabs(e0y) + (fabs(bx) + fabs(by)) * h;
+        const double ya = (double)lc.ylo, yb = (double)lc.yhi;
+        const double wa = fabs(wx0 + bx * ya) + fabs(wy0 + by * ya), wb = fabs(wx0 + bx * yb) + fabs(wy0 + by * yb);
+        lc.dlen0 = (float)((pl + (wa > wb ? wa : wb) + fabs(g.det_intercept)) * 1.000001);
+        lc.dlen1 = (float)((1.0 + fabs(ray.dx) + fabs(ray.dy)) * 1.000001);
+    }
     if (band) {
         band->lb0 = ray.dx * (e0x - cmx) + ray.dy * (e0y - cmy);
         band->lb1 = ray.dx * bx + ray.dy * by;
@@ -694,8 +738,13 @@ CPD_HD uint32_t trace64_chain(const cpd_design &g, const Chain64 &ch, double y, 
 // FP32 trace with error margins
 // ---------------------------------------------------------------------------
 #if defined(__HIP_DEVICE_COMPILE__)
-// max of two non-NaN values in one instruction (median of a, b and +inf)
-CPD_HD float max2f(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, __builtin_inff()); }
+// max of two non-NaN values in one instruction (fmaxf would first quiet
+// signalling NaNs, one more instruction per operand)
+CPD_HD float max2f(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 CPD_HD float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 CPD_HD float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // Whether any lane of the wavefront holds v (one lane on the host).
@@ -721,8 +770,10 @@ CPD_HD bool wave_any(bool v) { return v; }
 // so no register is set for it).
 #if defined(__clang__)
 #define kAnyFloat __builtin_nondeterministic_value(0.f)
+#define kAnyU32 __builtin_nondeterministic_value(0u)
 #else
 #define kAnyFloat 0.f
+#define kAnyU32 0u
 #endif
 
 // Error budget of an FP32 length per unit of path travelled, and of an FP32
@@ -741,9 +792,10 @@ static constexpr uint32_t kOpsBeam = 2, kOpsChain = 4, kOpsLensHit = 22, kOpsLen
                           kOpsHit = 1;
 
 struct Trace32Result {
-    uint32_t bin;  // nbin on a miss
-    float t;
+    uint32_t bin;  // the ray's bin, with hit
+    float t;       // its transmittance, with hit
     bool guard;    // the outcome is within the FP32 error margin of a decision
+    bool hit;      // the ray lands in a bin
     uint32_t ops;  // algorithmic operations of the stages the ray reached (COUNT)
 };
 
@@ -753,21 +805,29 @@ struct Trace32Result {
 struct alignas(4) HotRecord {
     uint32_t nbin, ar_coated;
     float y_mean, beam_w, ylo, yhi, ymarg;
+    float ymid, yhalf;  // the accepted heights' midpoint and half-width
     float disc_lim;  // margin of the lens discriminant for every accepted height
+    float blim;      // margin of the lens's bounds test
+    float coat_lim;  // margin of the coating's cosine test
     float cd_lim;    // margin of the acceptance cosine, times cos(theta_t), for every ray
-    float e0x, e0y, e1x, e1y, lb0, lb1, lc0, lc1, lc2;
-    float dx, dy, rdx, rdy, ts, tp, amp, path;
-    float num0k;     // kGuardRel amp
-    float cmx, cmy, inv_r, max_dist_sq, bound_scale;
-    float lr2r, lomr2, lr, nd0, nd1, ddn, rddn;
-    float mpx, mpy, dnx, dny, ddx, ddy, min_ci, half_len;
-    float slope, inv_slope, intercept, bin_size, nbin_f;
+    float lb0, lb1, lc0, lc1, lc2;
+    float g0, g1, cmd2;  // arc bounds: |q|^2 - max_dist_sq = g0 + g1 y + cmd2 s at the root s
+    float inv_r;
+    float lr2r, lomr2, lr, nd0, nd1, ddn, rddn, min_ci;
+    float wdn0, wdn1, wdn2, wdd0, wdd1, wdd2, ndd0, ndd1, ndd2, rdd;
+    float num0k;         // kGuardRel A
+    float dlen0, dlen1;  // the detector stage's length scale terms, times num0k
+    float hc, half_len, slope, inv_slope, bin_size;
+    float ts4, tp4;      // 4 tscale Ts, 4 tscale Tp through the planar faces
+    float tcoat;         // t of a ray through the coating: 0.99 tscale (Ts + Tp) + toff
+    float toff;
 };
 
-// tscale multiplies the planar faces' transmittances, so that trace32's t =
-// tscale (Ts + Tp) (0.5: the mean the reference takes; a power of two, so
-// any other scale gives the same bits scaled).
-CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc, float tscale = 0.5f) {
+// tscale multiplies the planar faces' transmittances and toff is added, so
+// that trace32's t = tscale (Ts + Tp) + toff (0.5 and 0: the mean the
+// reference takes; tscale a power of two, so any other scale gives the same
+// bits scaled).
+CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc, float tscale = 0.5f, float toff = 0.f) {
     HotRecord h;
     h.nbin = d.nbin;
     h.ar_coated = d.ar_coated;
@@ -780,38 +840,35 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc, float ts
     // kGuardRel) / (1 - kGuardRel), so kGuardRel (max |y_b| + 1) /
     // (1 - kGuardRel), rounded up, covers every such ray.
     h.ymarg = kGuardRel * (fmaxf(fabsf(lc.ylo), fabsf(lc.yhi)) + 1.f) * (1.f + 2.f * kGuardRel) * 1.000001f;
+    // ylo <= y <= yhi as |y - ymid| <= yhalf: the rounding of ymid, yhalf
+    // and the two differences moves the bounds by a few ulps of max |y_b|,
+    // far inside ymarg
+    h.ymid = (float)(((double)lc.ylo + (double)lc.yhi) * 0.5);
+    h.yhalf = (float)(((double)lc.yhi - (double)lc.ylo) * 0.5);
     // b(y) = lb0 + lb1 y is affine, so its largest magnitude over the accepted
     // heights bounds |b| for every ray the discriminant decides; the margin
     // 4 kGuardRel path A (|b| + lens_scale) at that bound covers each ray's.
     const float bmax = fmaxf(fabsf(fmaf(lc.lb1, lc.ylo, lc.lb0)), fabsf(fmaf(lc.lb1, lc.yhi, lc.lb0))) * 1.000001f;
     h.disc_lim = kGuardRel * lc.path * lc.amp * 4.f * (bmax + d.lens_scale) * 1.000001f;
+    // the arc bounds' value 2 q.cm + const moves by 2 |cm| |dq| for a hit
+    // point error dq of kGuardRel path A (bound_scale = 2 (|cm| + 1)), plus
+    // 8 ulps of the magnitudes its FP32 constants carry
+    h.blim = (kGuardRel * lc.amp * lc.path * d.bound_scale +
+              4.8e-7f * (fabsf(lc.lg0) + fabsf(lc.lg1) * fmaxf(fabsf(lc.ylo), fabsf(lc.yhi)))) * 1.000001f;
+    h.coat_lim = kGuardCos * lc.amp;
     // the acceptance cosine's margin kGuardCos A max(r ci, cr) / cr: with
     // ci, cr <= 1, A max(r, 1) bounds the numerator for every ray (FP32 ci
     // and cr can exceed 1 by ~1e-6, hence the extra 1e-4)
     h.cd_lim = kGuardCos * lc.amp * fmaxf(lc.lr, 1.f) * 1.0001f;
-    h.e0x = lc.e0x;
-    h.e0y = lc.e0y;
-    h.e1x = lc.e1x;
-    h.e1y = lc.e1y;
     h.lb0 = lc.lb0;
     h.lb1 = lc.lb1;
     h.lc0 = lc.lc0;
     h.lc1 = lc.lc1;
     h.lc2 = lc.lc2;
-    h.dx = lc.dx;
-    h.dy = lc.dy;
-    h.rdx = lc.rdx;
-    h.rdy = lc.rdy;
-    h.ts = lc.ts * tscale;
-    h.tp = lc.tp * tscale;
-    h.num0k = kGuardRel * lc.amp;
-    h.amp = lc.amp;
-    h.path = lc.path;
-    h.cmx = d.cmx;
-    h.cmy = d.cmy;
+    h.g0 = lc.lg0;
+    h.g1 = lc.lg1;
+    h.cmd2 = lc.lcmd2;
     h.inv_r = d.inv_r;
-    h.max_dist_sq = d.max_dist_sq;
-    h.bound_scale = d.bound_scale;
     h.lr2r = lc.lr2r;
     h.lomr2 = lc.lomr2;
     h.lr = lc.lr;
@@ -819,34 +876,58 @@ CPD_HD HotRecord make_hot(const TraceDesign &d, const LambdaConsts &lc, float ts
     h.nd1 = lc.nd1;
     h.ddn = lc.ddn;
     h.rddn = lc.rddn;
-    h.mpx = d.mpx;
-    h.mpy = d.mpy;
-    h.dnx = d.dnx;
-    h.dny = d.dny;
-    h.ddx = d.ddx;
-    h.ddy = d.ddy;
     h.min_ci = d.min_ci;
+    h.wdn0 = lc.wdn0;
+    h.wdn1 = lc.wdn1;
+    h.wdn2 = lc.wdn2;
+    h.wdd0 = lc.wdd0;
+    h.wdd1 = lc.wdd1;
+    h.wdd2 = lc.wdd2;
+    h.ndd0 = lc.ndd0;
+    h.ndd1 = lc.ndd1;
+    h.ndd2 = lc.ndd2;
+    h.rdd = lc.rdd;
+    h.num0k = kGuardRel * lc.amp;
+    h.dlen0 = lc.dlen0 * h.num0k * 1.000001f;
+    h.dlen1 = lc.dlen1 * h.num0k * 1.000001f;
+    h.hc = lc.hc;
     h.half_len = d.half_len;
     h.slope = d.slope;
     h.inv_slope = d.inv_slope;
-    h.intercept = d.intercept;
     h.bin_size = d.bin_size;
-    h.nbin_f = d.nbin_f;
+    h.ts4 = lc.ts * (4.f * tscale);
+    h.tp4 = lc.tp * (4.f * tscale);
+    h.tcoat = fmaf(lc.ts * tscale + lc.tp * tscale, 0.99f, toff);
+    h.toff = toff;
     return h;
 }
 
+// Saturating conversion of an integral float to u32 (negative and NaN give
+// 0, as the reference's `as u32` casts do).
+#if defined(__HIP_DEVICE_COMPILE__)
+CPD_HD uint32_t sat_u32(float x) {
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+#else
+CPD_HD uint32_t sat_u32(float x) { return x > 0.f ? (x < 4294967296.f ? (uint32_t)x : 0xffffffffu) : 0u; }
+#endif
+
 // One ray at beam height y.  Decisions are kept as per-lane booleans and
 // combined with bitwise operators, so the whole trace is straight-line code.
+// The ray lands in bin `bin` with transmittance t when `hit`; guard marks a
+// ray whose outcome is within the FP32 error margin of a decision.
 template <bool COUNT>
 CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
-    // t is read only with a bin below nbin, which only the end of the trace
-    // sets: the miss paths leave it without a value to materialise
-    Trace32Result res{d.nbin, kAnyFloat, false, 0};
+    // bin and t are read only for a hit, which only the end of the trace
+    // sets: the miss paths leave them without a value to materialise
+    Trace32Result res{__builtin_bit_cast(uint32_t, y), y, false, false, 0};
     const bool ar = d.ar_coated != 0;
     // planar chain: the ray survives the planar faces iff ylo <= y <= yhi
-    bool alive = (y >= d.ylo) & (y <= d.yhi);
-    bool guard = fminf(fabsf(y - d.ylo), fabsf(y - d.yhi)) <= d.ymarg;
-    const float ex = fmaf(d.e1x, y, d.e0x), ey = fmaf(d.e1y, y, d.e0y);
+    const float yo = fabsf(y - d.ymid) - d.yhalf;  // <= 0 inside, |yo| from the nearer bound
+    bool alive = yo <= 0.f;
+    bool guard = fabsf(yo) <= d.ymarg;
     if (COUNT) res.ops = kOpsBeam + kOpsChain + (alive ? kOpsLensHit : 0u);
     // wavefronts of banded rays often lie wholly outside the accepted heights
     if (!wave_any(alive | guard)) return res;
@@ -854,9 +935,6 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     // |e + s dir - cm|^2 = R^2  <=>  s^2 + 2 b s + c = 0; the root of larger
     // magnitude is -(b + sign(b) sqrt(disc)) and the other one is c over it,
     // so neither root cancels.
-    const float num0 = d.amp;
-    const float lim = kGuardRel * d.path * num0;
-    const float dx = d.dx, dy = d.dy;
     const float b = fmaf(d.lb1, y, d.lb0);
     const float c = fmaf(fmaf(d.lc2, y, d.lc1), y, d.lc0);
     const float disc = fmaf(b, b, -c);
@@ -869,15 +947,13 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     const float small = c * fast_rcp(big);
     const bool bpos = !signbit(b);
     const float d1 = bpos ? big : small, d2 = bpos ? small : big;
-    const float q1x = fmaf(dx, d1, ex), q1y = fmaf(dy, d1, ey);
-    const float q2x = fmaf(dx, d2, ex), q2y = fmaf(dy, d2, ey);
-    // |q|^2 - max_dist_sq, with the bound inside the fused products
-    const float s1 = fmaf(q1x, q1x, fmaf(q1y, q1y, -d.max_dist_sq));
-    const float s2 = fmaf(q2x, q2x, fmaf(q2y, q2y, -d.max_dist_sq));
+    // |q|^2 - max_dist_sq at q = e + s dir on the lens circle (affine in s)
+    const float gy = fmaf(d.g1, y, d.g0);
+    const float s1 = fmaf(d.cmd2, d1, gy);
+    const float s2 = fmaf(d.cmd2, d2, gy);
     const bool ok1 = (d1 >= 0.f) & (s1 <= 0.f);
     const bool ok2 = (d2 >= 0.f) & (s2 <= 0.f);
-    const float blim = lim * d.bound_scale;
-    guard |= alive & ((fabsf(s1) <= blim) | (!ok1 & (fabsf(s2) <= blim)));
+    guard |= alive & ((fabsf(s1) <= d.blim) | (!ok1 & (fabsf(s2) <= d.blim)));
     alive &= ok1 | ok2;
     if (COUNT) res.ops += alive ? kOpsLensRefract : 0u;
     const float tq = ok1 ? d1 : d2;  // the hit point is q = e + tq dir
@@ -894,7 +970,7 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
     const float k = rci - cr;
     guard |= alive & (fabsf(cr_sq) <= kGuardCos);
     alive &= cr_sq >= 0.f;
-    guard |= alive & ar & (fabsf(ci - 0.5f) <= kGuardCos * num0);
+    if (ar) guard |= alive & (fabsf(ci - 0.5f) <= d.coat_lim);
     const float den = cr;
     // detector array: w = o - P0 = (m - P0) + q
     if (COUNT) res.ops += alive ? kOpsDetector : 0u;
@@ -906,40 +982,51 @@ CPD_HD Trace32Result trace32(const HotRecord &d, float y) {
         const float cd = -fmaf(k, ndn, d.rddn);
         guard |= alive & (fabsf(cd - d.min_ci) * den <= d.cd_lim);
         alive &= cd > d.min_ci;
-        // most wavefronts lose every ray here (the acceptance cosine): skip
-        // the rest of the stage when no lane has a ray left or near a decision
-        if (!wave_any(alive) && !wave_any(guard)) return res;
-        const float qx = ok1 ? q1x : q2x, qy = ok1 ? q1y : q2y;
-        const float path = d.path + fabsf(tq);
-        const float nx = (d.cmx - qx) * d.inv_r, ny = (d.cmy - qy) * d.inv_r;
-        const float vx = fmaf(nx, k, d.rdx), vy = fmaf(ny, k, d.rdy);
-        // Fresnel: rs = (r ci - cr) / (r ci + cr), rp = (r cr - ci) / (r cr + ci)
-        // (the reference's amplitudes divided through by n2), one reciprocal
-        const float fa = rci + cr, fb = fmaf(d.lr, cr, ci);
-        const float inv = fast_rcp(fa * fb);
-        const float as = k * (fb * inv), ap = fmaf(d.lr, cr, -ci) * (fa * inv);
-        const bool coat = ar & (ci > 0.5f);
-        const float ts = d.ts * (coat ? 0.99f : fmaf(-as, as, 1.f));
-        const float tp = d.tp * (coat ? 0.99f : fmaf(-ap, ap, 1.f));
-        const float wx = d.mpx + qx, wy = d.mpy + qy;
-        const float dist = fmaf(wx, d.dnx, wy * d.dny) * fast_rcp(cd);
-        const float pos = fmaf(fmaf(vx, dist, wx), d.ddx, fmaf(vy, dist, wy) * d.ddy);
-        const float plim = (path + fabsf(dist) + fabsf(wx) + fabsf(wy)) * (d.num0k * max2f(rci, cr));
-        const float apos = pos - d.half_len;
-        guard |= alive & (fabsf(fabsf(apos) - d.half_len) * den <= plim);
-        alive &= fabsf(apos) <= d.half_len;
-        const float v = pos - d.intercept;
-        const float qb = floorf(v * d.inv_slope);
-        const float rem = fmaf(-qb, d.slope, v);
-        const float edge = fminf(fminf(fabsf(rem), fabsf(d.slope - rem)), fabsf(rem - d.bin_size));
-        guard |= alive & (edge * den <= plim);
-        // the quotient is cast to u32 saturating (bin_index, spectrometer.rs:
-        // 291-303): a position before the intercept has a negative quotient
-        // and counts in bin 0 when its remainder is below the bin size
-        alive &= (qb < d.nbin_f) & (rem < d.bin_size);
-        res.bin = alive ? (uint32_t)fmaxf(qb, 0.f) : d.nbin;
-        res.t = alive ? ts + tp : 0.f;  // ts, tp carry make_hot's tscale
-        if (COUNT) res.ops += alive ? kOpsHit : 0u;
+        // most wavefronts lose every ray here (the acceptance cosine): the
+        // rest of the stage runs on the lanes with a ray left or near a
+        // decision (a divergent branch, which a wavefront with none skips)
+        res.t = cr;  // (any value: a miss's t is not read)
+        if (alive | guard) {
+            // the ray meets the array's line at w + dist v, dist = -w.dn / v.dn,
+            // at u = (w + dist v).dd - intercept along it; every product with
+            // dn or dd is affine in y and tq (per-wavelength constants)
+            const float wdn = fmaf(d.wdn2, tq, fmaf(d.wdn1, y, d.wdn0));
+            const float wdd = fmaf(d.wdd2, tq, fmaf(d.wdd1, y, d.wdd0));
+            const float ndd = fmaf(d.ndd2, tq, fmaf(d.ndd1, y, d.ndd0));
+            const float vdd = fmaf(k, ndd, d.rdd);
+            const float dist = wdn * fast_rcp(cd);
+            const float u = fmaf(dist, vdd, wdd);
+            // position errors: kGuardRel A per unit of length (the path to the
+            // lens, |tq| and |w| through the planar faces' bound, |dist|), times
+            // the lens's angular gain max(r ci, cr) / cr (the 1 / cr multiplies
+            // the margins instead)
+            const float plim = fmaf(fabsf(dist), d.num0k, fmaf(d.dlen1, fabsf(tq), d.dlen0)) * max2f(rci, cr);
+            const float apos = u - d.hc;  // position minus half the length
+            guard |= alive & (fabsf(fabsf(apos) - d.half_len) * den <= plim);
+            alive &= fabsf(apos) <= d.half_len;
+            const float qb = floorf(u * d.inv_slope);
+            const float rem = fmaf(-qb, d.slope, u);
+            const float edge = fminf(fminf(fabsf(rem), fabsf(d.slope - rem)), fabsf(rem - d.bin_size));
+            guard |= alive & (edge * den <= plim);
+            // the quotient is cast to u32 saturating (bin_index, spectrometer.rs:
+            // 291-303): a position before the intercept has a negative quotient
+            // and counts in bin 0 when its remainder is below the bin size
+            const uint32_t bin = sat_u32(qb);
+            alive &= (bin < d.nbin) & (rem < d.bin_size);
+            // Fresnel: 1 - rs^2 = 4 r ci cr / (r ci + cr)^2 and 1 - rp^2 =
+            // 4 r ci cr / (r cr + ci)^2 (rs = (r ci - cr) / (r ci + cr), rp =
+            // (r cr - ci) / (r cr + ci), the reference's amplitudes divided
+            // through by n2), one reciprocal
+            const float fa = rci + cr, fb = fmaf(d.lr, cr, ci);
+            const float inv = fast_rcp(fa * fb);
+            const float ia = fb * inv, ib = fa * inv;  // 1 / fa, 1 / fb
+            const float sum = fmaf(d.ts4 * ia, ia, (d.tp4 * ib) * ib);
+            const bool coat = ar & (ci > 0.5f);
+            res.bin = bin;
+            res.t = coat ? d.tcoat : fmaf(rci * cr, sum, d.toff);
+            res.hit = alive;
+            if (COUNT) res.ops += alive ? kOpsHit : 0u;
+        }
     }
     res.guard = guard;
     return res;

@@ -99,7 +99,7 @@ int guard_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t
                     }
                     const float y = hot.y_mean - hot.beam_w * z32[k];
                     const cpd::Trace32Result r = cpd::trace32<false>(hot, y);
-                    const uint32_t b32 = r.bin < g.bin_count ? r.bin : g.bin_count;
+                    const uint32_t b32 = r.hit ? r.bin : g.bin_count;
                     if (r.guard) {
                         loc[1]++;
                         loc[5] += b32 != b64;
