@@ -490,7 +490,7 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
 }
 
 template <bool COUNT, int A>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel(BatchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void trace_kernel(BatchArgs a) {
     extern __shared__ __align__(8) uint32_t lds[];
     trace_lds_clear<A>(lds, a.hist_words, a.row_words);
     const uint32_t lane = threadIdx.x;
@@ -776,7 +776,7 @@ __device__ __attribute__((noinline)) void resident_finalize(const ResidentArgs &
 // only those counters, so the host link carries a handful of polls.
 constexpr uint32_t kResPollers = 4;
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void resident_kernel(ResidentArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void resident_kernel(ResidentArgs a) {
     extern __shared__ __align__(8) uint32_t lds[];
     trace_lds_clear<CPD_ACC_EXACT>(lds, a.hist_words, a.row_words);
     const uint32_t lane = threadIdx.x;

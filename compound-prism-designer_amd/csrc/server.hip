@@ -127,7 +127,7 @@ struct Server {
     int cu_count = 0;
     hipStream_t stream = nullptr;
     uint32_t step_items = 0;
-    uint32_t waves_per_cu = 16;
+    uint32_t waves_per_cu = 20;
     uint64_t max_designs = 0, acc_cap = 0, lb_cap = 0;
     DeviceBuffer d_designs, d_traces, d_acc_begin, d_lb_begin, d_acc, d_last_bin, d_results, d_step, d_counter;
     BeamTables beam;

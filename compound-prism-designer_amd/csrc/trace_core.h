@@ -245,8 +245,9 @@ CPD_HD uint32_t trace64(const cpd_design &g, double wavelength, double y, double
 // ---------------------------------------------------------------------------
 // Per-wavefront LDS histogram of one work item: `cols` columns per bin (lane
 // l adds into column l % cols), at most kHistWords words for bin counts up to
-// 2048, so the histogram stays near 8 KB and several wavefronts fit a CU.
-static constexpr uint32_t kHistWords = 2048;
+// 1024: 4 KB, so that a CU's LDS holds the twenty wavefronts (five per SIMD)
+// the trace kernel runs with.
+static constexpr uint32_t kHistWords = 1024;
 // Rays per work item: at most kMaxTileRays; kDefaultTileRays unless a
 // schedule caps it.
 static constexpr uint32_t kMaxTileRays = 1u << 20;
