@@ -184,8 +184,12 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
     if (lane < count) {
         const uint32_t i = c.queue[(head + lane) & (kQueueCap - 1)];  // position
         const double y = beam_y64(*c.g, i <= c.full_end ? c.zs64[i - 1] : c.z64[i - 1]);
+        // the affine FP64 trace decides nearly every queued ray; the few it
+        // leaves (a decision within its 1e-9 margin) take the reference's
+        // operation order
         double t;
-        const uint32_t b = trace64_chain(*c.g, *c.chain, y, t);
+        uint32_t b;
+        if (!trace64_affine(*c.band, *c.chain, *c.g, y, b, t)) b = trace64_chain(*c.g, *c.chain, y, t);
         if (b < c.nbin) atomicAdd((Cell *)c.hist + b * c.cols + (lane & (c.cols - 1)), Acc<A>::of64(t));
         if (i == c.last_pos) set_last_bin(c.last_bin, b);
     }

@@ -353,6 +353,7 @@ struct LensBand64 {
     double cmx, cmy;                 // lens centre minus the chord midpoint
     double mpx, mpy;                 // chord midpoint minus the array's start
     double dnx, dny, ddx, ddy, length;  // array normal, direction, length
+    double ylo, yhi, path;           // accepted beam heights; length scale of the planar chain
 };
 
 // Build the FP32 record from the FP64 design (host or device).
@@ -587,6 +588,9 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
         band->ddx = g.det_direction[0];
         band->ddy = g.det_direction[1];
         band->length = g.det_length;
+        band->ylo = lo;
+        band->yhi = hi;
+        band->path = path + fabs(e0x) + fabs(e0y) + (fabs(bx) + fabs(by)) * h;
     }
 }
 
@@ -733,6 +737,97 @@ CPD_HD uint32_t trace64_chain(const cpd_design &g, const Chain64 &ch, double y, 
     if (detector64(ray, g, bin, tt) != T_OK) return g.bin_count;
     t = tt;
     return bin;
+}
+
+// One ray in FP64 from the wavelength's affine planar chain and lens
+// constants (LensBand64: the quantities trace32 evaluates, without its FP32
+// rounding) and the planar faces' transmittances (Chain64).  It returns false
+// when a decision -- accepted height, lens discriminant, root sign, arc
+// bounds, reflection, coating, acceptance, the array's ends or a bin edge --
+// lies within a relative margin of kAffineRel of the quantities it is formed
+// from, far above the FP64 rounding of these few operations and of the
+// reference's own; the caller then re-traces the ray with trace64_chain, the
+// reference's operation order.  Otherwise bin (bin_count on a miss) is
+// trace64's and t agrees with trace64's to ~1e-12 relative (the reference's
+// 1 - rs^2 loses digits near grazing exit, the identity used here does not).
+static constexpr double kAffineRel = 1e-9;
+CPD_HD bool trace64_affine(const LensBand64 &L, const Chain64 &ch, const cpd_design &g, double y, uint32_t &bin,
+                           double &t) {
+    bin = g.bin_count;
+    t = 0.0;
+    // planar chain: the faces' conditions are affine in y
+    const double ym = kAffineRel * (fabs(y) + L.path);
+    if (fabs(y - L.ylo) <= ym || fabs(y - L.yhi) <= ym) return false;
+    if (!(y >= L.ylo && y <= L.yhi)) return true;
+    // exit lens, relative to its chord midpoint m (as trace32)
+    const double ex = fma(L.e1x, y, L.e0x), ey = fma(L.e1y, y, L.e0y);
+    const double b = fma(L.lb1, y, L.lb0);
+    const double c = fma(fma(L.lc2, y, L.lc1), y, L.lc0);
+    const double disc = fma(b, b, -c);
+    if (fabs(disc) <= kAffineRel * (b * b + fabs(c) + L.max_dist_sq)) return false;
+    if (!(disc > 0.0)) return true;
+    const double sd = sqrt(disc);
+    const double big = -(b + copysign(sd, b));
+    const double small = c / big;
+    const double d1 = b >= 0.0 ? big : small, d2 = b >= 0.0 ? small : big;  // ascending
+    const double dm = kAffineRel * (fabs(b) + sd + L.radius + L.path);
+    double qx = 0.0, qy = 0.0;
+    bool found = false;
+    for (int i = 0; i < 2 && !found; ++i) {
+        // nearest valid root first (geometry.rs:203-209)
+        const double d = i == 0 ? d1 : d2;
+        qx = fma(L.dx, d, ex);
+        qy = fma(L.dy, d, ey);
+        const double q2 = qx * qx + qy * qy;
+        if (fabs(d) <= dm || fabs(q2 - L.max_dist_sq) <= kAffineRel * (q2 + L.max_dist_sq + L.path * L.path))
+            return false;
+        found = d >= 0.0 && q2 <= L.max_dist_sq;
+    }
+    if (!found) return true;
+    // refraction out of the lens: n = (c - q) / R, n1 = r, n2 = 1
+    const double nx = (L.cmx - qx) * L.inv_r, ny = (L.cmy - qy) * L.inv_r;
+    const double ci = -(L.dx * nx + L.dy * ny);
+    const double cr_sq = 1.0 - L.r * L.r * (1.0 - ci * ci);
+    if (fabs(cr_sq) <= kAffineRel) return false;
+    if (!(cr_sq >= 0.0)) return true;
+    const double cr = sqrt(cr_sq);
+    const double k = L.r * ci - cr;
+    const double vx = L.dx * L.r + nx * k, vy = L.dy * L.r + ny * k;
+    const bool ar = g.ar_coated != 0;
+    if (ar && fabs(ci - 0.5) <= kAffineRel) return false;
+    // detector array: its acceptance cosine, the distance along v and the
+    // position from its start; errors grow with the lens's angular gain
+    const double gain = (fabs(L.r * ci) > cr ? fabs(L.r * ci) : cr) / cr;
+    const double cd = -(vx * L.dnx + vy * L.dny);
+    if (fabs(cd - L.min_ci) <= kAffineRel * gain) return false;
+    if (!(cd > L.min_ci)) return true;
+    const double wx = qx + L.mpx, wy = qy + L.mpy;
+    const double dist = (wx * L.dnx + wy * L.dny) / cd;
+    const double pos = (wx + vx * dist) * L.ddx + (wy + vy * dist) * L.ddy;
+    const double pm = kAffineRel * (fabs(wx) + fabs(wy) + fabs(dist) + L.path + L.radius) * gain / cd;
+    if (fabs(pos) <= pm || fabs(pos - L.length) <= pm) return false;
+    if (pos < 0.0 || L.length < pos) return true;
+    // bin: euclid_div_rem (src/spectrometer/utils.rs:177-192), saturating u32
+    const double v = pos - g.det_intercept;
+    double q = trunc(v / g.det_slope);
+    double rem = v - q * g.det_slope;
+    if (rem < 0.0) {
+        q = g.det_slope > 0.0 ? q - 1.0 : q + 1.0;
+        rem = rem + fabs(g.det_slope);
+    }
+    const double em = pm + kAffineRel * (fabs(v) + fabs(g.det_slope));
+    if (rem <= em || fabs(rem - fabs(g.det_slope)) <= em || fabs(rem - g.det_bin_size) <= em) return false;
+    const uint32_t bq = !(q > 0.0) ? 0u : (q >= 4294967295.0 ? 4294967295u : (uint32_t)q);
+    if (!(bq < g.bin_count && rem < g.det_bin_size)) return true;
+    bin = bq;
+    if (ar && ci > 0.5) {
+        t = (ch.ts * 0.99 + ch.tp * 0.99) * 0.5;
+    } else {
+        // 1 - rs^2 = 4 r ci cr / (r ci + cr)^2, 1 - rp^2 = 4 r ci cr / (r cr + ci)^2
+        const double fa = L.r * ci + cr, fb = L.r * cr + ci, rc4 = 4.0 * L.r * ci * cr;
+        t = (ch.ts * (rc4 / (fa * fa)) + ch.tp * (rc4 / (fb * fb))) * 0.5;
+    }
+    return true;
 }
 
 // ---------------------------------------------------------------------------

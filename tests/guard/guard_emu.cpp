@@ -214,3 +214,71 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// The affine FP64 trace of the queue drains (trace64_affine) against trace64,
+// the reference's operation order, on every ray of every (design,
+// wavelength) with a valid planar chain: stats[0] rays, [1] rays it leaves
+// to trace64_chain, [2] decided rays whose bin differs from trace64's (must
+// be 0), [3] decided hits; terr[0] the largest relative transmittance
+// difference of a decided hit.
+int affine_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t n_rays, int threads,
+                 uint64_t *stats, double *terr) {
+    std::vector<double> z64(n_rays);
+    for (uint32_t k = 0; k < n_rays; ++k) z64[k] = cpd::norminv64(cpd::qmc_u64((uint64_t)k + 1));
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> tot[4];
+    for (auto &t : tot) t = 0;
+    const int T = threads > 0 ? threads : 1;
+    std::vector<double> worst(T, 0.0);
+    auto work = [&](int tid) {
+        uint64_t loc[4] = {0, 0, 0, 0};
+        double w = 0.0;
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const cpd_design &g = designs[i];
+            for (uint32_t j = 1; j <= n_lambda; ++j) {
+                const double lambda = cpd::lambda_of(g, cpd::qmc_u64(j));
+                cpd::LambdaConsts lc;
+                cpd::Chain64 chain;
+                cpd::LensBand64 band;
+                cpd::make_lambda_consts(g, lambda, lc, &chain, &band);
+                if (!lc.ok) continue;
+                for (uint32_t k = 0; k < n_rays; ++k) {
+                    const double y = cpd::beam_y64(g, z64[k]);
+                    double t64 = 0.0, ta = 0.0;
+                    const uint32_t b64 = cpd::trace64(g, lambda, y, t64);
+                    uint32_t ba = 0;
+                    loc[0]++;
+                    if (!cpd::trace64_affine(band, chain, g, y, ba, ta)) {
+                        loc[1]++;
+                        continue;
+                    }
+                    if (ba != b64) {
+                        loc[2]++;
+                        continue;
+                    }
+                    if (b64 < g.bin_count) {
+                        loc[3]++;
+                        const double e = fabs(ta - t64) / t64;
+                        if (e > w) w = e;
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < 4; ++q) tot[q] += loc[q];
+        worst[tid] = w;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    for (int q = 0; q < 4; ++q) stats[q] = tot[q];
+    terr[0] = 0.0;
+    for (double v : worst) terr[0] = v > terr[0] ? v : terr[0];
+    return 0;
+}
+
+}  // extern "C"

@@ -120,3 +120,27 @@ def test_skipped_bands_hold_no_hit(config, transmit):
     print(f"config {config}: {st}, skipped {st['skipped'] / st['past_chain']:.3f} of the bands past the chain test")
     assert st["skipped_with_hit"] == 0, st
     assert st["skipped"] > 0
+
+
+@pytest.mark.parametrize("config", [0, 2, 4, 5, 6, 7])
+def test_affine_fp64_trace_decides_as_the_reference(config):
+    # the queue drains trace a flagged ray with trace64_affine, the affine
+    # FP64 chain; every ray it decides lands in trace64's bin (the
+    # reference's operation order) with the transmittance within 1e-10
+    # relative (the reference's 1 - rs^2 loses digits near grazing exit, the
+    # identity 4 r ci cr / (r ci + cr)^2 does not), and it decides almost
+    # every ray
+    L = guard_lib("fast-honor-pragmas")
+    L.affine_check.argtypes = [ctypes.POINTER(N.Design), ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.c_double * 1]
+    _, D, _, _ = transmitting(config, 24, first=1000)
+    arr = (N.Design * len(D))(*D)
+    st = (ctypes.c_uint64 * 4)()
+    terr = (ctypes.c_double * 1)()
+    L.affine_check(arr, len(D), 16, 16384, min(16, os.cpu_count() or 1), st, terr)
+    s = dict(zip(["rays", "undecided", "mismatch", "hits"], st[:]))
+    print(f"config {config}: {s}, max relative t difference {terr[0]:.2e}")
+    assert s["mismatch"] == 0, s
+    assert s["hits"] > 100_000 and s["undecided"] <= 1e-4 * s["rays"], s
+    assert terr[0] <= 1e-10
+
