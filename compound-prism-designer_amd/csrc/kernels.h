@@ -15,8 +15,9 @@ namespace cpd {
 static constexpr double kQuantScale = 4194304.0;
 static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 re-trace
 // Positions of a tile whose bands are tested at once before their rays (one
-// 64-bit skip mask per 4096 positions in LDS).
-static constexpr uint32_t kMaskChunk = 65536;
+// 64-bit skip mask per 4096 positions in LDS): four blocks of 4096, whose 32
+// super-bands of 512 positions one pass of the wavefront tests.
+static constexpr uint32_t kMaskChunk = 16384;
 // Readable entries past the last ray of the FP32 beam tables (z32, zs32) the
 // trace loop needs: it loads 64 consecutive entries from any position up to
 // the last ray.

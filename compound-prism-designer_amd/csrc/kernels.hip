@@ -40,9 +40,9 @@ namespace {
 __device__ __forceinline__ uint32_t quant32(float t) { return __float2uint_rn(t * (float)kQuantScale); }
 __device__ __forceinline__ uint32_t quant64(double t) { return __double2uint_rn(t * kQuantScale); }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    const uint32_t lane = threadIdx.x;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+// Set bits of m below this lane (mbcnt: no lane-mask register kept live).
+__device__ __forceinline__ uint32_t count_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Copy of a wave-uniform record into scalar registers.
@@ -230,24 +230,77 @@ __device__ void fold_columns(void *hist_v, void *row_v, uint32_t nbin, uint32_t 
     __syncthreads();
 }
 
+// Position of the n-th (from 0) set bit of m (n < popcount(m)).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t half = 32; half >= 1; half >>= 1) {
+        const uint32_t c = __popcll(m & ((1ull << half) - 1));
+        if (n >= c) {
+            n -= c;
+            m >>= half;
+            pos += half;
+        }
+    }
+    return pos;
+}
+
 // Skip masks of the bands of positions [chunk, chunk_end) (at most
 // kMaskChunk): band i (positions chunk + 64 i .. + 63) is skipped when all its
 // positions are banded and band_may_hit proves that the FP64 trace loses every
-// one of its rays.  Word g of `masks` holds bands 64 g .. 64 g + 63, one per
-// lane; every lane of the wavefront evaluates its own band of each word.
+// one of its rays.  Word g of `masks` holds bands 64 g .. 64 g + 63.
+// Within a block of 4096 banded positions the heights are sorted, so a band's
+// heights lie between those of its ends, and a run of bands is one interval
+// of heights: the wavefront first tests the super-bands of 8 bands (one lane
+// each), whose bands it skips together when one is lost (interval arithmetic
+// only widens with its interval, so every band of it is lost as well), then
+// the bands of the other super-bands, 64 to a pass, gathered over the blocks.
 __device__ void band_masks(const TileCtx &c, uint32_t chunk, uint32_t chunk_end, uint64_t *masks) {
+    static_assert(kMaskChunk == 4 * 4096, "one pass tests the 32 super-bands of a chunk");
     const uint32_t lane = threadIdx.x;
     const uint32_t words = (chunk_end - chunk + 4095u) >> 12;
-    for (uint32_t w = 0; w < words; ++w) {
-        const uint32_t b0 = chunk + (w << 12) + 64u * lane;  // this lane's band: positions b0 .. b0 + 63
-        bool sk = false;
-        if (b0 < chunk_end && b0 + 63u <= c.full_end) {
+    bool lost = false;
+    {
+        const uint32_t w = lane >> 3;
+        const uint32_t s0 = chunk + (w << 12) + ((lane & 7u) << 9);  // positions s0 .. s0 + 511
+        if (w < words && s0 < chunk_end && s0 + 511u <= c.full_end) {
+            const double ya = beam_y64(*c.g, c.zs64[s0 - 1]);
+            const double yb = beam_y64(*c.g, c.zs64[s0 + 510]);
+            lost = !band_may_hit(*c.band, ya, yb);
+        }
+    }
+    const uint64_t super_lost = __builtin_amdgcn_ballot_w64(lost);  // bit 8 w + s
+    // bands of the super-bands not lost, per word; the lost ones start the masks
+    uint64_t todo[4];
+    uint32_t first[5];
+    first[0] = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+        todo[w] = 0;
+        if (w < words) {
+            const uint32_t b0 = chunk + (w << 12) + 64u * lane;
+            const bool whole = b0 < chunk_end && b0 + 63u <= c.full_end;
+            const bool in_lost = (super_lost >> (8u * w + (lane >> 3))) & 1u;
+            todo[w] = __builtin_amdgcn_ballot_w64(whole && !in_lost);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(whole && in_lost);
+            if (lane == 0) masks[w] = m;
+        }
+        first[w + 1] = first[w] + (uint32_t)__popcll(todo[w]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (uint32_t p = 0; p < first[4]; p += 64) {
+        const uint32_t k = p + lane;
+        if (k < first[4]) {
+            uint32_t w = 0;
+#pragma unroll
+            for (uint32_t q = 1; q < 4; ++q) w += k >= first[q] ? 1u : 0u;
+            const uint32_t i = nth_set_bit(todo[w], k - first[w]);
+            const uint32_t b0 = chunk + (w << 12) + 64u * i;
             const double ya = beam_y64(*c.g, c.zs64[b0 - 1]);
             const double yb = beam_y64(*c.g, c.zs64[b0 + 62]);
-            sk = !band_may_hit(*c.band, ya, yb);
+            if (!band_may_hit(*c.band, ya, yb)) atomicOr((unsigned long long *)&masks[w], 1ull << i);
         }
-        const uint64_t m = __builtin_amdgcn_ballot_w64(sk);
-        if (lane == 0) masks[w] = m;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -334,7 +387,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
             const uint32_t k = base + lane;
             const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
             if (mask) {
-                if (flagged) c.queue[(tail + __popcll(mask & lanemask_lt())) & (kQueueCap - 1)] = k;
+                if (flagged) c.queue[(tail + count_below(mask)) & (kQueueCap - 1)] = k;
                 tail += __popcll(mask);
                 if (tail - head >= 64) {
                     __builtin_amdgcn_wave_barrier();

@@ -688,6 +688,9 @@ CPD_HD bool band_root_may_hit(const LensBand64 &L, Iv y, Iv b, Iv sd, Iv disc, b
 
 CPD_HD bool band_may_hit(const LensBand64 &L, double ya, double yb) {
     const Iv y{fmin(ya, yb), fmax(ya, yb)};
+    // every height outside the accepted interval: lost at the planar faces
+    const double ym = 1e-9 * (fmax(fabs(ya), fabs(yb)) + L.path);
+    if (y.hi < L.ylo - ym || y.lo > L.yhi + ym) return false;
     const Iv b = iv_affine(L.lb0, L.lb1, y);
     // c(y) = lc0 + y (lc1 + lc2 y)
     const Iv c = iv_add(Iv{L.lc0, L.lc0}, iv_mul(y, iv_affine(L.lc1, L.lc2, y)));

@@ -144,9 +144,11 @@ extern "C" {
 
 // The band test (band_may_hit, trace_core.h) over the banded beam order the
 // device traces in (within each block of 4096 rays sorted by height, 64 rays
-// to a band): stats[0] bands, [1] bands a wavefront would take past the
-// planar-chain test, [2] of those the band test skips, [3] skipped bands
-// holding an FP64 hit (must be 0), [4] bands holding an FP64 hit.
+// to a band), as the kernel applies it: a band is skipped when its
+// super-band of 8 bands (512 positions) or the band itself is proven lost.
+// stats[0] bands, [1] bands a wavefront would take past the planar-chain
+// test, [2] of those skipped, [3] skipped bands (any) holding an FP64 hit
+// (must be 0), [4] bands holding an FP64 hit.
 int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t n_rays, int threads,
                uint64_t *stats) {
     const uint32_t full = n_rays / cpd::kBandRays * cpd::kBandRays;
@@ -178,8 +180,15 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                 cpd::LensBand64 band;
                 cpd::make_lambda_consts(g, lambda, lc, &chain, &band);
                 const cpd::HotRecord hot = cpd::make_hot(td, lc);
+                bool super_lost = false;
                 for (uint32_t p0 = 0; p0 < full; p0 += 64) {
                     loc[0]++;
+                    if (p0 % 512 == 0) {
+                        // the super-band of 8 bands the kernel tests first
+                        const double sa = cpd::beam_y64(g, z64[order[p0]]);
+                        const double sb = cpd::beam_y64(g, z64[order[p0 + 511]]);
+                        super_lost = lc.ok && !cpd::band_may_hit(band, sa, sb);
+                    }
                     bool chain_pass = false, hit = false;
                     for (uint32_t q = 0; q < 64; ++q) {
                         const uint32_t k = order[p0 + q];
@@ -191,14 +200,15 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                         hit |= lc.ok && cpd::trace64(g, lambda, cpd::beam_y64(g, z64[k]), t) < g.bin_count;
                     }
                     loc[4] += hit;
-                    if (!chain_pass || !lc.ok) continue;
-                    loc[1]++;
+                    if (!lc.ok) continue;
                     const double ya = cpd::beam_y64(g, z64[order[p0]]);
                     const double yb = cpd::beam_y64(g, z64[order[p0 + 63]]);
-                    if (!cpd::band_may_hit(band, ya, yb)) {
-                        loc[2]++;
-                        loc[3] += hit;
+                    const bool skipped = super_lost || !cpd::band_may_hit(band, ya, yb);
+                    if (chain_pass) {
+                        loc[1]++;
+                        loc[2] += skipped;
                     }
+                    if (skipped) loc[3] += hit;
                 }
             }
         }
