@@ -230,6 +230,51 @@ __device__ void fold_columns(void *hist_v, void *row_v, uint32_t nbin, uint32_t 
     __syncthreads();
 }
 
+// Chunk layouts.  A chunk of kMaskChunk positions made of four whole blocks
+// of kBandRays banded positions is traced interleaved: iteration i takes the
+// ranks 16 i .. 16 i + 15 of each block (lane l: block l / 16, rank
+// 16 i + l % 16), so its 64 rays span about 1/256 of the beam's heights, as
+// if the chunk were one sorted block.  Any other chunk is traced in position
+// order, 64 consecutive positions per iteration.  Band i of a chunk is the
+// rays of its iteration i.
+struct ChunkMap {
+    uint32_t chunk, end;  // positions [chunk, end)
+    bool inter;           // interleaved layout
+};
+__device__ __forceinline__ ChunkMap chunk_map(uint32_t chunk, uint32_t end, uint32_t full_end) {
+    static_assert(kMaskChunk == 4 * kBandRays && kBandRays == 4096, "four blocks of 4096 per chunk");
+    const bool inter = end - chunk == kMaskChunk && ((chunk - 1u) & (kBandRays - 1u)) == 0 &&
+                       chunk + kMaskChunk - 1u <= full_end;
+    return ChunkMap{chunk, end, inter};
+}
+
+// Lowest and highest FP64 beam height of bands first .. first + n - 1 of the
+// chunk (n a power of two up to 8), and whether those bands are whole banded
+// positions lying in one sorted run (in position order: one block), so that
+// the heights of their ends bound theirs.
+__device__ bool band_heights(const TileCtx &c, const ChunkMap &m, uint32_t first, uint32_t n, double &ya,
+                             double &yb) {
+    if (m.inter) {
+        const uint32_t r0 = 16u * first, r1 = 16u * (first + n) - 1u;  // ranks in each block
+        double lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t b = m.chunk - 1u + j * kBandRays;
+            const double a = beam_y64(*c.g, c.zs64[b + r0]), e = beam_y64(*c.g, c.zs64[b + r1]);
+            lo = fmin(lo, fmin(a, e));
+            hi = fmax(hi, fmax(a, e));
+        }
+        ya = lo;
+        yb = hi;
+        return true;
+    }
+    const uint32_t p0 = m.chunk + 64u * first, p1 = p0 + 64u * n - 1u;
+    if (!(p0 < m.end && p1 <= c.full_end && ((p0 - 1u) >> 12) == ((p1 - 1u) >> 12))) return false;
+    ya = beam_y64(*c.g, c.zs64[p0 - 1]);
+    yb = beam_y64(*c.g, c.zs64[p1 - 1]);
+    return true;
+}
+
 // Position of the n-th (from 0) set bit of m (n < popcount(m)).
 __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
     uint32_t pos = 0;
@@ -245,32 +290,24 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
     return pos;
 }
 
-// Skip masks of the bands of positions [chunk, chunk_end) (at most
-// kMaskChunk): band i (positions chunk + 64 i .. + 63) is skipped when all its
-// positions are banded and band_may_hit proves that the FP64 trace loses every
-// one of its rays.  Word g of `masks` holds bands 64 g .. 64 g + 63.
-// Within a block of 4096 banded positions the heights are sorted, so a band's
-// heights lie between those of its ends, and a run of bands is one interval
-// of heights: the wavefront first tests the super-bands of 8 bands (one lane
-// each), whose bands it skips together when one is lost (interval arithmetic
-// only widens with its interval, so every band of it is lost as well), then
-// the bands of the other super-bands, 64 to a pass, gathered over the blocks.
-__device__ void band_masks(const TileCtx &c, uint32_t chunk, uint32_t chunk_end, uint64_t *masks) {
-    static_assert(kMaskChunk == 4 * 4096, "one pass tests the 32 super-bands of a chunk");
+// Skip masks of the bands of a chunk: band i is skipped when band_may_hit
+// proves that the FP64 trace loses every one of its rays.  Word g of `masks`
+// holds bands 64 g .. 64 g + 63.  The wavefront first classifies the 32
+// super-bands of 8 bands (one lane each).  It skips the bands of a lost one
+// together (interval arithmetic only widens with its interval, so every band
+// of it is lost as well) and tests none of a certain one (every band of it
+// may hit); the bands of the mixed ones it tests 64 to a pass.
+__device__ void band_masks(const TileCtx &c, const ChunkMap &m, uint64_t *masks) {
     const uint32_t lane = threadIdx.x;
-    const uint32_t words = (chunk_end - chunk + 4095u) >> 12;
-    bool lost = false;
-    {
-        const uint32_t w = lane >> 3;
-        const uint32_t s0 = chunk + (w << 12) + ((lane & 7u) << 9);  // positions s0 .. s0 + 511
-        if (w < words && s0 < chunk_end && s0 + 511u <= c.full_end) {
-            const double ya = beam_y64(*c.g, c.zs64[s0 - 1]);
-            const double yb = beam_y64(*c.g, c.zs64[s0 + 510]);
-            lost = !band_may_hit(*c.band, ya, yb);
-        }
+    const uint32_t words = (m.end - m.chunk + 4095u) >> 12;
+    int cls = kBandMixed;
+    if (lane < 8u * words) {
+        double ya, yb;
+        if (band_heights(c, m, 8u * lane, 8u, ya, yb)) cls = band_class(*c.band, ya, yb);
     }
-    const uint64_t super_lost = __builtin_amdgcn_ballot_w64(lost);  // bit 8 w + s
-    // bands of the super-bands not lost, per word; the lost ones start the masks
+    const uint64_t super_lost = __builtin_amdgcn_ballot_w64(cls == kBandLost);  // bit = super-band
+    const uint64_t super_mixed = __builtin_amdgcn_ballot_w64(cls == kBandMixed);
+    // bands of the mixed super-bands, per word; the lost ones start the masks
     uint64_t todo[4];
     uint32_t first[5];
     first[0] = 0;
@@ -278,12 +315,13 @@ __device__ void band_masks(const TileCtx &c, uint32_t chunk, uint32_t chunk_end,
     for (uint32_t w = 0; w < 4; ++w) {
         todo[w] = 0;
         if (w < words) {
-            const uint32_t b0 = chunk + (w << 12) + 64u * lane;
-            const bool whole = b0 < chunk_end && b0 + 63u <= c.full_end;
-            const bool in_lost = (super_lost >> (8u * w + (lane >> 3))) & 1u;
-            todo[w] = __builtin_amdgcn_ballot_w64(whole && !in_lost);
-            const uint64_t m = __builtin_amdgcn_ballot_w64(whole && in_lost);
-            if (lane == 0) masks[w] = m;
+            const uint32_t i = 64u * w + lane;  // band
+            const bool whole = m.inter || (m.chunk + 64u * i < m.end && m.chunk + 64u * i + 63u <= c.full_end);
+            const bool in_lost = (super_lost >> (i >> 3)) & 1u;
+            const bool in_mixed = (super_mixed >> (i >> 3)) & 1u;
+            todo[w] = __builtin_amdgcn_ballot_w64(whole && in_mixed);
+            const uint64_t lost = __builtin_amdgcn_ballot_w64(whole && in_lost);
+            if (lane == 0) masks[w] = lost;
         }
         first[w + 1] = first[w] + (uint32_t)__popcll(todo[w]);
     }
@@ -296,10 +334,9 @@ __device__ void band_masks(const TileCtx &c, uint32_t chunk, uint32_t chunk_end,
 #pragma unroll
             for (uint32_t q = 1; q < 4; ++q) w += k >= first[q] ? 1u : 0u;
             const uint32_t i = nth_set_bit(todo[w], k - first[w]);
-            const uint32_t b0 = chunk + (w << 12) + 64u * i;
-            const double ya = beam_y64(*c.g, c.zs64[b0 - 1]);
-            const double yb = beam_y64(*c.g, c.zs64[b0 + 62]);
-            if (!band_may_hit(*c.band, ya, yb)) atomicOr((unsigned long long *)&masks[w], 1ull << i);
+            double ya, yb;
+            if (band_heights(c, m, 64u * w + i, 1u, ya, yb) && !band_may_hit(*c.band, ya, yb))
+                atomicOr((unsigned long long *)&masks[w], 1ull << i);
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -328,7 +365,8 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     // last whole block) are never skipped, and op counting traces every band.
     for (uint32_t chunk = c.k_begin; chunk < k_end; chunk += kMaskChunk) {
         const uint32_t chunk_end = min(chunk + kMaskChunk, k_end);
-        if (!COUNT) band_masks(c, chunk, chunk_end, c.masks);
+        const ChunkMap cmap = chunk_map(chunk, chunk_end, full_end);
+        if (!COUNT) band_masks(c, cmap, c.masks);
         // register copies of the per-item records (made after the band tests,
         // whose FP64 registers they would otherwise share): the loop stores
         // through last_bin, so loads through c.td would be repeated every
@@ -352,9 +390,20 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         hot.slope = in_vgpr(hot.slope);
         hot.inv_slope = in_vgpr(hot.inv_slope);
         hot.bin_size = in_vgpr(hot.bin_size);
+        // this lane's position in iteration it is chunk + it * step + lane_off
+        const uint32_t step = cmap.inter ? 16u : 64u;
+        const uint32_t lane_off = cmap.inter ? ((lane >> 4) * kBandRays + (lane & 15u)) : lane;
+        // the iteration and lane of the last ray's position, when in the chunk
+        uint32_t last_it = ~0u, last_lane = 0;
+        if (last_pos >= chunk && last_pos < chunk_end) {
+            const uint32_t off = last_pos - chunk;
+            last_it = cmap.inter ? (off & (kBandRays - 1u)) >> 4 : off >> 6;
+            last_lane = cmap.inter ? ((off >> 12) << 4) + (off & 15u) : off & 63u;
+        }
         uint64_t skip = 0;
-        for (uint32_t base = chunk; base < chunk_end; base += 64) {
-            const uint32_t it = (base - chunk) >> 6;
+        const uint32_t n_it = (chunk_end - chunk + 63u) >> 6;
+        for (uint32_t it = 0; it < n_it; ++it) {
+            const uint32_t base = chunk + it * step;
             if (!COUNT) {
                 // the chunk's masks are wave-uniform: scalar registers
                 if ((it & 63u) == 0) {
@@ -366,25 +415,25 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
                 }
                 if ((skip >> (it & 63u)) & 1u) {
                     // every ray of the band misses: only the last ray's bin is recorded
-                    if (last_pos >= base && last_pos < base + 64u && lane == 0) set_last_bin(c.last_bin, nbin);
+                    if (it == last_it && lane == 0) set_last_bin(c.last_bin, nbin);
                     continue;
                 }
             }
-            // positions base .. base + 63 are all banded or all in ray order;
-            // the tables extend 63 entries past their last ray, so every lane
-            // loads from them (lanes at or past k_end only in the tile's last
-            // iteration, whose results they discard)
-            const float *__restrict__ zp = (base + 63 <= full_end ? c.zs32 : c.z32) + (base - 1u);
-            const bool active = lane < k_end - base;  // position base + lane
-            const float z = zp[lane];
+            // the iteration's positions are all banded or all in ray order
+            // (interleaved: all banded); the tables extend 63 entries past
+            // their last ray, so every lane loads from them (lanes at or past
+            // k_end only in the tile's last iteration, whose results they
+            // discard)
+            const float *__restrict__ zp = (cmap.inter || base + 63 <= full_end ? c.zs32 : c.z32) + (base - 1u);
+            const bool active = lane < (cmap.inter ? 64u : chunk_end - base);
+            const float z = zp[lane_off];
             const float y = hot.y_mean - hot.beam_w * z;
             const Trace32Result r = trace32<COUNT>(hot, y);
             if (COUNT && active) ops += r.ops;
             const bool flagged = active && r.guard;
             if (active && !r.guard && r.hit) atomicAdd((Cell *)(hist_col + (r.bin << cell_shift)), Acc<A>::of32(r.t));
-            if (last_pos - base < 64u && lane == last_pos - base && active && !r.guard)
-                set_last_bin(c.last_bin, r.hit ? r.bin : nbin);
-            const uint32_t k = base + lane;
+            if (it == last_it && lane == last_lane && !r.guard) set_last_bin(c.last_bin, r.hit ? r.bin : nbin);
+            const uint32_t k = base + lane_off;  // position
             const uint64_t mask = __builtin_amdgcn_ballot_w64(flagged);
             if (mask) {
                 if (flagged) c.queue[(tail + count_below(mask)) & (kQueueCap - 1)] = k;

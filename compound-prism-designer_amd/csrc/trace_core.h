@@ -638,21 +638,27 @@ CPD_HD double iv_mag(Iv a) { return fmax(fabs(a.lo), fabs(a.hi)); }
 // Whether a ray of the band can pass the acceptance test through lens root
 // `far` (0: -b - sqrt(disc), 1: -b + sqrt(disc)); *valid is set when the root
 // is certainly in front of the ray and inside the arc for every height of the
-// band (so the trace never looks at the other root).
-CPD_HD bool band_root_may_hit(const LensBand64 &L, Iv y, Iv b, Iv sd, Iv disc, bool far, bool *valid) {
+// band (so the trace never looks at the other root), *certain when, beyond
+// that, every ray of the band certainly refracts, passes the acceptance test
+// and meets the array's line between its ends, *never when the root is
+// behind the ray or outside the arc for every height of the band.
+CPD_HD bool band_root_may_hit(const LensBand64 &L, Iv y, Iv b, Iv sd, Iv disc, bool far, bool *valid,
+                              bool *certain, bool *never) {
     *valid = false;
+    *certain = false;
+    *never = false;
     // root distance d = -b -/+ sqrt(disc)
     const Iv nb{-b.hi, -b.lo};
     const Iv d = far ? iv_add(nb, sd) : iv_sub(nb, sd);
     const double dm = 1e-9 * (iv_mag(d) + iv_mag(b) + iv_mag(sd));
-    if (d.hi < -dm) return false;  // behind the ray
+    if (d.hi < -dm) return *never = true, false;  // behind the ray
     // bounds: |e + d dir|^2 <= max_dist_sq, e = e0 + e1 y
     const Iv qx = iv_add(iv_affine(L.e0x, L.e1x, y), iv_scale(d, L.dx));
     const Iv qy = iv_add(iv_affine(L.e0y, L.e1y, y), iv_scale(d, L.dy));
     const Iv qx2 = iv_sqr(qx), qy2 = iv_sqr(qy);
     const Iv s = iv_sub(iv_add(qx2, qy2), Iv{L.max_dist_sq, L.max_dist_sq});
     const double sm = 1e-9 * (L.max_dist_sq + qx2.hi + qy2.hi);
-    if (s.lo > sm) return false;  // outside the arc
+    if (s.lo > sm) return *never = true, false;  // outside the arc
     *valid = d.lo > dm && s.hi < -sm;
     // ci = -/+ sqrt(disc) / R, cr = sqrt(1 - r^2 (1 - ci^2)), k = r ci - cr
     const Iv cr2 = iv_add(Iv{L.lomr2, L.lomr2}, iv_scale(disc, L.lr2r));
@@ -683,14 +689,21 @@ CPD_HD bool band_root_may_hit(const LensBand64 &L, Iv y, Iv b, Iv sd, Iv disc, b
     const Iv px = iv_add(wx, iv_mul(vx, dist)), py = iv_add(wy, iv_mul(vy, dist));
     const Iv pos = iv_add(iv_scale(px, L.ddx), iv_scale(py, L.ddy));
     const double pm = 1e-9 * (iv_mag(px) + iv_mag(py) + L.length);
+    *certain = *valid && cr2.lo > 1e-9 && cd.lo > L.min_ci + cm && pos.lo > pm && pos.hi < L.length - pm;
     return !(pos.hi < -pm || pos.lo > L.length + pm);
 }
 
-CPD_HD bool band_may_hit(const LensBand64 &L, double ya, double yb) {
+// The band's class: kBandLost when band_may_hit would be false;
+// kBandCertain when every ray of it certainly passes the planar faces, takes
+// the near lens root and meets the array's line between its ends (so every
+// narrower band of heights inside it may hit as well); kBandMixed otherwise.
+enum { kBandLost = 0, kBandMixed = 1, kBandCertain = 2 };
+CPD_HD int band_class(const LensBand64 &L, double ya, double yb) {
     const Iv y{fmin(ya, yb), fmax(ya, yb)};
     // every height outside the accepted interval: lost at the planar faces
     const double ym = 1e-9 * (fmax(fabs(ya), fabs(yb)) + L.path);
-    if (y.hi < L.ylo - ym || y.lo > L.yhi + ym) return false;
+    if (y.hi < L.ylo - ym || y.lo > L.yhi + ym) return kBandLost;
+    const bool inside = y.lo > L.ylo + ym && y.hi < L.yhi - ym;
     const Iv b = iv_affine(L.lb0, L.lb1, y);
     // c(y) = lc0 + y (lc1 + lc2 y)
     const Iv c = iv_add(Iv{L.lc0, L.lc0}, iv_mul(y, iv_affine(L.lc1, L.lc2, y)));
@@ -702,17 +715,23 @@ CPD_HD bool band_may_hit(const LensBand64 &L, double ya, double yb) {
     const Iv ex = iv_affine(L.e0x, L.e1x, y), ey = iv_affine(L.e0y, L.e1y, y);
     const double em = iv_mag(ex) + iv_mag(ey);
     const double tol = 1e-14 * (L.radius + em) * (L.radius + em) + 1e-9 * (iv_mag(b) * iv_mag(b) + iv_mag(c));
-    if (disc.hi <= -tol) return false;
+    if (disc.hi <= -tol) return kBandLost;
     disc = Iv{disc.lo - tol, disc.hi + tol};
     const Iv sd = iv_sqrt0(disc);
     // the trace takes the near root wherever it is valid (nearest valid root
     // first, geometry.rs:203-209); where it is valid for the whole band the far
     // root is never looked at
-    bool near_valid, far_valid;
-    if (band_root_may_hit(L, y, b, sd, disc, false, &near_valid)) return true;
-    if (near_valid) return false;
-    return band_root_may_hit(L, y, b, sd, disc, true, &far_valid);
+    bool near_valid, near_certain, near_never, far_valid, far_certain, far_never;
+    if (band_root_may_hit(L, y, b, sd, disc, false, &near_valid, &near_certain, &near_never))
+        return inside && disc.lo > 0.0 && near_certain ? kBandCertain : kBandMixed;
+    if (near_valid) return kBandLost;
+    // the far root, which the trace takes where the near one is not valid
+    if (band_root_may_hit(L, y, b, sd, disc, true, &far_valid, &far_certain, &far_never))
+        return inside && disc.lo > 0.0 && near_never && far_certain ? kBandCertain : kBandMixed;
+    return kBandLost;
 }
+
+CPD_HD bool band_may_hit(const LensBand64 &L, double ya, double yb) { return band_class(L, ya, yb) != kBandLost; }
 
 // trace64 of one ray with the planar faces' directions and transmittances
 // taken from the wavelength's chain record (built by make_lambda_consts for a

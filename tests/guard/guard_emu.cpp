@@ -143,9 +143,12 @@ int guard_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t
 extern "C" {
 
 // The band test (band_may_hit, trace_core.h) over the banded beam order the
-// device traces in (within each block of 4096 rays sorted by height, 64 rays
-// to a band), as the kernel applies it: a band is skipped when its
-// super-band of 8 bands (512 positions) or the band itself is proven lost.
+// device traces in (within each block of 4096 rays sorted by height) and its
+// bands as the kernel forms them (an iteration's 64 rays: interleaved over
+// the four blocks of a whole 16384-position chunk, else 64 consecutive
+// positions): a band is skipped when its super-band of 8 bands or the band
+// itself is proven lost.  Heights are taken over every ray of a band (the
+// kernel bounds them by the ends of each sorted run).
 // stats[0] bands, [1] bands a wavefront would take past the planar-chain
 // test, [2] of those skipped, [3] skipped bands (any) holding an FP64 hit
 // (must be 0), [4] bands holding an FP64 hit.
@@ -162,6 +165,22 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
         std::sort(order.begin() + b, order.begin() + b + cpd::kBandRays, [&](uint32_t x, uint32_t y) {
             return z64[x] < z64[y] || (z64[x] == z64[y] && x < y);
         });
+    // the kernel's bands (0-based positions): a chunk of 16384 whole banded
+    // positions interleaved (band i: ranks 16 i .. 16 i + 15 of each block of
+    // 4096), otherwise 64 consecutive positions
+    const uint32_t kChunk = 4 * cpd::kBandRays;
+    std::vector<uint32_t> band_pos;  // 64 positions per band, in band order
+    std::vector<uint32_t> band_chunk;
+    for (uint32_t c0 = 0; c0 < full; c0 += kChunk) {
+        const bool inter = c0 + kChunk <= full;
+        const uint32_t len = inter ? kChunk : full - c0;
+        for (uint32_t b = 0; b < len / 64; ++b) {
+            for (uint32_t l = 0; l < 64; ++l)
+                band_pos.push_back(inter ? c0 + (l >> 4) * cpd::kBandRays + 16 * b + (l & 15) : c0 + 64 * b + l);
+            band_chunk.push_back(c0);
+        }
+    }
+    const size_t n_bands = band_chunk.size();
     std::atomic<size_t> next{0};
     std::atomic<uint64_t> tot[5];
     for (auto &t : tot) t = 0;
@@ -180,18 +199,28 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                 cpd::LensBand64 band;
                 cpd::make_lambda_consts(g, lambda, lc, &chain, &band);
                 const cpd::HotRecord hot = cpd::make_hot(td, lc);
+                // heights of bands b .. b + nb - 1 (one chunk, one super-band at most)
+                auto heights = [&](size_t b, size_t nb, double &lo, double &hi) {
+                    lo = INFINITY;
+                    hi = -INFINITY;
+                    for (size_t q = 64 * b; q < 64 * (b + nb); ++q) {
+                        const double y = cpd::beam_y64(g, z64[order[band_pos[q]]]);
+                        lo = fmin(lo, y);
+                        hi = fmax(hi, y);
+                    }
+                };
                 bool super_lost = false;
-                for (uint32_t p0 = 0; p0 < full; p0 += 64) {
+                for (size_t b = 0; b < n_bands; ++b) {
                     loc[0]++;
-                    if (p0 % 512 == 0) {
+                    if (b % 8 == 0) {
                         // the super-band of 8 bands the kernel tests first
-                        const double sa = cpd::beam_y64(g, z64[order[p0]]);
-                        const double sb = cpd::beam_y64(g, z64[order[p0 + 511]]);
+                        double sa, sb;
+                        heights(b, 8, sa, sb);
                         super_lost = lc.ok && !cpd::band_may_hit(band, sa, sb);
                     }
                     bool chain_pass = false, hit = false;
                     for (uint32_t q = 0; q < 64; ++q) {
-                        const uint32_t k = order[p0 + q];
+                        const uint32_t k = order[band_pos[64 * b + q]];
                         const float y = hot.y_mean - hot.beam_w * (float)z64[k];
                         const bool alive = (y >= hot.ylo) & (y <= hot.yhi);
                         const bool guard = fminf(fabsf(y - hot.ylo), fabsf(y - hot.yhi)) <= hot.ymarg;
@@ -201,8 +230,8 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                     }
                     loc[4] += hit;
                     if (!lc.ok) continue;
-                    const double ya = cpd::beam_y64(g, z64[order[p0]]);
-                    const double yb = cpd::beam_y64(g, z64[order[p0 + 63]]);
+                    double ya, yb;
+                    heights(b, 1, ya, yb);
                     const bool skipped = super_lost || !cpd::band_may_hit(band, ya, yb);
                     if (chain_pass) {
                         loc[1]++;

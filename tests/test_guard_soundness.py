@@ -108,15 +108,16 @@ def band_stats(designs, n_lambda, n_rays):
     return dict(zip(["bands", "past_chain", "skipped", "skipped_with_hit", "with_hit"], st[:]))
 
 
-@pytest.mark.parametrize("config,transmit", [(1, False), (2, False), (4, False), (0, True), (5, True), (6, True),
-                                             (7, True)])
-def test_skipped_bands_hold_no_hit(config, transmit):
+@pytest.mark.parametrize("config,transmit,n_rays", [(1, False, 16384), (2, False, 16384), (4, False, 16384),
+                                                    (0, True, 16384), (5, True, 16384), (6, True, 16384),
+                                                    (7, True, 16384), (1, False, 12288), (0, True, 12288)])
+def test_skipped_bands_hold_no_hit(config, transmit, n_rays):
     # the trace kernel skips a band of 64 banded rays when band_may_hit proves
     # (interval arithmetic on the lens and the acceptance cosine) that the FP64
     # trace loses every ray of it; no skipped band may hold an FP64 hit
     import compound_prism_designer as cpd
     D = transmitting(config, 48)[1] if transmit else cpd.generate_designs(config, 0, 96)[1]
-    st = band_stats(D, 16, 16384)
+    st = band_stats(D, 16, n_rays)
     print(f"config {config}: {st}, skipped {st['skipped'] / st['past_chain']:.3f} of the bands past the chain test")
     assert st["skipped_with_hit"] == 0, st
     assert st["skipped"] > 0
