@@ -54,6 +54,13 @@ struct BatchArgs {
     unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
     uint32_t items_per_fetch;    // consecutive items one counter fetch claims (0: 1)
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)
+    // per-wavelength records, which launch_trace fills before the trace (null:
+    // each work item builds its own): those of entry e at
+    // lrec[lrec_begin[e] + j - j0], j0 = item_offset[e] / n_tiles the first
+    // wavelength sample its items cover, or at lrec[lb_begin[slot] + j] when
+    // lrec_begin is null
+    LambdaRecord *lrec;
+    const uint64_t *lrec_begin;  // [n_entries] (may be null)
     uint32_t n_designs;          // entries
     uint32_t hist_words;         // max over the entries of nbin * hist_cols
     uint32_t row_words;          // max over the entries of nbin

@@ -72,14 +72,21 @@ __device__ __forceinline__ T in_vgpr(T v) {
     return v;
 }
 
+// The entry whose items hold `item`: the last e < n with begin[e] <= item
+// (begin ascending, begin[0] = 0 <= item).  All 64 lanes search together: each
+// round every lane tests one of 64 evenly spaced candidates, so n <= 4096
+// takes two rounds of one load each (a binary search takes log2 n dependent
+// loads).
 __device__ __forceinline__ uint32_t find_design(const uint64_t *begin, uint32_t n, uint64_t item) {
-    uint32_t lo = 0, hi = n;  // begin[lo] <= item < begin[hi]
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (begin[mid] <= item)
-            lo = mid;
-        else
-            hi = mid;
+    const uint32_t lane = threadIdx.x;
+    uint32_t lo = 0, cnt = n;  // the entry lies in [lo, lo + cnt)
+    while (cnt > 1) {
+        const uint32_t stride = (cnt + 63u) >> 6;
+        const uint32_t k = lane * stride;
+        const bool le = k < cnt && begin[lo + k] <= item;
+        const uint32_t c = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(le));  // >= 1 (lane 0)
+        lo += (c - 1u) * stride;
+        cnt = min(stride, cnt - (c - 1u) * stride);
     }
     return lo;
 }
@@ -545,16 +552,25 @@ __device__ __forceinline__ void trace_lds_clear(uint32_t *lds, uint32_t hist_wor
     __syncthreads();
 }
 
+// Copies a record of 8-byte words from global memory into LDS, 64 lanes at a
+// time (the caller synchronises before reading it).
+__device__ __forceinline__ void copy_to_lds(void *dst, const void *src, uint32_t bytes) {
+    const uint2 *s = (const uint2 *)src;
+    uint2 *d = (uint2 *)dst;
+    for (uint32_t i = threadIdx.x; i < bytes / 8u; i += 64) d[i] = s[i];
+}
+
 // One work item: wavelength sample j (at `lambda`) and ray tile `tile` of the
 // design (g, td); its row is added to acc[acc_j + b] and the bin of the
 // design's last ray, when the tile holds it, stored at *last_bin_j.  Args
 // supplies the beam tables (z32, zs32, z64, zs64, inv) and the LDS shape
 // (hist_words, row_words); the LDS regions are derived here from the kernel's
 // arguments, which keeps them out of the registers live across items.
-template <bool COUNT, int A, class Args>
+template <bool COUNT, int A, bool REC, class Args>
 __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd_design *g, const TraceDesign *td,
-                                         uint32_t j, uint32_t tile, double lambda, unsigned long long *acc,
-                                         uint64_t acc_j, uint32_t *last_bin_j, unsigned long long &ops) {
+                                         uint32_t j, uint32_t tile, double lambda, const LambdaRecord *rec,
+                                         unsigned long long *acc, uint64_t acc_j, uint32_t *last_bin_j,
+                                         unsigned long long &ops) {
     const TraceLds L = trace_lds<A>(lds, a.hist_words, a.row_words);
     const uint32_t lane = threadIdx.x;
     const uint32_t M = td->n_rays, nbin = td->nbin;
@@ -562,10 +578,20 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     TileCtx c;
     c.g = g;
     c.td = td;
-    // every lane computes the same constants; lane 0 stores the chain record
-    make_lambda_consts(*g, lambda, c.lc, lane == 0 ? L.chain : nullptr, lane == 0 ? L.band : nullptr);
-    if (lane == 0) *L.lc_lds = c.lc;
+    if constexpr (REC) {
+        // the wavelength's records, built before the launch's rays
+        (void)lambda;
+        copy_to_lds(L.chain, &rec->chain, sizeof(Chain64));
+        copy_to_lds(L.band, &rec->band, sizeof(LensBand64));
+        copy_to_lds(L.lc_lds, &rec->lc, sizeof(LambdaConsts));
+    } else {
+        // every lane computes the same constants; lane 0 stores the records
+        (void)rec;
+        make_lambda_consts(*g, lambda, c.lc, lane == 0 ? L.chain : nullptr, lane == 0 ? L.band : nullptr);
+        if (lane == 0) *L.lc_lds = c.lc;
+    }
     __syncthreads();  // the queue drains and the band tests of this item read the records
+    const bool lc_ok = REC ? L.lc_lds->ok != 0 : c.lc.ok != 0;
     c.chain = L.chain;
     c.band = L.band;
     c.masks = L.masks;
@@ -585,7 +611,7 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     c.zs64 = a.zs64;
     c.full_end = M / kBandRays * kBandRays;
     c.last_pos = M <= c.full_end ? a.inv[M - 1] : M;
-    if (!c.lc.ok || td->skip) {
+    if (!lc_ok || td->skip) {
         // every ray of this wavelength is lost in the planar faces, or the
         // design is a stand-in for one that failed construction
         if (c.k_end == M + 1 && lane == 0) set_last_bin(c.last_bin, nbin);
@@ -595,7 +621,9 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     }
 }
 
-template <bool COUNT, int A>
+// REC: the wavelength records come from a.lrec (built by lambda_records_kernel
+// before the launch) instead of each item building its own.
+template <bool COUNT, int A, bool REC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void trace_kernel(BatchArgs a) {
     extern __shared__ __align__(8) uint32_t lds[];
     trace_lds_clear<A>(lds, a.hist_words, a.row_words);
@@ -623,14 +651,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void tr
         const uint32_t n_tiles = td->n_tiles;
         const uint64_t local = item - a.item_begin[e] + (a.item_offset ? a.item_offset[e] : 0);
         const uint32_t j = (uint32_t)(local / n_tiles), tile = (uint32_t)(local % n_tiles);
-        const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
-        run_item<COUNT, A>(a, lds, g, td, j, tile, lambda, a.acc, a.acc_begin[d] + (uint64_t)j * td->nbin,
-                           a.last_bin + a.lb_begin[d] + j, ops);
+        const uint64_t lb = a.lb_begin[d] + j;
+        const double lambda = REC ? 0.0 : a.wavelengths ? a.wavelengths[j] : lambda_of(*g, qmc_u64(j + 1));
+        const LambdaRecord *rec = nullptr;
+        if (REC) {
+            const uint64_t j0 = a.item_offset ? a.item_offset[e] / n_tiles : 0;  // first sample of the entry
+            rec = a.lrec + (a.lrec_begin ? a.lrec_begin[e] + (j - j0) : lb);
+        }
+        run_item<COUNT, A, REC>(a, lds, g, td, j, tile, lambda, rec, a.acc, a.acc_begin[d] + (uint64_t)j * td->nbin,
+                                a.last_bin + lb, ops);
         if (!a.work_counter) break;
     }
     if (COUNT) {
         for (int off = 32; off > 0; off >>= 1) ops += __shfl_xor(ops, off);
         if (lane == 0 && ops) atomicAdd(a.ops, ops);
+    }
+}
+
+// The wavelength records of a launch (a.lrec): one workgroup per entry, one
+// lane per wavelength sample the entry's items cover, each running the FP64
+// make_lambda_consts an item would otherwise run on all 64 of its lanes.
+__global__ __launch_bounds__(64) void lambda_records_kernel(BatchArgs a) {
+    const uint32_t e = blockIdx.x;
+    const uint32_t d = a.slot ? a.slot[e] : e;
+    const cpd_design &g = a.designs[d];
+    const TraceDesign &td = a.traces[d];
+    const uint64_t n_items = a.item_begin[e + 1] - a.item_begin[e];
+    if (n_items == 0 || td.n_tiles == 0) return;
+    const uint64_t off = a.item_offset ? a.item_offset[e] : 0;
+    const uint32_t j0 = (uint32_t)(off / td.n_tiles), j1 = (uint32_t)((off + n_items - 1) / td.n_tiles);
+    for (uint32_t j = j0 + threadIdx.x; j <= j1; j += 64) {
+        const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(g, qmc_u64(j + 1));
+        LambdaRecord &r = a.lrec[a.lrec_begin ? a.lrec_begin[e] + (j - j0) : a.lb_begin[d] + j];
+        LambdaConsts lc;
+        make_lambda_consts(g, lambda, lc, &r.chain, &r.band);
+        r.lc = lc;
     }
 }
 
@@ -987,9 +1042,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void re
         const uint32_t local = x - pre[q];
         const uint32_t j = local / td->n_tiles, tile = local - j * td->n_tiles;
         if (j < td->n_lambda) {
-            run_item<false, CPD_ACC_EXACT>(a, lds, g, td, j, tile, lambda_of(*g, qmc_u64(j + 1)), a.acc,
-                                           a.acc_begin[slot] + (uint64_t)j * td->nbin,
-                                           a.last_bin + a.lb_begin[slot] + j, ops);
+            run_item<false, CPD_ACC_EXACT, false>(a, lds, g, td, j, tile, lambda_of(*g, qmc_u64(j + 1)), nullptr,
+                                                  a.acc, a.acc_begin[slot] + (uint64_t)j * td->nbin,
+                                                  a.last_bin + a.lb_begin[slot] + j, ops);
         } else if (lane == 0) {
             a.host->error = 2;
         }
@@ -1082,15 +1137,29 @@ hipError_t launch_build_traces(hipStream_t s, const cpd_design *designs, const c
 
 
 namespace {
-template <bool COUNT, int A>
-hipError_t launch_trace_t(hipStream_t s, const BatchArgs &a, uint32_t grid, hipEvent_t start, hipEvent_t stop) {
+template <bool COUNT, int A, bool REC>
+hipError_t launch_trace_r(hipStream_t s, const BatchArgs &a, uint32_t grid, hipEvent_t start, hipEvent_t stop) {
     const size_t lds = trace_lds_bytes(a.hist_words, a.row_words, A);
     if (!start && !stop)
         // plain launch: the form a stream capture records as a kernel node
-        hipLaunchKernelGGL((trace_kernel<COUNT, A>), dim3(grid), dim3(64), lds, s, a);
+        hipLaunchKernelGGL((trace_kernel<COUNT, A, REC>), dim3(grid), dim3(64), lds, s, a);
     else
-        hipExtLaunchKernelGGL((trace_kernel<COUNT, A>), dim3(grid), dim3(64), lds, s, start, stop, 0, a);
+        hipExtLaunchKernelGGL((trace_kernel<COUNT, A, REC>), dim3(grid), dim3(64), lds, s, start, stop, 0, a);
     return hipGetLastError();
+}
+template <bool COUNT, int A>
+hipError_t launch_trace_t(hipStream_t s, const BatchArgs &a, uint32_t grid, hipEvent_t start, hipEvent_t stop) {
+    if (a.lrec) {
+        // the wavelength records first (outside the events, which time the trace)
+        hipLaunchKernelGGL(lambda_records_kernel, dim3(a.n_designs), dim3(64), 0, s, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_trace_r<COUNT, A, true>(s, a, grid, start, stop);
+    }
+    // only the batches without record storage (the allocation-free entry
+    // point) build the records per item; they do not count operations
+    if (COUNT) return hipErrorInvalidValue;
+    return launch_trace_r<false, A, false>(s, a, grid, start, stop);
 }
 template <bool COUNT>
 hipError_t launch_trace_c(hipStream_t s, const BatchArgs &a, uint32_t grid, int acc, hipEvent_t start,

@@ -46,7 +46,7 @@ struct Plan {
     std::vector<cpd_design> shape;      // per slot: a valid record of the slot's shape
     std::vector<cpd_sampling> sampling;  // per slot
     std::vector<uint32_t> perm;         // staging position -> slot
-    DeviceBuffer d_fixed, d_designs, d_traces, d_acc, d_last_bin, d_results, d_counter;
+    DeviceBuffer d_fixed, d_designs, d_traces, d_acc, d_last_bin, d_results, d_counter, d_lrec;
     BeamTables beam;
     HostBuffer h_designs, h_out;
     hipGraph_t graph = nullptr;
@@ -89,6 +89,7 @@ Plan::~Plan() {
     d_traces.release();
     d_acc.release();
     d_last_bin.release();
+    d_lrec.release();
     d_results.release();
     d_counter.release();
     beam.release();
@@ -164,6 +165,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     if ((st = d_traces.reserve(n * sizeof(TraceDesign), false, stream)) != CPD_OK) return st;
     if ((st = d_acc.reserve(acc_n * sizeof(unsigned long long), true, stream)) != CPD_OK) return st;
     if ((st = d_last_bin.reserve(lb_n * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
+    if ((st = d_lrec.reserve(lb_n * sizeof(LambdaRecord), false, stream)) != CPD_OK) return st;
     if ((st = d_results.reserve(n * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
     if ((st = d_counter.reserve(256, true, stream)) != CPD_OK) return st;
     if ((st = h_designs.reserve(n * sizeof(cpd_design))) != CPD_OK) return st;
@@ -191,6 +193,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     a.zs64 = beam.zs64.as<double>();
     a.inv = beam.inv.as<uint32_t>();
     a.work_counter = d_counter.as<unsigned int>();
+    a.lrec = d_lrec.as<LambdaRecord>();
     a.n_designs = n;
     a.hist_words = hist_words;
     a.row_words = row_words;

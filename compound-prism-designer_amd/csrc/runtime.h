@@ -67,6 +67,7 @@ struct Context {
     BeamTables beam;
     // batch state
     DeviceBuffer stage, acc, last_bin, results, counters, row_buf, prop_buf, build_buf;
+    DeviceBuffer lrec;         // per-(design, wavelength) records of a batch (LambdaRecord)
     // adaptive estimators: row descriptors, design records, seeded beam tables
     DeviceBuffer row_stage, design_stage, seed_z32, seed_z64;
     HostBuffer host_stage, host_out;

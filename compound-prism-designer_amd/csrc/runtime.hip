@@ -312,6 +312,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     if ((st = acc.reserve(acc_n * sizeof(unsigned long long), true, stream)) != CPD_OK) return st;
     if ((st = last_bin.reserve(lb_n * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
     if ((st = results.reserve(n * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
+    if ((st = lrec.reserve(lb_n * sizeof(LambdaRecord), false, stream)) != CPD_OK) return st;
     if ((st = counters.reserve(64, true, stream)) != CPD_OK) return st;
     if (rows && (st = row_buf.reserve(acc_n * sizeof(double), false, stream)) != CPD_OK) return st;
 
@@ -337,6 +338,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.inv = beam.inv.as<uint32_t>();
     a.ops = want_ops ? dops : nullptr;
     a.wavelengths = n_wl ? (const double *)(dstage + off_wl) : nullptr;
+    a.lrec = lrec.as<LambdaRecord>();
     a.n_designs = n;
     a.hist_words = hist_words;
     a.row_words = row_words;
@@ -405,6 +407,7 @@ Context::~Context() {
     stage.release();
     acc.release();
     last_bin.release();
+    lrec.release();
     results.release();
     counters.release();
     row_buf.release();

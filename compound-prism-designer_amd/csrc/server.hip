@@ -118,6 +118,7 @@ struct Request {
 struct Entry {
     uint32_t slot;
     uint64_t offset, count;
+    uint64_t n_rec;  // wavelength samples its items cover (its records in the step)
 };
 
 }  // namespace
@@ -130,6 +131,7 @@ struct Server {
     uint32_t waves_per_cu = 20;
     uint64_t max_designs = 0, acc_cap = 0, lb_cap = 0;
     DeviceBuffer d_designs, d_traces, d_acc_begin, d_lb_begin, d_acc, d_last_bin, d_results, d_step, d_counter;
+    DeviceBuffer d_lrec;  // the wavelength records of a step
     BeamTables beam;
     HostBuffer h_stage, h_results;
     RangeAlloc slots, accs, lbs;
@@ -574,7 +576,9 @@ cpd_status Server::step() {
         while (left) {
             const uint64_t avail = r.items[r.cursor] - r.cursor_item;
             const uint64_t t = std::min(avail, left);
-            entries.push_back(Entry{(uint32_t)(r.slot0 + r.cursor), r.cursor_item, t});
+            const uint64_t nt = r.traces[r.cursor].n_tiles;
+            entries.push_back(Entry{(uint32_t)(r.slot0 + r.cursor), r.cursor_item, t,
+                                    (r.cursor_item + t - 1) / nt - r.cursor_item / nt + 1});
             hist_words = std::max(hist_words, r.traces[r.cursor].hist_cols * r.traces[r.cursor].nbin);
             row_words = std::max(row_words, r.traces[r.cursor].nbin);
             left -= t;
@@ -590,11 +594,12 @@ cpd_status Server::step() {
             for (size_t q = 0; q < r.designs.size(); ++q) finishing.push_back((uint32_t)(r.slot0 + q));
         }
     }
-    // step arrays: slot[E] | item_begin[E+1] | item_offset[E] | finalize slots[F]
+    // step arrays: slot[E] | item_begin[E+1] | item_offset[E] | lrec_begin[E] | finalize slots[F]
     const size_t E = entries.size(), F = finishing.size();
     const size_t off_begin = (E * sizeof(uint32_t) + 255) / 256 * 256;
     const size_t off_offset = off_begin + ((E + 1) * sizeof(uint64_t) + 255) / 256 * 256;
-    const size_t off_fin = off_offset + (E * sizeof(uint64_t) + 255) / 256 * 256;
+    const size_t off_rec = off_offset + (E * sizeof(uint64_t) + 255) / 256 * 256;
+    const size_t off_fin = off_rec + (E * sizeof(uint64_t) + 255) / 256 * 256;
     const size_t bytes = off_fin + F * sizeof(uint32_t) + 256;
     SRV_TRY(hipStreamSynchronize(stream));  // staging reuse after uploads
     if ((st = h_stage.reserve(bytes)) != CPD_OK) return st;
@@ -603,14 +608,18 @@ cpd_status Server::step() {
     uint32_t *hslot = (uint32_t *)h;
     uint64_t *hbegin = (uint64_t *)(h + off_begin);
     uint64_t *hoff = (uint64_t *)(h + off_offset);
+    uint64_t *hrec = (uint64_t *)(h + off_rec);
     uint32_t *hfin = (uint32_t *)(h + off_fin);
-    uint64_t items = 0;
+    uint64_t items = 0, n_rec = 0;
     for (size_t e = 0; e < E; ++e) {
         hslot[e] = entries[e].slot;
         hbegin[e] = items;
         hoff[e] = entries[e].offset;
+        hrec[e] = n_rec;
         items += entries[e].count;
+        n_rec += entries[e].n_rec;
     }
+    if ((st = d_lrec.reserve(n_rec * sizeof(LambdaRecord), false, stream)) != CPD_OK) return st;
     hbegin[E] = items;
     for (size_t f = 0; f < F; ++f) hfin[f] = finishing[f];
     char *d = (char *)d_step.ptr;
@@ -632,6 +641,8 @@ cpd_status Server::step() {
     a.zs64 = beam.zs64.as<double>();
     a.inv = beam.inv.as<uint32_t>();
     a.work_counter = d_counter.as<unsigned int>();
+    a.lrec = d_lrec.as<LambdaRecord>();
+    a.lrec_begin = (const uint64_t *)(d + off_rec);
     a.n_designs = (uint32_t)E;
     a.hist_words = hist_words;
     a.row_words = row_words;
@@ -882,7 +893,7 @@ void cpd_server_destroy(cpd_server *srv) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (cpd::DeviceBuffer *b : {&s->d_designs, &s->d_traces, &s->d_acc_begin, &s->d_lb_begin, &s->d_acc,
-                                 &s->d_last_bin, &s->d_results, &s->d_step, &s->d_counter, &s->d_items_total,
+                                 &s->d_last_bin, &s->d_results, &s->d_step, &s->d_counter, &s->d_lrec, &s->d_items_total,
                                  &s->d_items_done, &s->d_slot_tag, &s->d_ingested, &s->d_claim, &s->d_chunk_slot0,
                                  &s->d_chunk_n, &s->d_prefix, &s->d_ctrl})
         b->release();

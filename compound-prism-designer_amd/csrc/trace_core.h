@@ -356,6 +356,16 @@ struct LensBand64 {
     double ylo, yhi, path;           // accepted beam heights; length scale of the planar chain
 };
 
+// The three per-wavelength records of one (design, wavelength sample), as
+// make_lambda_consts builds them; a batch computes them once per wavelength,
+// one lane each, before the rays (every work item of that wavelength copies
+// them into its LDS).
+struct alignas(16) LambdaRecord {
+    LambdaConsts lc;
+    Chain64 chain;
+    LensBand64 band;
+};
+
 // Build the FP32 record from the FP64 design (host or device).
 // tile_cap (a multiple of 64, 0 for none) bounds the rays of one work item.
 CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n_rays, TraceDesign &t,
