@@ -186,6 +186,11 @@ CPD_HD inline size_t trace_lds_bytes_d(uint32_t hist_words, uint32_t row_words, 
            (kMaskChunk >> 12) * sizeof(uint64_t) + sizeof(LambdaConsts);
 }
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc);
+// The resident loop's LDS: the trace regions, then copies of the claimed
+// design's records (cpd_design, TraceDesign) at this 16-byte aligned offset.
+CPD_HD inline size_t resident_records_offset(uint32_t hist_words, uint32_t row_words) {
+    return (trace_lds_bytes_d(hist_words, row_words, CPD_ACC_EXACT) + 15) & ~(size_t)15;
+}
 
 // start/stop (may be null) are recorded immediately around the kernel; with
 // neither, the launch is a plain one that stream capture records.

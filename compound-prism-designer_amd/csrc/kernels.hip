@@ -854,6 +854,25 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long v) {
 __device__ __forceinline__ unsigned long long agent_load(unsigned long long *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint64_t agent_load(const uint64_t *p) {
+    return __hip_atomic_load((uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t agent_load32(const uint32_t *p) {
+    return __hip_atomic_load((uint32_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A record of 8-byte words in device memory copied into LDS with agent-scope
+// loads (the caller synchronises before reading it).
+__device__ __forceinline__ void agent_copy_to_lds(void *dst, const void *src, uint32_t bytes) {
+    const uint64_t *s = (const uint64_t *)src;
+    uint64_t *d = (uint64_t *)dst;
+    for (uint32_t i = threadIdx.x; i < bytes / 8u; i += 64) d[i] = agent_load(s + i);
+}
+static_assert(sizeof(cpd_design) % 16 == 0 && sizeof(TraceDesign) % 8 == 0,
+              "record copies of 8-byte words, the trace record 16-byte aligned after the design");
+// The LDS copies of the claimed design's records follow the trace regions.
+__device__ __forceinline__ cpd_design *resident_design_lds(uint32_t *lds, uint32_t hist_words, uint32_t row_words) {
+    return (cpd_design *)((char *)lds + resident_records_offset(hist_words, row_words));
+}
 
 // Copies chunk c's staged records (host memory) into the pools, builds its
 // item table and publishes its claim word.  Returns false on an inconsistent
@@ -1025,26 +1044,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void re
         }
         idle = 0;
         pos_x = bcast64(pos_x);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the chunk's tables and records
+        // No acquire fence here: one at agent scope would invalidate the L2 of
+        // this wavefront's XCD (the beam tables the other wavefronts read
+        // included) at every claim.  What the ingesting wavefront wrote before
+        // releasing the claim word -- the chunk's tables and the slot's records
+        // -- is read with agent-scope loads instead, which are coherent across
+        // the XCDs, and the records are copied into LDS for the item.
         const uint32_t pos = (uint32_t)(pos_x >> 32), x = (uint32_t)pos_x;
-        const uint32_t *pre = a.prefix + (size_t)pos * 65;
-        const uint32_t n = a.chunk_n[pos];
-        // the design holding item x: the number of designs whose items end at or before x
-        const uint64_t before = __builtin_amdgcn_ballot_w64(lane < n && pre[lane + 1] <= x);
+        const uint32_t n = agent_load32(a.chunk_n + pos);
+        const uint32_t slot0 = agent_load32(a.chunk_slot0 + pos);
+        // the design holding item x: the number of designs whose items end at
+        // or before x (pre_l: lane's inclusive prefix)
+        const uint32_t pre_l = lane < n ? agent_load32(a.prefix + (size_t)pos * 65 + lane + 1) : 0xffffffffu;
+        const uint64_t before = __builtin_amdgcn_ballot_w64(lane < n && pre_l <= x);
         const uint32_t q = (uint32_t)__builtin_popcountll(before);
-        const uint32_t slot = a.chunk_slot0[pos] + q;
+        const uint32_t slot = slot0 + q;
         if (q >= n || slot >= a.max_designs) {
             if (lane == 0) a.host->error = 3;
             continue;
         }
-        const cpd_design *g = a.designs + slot;
-        const TraceDesign *td = a.traces + slot;
-        const uint32_t local = x - pre[q];
-        const uint32_t j = local / td->n_tiles, tile = local - j * td->n_tiles;
+        const uint32_t pre_q = q ? (uint32_t)__shfl(pre_l, (int)q - 1) : 0u;  // items before design q
+        cpd_design *g = resident_design_lds(lds, a.hist_words, a.row_words);
+        TraceDesign *td = (TraceDesign *)(g + 1);
+        agent_copy_to_lds(g, a.designs + slot, sizeof(cpd_design));
+        agent_copy_to_lds(td, a.traces + slot, sizeof(TraceDesign));
+        uint64_t acc_b = 0, lb_b = 0;
+        if (lane == 0) {
+            acc_b = agent_load(a.acc_begin + slot);
+            lb_b = agent_load(a.lb_begin + slot);
+        }
+        acc_b = bcast64(acc_b);
+        lb_b = bcast64(lb_b);
+        __syncthreads();  // the record copies
+        const uint32_t local = x - pre_q;
+        const uint32_t n_tiles = td->n_tiles;
+        const uint32_t j = local / n_tiles, tile = local - j * n_tiles;
         if (j < td->n_lambda) {
             run_item<false, CPD_ACC_EXACT, false>(a, lds, g, td, j, tile, lambda_of(*g, qmc_u64(j + 1)), nullptr,
-                                                  a.acc, a.acc_begin[slot] + (uint64_t)j * td->nbin,
-                                                  a.last_bin + a.lb_begin[slot] + j, ops);
+                                                  a.acc, acc_b + (uint64_t)j * td->nbin, a.last_bin + lb_b + j, ops);
         } else if (lane == 0) {
             a.host->error = 2;
         }
@@ -1053,7 +1090,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void re
         if (lane == 0) {
             const unsigned int done =
                 __hip_atomic_fetch_add(a.items_done + slot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
-            last = done == a.items_total[slot];
+            last = done == agent_load32(a.items_total + slot);
         }
         if (bcast64(last)) resident_finalize(a, slot);
     }
@@ -1196,7 +1233,7 @@ hipError_t launch_finalize(hipStream_t s, const FinalizeArgs &a, int acc) {
 }
 
 size_t resident_lds_bytes(uint32_t hist_words, uint32_t row_words) {
-    return trace_lds_bytes(hist_words, row_words, CPD_ACC_EXACT);
+    return resident_records_offset(hist_words, row_words) + sizeof(cpd_design) + sizeof(TraceDesign);
 }
 
 hipError_t resident_occupancy(size_t lds, int *per_cu) {
