@@ -159,8 +159,13 @@ template <> struct Acc<CPD_ACC_F64> {
     static __device__ uint32_t fold_every(uint32_t) { return 0xffffffffu; }
 };
 
-// The bin of a design's last ray for one wavelength.
-__device__ __forceinline__ void set_last_bin(uint32_t *p, uint32_t b) { *p = b; }
+// The bin of a design's last ray for one wavelength: an agent-scope store,
+// which reaches the memory side before it completes (the resident loop hands
+// a finished item to its design's finalizer, possibly on another XCD, without
+// writing back its L2).
+__device__ __forceinline__ void set_last_bin(uint32_t *p, uint32_t b) {
+    __hip_atomic_store(p, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 struct TileCtx {
     const cpd_design *g;
@@ -1085,7 +1090,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void re
         } else if (lane == 0) {
             a.host->error = 2;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this item's sums and last bin
+        // The item's outputs are agent-scope atomics (its sums) and an
+        // agent-scope store (its last bin), performed on the memory side; once
+        // they have completed, the count below can hand the design to its
+        // finalizer.  A release fence at agent scope would instead write back
+        // this XCD's whole L2 at every item (buffer_wbl2), which stalls every
+        // wavefront of the XCD: it halved the loop's throughput.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
         unsigned long long last = 0;
         if (lane == 0) {
             const unsigned int done =
