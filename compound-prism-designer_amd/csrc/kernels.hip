@@ -355,6 +355,53 @@ __device__ void band_masks(const TileCtx &c, const ChunkMap &m, uint64_t *masks)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 }
 
+// The iterations a chunk traces, in order: those below n_it whose band is not
+// skipped (MASKS: the chunk's skip masks, one 64-bit word per 64 iterations,
+// in LDS; otherwise every iteration).  The masks are wave-uniform: the walk
+// keeps the current word in scalar registers.
+template <bool MASKS>
+struct LiveWalk {
+    const uint64_t *masks;
+    uint64_t live;  // untraced iterations of word w still to come
+    uint32_t w, words, n_it;
+    __device__ __forceinline__ LiveWalk(const uint64_t *m, uint32_t n) : masks(m), live(0), w(0), n_it(n) {
+        words = (n + 63u) >> 6;
+        if (words) live = word(0);
+    }
+    __device__ __forceinline__ uint64_t word(uint32_t q) const {
+        uint64_t m = 0;
+        if (MASKS) {
+            const uint64_t v = masks[q];
+            // (readfirstlane returns int: each half is taken as u32, so the
+            // low half is not sign-extended into the high one)
+            m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+        }
+        const uint32_t left = n_it - 64u * q;
+        return ~m & (left >= 64u ? ~0ull : (1ull << left) - 1ull);
+    }
+    // whether iteration i (< n_it) is traced
+    __device__ __forceinline__ bool traced(uint32_t i) const { return (word(i >> 6) >> (i & 63u)) & 1u; }
+    // the next traced iteration, n_it when none is left
+    __device__ __forceinline__ uint32_t next() {
+        while (!live) {
+            if (++w >= words) return n_it;
+            live = word(w);
+        }
+        const uint32_t i = 64u * w + (uint32_t)__builtin_ctzll(live);
+        live &= live - 1ull;
+        return i;
+    }
+};
+
+// Beam quantile of this lane's ray in the iteration whose positions start at
+// base (1-based): banded when the iteration lies within the banded blocks.
+__device__ __forceinline__ float beam_z(const float *__restrict__ zs32, const float *__restrict__ z32, uint32_t base,
+                                        uint32_t full_end, bool inter, uint32_t lane_off) {
+    const float *__restrict__ zp = (inter || base + 63 <= full_end ? zs32 : z32) + (base - 1u);
+    return zp[lane_off];
+}
+
 template <bool COUNT, int A>
 __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     using Cell = typename Acc<A>::Cell;
@@ -412,33 +459,29 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
             last_it = cmap.inter ? (off & (kBandRays - 1u)) >> 4 : off >> 6;
             last_lane = cmap.inter ? ((off >> 12) << 4) + (off & 15u) : off & 63u;
         }
-        uint64_t skip = 0;
         const uint32_t n_it = (chunk_end - chunk + 63u) >> 6;
-        for (uint32_t it = 0; it < n_it; ++it) {
+        LiveWalk<!COUNT> walk(c.masks, n_it);
+        // the last ray in a skipped band misses with the rest of it
+        if (!COUNT && last_it != ~0u && !walk.traced(last_it) && lane == 0) set_last_bin(c.last_bin, nbin);
+        // the iteration's positions are all banded or all in ray order
+        // (interleaved: all banded); the tables extend 63 entries past their
+        // last ray, so every lane loads from them (lanes at or past k_end only
+        // in the tile's last iteration, whose results they discard)
+        const bool inter = cmap.inter;
+        const float *__restrict__ zs32 = c.zs32;
+        const float *__restrict__ z32 = c.z32;
+        uint32_t it = walk.next();
+        float z = it < n_it ? beam_z(zs32, z32, chunk + it * step, full_end, inter, lane_off) : 0.f;
+        while (it < n_it) {
+            // the next traced iteration's beam height is loaded a whole
+            // iteration ahead, so the trace never waits for it
+            // (loaded unconditionally -- the current iteration's positions
+            // stand in past the chunk's end -- so one load is in flight on
+            // every path and the trace waits only for the older one)
+            const uint32_t next = walk.next();
+            const float z_next = beam_z(zs32, z32, chunk + (next < n_it ? next : it) * step, full_end, inter, lane_off);
             const uint32_t base = chunk + it * step;
-            if (!COUNT) {
-                // the chunk's masks are wave-uniform: scalar registers
-                if ((it & 63u) == 0) {
-                    const uint64_t m = c.masks[it >> 6];
-                    // (readfirstlane returns int: each half is taken as u32,
-                    // so the low half is not sign-extended into the high one)
-                    skip = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m >> 32)) << 32) |
-                           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m);
-                }
-                if ((skip >> (it & 63u)) & 1u) {
-                    // every ray of the band misses: only the last ray's bin is recorded
-                    if (it == last_it && lane == 0) set_last_bin(c.last_bin, nbin);
-                    continue;
-                }
-            }
-            // the iteration's positions are all banded or all in ray order
-            // (interleaved: all banded); the tables extend 63 entries past
-            // their last ray, so every lane loads from them (lanes at or past
-            // k_end only in the tile's last iteration, whose results they
-            // discard)
-            const float *__restrict__ zp = (cmap.inter || base + 63 <= full_end ? c.zs32 : c.z32) + (base - 1u);
             const bool active = lane < (cmap.inter ? 64u : chunk_end - base);
-            const float z = zp[lane_off];
             const float y = hot.y_mean - hot.beam_w * z;
             const Trace32Result r = trace32<COUNT>(hot, y);
             if (COUNT && active) ops += r.ops;
@@ -461,6 +504,8 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
                 fold_columns<A>(c.hist, c.row, nbin, cols);
                 since_fold = 0;
             }
+            it = next;
+            z = z_next;
         }
     }
     if (tail != head) {
