@@ -672,9 +672,11 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
 }
 
 // REC: the wavelength records come from a.lrec (built by lambda_records_kernel
-// before the launch) instead of each item building its own.
+// before the launch) instead of each item building its own.  Six wavefronts
+// per SIMD (80 registers; the ray loop holds no spill at that budget, and the
+// LDS of 24 wavefronts fits a CU up to 128 bins).
 template <bool COUNT, int A, bool REC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void trace_kernel(BatchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void trace_kernel(BatchArgs a) {
     extern __shared__ __align__(8) uint32_t lds[];
     trace_lds_clear<A>(lds, a.hist_words, a.row_words);
     const uint32_t lane = threadIdx.x;

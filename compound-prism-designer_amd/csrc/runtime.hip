@@ -188,7 +188,7 @@ cpd_status Context::autotune(const cpd_design *designs, const cpd_sampling *samp
     double cur_ms;
     st = time_of(cur, cur_ms);
     const uint32_t orders[] = {CPD_ORDER_GIVEN};
-    const uint32_t waves[] = {4, 8, 16, 20};
+    const uint32_t waves[] = {4, 8, 16, 20, 24};
     const uint32_t tiles[] = {16384, 8192, 4096};
     for (uint32_t o : orders) {
         if (st != CPD_OK) break;

@@ -245,7 +245,7 @@ CPD_HD uint32_t trace64(const cpd_design &g, double wavelength, double y, double
 // ---------------------------------------------------------------------------
 // Per-wavefront LDS histogram of one work item: `cols` columns per bin (lane
 // l adds into column l % cols), at most kHistWords words for bin counts up to
-// 1024: 4 KB, so that a CU's LDS holds the twenty wavefronts (five per SIMD)
+// 1024: 4 KB, so that a CU's LDS holds the 24 wavefronts (six per SIMD)
 // the trace kernel runs with.
 static constexpr uint32_t kHistWords = 1024;
 // Rays per work item: at most kMaxTileRays; kDefaultTileRays unless a

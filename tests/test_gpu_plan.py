@@ -55,7 +55,7 @@ def test_autotune_returns_a_candidate_schedule():
     best, ms = cpd.autotune(D, S)
     assert ms > 0
     assert best.tile_rays in (0, 16384, 8192, 4096)
-    assert best.waves_per_cu in (4, 8, 16, 20)
+    assert best.waves_per_cu in (4, 8, 16, 20, 24)
     assert best.order in (N.ORDER_GIVEN, N.ORDER_COST)
 
 
