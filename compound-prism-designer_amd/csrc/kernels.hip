@@ -743,12 +743,30 @@ __global__ __launch_bounds__(64) void lambda_records_kernel(BatchArgs a) {
 
 // p(d|l) of one accumulator cell (the bin of the last ray divides by M + 1).
 template <int A>
-__device__ double cell_p(const unsigned long long *acc, uint64_t idx, double div) {
-    if (A == CPD_ACC_EXACT) return (double)acc[idx] * (1.0 / kQuantScale) / div;
-    if (A == CPD_ACC_F64) return *(const double *)(acc + idx) / div;
-    return (double)(*(const float *)(acc + idx) / (float)div);
+__device__ double cell_p(unsigned long long raw, double div) {
+    if (A == CPD_ACC_EXACT) return (double)raw * (1.0 / kQuantScale) / div;
+    if (A == CPD_ACC_F64) return __longlong_as_double((long long)raw) / div;
+    return (double)(__uint_as_float((uint32_t)raw) / (float)div);  // the cell's first four bytes
 }
 
+// The cells (j, b), j = j0 .. j0 + n - 1 (n <= kFinalizeBatch), of one bin and
+// the last-ray bins of their wavelengths, loaded together before they are
+// summed in order (the summation would otherwise wait for each load in turn).
+constexpr uint32_t kFinalizeBatch = 16;
+struct CellBatch {
+    unsigned long long raw[kFinalizeBatch];
+    uint32_t last[kFinalizeBatch];
+};
+__device__ __forceinline__ void load_cells(const unsigned long long *acc, const uint32_t *lb, uint32_t nbin, uint32_t b,
+                                           uint32_t j0, uint32_t n, CellBatch &c) {
+#pragma unroll
+    for (uint32_t q = 0; q < kFinalizeBatch; ++q) {
+        if (q < n) {
+            c.raw[q] = acc[(uint64_t)(j0 + q) * nbin + b];
+            c.last[q] = lb[j0 + q];
+        }
+    }
+}
 // One wavefront reduces a design's sums (N wavelengths x nbin cells at acc,
 // the last ray's bins at lb) to its information, writes p(d|l) rows when
 // `rows` is set, and leaves the cells zero and the last bins at nbin for the
@@ -762,13 +780,21 @@ __device__ __forceinline__ double finalize_design(unsigned long long *acc, uint3
         CSum h_cond, h_det;
         for (uint32_t b = lane; b < nbin; b += 64) {
             CSum pb;
-            for (uint32_t j = 0; j < N; ++j) {
-                const uint64_t idx = (uint64_t)j * nbin + b;
-                const double p = cell_p<A>(acc, idx, lb[j] == b ? (double)M + 1.0 : (double)M);
-                h_cond.add(plog2p(p));
-                pb.add(p);
-                if (rows) rows[idx] = p;
-                acc[idx] = 0;
+            for (uint32_t j0 = 0; j0 < N; j0 += kFinalizeBatch) {
+                const uint32_t n = min(kFinalizeBatch, N - j0);
+                CellBatch cb;
+                load_cells(acc, lb, nbin, b, j0, n, cb);
+#pragma unroll
+                for (uint32_t q = 0; q < kFinalizeBatch; ++q) {
+                    if (q < n) {
+                        const uint64_t idx = (uint64_t)(j0 + q) * nbin + b;
+                        const double p = cell_p<A>(cb.raw[q], cb.last[q] == b ? (double)M + 1.0 : (double)M);
+                        h_cond.add(plog2p(p));
+                        pb.add(p);
+                        if (rows) rows[idx] = p;
+                        acc[idx] = 0;
+                    }
+                }
             }
             h_det.add(plog2p(pb.value() / (double)N));
         }
@@ -795,13 +821,21 @@ __device__ __forceinline__ double finalize_design(unsigned long long *acc, uint3
         float h_cond = 0.f, h_det = 0.f;
         for (uint32_t b = lane; b < nbin; b += 64) {
             float pb = 0.f;
-            for (uint32_t j = 0; j < N; ++j) {
-                const uint64_t idx = (uint64_t)j * nbin + b;
-                const float p = (float)cell_p<A>(acc, idx, lb[j] == b ? (double)M + 1.0 : (double)M);
-                h_cond += plog2pf(p);
-                pb += p;
-                if (rows) rows[idx] = p;
-                acc[idx] = 0;
+            for (uint32_t j0 = 0; j0 < N; j0 += kFinalizeBatch) {
+                const uint32_t n = min(kFinalizeBatch, N - j0);
+                CellBatch cb;
+                load_cells(acc, lb, nbin, b, j0, n, cb);
+#pragma unroll
+                for (uint32_t q = 0; q < kFinalizeBatch; ++q) {
+                    if (q < n) {
+                        const uint64_t idx = (uint64_t)(j0 + q) * nbin + b;
+                        const float p = (float)cell_p<A>(cb.raw[q], cb.last[q] == b ? (double)M + 1.0 : (double)M);
+                        h_cond += plog2pf(p);
+                        pb += p;
+                        if (rows) rows[idx] = p;
+                        acc[idx] = 0;
+                    }
+                }
             }
             h_det += plog2pf(pb / (float)N);
         }
@@ -814,13 +848,21 @@ __device__ __forceinline__ double finalize_design(unsigned long long *acc, uint3
         CSumT<float> h_cond, h_det;
         for (uint32_t b = lane; b < nbin; b += 64) {
             CSumT<float> pb;
-            for (uint32_t j = 0; j < N; ++j) {
-                const uint64_t idx = (uint64_t)j * nbin + b;
-                const float p = (float)cell_p<A>(acc, idx, lb[j] == b ? (double)M + 1.0 : (double)M);
-                h_cond.add(plog2pf(p));
-                pb.add(p);
-                if (rows) rows[idx] = p;
-                acc[idx] = 0;
+            for (uint32_t j0 = 0; j0 < N; j0 += kFinalizeBatch) {
+                const uint32_t n = min(kFinalizeBatch, N - j0);
+                CellBatch cb;
+                load_cells(acc, lb, nbin, b, j0, n, cb);
+#pragma unroll
+                for (uint32_t q = 0; q < kFinalizeBatch; ++q) {
+                    if (q < n) {
+                        const uint64_t idx = (uint64_t)(j0 + q) * nbin + b;
+                        const float p = (float)cell_p<A>(cb.raw[q], cb.last[q] == b ? (double)M + 1.0 : (double)M);
+                        h_cond.add(plog2pf(p));
+                        pb.add(p);
+                        if (rows) rows[idx] = p;
+                        acc[idx] = 0;
+                    }
+                }
             }
             h_det.add(plog2pf(pb.value() / (float)N));
         }
