@@ -18,9 +18,8 @@ static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 r
 // 64-bit skip mask per 4096 positions in LDS): four blocks of 4096, whose 32
 // super-bands of 512 positions one pass of the wavefront tests.
 static constexpr uint32_t kMaskChunk = 16384;
-// Readable entries past the last ray of the FP32 beam tables (z32, zs32) the
-// trace loop needs: it loads 64 consecutive entries from any position up to
-// the last ray.
+// Entries past the last ray of the FP32 beam tables (z32, zk), kept readable
+// (zk's carry a ray index no design reaches).
 static constexpr uint32_t kBeamPad = 64;
 
 // One launch's work on the device.  Design records live in a pool
@@ -47,8 +46,8 @@ struct BatchArgs {
     uint32_t *last_bin;
     const float *z32;            // beam quantiles z_k = norminv(u_k), k = 1..n, FP32
     const double *z64;           // the same in FP64
-    const float *zs32;           // banded: within blocks of kBandRays rays sorted by z
-    const double *zs64;
+    const uint2 *zk;             // banded (within blocks of kBandRays rays sorted by z): (FP32 z bits, ray index k)
+    const double *zs64;          // banded, FP64
     const uint32_t *inv;         // 1-based banded position of ray k (index k - 1)
     unsigned long long *ops;     // algorithmic operation count (may be null)
     unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
@@ -140,7 +139,8 @@ struct ResidentArgs {
     uint32_t *chunk_slot0, *chunk_n;         // per ring position
     uint32_t *prefix;                        // per ring position: 65 inclusive item prefix sums over its designs
     unsigned long long *ctrl;                // kRes* counters
-    const float *z32, *zs32;
+    const float *z32;
+    const uint2 *zk;
     const double *z64, *zs64;
     const uint32_t *inv;
     uint32_t max_rays;                       // beam table length

@@ -176,11 +176,9 @@ struct TileCtx {
     void *row;       // the wavefront's row per bin
     uint32_t *queue;
     uint32_t *last_bin;
-    const float *z32;    // ray order
-    const double *z64;
-    const float *zs32;   // banded order
-    const double *zs64;
-    uint32_t full_end;   // positions 1..full_end are banded, the rest in ray order
+    const uint2 *zk;     // banded order: (FP32 z bits, 1-based ray index)
+    const double *zs64;  // banded order, FP64
+    uint32_t full_end;   // positions 1..full_end: whole banded blocks (rays past n_rays in the last are idle)
     uint32_t last_pos;   // position of ray k = n_rays
     const Chain64 *chain;  // the wavelength's planar faces in FP64 (LDS)
     const LensBand64 *band;  // the wavelength's lens and acceptance constants in FP64 (LDS)
@@ -195,7 +193,7 @@ __device__ void drain_queue(const TileCtx &c, uint32_t head, uint32_t count) {
     const uint32_t lane = threadIdx.x;
     if (lane < count) {
         const uint32_t i = c.queue[(head + lane) & (kQueueCap - 1)];  // position
-        const double y = beam_y64(*c.g, i <= c.full_end ? c.zs64[i - 1] : c.z64[i - 1]);
+        const double y = beam_y64(*c.g, c.zs64[i - 1]);
         // the affine FP64 trace decides nearly every queued ray; the few it
         // leaves (a decision within its 1e-9 margin) take the reference's
         // operation order
@@ -394,12 +392,10 @@ struct LiveWalk {
     }
 };
 
-// Beam quantile of this lane's ray in the iteration whose positions start at
-// base (1-based): banded when the iteration lies within the banded blocks.
-__device__ __forceinline__ float beam_z(const float *__restrict__ zs32, const float *__restrict__ z32, uint32_t base,
-                                        uint32_t full_end, bool inter, uint32_t lane_off) {
-    const float *__restrict__ zp = (inter || base + 63 <= full_end ? zs32 : z32) + (base - 1u);
-    return zp[lane_off];
+// Beam quantile (FP32 bits) and ray index of this lane's position in the
+// iteration whose positions start at base (1-based).
+__device__ __forceinline__ uint2 beam_zk(const uint2 *__restrict__ zk, uint32_t base, uint32_t lane_off) {
+    return zk[base - 1u + lane_off];
 }
 
 template <bool COUNT, int A>
@@ -463,15 +459,13 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
         LiveWalk<!COUNT> walk(c.masks, n_it);
         // the last ray in a skipped band misses with the rest of it
         if (!COUNT && last_it != ~0u && !walk.traced(last_it) && lane == 0) set_last_bin(c.last_bin, nbin);
-        // the iteration's positions are all banded or all in ray order
-        // (interleaved: all banded); the tables extend 63 entries past their
-        // last ray, so every lane loads from them (lanes at or past k_end only
-        // in the tile's last iteration, whose results they discard)
-        const bool inter = cmap.inter;
-        const float *__restrict__ zs32 = c.zs32;
-        const float *__restrict__ z32 = c.z32;
+        // every iteration holds 64 positions of whole banded blocks; a lane
+        // whose ray index exceeds n_rays (the last block of a design whose ray
+        // count is not a multiple of kBandRays) traces nothing
+        const uint2 *__restrict__ zk = c.zk;
+        const uint32_t n_rays = c.n_rays;
         uint32_t it = walk.next();
-        float z = it < n_it ? beam_z(zs32, z32, chunk + it * step, full_end, inter, lane_off) : 0.f;
+        uint2 zr = beam_zk(zk, chunk + (it < n_it ? it : 0u) * step, lane_off);
         while (it < n_it) {
             // the next traced iteration's beam height is loaded a whole
             // iteration ahead, so the trace never waits for it
@@ -479,10 +473,10 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
             // stand in past the chunk's end -- so one load is in flight on
             // every path and the trace waits only for the older one)
             const uint32_t next = walk.next();
-            const float z_next = beam_z(zs32, z32, chunk + (next < n_it ? next : it) * step, full_end, inter, lane_off);
+            const uint2 zr_next = beam_zk(zk, chunk + (next < n_it ? next : it) * step, lane_off);
             const uint32_t base = chunk + it * step;
-            const bool active = lane < (cmap.inter ? 64u : chunk_end - base);
-            const float y = hot.y_mean - hot.beam_w * z;
+            const bool active = zr.y <= n_rays;
+            const float y = hot.y_mean - hot.beam_w * __uint_as_float(zr.x);
             const Trace32Result r = trace32<COUNT>(hot, y);
             if (COUNT && active) ops += r.ops;
             const bool flagged = active && r.guard;
@@ -505,7 +499,7 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
                 since_fold = 0;
             }
             it = next;
-            z = z_next;
+            zr = zr_next;
         }
     }
     if (tail != head) {
@@ -613,7 +607,7 @@ __device__ __forceinline__ void copy_to_lds(void *dst, const void *src, uint32_t
 // One work item: wavelength sample j (at `lambda`) and ray tile `tile` of the
 // design (g, td); its row is added to acc[acc_j + b] and the bin of the
 // design's last ray, when the tile holds it, stored at *last_bin_j.  Args
-// supplies the beam tables (z32, zs32, z64, zs64, inv) and the LDS shape
+// supplies the beam tables (zk, zs64, inv) and the LDS shape
 // (hist_words, row_words); the LDS regions are derived here from the kernel's
 // arguments, which keeps them out of the registers live across items.
 template <bool COUNT, int A, bool REC, class Args>
@@ -647,7 +641,9 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     c.masks = L.masks;
     c.lc_lds = L.lc_lds;
     c.k_begin = tile * tile_rays + 1;
-    c.k_end = min(c.k_begin + tile_rays, M + 1);
+    // positions 1..P, the rays rounded up to whole banded blocks
+    const uint32_t P = (M + kBandRays - 1u) / kBandRays * kBandRays;
+    c.k_end = min(c.k_begin + tile_rays, P + 1u);
     c.n_rays = M;
     c.nbin = nbin;
     c.cols = cols;
@@ -655,16 +651,14 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     c.row = L.row32;
     c.queue = L.queue;
     c.last_bin = last_bin_j;
-    c.z32 = a.z32;
-    c.z64 = a.z64;
-    c.zs32 = a.zs32;
+    c.zk = a.zk;
     c.zs64 = a.zs64;
-    c.full_end = M / kBandRays * kBandRays;
-    c.last_pos = M <= c.full_end ? a.inv[M - 1] : M;
+    c.full_end = P;
+    c.last_pos = a.inv[M - 1];
     if (!lc_ok || td->skip) {
         // every ray of this wavelength is lost in the planar faces, or the
         // design is a stand-in for one that failed construction
-        if (c.k_end == M + 1 && lane == 0) set_last_bin(c.last_bin, nbin);
+        if (c.last_pos >= c.k_begin && c.last_pos < c.k_end && lane == 0) set_last_bin(c.last_bin, nbin);
     } else {
         tile_loop<COUNT, A>(c, ops);
         flush_row<A>(L.hist, L.row32, nbin, cols, acc, acc_j);

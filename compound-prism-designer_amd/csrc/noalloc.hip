@@ -38,12 +38,11 @@ constexpr size_t kStAcc = up(kStItems + (kMaxDesigns + 1) * sizeof(uint64_t));
 constexpr size_t kStLb = up(kStAcc + kMaxDesigns * sizeof(uint64_t));
 constexpr size_t kStageBytes = up(kStLb + kMaxDesigns * sizeof(uint64_t));
 // the scratch array: beam tables | stage | accumulators | last-ray bins | results | counter
-// (the trace loop's reads past the last ray of z32 and zs32, kBeamPad entries
-// at most, land in z64 and zs64, which follow them: values it discards)
+// (kMaxRays is a multiple of kBandRays: whole banded blocks)
 constexpr size_t kOffZ32 = 0;
 constexpr size_t kOffZ64 = up(kOffZ32 + kMaxRays * sizeof(float));
-constexpr size_t kOffZs32 = up(kOffZ64 + kMaxRays * sizeof(double));
-constexpr size_t kOffZs64 = up(kOffZs32 + kMaxRays * sizeof(float));
+constexpr size_t kOffZk = up(kOffZ64 + kMaxRays * sizeof(double));
+constexpr size_t kOffZs64 = up(kOffZk + kMaxRays * sizeof(uint2));
 constexpr size_t kOffInv = up(kOffZs64 + kMaxRays * sizeof(double));
 constexpr size_t kOffStage = up(kOffInv + kMaxRays * sizeof(uint32_t));
 constexpr size_t kOffAcc = up(kOffStage + kStageBytes);
@@ -52,9 +51,7 @@ constexpr size_t kOffResults = up(kOffLastBin + kMaxRows * sizeof(uint32_t));
 constexpr size_t kOffCounter = up(kOffResults + kMaxDesigns * sizeof(cpd_fitness));
 constexpr size_t kScratchBytes = up(kOffCounter + 256);
 static_assert(kScratchBytes == CPD_NOALLOC_SCRATCH_BYTES, "cpd.h states the scratch size");
-static_assert(kOffZ32 + (kMaxRays + kBeamPad) * sizeof(float) <= kOffZs32 &&
-                  kOffZs32 + (kMaxRays + kBeamPad) * sizeof(float) <= kOffInv,
-              "the FP32 beam tables' padding reads stay inside the beam tables");
+static_assert(kMaxRays % kBandRays == 0, "whole banded blocks");
 
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -67,7 +64,8 @@ uint32_t g_valid[kMaxDesigns];
 bool g_ok[kMaxDesigns];
 // beam tables as the batch path builds them for 65536 rays (BeamTables::ensure)
 double g_z[kMaxRays], g_zs[kMaxRays];
-float g_zf[kMaxRays], g_zsf[kMaxRays];
+float g_zf[kMaxRays];
+uint2 g_zk[kMaxRays];
 uint32_t g_order[kMaxRays], g_pos[kMaxRays];
 
 #define NA_TRY(expr)                                                                                   \
@@ -99,13 +97,16 @@ cpd_status setup(int device, hipStream_t s, unsigned char **out) {
                   [](uint32_t x, uint32_t y) { return g_z[x] < g_z[y] || (g_z[x] == g_z[y] && x < y); });
     for (uint32_t i = 0; i < kMaxRays; ++i) {
         g_zs[i] = g_z[g_order[i]];
-        g_zsf[i] = g_zf[g_order[i]];
+        float f = g_zf[g_order[i]];
+        uint32_t bits;
+        memcpy(&bits, &f, sizeof bits);
+        g_zk[i] = uint2{bits, g_order[i] + 1};
         g_pos[g_order[i]] = i + 1;
     }
     NA_TRY(hipMemcpyAsync(base + kOffZ64, g_z, sizeof g_z, hipMemcpyHostToDevice, s));
     NA_TRY(hipMemcpyAsync(base + kOffZ32, g_zf, sizeof g_zf, hipMemcpyHostToDevice, s));
     NA_TRY(hipMemcpyAsync(base + kOffZs64, g_zs, sizeof g_zs, hipMemcpyHostToDevice, s));
-    NA_TRY(hipMemcpyAsync(base + kOffZs32, g_zsf, sizeof g_zsf, hipMemcpyHostToDevice, s));
+    NA_TRY(hipMemcpyAsync(base + kOffZk, g_zk, sizeof g_zk, hipMemcpyHostToDevice, s));
     NA_TRY(hipMemcpyAsync(base + kOffInv, g_pos, sizeof g_pos, hipMemcpyHostToDevice, s));
     NA_TRY(hipStreamSynchronize(s));
     g_ptr[device] = base;
@@ -204,7 +205,7 @@ extern "C" cpd_status cpd_fitness_batch_noalloc(cpd_ctx *ctx, const cpd_design *
     a.last_bin = (uint32_t *)(base + kOffLastBin);
     a.z32 = (const float *)(base + kOffZ32);
     a.z64 = (const double *)(base + kOffZ64);
-    a.zs32 = (const float *)(base + kOffZs32);
+    a.zk = (const uint2 *)(base + kOffZk);
     a.zs64 = (const double *)(base + kOffZs64);
     a.inv = (const uint32_t *)(base + kOffInv);
     a.n_designs = nv;

@@ -189,7 +189,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     a.last_bin = d_last_bin.as<uint32_t>();
     a.z32 = beam.z32.as<float>();
     a.z64 = beam.z64.as<double>();
-    a.zs32 = beam.zs32.as<float>();
+    a.zk = beam.zk.as<uint2>();
     a.zs64 = beam.zs64.as<double>();
     a.inv = beam.inv.as<uint32_t>();
     a.work_counter = d_counter.as<unsigned int>();

@@ -31,15 +31,18 @@ struct HostBuffer {
     void release();
 };
 
-// Beam quantiles z_k = norminv(u_k), k = 1..n, of the R1 sequence from 0.5:
-// in ray order (FP32, FP64), and banded: within each block of kBandRays rays
-// sorted by z (so by beam height), with inv[k - 1] the 1-based position of
-// ray k in the banded tables.  A wavefront that traces 64 consecutive banded
-// positions covers a narrow band of heights, so whole wavefronts miss
-// together; the per-bin sums do not depend on the order rays are added in.
+// Beam quantiles z_k = norminv(u_k), k = 1..n (n a multiple of kBandRays), of
+// the R1 sequence from 0.5: in ray order (FP32, FP64), and banded: within
+// each block of kBandRays rays sorted by z (so by beam height), as FP64 and
+// as (FP32 z bits, k) pairs, with inv[k - 1] the 1-based position of ray k in
+// the banded tables.  A wavefront that traces 64 consecutive banded positions
+// covers a narrow band of heights, so whole wavefronts miss together; the
+// per-bin sums do not depend on the order rays are added in.  A design traces
+// whole blocks; in its last one the positions whose ray index exceeds its ray
+// count are idle.
 struct BeamTables {
     uint32_t n = 0;
-    DeviceBuffer z32, z64, zs32, zs64, inv;
+    DeviceBuffer z32, z64, zk, zs64, inv;
     cpd_status ensure(uint32_t want, hipStream_t s);  // grow-only, at least 65536 rays
     void release();
 };

@@ -81,17 +81,20 @@ cpd_status upload_beam_table(hipStream_t s, uint64_t seed_fx, uint32_t n, float 
 
 cpd_status BeamTables::ensure(uint32_t want, hipStream_t s) {
     if (want <= n) return CPD_OK;
-    const uint32_t m = std::max<uint32_t>(want, 65536);
+    // whole banded blocks: a design's last block is traced whole, its rays
+    // past n_rays idle
+    const uint32_t m = (std::max<uint32_t>(want, 65536) + kBandRays - 1) / kBandRays * kBandRays;
     cpd_status st;
-    // the FP32 tables carry kBeamPad zeros past ray m: the trace loop loads a
-    // whole wavefront's entries from any position up to the last ray
+    // the FP32 tables carry kBeamPad entries past ray m (zk's with a ray
+    // index no design reaches)
     if ((st = z32.reserve((size_t)(m + kBeamPad) * sizeof(float), false, s)) != CPD_OK) return st;
     if ((st = z64.reserve((size_t)m * sizeof(double), false, s)) != CPD_OK) return st;
-    if ((st = zs32.reserve((size_t)(m + kBeamPad) * sizeof(float), false, s)) != CPD_OK) return st;
+    if ((st = zk.reserve((size_t)(m + kBeamPad) * sizeof(uint2), false, s)) != CPD_OK) return st;
     if ((st = zs64.reserve((size_t)m * sizeof(double), false, s)) != CPD_OK) return st;
     if ((st = inv.reserve((size_t)m * sizeof(uint32_t), false, s)) != CPD_OK) return st;
     std::vector<double> z(m), zs(m);
-    std::vector<float> zf(m + kBeamPad, 0.f), zsf(m + kBeamPad, 0.f);
+    std::vector<float> zf(m + kBeamPad, 0.f);
+    std::vector<uint2> zkh(m + kBeamPad, uint2{0u, 0xffffffffu});
     std::vector<uint32_t> order(m), pos(m);
     for (uint32_t i = 0; i < m; ++i) {
         z[i] = norminv64(qmc_seeded(kHalfFx, (uint64_t)i + 1));
@@ -103,13 +106,16 @@ cpd_status BeamTables::ensure(uint32_t want, hipStream_t s) {
                   [&](uint32_t x, uint32_t y) { return z[x] < z[y] || (z[x] == z[y] && x < y); });
     for (uint32_t i = 0; i < m; ++i) {
         zs[i] = z[order[i]];
-        zsf[i] = zf[order[i]];
+        float f = zf[order[i]];
+        uint32_t bits;
+        memcpy(&bits, &f, sizeof bits);
+        zkh[i] = uint2{bits, order[i] + 1};
         pos[order[i]] = i + 1;
     }
     HIP_TRY(hipMemcpyAsync(z64.ptr, z.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(z32.ptr, zf.data(), (size_t)(m + kBeamPad) * sizeof(float), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(zs64.ptr, zs.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(zs32.ptr, zsf.data(), (size_t)(m + kBeamPad) * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(zk.ptr, zkh.data(), (size_t)(m + kBeamPad) * sizeof(uint2), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(inv.ptr, pos.data(), (size_t)m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     n = m;
@@ -119,7 +125,7 @@ cpd_status BeamTables::ensure(uint32_t want, hipStream_t s) {
 void BeamTables::release() {
     z32.release();
     z64.release();
-    zs32.release();
+    zk.release();
     zs64.release();
     inv.release();
     n = 0;
@@ -226,7 +232,8 @@ double item_cost(const cpd_design &g, const cpd_sampling &s, uint32_t tile_cap) 
     // faces are hoisted per wavelength); per item the wavelength set-up adds
     // a few hundred FP64 operations per face
     const double per_ray = kOpsBeam + kOpsChain + kOpsLensHit + kOpsLensRefract + kOpsDetector + kOpsHit;
-    return per_ray * tile_for(g.bin_count, s.n_rays, tile_cap) + 300.0 * g.n_glass;
+    return per_ray * tile_for(g.bin_count, (s.n_rays + kBandRays - 1) / kBandRays * kBandRays, tile_cap) +
+           300.0 * g.n_glass;
 }
 
 void order_designs(const cpd_design *designs, const cpd_sampling *sampling, std::vector<uint32_t> &valid,
@@ -333,7 +340,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.last_bin = last_bin.as<uint32_t>();
     a.z32 = beam.z32.as<float>();
     a.z64 = beam.z64.as<double>();
-    a.zs32 = beam.zs32.as<float>();
+    a.zk = beam.zk.as<uint2>();
     a.zs64 = beam.zs64.as<double>();
     a.inv = beam.inv.as<uint32_t>();
     a.ops = want_ops ? dops : nullptr;

@@ -280,7 +280,7 @@ cpd_status Server::init_resident(const cpd_server_opts *o) {
     a.prefix = d_prefix.as<uint32_t>();
     a.ctrl = d_ctrl.as<unsigned long long>();
     a.z32 = beam.z32.as<float>();
-    a.zs32 = beam.zs32.as<float>();
+    a.zk = beam.zk.as<uint2>();
     a.z64 = beam.z64.as<double>();
     a.zs64 = beam.zs64.as<double>();
     a.inv = beam.inv.as<uint32_t>();
@@ -637,7 +637,7 @@ cpd_status Server::step() {
     a.last_bin = d_last_bin.as<uint32_t>();
     a.z32 = beam.z32.as<float>();
     a.z64 = beam.z64.as<double>();
-    a.zs32 = beam.zs32.as<float>();
+    a.zk = beam.zk.as<uint2>();
     a.zs64 = beam.zs64.as<double>();
     a.inv = beam.inv.as<uint32_t>();
     a.work_counter = d_counter.as<unsigned int>();

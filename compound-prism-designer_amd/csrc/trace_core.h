@@ -268,12 +268,12 @@ CPD_HD constexpr uint32_t hist_cols(uint32_t nbin) {
 // column below 2^32.
 CPD_HD constexpr uint32_t fold_rays(uint32_t cols) { return (512u * cols - 65u) & ~63u; }
 
-// Rays per work item of a design: the whole wavelength sample rounded up to
-// a wavefront multiple, capped by tile_cap (a multiple of 64; 0 for the
-// default cap).
-CPD_HD uint32_t tile_for(uint32_t nbin, uint32_t n_rays, uint32_t tile_cap) {
+// Positions per work item of a design: the whole wavelength sample's
+// positions (n_pos) rounded up to a wavefront multiple, capped by tile_cap (a
+// multiple of 64; 0 for the default cap).
+CPD_HD uint32_t tile_for(uint32_t nbin, uint32_t n_pos, uint32_t tile_cap) {
     (void)nbin;
-    uint32_t tile = (n_rays + 63) / 64 * 64;
+    uint32_t tile = (n_pos + 63) / 64 * 64;
     const uint32_t cap = tile_cap ? tile_cap : kDefaultTileRays;
     if (tile > cap) tile = cap;
     return tile < 64 ? 64 : tile;
@@ -281,7 +281,7 @@ CPD_HD uint32_t tile_for(uint32_t nbin, uint32_t n_rays, uint32_t tile_cap) {
 
 struct alignas(16) TraceDesign {
     uint32_t n_glass, nbin, ar_coated, n_lambda;
-    uint32_t n_rays, hist_cols, tile_rays, n_tiles;  // LDS columns; rays per work item; items per wavelength
+    uint32_t n_rays, hist_cols, tile_rays, n_tiles;  // LDS columns; positions per work item; items per wavelength
     // exit lens relative to its chord midpoint m: centre offset cm = c - m,
     // 1/R, bound radius^2, error scales
     float cmx, cmy, inv_r, max_dist_sq;
@@ -377,8 +377,11 @@ CPD_HD void make_trace_design(const cpd_design &g, uint32_t n_lambda, uint32_t n
     t.n_lambda = n_lambda;
     t.n_rays = n_rays;
     t.hist_cols = hist_cols(g.bin_count);
-    t.tile_rays = tile_for(g.bin_count, n_rays, tile_cap);
-    t.n_tiles = (n_rays + t.tile_rays - 1) / t.tile_rays;
+    // work items cover the banded positions 1..P, the rays rounded up to whole
+    // blocks of kBandRays (positions past the rays in the last block idle)
+    const uint32_t positions = (uint32_t)(((uint64_t)n_rays + kBandRays - 1) / kBandRays * kBandRays);
+    t.tile_rays = tile_for(g.bin_count, positions, tile_cap);
+    t.n_tiles = (positions + t.tile_rays - 1) / t.tile_rays;
     t.cmx = (float)(g.arc_center[0] - g.arc_midpt[0]);
     t.cmy = (float)(g.arc_center[1] - g.arc_midpt[1]);
     t.inv_r = (float)(1.0 / g.arc_radius);

@@ -147,14 +147,16 @@ extern "C" {
 // bands as the kernel forms them (an iteration's 64 rays: interleaved over
 // the four blocks of a whole 16384-position chunk, else 64 consecutive
 // positions): a band is skipped when its super-band of 8 bands or the band
-// itself is proven lost.  Heights are taken over every ray of a band (the
+// itself is proven lost.  The positions cover the rays rounded up to whole
+// blocks; in the last block the positions of rays past n_rays are idle.
+// Heights are taken over every position of a band, idle ones included (the
 // kernel bounds them by the ends of each sorted run).
 // stats[0] bands, [1] bands a wavefront would take past the planar-chain
 // test, [2] of those skipped, [3] skipped bands (any) holding an FP64 hit
 // (must be 0), [4] bands holding an FP64 hit.
 int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t n_rays, int threads,
                uint64_t *stats) {
-    const uint32_t full = n_rays / cpd::kBandRays * cpd::kBandRays;
+    const uint32_t full = (n_rays + cpd::kBandRays - 1) / cpd::kBandRays * cpd::kBandRays;
     std::vector<double> z64(full);
     std::vector<uint32_t> order(full);
     for (uint32_t k = 0; k < full; ++k) {
@@ -221,6 +223,7 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                     bool chain_pass = false, hit = false;
                     for (uint32_t q = 0; q < 64; ++q) {
                         const uint32_t k = order[band_pos[64 * b + q]];
+                        if (k >= n_rays) continue;  // idle position
                         const float y = hot.y_mean - hot.beam_w * (float)z64[k];
                         const bool alive = (y >= hot.ylo) & (y <= hot.yhi);
                         const bool guard = fminf(fabsf(y - hot.ylo), fabsf(y - hot.yhi)) <= hot.ymarg;

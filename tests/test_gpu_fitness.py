@@ -226,3 +226,41 @@ def test_config4_rows_of_a_million_rays(config4_rows, acc, tile):
     print(f"policy {acc} tile {tile}: dI {got[:, 1] - ref[:, 1]}, info {ref[:, 1]}", flush=True)
     for i in range(len(D)):
         check_close(got[i], ref[i], f"policy {acc} design {i}")
+
+
+@pytest.mark.parametrize("n_rays", [100, 2500, 4097, 10000, 20000])
+def test_ray_counts_off_the_block_size_match_oracle(n_rays):
+    # The trace covers whole banded blocks of 4096 positions; in a design's
+    # last block the positions of rays past n_rays are idle.  Every ray must be
+    # traced exactly once: one ray missed or counted twice moves a cell by
+    # t / n_rays (>= 1e-5 here), far above the 2e-6 FP32 transmittances leave.
+    _, D, _, _ = transmitting(0, 6)
+    dp = ctypes.POINTER(ctypes.c_double)
+    for d in D:
+        w = np.linspace(d.wavelength_lo, d.wavelength_hi, 5)[1:-1].copy()
+        ref = np.array([O.p_dets_l_wavelength(to_oracle(d), float(x), n_rays, qrng=O.QRNG_FIXED_POINT)[0]
+                        for x in w])
+        out = np.empty((3, d.bin_count))
+        st = N.lib.cpd_p_dets_l_wavelength(None, ctypes.byref(d), w.ctypes.data_as(dp), 3, n_rays, N.MODE_FIXED,
+                                           out.ctypes.data_as(dp))
+        assert st == 0, N.last_error()
+        np.testing.assert_allclose(out, ref, atol=2e-6, err_msg=f"{n_rays} rays")
+
+
+def test_tiles_that_split_the_last_block():
+    # work items of 1024 positions split a design's last, partly idle block
+    # across items; the sums (exact integers) match one item per wavelength
+    # and the oracle
+    _, D, _, _ = transmitting(0, 8)
+    S = [N.Sampling(8, 5000 + 320 * i, N.MODE_FIXED, 0) for i in range(len(D))]
+    ctx = cpd.Context(0)
+    whole, st = cpd.fitness_batch(D, S, ctx=ctx)
+    assert (st == 0).all()
+    ctx.set_schedule(N.Schedule(1024, 16, N.ORDER_GIVEN, 0))
+    split, st = cpd.fitness_batch(D, S, ctx=ctx)
+    ctx.set_schedule(None)
+    assert (st == 0).all()
+    assert np.array_equal(whole, split)
+    for i, d in enumerate(D):
+        r, _ = O.fitness(to_oracle(d), 8, S[i].n_rays, 64, adaptive=False, qrng=O.QRNG_FIXED_POINT)
+        assert abs(split[i][1] - r[1]) <= INFO_TOL, (i, split[i], r)
