@@ -264,3 +264,18 @@ def test_tiles_that_split_the_last_block():
     for i, d in enumerate(D):
         r, _ = O.fitness(to_oracle(d), 8, S[i].n_rays, 64, adaptive=False, qrng=O.QRNG_FIXED_POINT)
         assert abs(split[i][1] - r[1]) <= INFO_TOL, (i, split[i], r)
+
+
+def test_many_entries_find_their_items():
+    # a launch of 5000 designs of 1-3 items each: work items find their design
+    # by a 64-lane search over the item prefix in three rounds (two cover at
+    # most 4096 entries); every design matches its own result in launches of
+    # 1000 designs
+    _, D, _, _ = cpd.generate_designs(0, 0, 5000)
+    S = [N.Sampling(1 + i % 3, 256, N.MODE_FIXED, 0) for i in range(len(D))]
+    whole, st = cpd.fitness_batch(D, S)
+    assert (st == 0).all()
+    for q in range(0, len(D), 1000):
+        part, st = cpd.fitness_batch(D[q:q + 1000], S[q:q + 1000])
+        assert (st == 0).all()
+        assert np.array_equal(part, whole[q:q + 1000]), q
