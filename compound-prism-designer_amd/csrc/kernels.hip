@@ -635,7 +635,7 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
         if (lane == 0) *L.lc_lds = c.lc;
     }
     __syncthreads();  // the queue drains and the band tests of this item read the records
-    const bool lc_ok = REC ? L.lc_lds->ok != 0 : c.lc.ok != 0;
+    const bool lc_ok = REC ? L.lc_lds->ok != 0 && (COUNT || L.lc_lds->dead == 0) : c.lc.ok != 0;
     c.chain = L.chain;
     c.band = L.band;
     c.masks = L.masks;
@@ -656,8 +656,9 @@ __device__ __forceinline__ void run_item(const Args &a, uint32_t *lds, const cpd
     c.full_end = P;
     c.last_pos = a.inv[M - 1];
     if (!lc_ok || td->skip) {
-        // every ray of this wavelength is lost in the planar faces, or the
-        // design is a stand-in for one that failed construction
+        // every ray of this wavelength is lost in the planar faces or proved
+        // lost beyond them, or the design is a stand-in for one that failed
+        // construction
         if (c.last_pos >= c.k_begin && c.last_pos < c.k_end && lane == 0) set_last_bin(c.last_bin, nbin);
     } else {
         tile_loop<COUNT, A>(c, ops);
@@ -726,11 +727,24 @@ __global__ __launch_bounds__(64) void lambda_records_kernel(BatchArgs a) {
     if (n_items == 0 || td.n_tiles == 0) return;
     const uint64_t off = a.item_offset ? a.item_offset[e] : 0;
     const uint32_t j0 = (uint32_t)(off / td.n_tiles), j1 = (uint32_t)((off + n_items - 1) / td.n_tiles);
+    // the heights of every ray the design traces: its positions 1..P are whole
+    // blocks sorted by z, so their first and last positions bound them
+    const uint32_t P = (td.n_rays + kBandRays - 1u) / kBandRays * kBandRays;
+    double ya = INFINITY, yb = -INFINITY;
+    for (uint32_t b = 0; b < P; b += kBandRays) {
+        const double y0 = beam_y64(g, a.zs64[b]), y1 = beam_y64(g, a.zs64[b + kBandRays - 1u]);
+        ya = fmin(ya, fmin(y0, y1));
+        yb = fmax(yb, fmax(y0, y1));
+    }
     for (uint32_t j = j0 + threadIdx.x; j <= j1; j += 64) {
         const double lambda = a.wavelengths ? a.wavelengths[j] : lambda_of(g, qmc_u64(j + 1));
         LambdaRecord &r = a.lrec[a.lrec_begin ? a.lrec_begin[e] + (j - j0) : a.lb_begin[d] + j];
         LambdaConsts lc;
         make_lambda_consts(g, lambda, lc, &r.chain, &r.band);
+        // one band test over every height: a wavelength it proves lost (the
+        // lens, its acceptance cosine or the array ends lose every ray) is not
+        // traced at all (the items of a launch that counts operations still are)
+        if (lc.ok && band_class(r.band, ya, yb) == kBandLost) lc.dead = 1;
         r.lc = lc;
     }
 }

@@ -326,6 +326,7 @@ struct LambdaConsts {
     float hc;                  // half the array's length minus the intercept
     float dlen0, dlen1;        // the detector stage's length scale is dlen0 + dlen1 |tq| + |dist|
     int ok;                    // 0 when every ray is lost in the planar faces (TIR, parallel face)
+    int dead;                  // 1 when a band test over every beam height proves every ray lost
     double lambda;
 };
 
@@ -499,6 +500,7 @@ CPD_HD void make_lambda_consts(const cpd_design &g, double lambda, LambdaConsts 
         ok = refract64(ray, nx, ny, n[i - 1], n[i], ar) == T_OK;
     }
     lc.ok = ok ? 1 : 0;
+    lc.dead = 0;
     if (chain) {
         chain->ldx = ray.dx;
         chain->ldy = ray.dy;

@@ -153,7 +153,9 @@ extern "C" {
 // kernel bounds them by the ends of each sorted run).
 // stats[0] bands, [1] bands a wavefront would take past the planar-chain
 // test, [2] of those skipped, [3] skipped bands (any) holding an FP64 hit
-// (must be 0), [4] bands holding an FP64 hit.
+// (must be 0), [4] bands holding an FP64 hit, [5] wavelengths the test over
+// every height proves lost (the trace skips them whole), [6] of those holding
+// an FP64 hit (must be 0).
 int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t n_rays, int threads,
                uint64_t *stats) {
     const uint32_t full = (n_rays + cpd::kBandRays - 1) / cpd::kBandRays * cpd::kBandRays;
@@ -184,10 +186,10 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
     }
     const size_t n_bands = band_chunk.size();
     std::atomic<size_t> next{0};
-    std::atomic<uint64_t> tot[5];
+    std::atomic<uint64_t> tot[7];
     for (auto &t : tot) t = 0;
     auto work = [&]() {
-        uint64_t loc[5] = {0, 0, 0, 0, 0};
+        uint64_t loc[7] = {0, 0, 0, 0, 0, 0, 0};
         for (;;) {
             const size_t i = next.fetch_add(1);
             if (i >= n) break;
@@ -211,6 +213,17 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                         hi = fmax(hi, y);
                     }
                 };
+                // the records kernel's test over every height of the design's
+                // positions (idle ones included)
+                double wa = INFINITY, wb = -INFINITY;
+                for (uint32_t q = 0; q < full; ++q) {
+                    const double y = cpd::beam_y64(g, z64[q]);
+                    wa = fmin(wa, y);
+                    wb = fmax(wb, y);
+                }
+                const bool dead = lc.ok && cpd::band_class(band, wa, wb) == cpd::kBandLost;
+                loc[5] += dead;
+                bool any_hit = false;
                 bool super_lost = false;
                 for (size_t b = 0; b < n_bands; ++b) {
                     loc[0]++;
@@ -232,6 +245,7 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                         hit |= lc.ok && cpd::trace64(g, lambda, cpd::beam_y64(g, z64[k]), t) < g.bin_count;
                     }
                     loc[4] += hit;
+                    any_hit |= hit;
                     if (!lc.ok) continue;
                     double ya, yb;
                     heights(b, 1, ya, yb);
@@ -242,16 +256,17 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                     }
                     if (skipped) loc[3] += hit;
                 }
+                if (dead) loc[6] += any_hit;
             }
         }
-        for (int q = 0; q < 5; ++q) tot[q] += loc[q];
+        for (int q = 0; q < 7; ++q) tot[q] += loc[q];
     };
     const int T = threads > 0 ? threads : 1;
     std::vector<std::thread> pool;
     for (int t = 1; t < T; ++t) pool.emplace_back(work);
     work();
     for (auto &th : pool) th.join();
-    for (int q = 0; q < 5; ++q) stats[q] = tot[q];
+    for (int q = 0; q < 7; ++q) stats[q] = tot[q];
     return 0;
 }
 

@@ -103,9 +103,10 @@ def band_stats(designs, n_lambda, n_rays):
                              ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     n = len(designs)
     arr = (N.Design * n)(*designs)
-    st = (ctypes.c_uint64 * 5)()
+    st = (ctypes.c_uint64 * 7)()
     L.band_check(arr, n, n_lambda, n_rays, min(16, os.cpu_count() or 1), st)
-    return dict(zip(["bands", "past_chain", "skipped", "skipped_with_hit", "with_hit"], st[:]))
+    return dict(zip(["bands", "past_chain", "skipped", "skipped_with_hit", "with_hit", "dead_wavelengths",
+                     "dead_with_hit"], st[:]))
 
 
 @pytest.mark.parametrize("config,transmit,n_rays", [(1, False, 16384), (2, False, 16384), (4, False, 16384),
@@ -124,6 +125,8 @@ def test_skipped_bands_hold_no_hit(config, transmit, n_rays):
     print(f"config {config}: {st}, skipped {st['skipped'] / st['past_chain']:.3f} of the bands past the chain test")
     assert st["skipped_with_hit"] == 0, st
     assert st["skipped"] > 0
+    # a wavelength the test over every beam height proves lost is not traced
+    assert st["dead_with_hit"] == 0, st
 
 
 @pytest.mark.parametrize("config", [0, 2, 4, 5, 6, 7])
