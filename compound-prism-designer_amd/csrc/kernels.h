@@ -18,6 +18,13 @@ static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 r
 // 64-bit skip mask per 4096 positions in LDS): four blocks of 4096, whose 32
 // super-bands of 512 positions one pass of the wavefront tests.
 static constexpr uint32_t kMaskChunk = 16384;
+// The persistent grid's work counters: kWorkCounters of them, kCounterStride
+// u32 apart (a 128-byte line each); counter x hands out items x,
+// x + kWorkCounters, x + 2 kWorkCounters, ... so that the wavefronts of the
+// chip do not all meet at one atomic (one counter for the 1024-design batch
+// held it back by a fifth).  A launch clears kWorkCounterWords words.
+static constexpr uint32_t kWorkCounters = 8, kCounterStride = 32;
+static constexpr uint32_t kWorkCounterWords = kWorkCounters * kCounterStride;
 // Entries past the last ray of the FP32 beam tables (z32, zk), kept readable
 // (zk's carry a ray index no design reaches).
 static constexpr uint32_t kBeamPad = 64;
@@ -50,7 +57,7 @@ struct BatchArgs {
     const double *zs64;          // banded, FP64
     const uint32_t *inv;         // 1-based banded position of ray k (index k - 1)
     unsigned long long *ops;     // algorithmic operation count (may be null)
-    unsigned int *work_counter;  // persistent scheduling (null: one item per workgroup)
+    unsigned int *work_counter;  // persistent scheduling: kWorkCounters counters (null: one item per workgroup)
     uint32_t items_per_fetch;    // consecutive items one counter fetch claims (0: 1)
     const double *wavelengths;   // explicit wavelength per sample j (null: the R1 samples)
     // per-wavelength records, which launch_trace fills before the trace (null:

@@ -677,18 +677,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void tr
     const uint32_t lane = threadIdx.x;
     unsigned long long ops = 0;
     uint64_t item = blockIdx.x;
-    // items claimed by the wavefront's last counter fetch: [next, end)
+    // The wavefront takes items from counter grp (its workgroup's index mod
+    // kWorkCounters, which is also its XCD's); slots [next, end) of that
+    // counter are its last fetch, slot q being item grp + kWorkCounters q.
+    // Slots ascend, so the first item past n_items means the counter has none
+    // left and the wavefront moves on to the next counter; it ends when all
+    // kWorkCounters are spent.
     const uint32_t per_fetch = a.items_per_fetch > 1 ? a.items_per_fetch : 1u;
+    uint32_t grp = blockIdx.x & (kWorkCounters - 1u), spent = 0;
     uint64_t next = 0, end = 0;
     for (;;) {
         if (a.work_counter) {
-            if (next == end) {
-                uint32_t v = 0;
-                if (lane == 0) v = atomicAdd(a.work_counter, per_fetch);
-                next = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(v);
-                end = next + per_fetch;
+            item = a.n_items;
+            while (spent < kWorkCounters) {
+                if (next == end) {
+                    uint32_t v = 0;
+                    if (lane == 0) v = atomicAdd(a.work_counter + grp * kCounterStride, per_fetch);
+                    next = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(v);
+                    end = next + per_fetch;
+                }
+                const uint64_t cand = grp + (uint64_t)kWorkCounters * next++;
+                if (cand < a.n_items) {
+                    item = cand;
+                    break;
+                }
+                ++spent;
+                grp = (grp + 1u) & (kWorkCounters - 1u);
+                next = end = 0;
             }
-            item = next++;  // claims ascend, so the first item past n_items ends the loop
         }
         if (item >= a.n_items) break;
         const uint32_t e = find_design(a.item_begin, a.n_designs, item);

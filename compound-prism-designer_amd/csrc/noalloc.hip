@@ -49,7 +49,7 @@ constexpr size_t kOffAcc = up(kOffStage + kStageBytes);
 constexpr size_t kOffLastBin = up(kOffAcc + kMaxCells * sizeof(unsigned long long));
 constexpr size_t kOffResults = up(kOffLastBin + kMaxRows * sizeof(uint32_t));
 constexpr size_t kOffCounter = up(kOffResults + kMaxDesigns * sizeof(cpd_fitness));
-constexpr size_t kScratchBytes = up(kOffCounter + 256);
+constexpr size_t kScratchBytes = up(kOffCounter + kWorkCounterWords * sizeof(unsigned int));
 static_assert(kScratchBytes == CPD_NOALLOC_SCRATCH_BYTES, "cpd.h states the scratch size");
 static_assert(kMaxRays % kBandRays == 0, "whole banded blocks");
 
@@ -193,7 +193,7 @@ extern "C" cpd_status cpd_fitness_batch_noalloc(cpd_ctx *ctx, const cpd_design *
     // the accumulators start at zero (a failed earlier call may have left
     // partial sums) and so does the work counter; fill kernels, no memset
     NA_TRY(launch_zero_u32(s, (unsigned int *)acc, (uint32_t)(2 * acc_n)));
-    NA_TRY(launch_zero_u32(s, counter, 1));
+    NA_TRY(launch_zero_u32(s, counter, kWorkCounterWords));
 
     BatchArgs a{};
     a.designs = (const cpd_design *)(base + kOffStage + kStDesigns);

@@ -167,7 +167,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
     if ((st = d_last_bin.reserve(lb_n * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
     if ((st = d_lrec.reserve(lb_n * sizeof(LambdaRecord), false, stream)) != CPD_OK) return st;
     if ((st = d_results.reserve(n * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
-    if ((st = d_counter.reserve(256, true, stream)) != CPD_OK) return st;
+    if ((st = d_counter.reserve(kWorkCounterWords * sizeof(unsigned int), true, stream)) != CPD_OK) return st;
     if ((st = h_designs.reserve(n * sizeof(cpd_design))) != CPD_OK) return st;
     if ((st = h_out.reserve(n * sizeof(cpd_fitness))) != CPD_OK) return st;
     char *fx = d_fixed.as<char>();
@@ -229,7 +229,7 @@ cpd_status Plan::init(Context *ctx, const cpd_design *shape_in, const cpd_sampli
 hipError_t Plan::capture_evaluation(const int *status) {
     hipError_t e = launch_build_traces(stream, args.designs, d_sampling, n, sched.tile_rays, d_traces.as<TraceDesign>(),
                                        status);
-    if (e == hipSuccess) e = launch_zero_u32(stream, args.work_counter, 1);
+    if (e == hipSuccess) e = launch_zero_u32(stream, args.work_counter, kWorkCounterWords);
     if (e == hipSuccess) e = launch_trace(stream, args, grid, false);
     if (e == hipSuccess) e = launch_finalize(stream, fin);
     if (e == hipSuccess)

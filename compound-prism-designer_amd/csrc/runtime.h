@@ -54,7 +54,7 @@ struct Context {
     int cu_count = 0;
     int clock_khz = 0;
     char name[256] = {0};
-    uint32_t waves_per_cu = 20;   // resident wavefronts per CU of the persistent grid (the autotuner tries up to 24)
+    uint32_t waves_per_cu = 24;   // resident wavefronts per CU of the persistent grid (six per SIMD)
     uint32_t tile_rays_hint = 0;  // rays per work item (0: the default)
     uint32_t order = CPD_ORDER_GIVEN;
     int acc_kind = CPD_ACC_EXACT;  // accumulation policy of fixed-count batches

@@ -320,13 +320,14 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     if ((st = last_bin.reserve(lb_n * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
     if ((st = results.reserve(n * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
     if ((st = lrec.reserve(lb_n * sizeof(LambdaRecord), false, stream)) != CPD_OK) return st;
-    if ((st = counters.reserve(64, true, stream)) != CPD_OK) return st;
+    // the operation count, then the work counters from byte 256
+    if ((st = counters.reserve(256 + kWorkCounterWords * sizeof(unsigned int), true, stream)) != CPD_OK) return st;
     if (rows && (st = row_buf.reserve(acc_n * sizeof(double), false, stream)) != CPD_OK) return st;
 
     char *dstage = (char *)stage.ptr;
     HIP_TRY(hipMemcpyAsync(dstage, h, staged, hipMemcpyHostToDevice, stream));
     unsigned long long *dops = (unsigned long long *)counters.ptr;
-    unsigned int *dwork = (unsigned int *)((char *)counters.ptr + 16);
+    unsigned int *dwork = (unsigned int *)((char *)counters.ptr + 256);
     const bool want_ops = ops_out != nullptr || count_ops;
     if (want_ops) HIP_TRY(hipMemsetAsync(dops, 0, sizeof(unsigned long long), stream));
 
@@ -352,7 +353,7 @@ cpd_status Context::run(const cpd_design *designs, const cpd_sampling *sampling,
     a.n_items = items;
     uint32_t grid;
     if (persistent) {
-        HIP_TRY(launch_zero_u32(stream, dwork, 1));
+        HIP_TRY(launch_zero_u32(stream, dwork, kWorkCounterWords));
         a.work_counter = dwork;
         const uint64_t resident = (uint64_t)cu_count * waves_per_cu;
         grid = (uint32_t)std::min<uint64_t>(items, resident);

@@ -128,7 +128,7 @@ struct Server {
     int cu_count = 0;
     hipStream_t stream = nullptr;
     uint32_t step_items = 0;
-    uint32_t waves_per_cu = 20;
+    uint32_t waves_per_cu = 24;
     uint64_t max_designs = 0, acc_cap = 0, lb_cap = 0;
     DeviceBuffer d_designs, d_traces, d_acc_begin, d_lb_begin, d_acc, d_last_bin, d_results, d_step, d_counter;
     DeviceBuffer d_lrec;  // the wavelength records of a step
@@ -207,7 +207,7 @@ cpd_status Server::init(Context *ctx, const cpd_server_opts *o) {
     if ((st = d_results.reserve(max_designs * sizeof(cpd_fitness), false, stream)) != CPD_OK) return st;
     if ((st = d_acc.reserve(acc_cap * sizeof(unsigned long long), true, stream)) != CPD_OK) return st;
     if ((st = d_last_bin.reserve(lb_cap * sizeof(uint32_t), false, stream)) != CPD_OK) return st;
-    if ((st = d_counter.reserve(256, true, stream)) != CPD_OK) return st;
+    if ((st = d_counter.reserve(kWorkCounterWords * sizeof(unsigned int), true, stream)) != CPD_OK) return st;
     if ((st = h_results.reserve(max_designs * sizeof(cpd_fitness))) != CPD_OK) return st;
     SRV_TRY(hipStreamSynchronize(stream));
     slots.reset(max_designs);
@@ -647,7 +647,7 @@ cpd_status Server::step() {
     a.hist_words = hist_words;
     a.row_words = row_words;
     a.n_items = items;
-    SRV_TRY(launch_zero_u32(stream, a.work_counter, 1));
+    SRV_TRY(launch_zero_u32(stream, a.work_counter, kWorkCounterWords));
     const uint32_t grid = (uint32_t)std::min<uint64_t>(items, (uint64_t)cu_count * waves_per_cu);
     SRV_TRY(launch_trace(stream, a, grid, false));
     if (F) {

@@ -327,7 +327,7 @@ cpd_status cpd_fitness_batch(cpd_ctx *ctx, const cpd_design *designs, const cpd_
 #define CPD_NOALLOC_MAX_RAYS 65536
 #define CPD_NOALLOC_MAX_ROWS (1u << 18)
 #define CPD_NOALLOC_MAX_CELLS (1u << 21)
-#define CPD_NOALLOC_SCRATCH_BYTES 23396864
+#define CPD_NOALLOC_SCRATCH_BYTES 23397632
 cpd_status cpd_fitness_batch_noalloc(cpd_ctx *ctx, const cpd_design *designs, const cpd_sampling *sampling,
                                      size_t n, cpd_fitness *out, cpd_status *per_design);
 
