@@ -229,7 +229,7 @@ def uneven_run(cpd, N, np, ctx, args):
     reference_ops = sum(s.n_wavelengths * s.n_rays * (85 + 42 * d.n_glass) for d, s in zip(D, S))
     log(f"uneven: trace kernel {kernel_ms:.2f} ms, {achieved:.2f} TFLOP/s; items {stats['items']}")
 
-    # plain path on a prefix
+    # plain path on the first n_plain designs (the whole batch by default)
     n_plain = min(args.uneven_plain, n)
     f1 = N.Fitness()
     N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(darr[0]), ctypes.byref(sarr[0]), ctypes.byref(f1))
@@ -238,7 +238,8 @@ def uneven_run(cpd, N, np, ctx, args):
         if N.lib.cpd_fitness_gpu(ctx.handle, ctypes.byref(darr[i]), ctypes.byref(sarr[i]), ctypes.byref(f1)) != 0:
             raise RuntimeError(N.last_error())
     plain_s = time.perf_counter() - t
-    # the prefix's rays per second, expressed in designs of the batch's mean size
+    # those designs' rays per second, expressed in designs of the batch's mean size
+    # (with the whole batch: its designs per second)
     prefix_rays = sum(S[i].n_wavelengths * S[i].n_rays for i in range(n_plain))
     plain_value = prefix_rays / plain_s / (rays / n)
     log(f"uneven: plain {n_plain / plain_s:.0f} designs/s on the prefix, {plain_value:.0f} scaled to the batch mix")
@@ -613,7 +614,8 @@ def main() -> int:
                     help="designs of the uneven batch (1-6 prisms, 16-128 bins, 32-256 wavelengths x 4096-65536 "
                          "rays), autotuned and replayed as a captured generation (0: skip)")
     ap.add_argument("--uneven-steps", type=int, default=5)
-    ap.add_argument("--uneven-plain", type=int, default=32, help="uneven designs timed through the plain path")
+    ap.add_argument("--uneven-plain", type=int, default=4096,
+                    help="uneven designs timed through the plain path (the first ones of the batch)")
     ap.add_argument("--uneven-check", type=int, default=6, help="uneven designs checked against the FP64 oracle")
     ap.add_argument("--adaptive", type=int, default=1024,
                     help="designs of the adaptive phase: the reference GPU estimator as the optimiser calls it "

@@ -14,9 +14,9 @@ namespace cpd {
 // integers and independent of the order in which rays and tiles are added.
 static constexpr double kQuantScale = 4194304.0;
 static constexpr int kQueueCap = 128;   // per-wave queue of rays for the FP64 re-trace
-// Positions of a tile whose bands are tested at once before their rays (one
-// 64-bit skip mask per 4096 positions in LDS): four blocks of 4096, whose 32
-// super-bands of 512 positions one pass of the wavefront tests.
+// Positions of a chunk, whose bands' skip masks are set at once before their
+// rays (one 64-bit mask per 4096 positions in LDS, four words; four more hold
+// the tile's lost sub-bands): four blocks of 4096.
 static constexpr uint32_t kMaskChunk = 16384;
 // The persistent grid's work counters: kWorkCounters of them, kCounterStride
 // u32 apart (a 128-byte line each); counter x hands out items x,
@@ -190,7 +190,7 @@ CPD_HD uint32_t trace_chain_offset(uint32_t hist_words, uint32_t row_words, int 
 }
 CPD_HD inline size_t trace_lds_bytes_d(uint32_t hist_words, uint32_t row_words, int acc) {
     return (size_t)trace_chain_offset(hist_words, row_words, acc) + sizeof(Chain64) + sizeof(LensBand64) +
-           (kMaskChunk >> 12) * sizeof(uint64_t) + sizeof(LambdaConsts);
+           2 * (kMaskChunk >> 12) * sizeof(uint64_t) + sizeof(LambdaConsts);
 }
 size_t trace_lds_bytes(uint32_t hist_words, uint32_t row_words, int acc);
 // The resident loop's LDS: the trace regions, then copies of the claimed

@@ -182,7 +182,7 @@ struct TileCtx {
     uint32_t last_pos;   // position of ray k = n_rays
     const Chain64 *chain;  // the wavelength's planar faces in FP64 (LDS)
     const LensBand64 *band;  // the wavelength's lens and acceptance constants in FP64 (LDS)
-    uint64_t *masks;         // skip masks of the current chunk's bands (LDS)
+    uint64_t *masks;         // skip masks of the current chunk's bands, then the tile's lost sub-bands (LDS)
     const LambdaConsts *lc_lds;  // lc, as lane 0 stored it (LDS)
 };
 
@@ -258,33 +258,6 @@ __device__ __forceinline__ ChunkMap chunk_map(uint32_t chunk, uint32_t end, uint
     return ChunkMap{chunk, end, inter};
 }
 
-// Lowest and highest FP64 beam height of bands first .. first + n - 1 of the
-// chunk (n a power of two up to 8), and whether those bands are whole banded
-// positions lying in one sorted run (in position order: one block), so that
-// the heights of their ends bound theirs.
-__device__ bool band_heights(const TileCtx &c, const ChunkMap &m, uint32_t first, uint32_t n, double &ya,
-                             double &yb) {
-    if (m.inter) {
-        const uint32_t r0 = 16u * first, r1 = 16u * (first + n) - 1u;  // ranks in each block
-        double lo = INFINITY, hi = -INFINITY;
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-            const uint32_t b = m.chunk - 1u + j * kBandRays;
-            const double a = beam_y64(*c.g, c.zs64[b + r0]), e = beam_y64(*c.g, c.zs64[b + r1]);
-            lo = fmin(lo, fmin(a, e));
-            hi = fmax(hi, fmax(a, e));
-        }
-        ya = lo;
-        yb = hi;
-        return true;
-    }
-    const uint32_t p0 = m.chunk + 64u * first, p1 = p0 + 64u * n - 1u;
-    if (!(p0 < m.end && p1 <= c.full_end && ((p0 - 1u) >> 12) == ((p1 - 1u) >> 12))) return false;
-    ya = beam_y64(*c.g, c.zs64[p0 - 1]);
-    yb = beam_y64(*c.g, c.zs64[p1 - 1]);
-    return true;
-}
-
 // Position of the n-th (from 0) set bit of m (n < popcount(m)).
 __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
     uint32_t pos = 0;
@@ -300,53 +273,78 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
     return pos;
 }
 
-// Skip masks of the bands of a chunk: band i is skipped when band_may_hit
-// proves that the FP64 trace loses every one of its rays.  Word g of `masks`
-// holds bands 64 g .. 64 g + 63.  The wavefront first classifies the 32
-// super-bands of 8 bands (one lane each).  It skips the bands of a lost one
-// together (interval arithmetic only widens with its interval, so every band
-// of it is lost as well) and tests none of a certain one (every band of it
-// may hit); the bands of the mixed ones it tests 64 to a pass.
-__device__ void band_masks(const TileCtx &c, const ChunkMap &m, uint64_t *masks) {
+// Sub-bands of a tile: sub-band u (0..255) is the ranks 16 u .. 16 u + 15 of
+// every banded block the tile's positions touch, group s (0..63) the ranks
+// 64 s .. 64 s + 63.  Each block is sorted by z, so the heights of a group or
+// sub-band lie between the FP64 heights of its lowest and highest rank over
+// those blocks.  The wavefront classifies the 64 groups (one lane each), then
+// the four sub-bands of every mixed group (64 to a pass), and sets bit u of
+// `sub` (four words) when band_class proves sub-band u lost: every sub-band
+// of a lost group, and those of the mixed groups that test lost.  Interval
+// arithmetic only widens with its interval, and every position of a chunk's
+// band lies in the sub-bands its ranks fall in, so a band all of whose
+// sub-bands are lost is lost (band_masks).
+__device__ void tile_band_classes(const TileCtx &c, uint64_t *sub) {
     const uint32_t lane = threadIdx.x;
-    const uint32_t words = (m.end - m.chunk + 4095u) >> 12;
-    int cls = kBandMixed;
-    if (lane < 8u * words) {
-        double ya, yb;
-        if (band_heights(c, m, 8u * lane, 8u, ya, yb)) cls = band_class(*c.band, ya, yb);
+    const uint32_t b0 = (c.k_begin - 1u) >> 12, b1 = (c.k_end - 2u) >> 12;  // blocks of positions k_begin .. k_end - 1
+    double zlo = INFINITY, zhi = -INFINITY;
+    for (uint32_t b = b0; b <= b1; ++b) {
+        zlo = fmin(zlo, c.zs64[(b << 12) + 64u * lane]);
+        zhi = fmax(zhi, c.zs64[(b << 12) + 64u * lane + 63u]);
     }
-    const uint64_t super_lost = __builtin_amdgcn_ballot_w64(cls == kBandLost);  // bit = super-band
-    const uint64_t super_mixed = __builtin_amdgcn_ballot_w64(cls == kBandMixed);
-    // bands of the mixed super-bands, per word; the lost ones start the masks
-    uint64_t todo[4];
-    uint32_t first[5];
-    first[0] = 0;
+    const int cls = band_class(*c.band, beam_y64(*c.g, zlo), beam_y64(*c.g, zhi));
+    const uint64_t lost = __builtin_amdgcn_ballot_w64(cls == kBandLost);  // bit = group
+    const uint64_t mixed = __builtin_amdgcn_ballot_w64(cls == kBandMixed);
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) {
-        todo[w] = 0;
-        if (w < words) {
-            const uint32_t i = 64u * w + lane;  // band
-            const bool whole = m.inter || (m.chunk + 64u * i < m.end && m.chunk + 64u * i + 63u <= c.full_end);
-            const bool in_lost = (super_lost >> (i >> 3)) & 1u;
-            const bool in_mixed = (super_mixed >> (i >> 3)) & 1u;
-            todo[w] = __builtin_amdgcn_ballot_w64(whole && in_mixed);
-            const uint64_t lost = __builtin_amdgcn_ballot_w64(whole && in_lost);
-            if (lane == 0) masks[w] = lost;
-        }
-        first[w + 1] = first[w] + (uint32_t)__popcll(todo[w]);
+        const uint64_t v = __builtin_amdgcn_ballot_w64((lost >> (16u * w + (lane >> 2))) & 1u);
+        if (lane == 0) sub[w] = v;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    for (uint32_t p = 0; p < first[4]; p += 64) {
+    const uint32_t n_sub = 4u * (uint32_t)__popcll(mixed);
+    for (uint32_t p = 0; p < n_sub; p += 64) {
         const uint32_t k = p + lane;
-        if (k < first[4]) {
-            uint32_t w = 0;
+        if (k < n_sub) {
+            const uint32_t u = 4u * nth_set_bit(mixed, k >> 2) + (k & 3u);
+            double lo = INFINITY, hi = -INFINITY;
+            for (uint32_t b = b0; b <= b1; ++b) {
+                lo = fmin(lo, c.zs64[(b << 12) + 16u * u]);
+                hi = fmax(hi, c.zs64[(b << 12) + 16u * u + 15u]);
+            }
+            if (!band_may_hit(*c.band, beam_y64(*c.g, lo), beam_y64(*c.g, hi)))
+                atomicOr((unsigned long long *)&sub[u >> 6], 1ull << (u & 63u));
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+}
+
+// Skip masks of the bands of a chunk from its tile's lost sub-bands `sub`:
+// word g of `masks` holds bands 64 g .. 64 g + 63.  Band i of an interleaved
+// chunk is the ranks 16 i .. 16 i + 15 of its four blocks, inside sub-band i;
+// a band of 64 consecutive positions in one block covers the sub-bands of its
+// ranks.  A band is skipped when every sub-band it covers is lost.  Bands
+// reaching past the whole blocks are never skipped.
+__device__ void band_masks(const TileCtx &c, const ChunkMap &m, uint64_t *masks, const uint64_t *sub) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t words = (m.end - m.chunk + 4095u) >> 12;
 #pragma unroll
-            for (uint32_t q = 1; q < 4; ++q) w += k >= first[q] ? 1u : 0u;
-            const uint32_t i = nth_set_bit(todo[w], k - first[w]);
-            double ya, yb;
-            if (band_heights(c, m, 64u * w + i, 1u, ya, yb) && !band_may_hit(*c.band, ya, yb))
-                atomicOr((unsigned long long *)&masks[w], 1ull << i);
+    for (uint32_t w = 0; w < 4; ++w) {
+        if (w < words) {
+            bool lost;
+            if (m.inter) {
+                lost = (sub[w] >> lane) & 1u;
+            } else {
+                const uint32_t p0 = m.chunk + 64u * (64u * w + lane), p1 = p0 + 63u;
+                lost = p0 < m.end && p1 <= c.full_end && ((p0 - 1u) >> 12) == ((p1 - 1u) >> 12);
+                if (lost) {
+                    const uint32_t u0 = ((p0 - 1u) & 4095u) >> 4, u1 = ((p1 - 1u) & 4095u) >> 4;  // u1 - u0 <= 4
+                    for (uint32_t u = u0; u <= u1; ++u) lost = lost && ((sub[u >> 6] >> (u & 63u)) & 1u);
+                }
+            }
+            const uint64_t v = __builtin_amdgcn_ballot_w64(lost);
+            if (lane == 0) masks[w] = v;
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -413,15 +411,17 @@ __device__ void tile_loop(const TileCtx &c, unsigned long long &ops) {
     uint32_t head = 0, tail = 0;  // ring of queued ray indices (wave-uniform)
     const uint32_t fold = Acc<A>::fold_every(cols);
     uint32_t since_fold = 0;
-    // The tile runs in chunks of kMaskChunk positions.  Before a chunk's rays,
-    // the wavefront marks the bands of 64 banded positions whose every ray the
-    // FP64 trace provably loses at the lens, its acceptance test or the array
-    // ends (band_masks); the ray loop skips them.  Rays in ray order (past the
-    // last whole block) are never skipped, and op counting traces every band.
+    // The tile runs in chunks of kMaskChunk positions.  Before its rays, the
+    // wavefront proves sub-bands of the tile's ranks lost where the FP64 trace
+    // loses every ray of them at the lens, its acceptance test or the array
+    // ends (tile_band_classes); before each chunk's rays it marks the chunk's
+    // bands of 64 positions that lie in lost sub-bands (band_masks), and the
+    // ray loop skips them.  Op counting traces every band.
+    if (!COUNT) tile_band_classes(c, c.masks + (kMaskChunk >> 12));
     for (uint32_t chunk = c.k_begin; chunk < k_end; chunk += kMaskChunk) {
         const uint32_t chunk_end = min(chunk + kMaskChunk, k_end);
         const ChunkMap cmap = chunk_map(chunk, chunk_end, full_end);
-        if (!COUNT) band_masks(c, cmap, c.masks);
+        if (!COUNT) band_masks(c, cmap, c.masks, c.masks + (kMaskChunk >> 12));
         // register copies of the per-item records (made after the band tests,
         // whose FP64 registers they would otherwise share): the loop stores
         // through last_bin, so loads through c.td would be repeated every
@@ -583,7 +583,7 @@ __device__ __forceinline__ TraceLds trace_lds(uint32_t *lds, uint32_t hist_words
     L.chain = (Chain64 *)((char *)lds + trace_chain_offset(hist_words, row_words, A));
     L.band = (LensBand64 *)(L.chain + 1);
     L.masks = (uint64_t *)(L.band + 1);
-    L.lc_lds = (LambdaConsts *)(L.masks + (kMaskChunk >> 12));
+    L.lc_lds = (LambdaConsts *)(L.masks + 2u * (kMaskChunk >> 12));
     return L;
 }
 template <int A>

@@ -142,15 +142,17 @@ int guard_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t
 
 extern "C" {
 
-// The band test (band_may_hit, trace_core.h) over the banded beam order the
-// device traces in (within each block of 4096 rays sorted by height) and its
-// bands as the kernel forms them (an iteration's 64 rays: interleaved over
-// the four blocks of a whole 16384-position chunk, else 64 consecutive
-// positions): a band is skipped when its super-band of 8 bands or the band
-// itself is proven lost.  The positions cover the rays rounded up to whole
-// blocks; in the last block the positions of rays past n_rays are idle.
-// Heights are taken over every position of a band, idle ones included (the
-// kernel bounds them by the ends of each sorted run).
+// The band test (band_class, trace_core.h) over the banded beam order the
+// device traces in (within each block of 4096 rays sorted by height), as the
+// kernel applies it to a design whose rays form one tile: sub-band u is the
+// ranks 16 u .. 16 u + 15 of every block, group s the ranks 64 s .. 64 s + 63,
+// each bounded by the heights of its lowest and highest rank over the blocks;
+// the 64 groups are classified first, then the sub-bands of the mixed ones.
+// The kernel's bands are an iteration's 64 rays (interleaved over the four
+// blocks of a whole 16384-position chunk, else 64 consecutive positions); a
+// band is skipped when every sub-band its positions' ranks fall in is lost.
+// The positions cover the rays rounded up to whole blocks; in the last block
+// the positions of rays past n_rays are idle (and count towards the bounds).
 // stats[0] bands, [1] bands a wavefront would take past the planar-chain
 // test, [2] of those skipped, [3] skipped bands (any) holding an FP64 hit
 // (must be 0), [4] bands holding an FP64 hit, [5] wavelengths the test over
@@ -203,16 +205,31 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                 cpd::LensBand64 band;
                 cpd::make_lambda_consts(g, lambda, lc, &chain, &band);
                 const cpd::HotRecord hot = cpd::make_hot(td, lc);
-                // heights of bands b .. b + nb - 1 (one chunk, one super-band at most)
-                auto heights = [&](size_t b, size_t nb, double &lo, double &hi) {
-                    lo = INFINITY;
-                    hi = -INFINITY;
-                    for (size_t q = 64 * b; q < 64 * (b + nb); ++q) {
-                        const double y = cpd::beam_y64(g, z64[order[band_pos[q]]]);
-                        lo = fmin(lo, y);
-                        hi = fmax(hi, y);
+                // lowest and highest height of ranks r0 .. r1 over every block
+                const uint32_t nblk = full / cpd::kBandRays;
+                auto hull = [&](uint32_t r0, uint32_t r1, double &lo, double &hi) {
+                    double zl = INFINITY, zh = -INFINITY;
+                    for (uint32_t q = 0; q < nblk; ++q) {
+                        zl = fmin(zl, z64[order[q * cpd::kBandRays + r0]]);
+                        zh = fmax(zh, z64[order[q * cpd::kBandRays + r1]]);
                     }
+                    lo = cpd::beam_y64(g, zl);
+                    hi = cpd::beam_y64(g, zh);
                 };
+                bool sub_lost[256];
+                for (uint32_t gs = 0; gs < 64; ++gs) {
+                    double ya, yb;
+                    hull(64 * gs, 64 * gs + 63, ya, yb);
+                    const int cls = cpd::band_class(band, ya, yb);
+                    for (uint32_t q = 0; q < 4; ++q) {
+                        const uint32_t u = 4 * gs + q;
+                        sub_lost[u] = cls == cpd::kBandLost;
+                        if (cls == cpd::kBandMixed) {
+                            hull(16 * u, 16 * u + 15, ya, yb);
+                            sub_lost[u] = !cpd::band_may_hit(band, ya, yb);
+                        }
+                    }
+                }
                 // the records kernel's test over every height of the design's
                 // positions (idle ones included)
                 double wa = INFINITY, wb = -INFINITY;
@@ -224,15 +241,8 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                 const bool dead = lc.ok && cpd::band_class(band, wa, wb) == cpd::kBandLost;
                 loc[5] += dead;
                 bool any_hit = false;
-                bool super_lost = false;
                 for (size_t b = 0; b < n_bands; ++b) {
                     loc[0]++;
-                    if (b % 8 == 0) {
-                        // the super-band of 8 bands the kernel tests first
-                        double sa, sb;
-                        heights(b, 8, sa, sb);
-                        super_lost = lc.ok && !cpd::band_may_hit(band, sa, sb);
-                    }
                     bool chain_pass = false, hit = false;
                     for (uint32_t q = 0; q < 64; ++q) {
                         const uint32_t k = order[band_pos[64 * b + q]];
@@ -247,9 +257,8 @@ int band_check(const cpd_design *designs, size_t n, uint32_t n_lambda, uint32_t 
                     loc[4] += hit;
                     any_hit |= hit;
                     if (!lc.ok) continue;
-                    double ya, yb;
-                    heights(b, 1, ya, yb);
-                    const bool skipped = super_lost || !cpd::band_may_hit(band, ya, yb);
+                    bool skipped = true;
+                    for (uint32_t q = 0; q < 64; ++q) skipped = skipped && sub_lost[(band_pos[64 * b + q] % cpd::kBandRays) / 16];
                     if (chain_pass) {
                         loc[1]++;
                         loc[2] += skipped;

@@ -114,11 +114,13 @@ def band_stats(designs, n_lambda, n_rays):
                                                     (7, True, 16384), (1, False, 12288), (0, True, 12288),
                                                     (1, False, 10000), (0, True, 10000), (5, True, 2500)])
 def test_skipped_bands_hold_no_hit(config, transmit, n_rays):
-    # the trace kernel skips a band of 64 banded rays when band_may_hit proves
+    # the trace kernel skips a band of 64 banded rays when band_class proves
     # (interval arithmetic on the lens and the acceptance cosine) that the FP64
-    # trace loses every ray of it; no skipped band may hold an FP64 hit (ray
-    # counts that are not a multiple of 4096 leave idle positions in the last
-    # block, whose heights the band bounds include)
+    # trace loses every ray of each sub-band its rays fall in (ranks
+    # 16 u .. 16 u + 15 of every block of the tile, tested once per tile); no
+    # skipped band may hold an FP64 hit (ray counts that are not a multiple of
+    # 4096 leave idle positions in the last block, whose heights the
+    # sub-band bounds include)
     import compound_prism_designer as cpd
     D = transmitting(config, 48)[1] if transmit else cpd.generate_designs(config, 0, 96)[1]
     st = band_stats(D, 16, n_rays)
