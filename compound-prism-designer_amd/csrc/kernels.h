@@ -142,7 +142,7 @@ struct ResidentArgs {
     unsigned int *items_total, *items_done;  // per slot
     uint32_t *slot_tag;                      // per slot
     unsigned long long *ingested;            // per ring position: c + 1 once chunk c's records are in the pools
-    unsigned long long *claim_word;          // per ring position: (c mod 2^24, items, claimed)
+    unsigned long long *claim_word;          // [kWorkCounters][chunk_cap]: (c mod 2^24, items, claimed) per group
     uint32_t *chunk_slot0, *chunk_n;         // per ring position
     uint32_t *prefix;                        // per ring position: 65 inclusive item prefix sums over its designs
     unsigned long long *ctrl;                // kRes* counters

@@ -939,13 +939,19 @@ __global__ __launch_bounds__(64) void finalize_kernel(FinalizeArgs a) {
 // Device-resident request loop (ResidentArgs in kernels.h).
 //
 // Chunk c (at most 64 designs) sits at ring position c % chunk_cap.  Its
-// claim word packs (c mod 2^24, items of the chunk, items claimed) as bits
-// 63..40, 39..20, 19..0; ingest stores it last, with release, so a wavefront
-// whose atomic add returns chunk c's word owns item `claimed` of chunk c and
-// sees c's tables, even if it meant to claim from an older chunk at that
-// position.  A chunk holds fewer than 2^18 items, and the count never nears
-// 2^20: past the total only wavefronts that find the chunk exhausted add to
-// it, once each, before the head moves on.  Items per design are counted down
+// items are handed out by kWorkCounters claim words, one per group of
+// wavefronts (the workgroup index mod kWorkCounters, which is also its XCD),
+// so the wavefronts of the chip do not all meet at one atomic: word g hands
+// out items g, g + kWorkCounters, ... and packs (c mod 2^24, items it holds,
+// items claimed) as bits 63..40, 39..20, 19..0.  Ingest stores the words
+// last, with release, so a wavefront whose atomic add returns chunk c's word
+// owns the item of its count in chunk c and sees c's tables, even if it meant
+// to claim from an older chunk at that position.  A wavefront tries its own
+// group's word first and moves on to the next when it is spent.  A chunk
+// holds fewer than 2^18 items, and no count nears 2^20: past a word's total
+// only claims that go on to another word add to it, and once every word is
+// spent each wavefront adds to each at most once before the head moves on.
+// Items per design are counted down
 // in device memory; the wavefront that finishes a design's last item
 // finalizes it.  Host memory is read only after a system-scope acquire and
 // written only with plain stores followed by a system-scope release.
@@ -1038,9 +1044,15 @@ __device__ __attribute__((noinline)) bool resident_ingest(const ResidentArgs &a,
     a.chunk_n[pos] = n;
     const bool fits = total < (1u << 18);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane < kWorkCounters) {
+        // word `lane`: items lane, lane + kWorkCounters, ... below the total
+        const uint32_t held = fits ? (total + kWorkCounters - 1u - lane) / kWorkCounters : 0u;
+        const unsigned long long word = ((c & 0xffffffull) << 40) | ((unsigned long long)held << kResCountBits);
+        __hip_atomic_store(a.claim_word + (size_t)lane * a.chunk_cap + pos, word, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (lane == 0) {
-        const unsigned long long word = ((c & 0xffffffull) << 40) | ((unsigned long long)(fits ? total : 0) << kResCountBits);
-        __hip_atomic_store(a.claim_word + pos, word, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(a.ingested + pos, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         if (!ok || !fits) a.host->error = 1;
     }
@@ -1069,17 +1081,58 @@ __device__ __attribute__((noinline)) void resident_finalize(const ResidentArgs &
                                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// One work item of the resident loop, compiled as a function of its own: its
+// registers are allocated apart from the loop's claim and ingest state, which
+// inlined leaves the ray loop spilling.  A function's arguments arrive as
+// per-lane values, so the wave-uniform ones are taken back into scalar
+// registers, and the LDS is reached through the kernel's dynamic shared array
+// (declared here as well), so its accesses stay LDS instructions.
+struct ResidentTables {
+    uint32_t hist_words, row_words;
+    const uint2 *zk;
+    const double *zs64;
+    const uint32_t *inv;
+};
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane(v); }
+// A wave-uniform pointer to global memory: the cast through the global
+// address space lets the compiler use global (not flat) instructions, which
+// LDS waits do not wait for.
+template <class T> __device__ __forceinline__ T *uniform_ptr(T *p) {
+    const uint64_t v = (uint64_t)p;
+    using GT = __attribute__((address_space(1))) T;
+    GT *q = (GT *)(((uint64_t)uniform32((uint32_t)(v >> 32)) << 32) | uniform32((uint32_t)v));
+    return (T *)q;
+}
+__device__ __attribute__((noinline)) void resident_item(ResidentTables t, uint32_t j, uint32_t tile,
+                                                        unsigned long long *acc, uint64_t acc_j, uint32_t *last_bin_j) {
+    extern __shared__ __align__(8) uint32_t lds[];
+    t.hist_words = uniform32(t.hist_words);
+    t.row_words = uniform32(t.row_words);
+    t.zk = uniform_ptr(t.zk);
+    t.zs64 = uniform_ptr(t.zs64);
+    t.inv = uniform_ptr(t.inv);
+    j = uniform32(j);
+    tile = uniform32(tile);
+    acc = uniform_ptr(acc);
+    acc_j = ((uint64_t)uniform32((uint32_t)(acc_j >> 32)) << 32) | uniform32((uint32_t)acc_j);
+    last_bin_j = uniform_ptr(last_bin_j);
+    const cpd_design *g = resident_design_lds(lds, t.hist_words, t.row_words);
+    const TraceDesign *td = (const TraceDesign *)(g + 1);
+    unsigned long long ops = 0;
+    run_item<false, CPD_ACC_EXACT, false>(t, lds, g, td, j, tile, lambda_of(*g, qmc_u64(j + 1)), nullptr, acc, acc_j,
+                                          last_bin_j, ops);
+}
+
 // Wavefronts 0..kResPollers-1 poll host memory (new chunks, the stop epoch)
 // once per loop and copy what they read into device counters; the others read
 // only those counters, so the host link carries a handful of polls.
 constexpr uint32_t kResPollers = 4;
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void resident_kernel(ResidentArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void resident_kernel(ResidentArgs a) {
     extern __shared__ __align__(8) uint32_t lds[];
     trace_lds_clear<CPD_ACC_EXACT>(lds, a.hist_words, a.row_words);
     const uint32_t lane = threadIdx.x;
     const bool poller = blockIdx.x < kResPollers;
-    unsigned long long ops = 0;
     uint32_t idle = 0;
     for (;;) {
         if (poller && lane == 0) {
@@ -1132,14 +1185,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void re
                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     st = 1;
                 } else if (ing == h + 1) {
-                    const unsigned long long w = __hip_atomic_fetch_add(a.claim_word + pos, 1ull, __ATOMIC_RELAXED,
-                                                                        __HIP_MEMORY_SCOPE_AGENT);
-                    const unsigned long long x = w & kResCountMask, total = (w >> kResCountBits) & kResCountMask;
-                    if (x < total) {
-                        st = 2;
-                        pos_x = ((unsigned long long)pos << 32) | x;
-                    } else {
-                        if ((w >> 40) == (h & 0xffffffull))
+                    // the group's own word first, then the others
+                    uint32_t grp = blockIdx.x & (kWorkCounters - 1u);
+                    bool spent = true;  // every word of chunk h is spent
+                    for (uint32_t t = 0; t < kWorkCounters; ++t, grp = (grp + 1u) & (kWorkCounters - 1u)) {
+                        const unsigned long long w = __hip_atomic_fetch_add(
+                            a.claim_word + (size_t)grp * a.chunk_cap + pos, 1ull, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned long long x = w & kResCountMask, held = (w >> kResCountBits) & kResCountMask;
+                        if (x < held) {
+                            st = 2;
+                            pos_x = ((unsigned long long)pos << 32) | (grp + (unsigned long long)kWorkCounters * x);
+                            break;
+                        }
+                        if ((w >> 40) != (h & 0xffffffull)) {
+                            spent = false;  // the position holds another chunk now
+                            break;
+                        }
+                    }
+                    if (st != 2) {
+                        if (spent)
                             __hip_atomic_compare_exchange_strong(a.ctrl + kResHead * kResLine, &h, h + 1, __ATOMIC_RELAXED,
                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         st = 1;
@@ -1198,8 +1263,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void re
         const uint32_t n_tiles = td->n_tiles;
         const uint32_t j = local / n_tiles, tile = local - j * n_tiles;
         if (j < td->n_lambda) {
-            run_item<false, CPD_ACC_EXACT, false>(a, lds, g, td, j, tile, lambda_of(*g, qmc_u64(j + 1)), nullptr,
-                                                  a.acc, acc_b + (uint64_t)j * td->nbin, a.last_bin + lb_b + j, ops);
+            resident_item(ResidentTables{a.hist_words, a.row_words, a.zk, a.zs64, a.inv}, j, tile, a.acc,
+                          acc_b + (uint64_t)j * td->nbin, a.last_bin + lb_b + j);
         } else if (lane == 0) {
             a.host->error = 2;
         }

@@ -240,7 +240,7 @@ cpd_status Server::init_resident(const cpd_server_opts *o) {
     if ((st = d_items_done.reserve(D * 4, false, stream)) != CPD_OK) return st;
     if ((st = d_slot_tag.reserve(D * 4, false, stream)) != CPD_OK) return st;
     if ((st = d_ingested.reserve(C * 8, true, stream)) != CPD_OK) return st;
-    if ((st = d_claim.reserve(C * 8, true, stream)) != CPD_OK) return st;
+    if ((st = d_claim.reserve(C * 8 * kWorkCounters, true, stream)) != CPD_OK) return st;
     if ((st = d_chunk_slot0.reserve(C * 4, false, stream)) != CPD_OK) return st;
     if ((st = d_chunk_n.reserve(C * 4, false, stream)) != CPD_OK) return st;
     if ((st = d_prefix.reserve(C * 65 * 4, false, stream)) != CPD_OK) return st;
