@@ -47,11 +47,11 @@ METRIC = "designs/sec (batched vs plain GPU) at ≤1e-4 bit MI error; p50/p99 re
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the batched trace kernel on this workload, from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy's readings
-TRACE_TRAFFIC_BYTES = 253.2e6
+TRACE_TRAFFIC_BYTES = 180.5e6
 # rocprofv3 counters of the same kernel and workload (the committed profile of this command)
-TRACE_COUNTERS = {"valu_insts_per_64_ray_iteration": 15.7, "salu_insts_per_64_ray_iteration": 10.0,
+TRACE_COUNTERS = {"valu_insts_per_64_ray_iteration": 15.4, "salu_insts_per_64_ray_iteration": 9.8,
                   "valu_issue_per_cu_cycle": 0.63, "trans_f32_insts_per_64_ray_iteration": 0.43,
-                  "hw_fp32_tflops": 10.8, "hw_fp32_frac": 0.069,
+                  "hw_fp32_tflops": 11.0, "hw_fp32_frac": 0.070,
                   "source": "profiles/trace_kernel_batch1024_p3.md (SQ_INSTS_VALU, SQ_INSTS_SALU, "
                             "SQ_INSTS_VALU_TRANS_F32, SQ_INSTS_VALU_{FMA,MUL,ADD}_F32)"}
 TRACE_TRAFFIC_SOURCE = ("profiles/trace_kernel_batch1024_p3.md (rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x "
