@@ -247,16 +247,19 @@ def test_ray_counts_off_the_block_size_match_oracle(n_rays):
         np.testing.assert_allclose(out, ref, atol=2e-6, err_msg=f"{n_rays} rays")
 
 
-def test_tiles_that_split_the_last_block():
+@pytest.mark.parametrize("tile", [1024, 1088, 4160, 12352])
+def test_tiles_that_split_the_last_block(tile):
     # work items of 1024 positions split a design's last, partly idle block
-    # across items; the sums (exact integers) match one item per wavelength
-    # and the oracle
+    # across items; tiles that are no multiple of the 4096-position block start
+    # inside blocks and span two or more, so their band tests bound ranks over
+    # partial blocks and their chunks are traced in position order.  The sums
+    # (exact integers) match one item per wavelength and the oracle
     _, D, _, _ = transmitting(0, 8)
     S = [N.Sampling(8, 5000 + 320 * i, N.MODE_FIXED, 0) for i in range(len(D))]
     ctx = cpd.Context(0)
     whole, st = cpd.fitness_batch(D, S, ctx=ctx)
     assert (st == 0).all()
-    ctx.set_schedule(N.Schedule(1024, 16, N.ORDER_GIVEN, 0))
+    ctx.set_schedule(N.Schedule(tile, 16, N.ORDER_GIVEN, 0))
     split, st = cpd.fitness_batch(D, S, ctx=ctx)
     ctx.set_schedule(None)
     assert (st == 0).all()
